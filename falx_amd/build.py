@@ -1,0 +1,66 @@
+"""Build the native pieces in-tree: libfalx_gpu.so (hipcc, gfx950) and the
+dataset generator libfalx_datagen.so (g++). Outputs land in falx_amd/_build/.
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+BUILD = os.path.join(HERE, "_build")
+REPO = os.path.dirname(HERE)
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+
+GPU_SOURCES = ["csrc/falx_gpu.hip"]
+GPU_DEPS = ["csrc/falx_gpu.hip", "csrc/scan.hip", "csrc/cells.h", "csrc/ndjson_walk.h"]
+# Packed-FP32 instructions stay off in every device compile.
+HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
+             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+             "-Wno-unused-command-line-argument"]
+
+
+def gpu_lib_path() -> str:
+    return os.path.join(BUILD, "libfalx_gpu.so")
+
+
+def datagen_lib_path() -> str:
+    return os.path.join(BUILD, "libfalx_datagen.so")
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, cwd=HERE, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError(f"build failed: {' '.join(cmd)}")
+    return r
+
+
+def build_gpu() -> str:
+    os.makedirs(BUILD, exist_ok=True)
+    out = gpu_lib_path()
+    tmp = out + ".tmp"
+    _run([HIPCC, *HIP_FLAGS, "-I", os.path.join(REPO, "include"), "-o", tmp,
+          *[os.path.join(HERE, s) for s in GPU_SOURCES]])
+    os.replace(tmp, out)
+    return out
+
+
+def build_datagen() -> str:
+    os.makedirs(BUILD, exist_ok=True)
+    out = datagen_lib_path()
+    tmp = out + ".tmp"
+    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-o", tmp,
+          os.path.join(HERE, "datagen", "datagen.cpp")])
+    os.replace(tmp, out)
+    return out
+
+
+def build_all() -> None:
+    build_datagen()
+    build_gpu()
+
+
+if __name__ == "__main__":
+    build_all()
+    print("built", gpu_lib_path(), datagen_lib_path())

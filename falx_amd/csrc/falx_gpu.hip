@@ -1,0 +1,564 @@
+// falx_gpu.hip — C ABI of include/falx_gpu.h over the gfx950 kernels in
+// scan.hip. One context = one device + one HIP stream + reusable scratch
+// (look-back state, per-tile partials, result slots).
+#include "scan.hip"
+
+#include <mutex>
+#include <new>
+
+using namespace fx;
+
+struct falx_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    float last_ms = 0.f;
+    unsigned long long* d_ticket = nullptr;
+    unsigned long long ticket_next = 0;
+    uint32_t epoch = 0;
+    unsigned long long* d_result = nullptr;
+    unsigned long long* h_result = nullptr;
+    void* d_lb = nullptr;
+    size_t lb_cap = 0;
+    void* d_part = nullptr;
+    size_t part_cap = 0;
+    void* d_red = nullptr;       // reduction outputs (StatPart[32] + extras)
+};
+
+namespace {
+
+constexpr size_t RED_BYTES = 64 * 1024;
+
+inline falx_status hip_status(hipError_t e) { return e == hipSuccess ? FALX_OK : FALX_E_DEVICE; }
+
+#define HIP_TRY(x)                                     \
+    do {                                               \
+        hipError_t _e = (x);                           \
+        if (_e != hipSuccess) return FALX_E_DEVICE;    \
+    } while (0)
+
+inline size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+struct LbLayout {
+    size_t lb1, lb2_flag, lb2_agg, lb2_inc, lb3_flag, lb3_agg, lb3_inc, total;
+};
+
+LbLayout lb_layout(size_t n) {
+    LbLayout L;
+    size_t o = 0;
+    L.lb1 = o; o = align_up(o + 8 * n, 256);
+    L.lb2_flag = o; o = align_up(o + 4 * n, 256);
+    L.lb2_agg = o; o = align_up(o + sizeof(Counts) * n, 256);
+    L.lb2_inc = o; o = align_up(o + sizeof(Counts) * n, 256);
+    L.lb3_flag = o; o = align_up(o + 4 * n, 256);
+    L.lb3_agg = o; o = align_up(o + 4 * FALX_MAX_COLUMNS * n, 256);
+    L.lb3_inc = o; o = align_up(o + 4 * FALX_MAX_COLUMNS * n, 256);
+    L.total = o;
+    return L;
+}
+
+falx_status ensure_lb(falx_ctx* c, uint32_t n_tiles) {
+    LbLayout L = lb_layout(n_tiles);
+    if (L.total <= c->lb_cap) return FALX_OK;
+    if (c->d_lb) HIP_TRY(hipFree(c->d_lb));
+    c->d_lb = nullptr;
+    c->lb_cap = 0;
+    size_t cap = align_up(L.total + L.total / 4, 1 << 20);
+    HIP_TRY(hipMalloc(&c->d_lb, cap));
+    HIP_TRY(hipMemsetAsync(c->d_lb, 0, cap, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    c->lb_cap = cap;
+    return FALX_OK;
+}
+
+falx_status ensure_part(falx_ctx* c, size_t bytes) {
+    if (bytes <= c->part_cap) return FALX_OK;
+    if (c->d_part) HIP_TRY(hipFree(c->d_part));
+    c->d_part = nullptr;
+    c->part_cap = 0;
+    size_t cap = align_up(bytes + bytes / 4, 1 << 20);
+    HIP_TRY(hipMalloc(&c->d_part, cap));
+    c->part_cap = cap;
+    return FALX_OK;
+}
+
+int dialect_kind(const falx_spec* s) {
+    bool hq = s->quote >= 0, esc = hq && s->escape >= 0, cmt = s->comment >= 0;
+    if (!hq) return cmt ? 1 : 0;
+    if (!cmt) return esc ? 3 : 2;
+    return esc ? 5 : 4;
+}
+
+template <int MODE>
+hipError_t launch_scan(const Params& P, int kind, hipStream_t s) {
+    dim3 g(P.n_tiles), b(WG);
+    switch (kind) {
+    case 0: hipLaunchKernelGGL((k_scan<false, false, false, MODE>), g, b, 0, s, P); break;
+    case 1: hipLaunchKernelGGL((k_scan<false, false, true, MODE>), g, b, 0, s, P); break;
+    case 2: hipLaunchKernelGGL((k_scan<true, false, false, MODE>), g, b, 0, s, P); break;
+    case 3: hipLaunchKernelGGL((k_scan<true, true, false, MODE>), g, b, 0, s, P); break;
+    case 4: hipLaunchKernelGGL((k_scan<true, false, true, MODE>), g, b, 0, s, P); break;
+    default: hipLaunchKernelGGL((k_scan<true, true, true, MODE>), g, b, 0, s, P); break;
+    }
+    return hipGetLastError();
+}
+
+// Logfmt and NDJSON-line counting use their fixed dialect (kind 3).
+template <int MODE>
+hipError_t launch_scan_fixed3(const Params& P, hipStream_t s) {
+    hipLaunchKernelGGL((k_scan<true, true, false, MODE>), dim3(P.n_tiles), dim3(WG), 0, s, P);
+    return hipGetLastError();
+}
+
+falx_status check_input(const falx_ctx* c, const uint8_t* d, uint64_t len) {
+    if (!c) return FALX_E_INVALID;
+    if (len >= (1ULL << 32)) return FALX_E_TOO_LARGE;
+    if (len > 0 && (!d || ((uintptr_t)d & 15u))) return FALX_E_INVALID;
+    return FALX_OK;
+}
+
+// Common Params for a launch over d_data[0, len).
+falx_status prepare(falx_ctx* c, const falx_spec* spec, const uint8_t* d, uint64_t len, Params& P) {
+    memset(&P, 0, sizeof P);
+    P.data = d;
+    P.len = (uint32_t)len;
+    P.n_tiles = (uint32_t)((len + TILE - 1) / TILE);
+    if (spec) {
+        for (uint32_t i = 0; i < spec->n_structural; ++i) {
+            uint8_t b = spec->structural[i];
+            if (b != '\n') P.cand[b >> 5] |= 1u << (b & 31);
+        }
+        bool nl = false, sep = false;
+        for (uint32_t i = 0; i < spec->n_structural; ++i) (spec->structural[i] == '\n' ? nl : sep) = true;
+        if (nl || !sep) P.cand['\n' >> 5] |= 1u << ('\n' & 31);
+        P.quote = spec->quote;
+        P.escape = spec->quote >= 0 ? spec->escape : -1;
+        P.comment = spec->comment;
+        P.n_cols = spec->n_columns;
+        uint32_t ns = 0;
+        for (uint32_t k = 0; k < spec->n_columns; ++k) {
+            P.cols[k].index = spec->columns[k].index;
+            P.cols[k].type = spec->columns[k].type;
+            P.cols[k].str_slot = spec->columns[k].type == FALX_COL_STR ? ns++ : 0;
+        }
+        P.n_str = ns;
+    }
+    falx_status st = ensure_lb(c, P.n_tiles);
+    if (st != FALX_OK) return st;
+    LbLayout L = lb_layout(P.n_tiles);
+    char* base = static_cast<char*>(c->d_lb);
+    P.lb1 = reinterpret_cast<unsigned long long*>(base + L.lb1);
+    P.lb2_flag = reinterpret_cast<uint32_t*>(base + L.lb2_flag);
+    P.lb2_agg = reinterpret_cast<Counts*>(base + L.lb2_agg);
+    P.lb2_inc = reinterpret_cast<Counts*>(base + L.lb2_inc);
+    P.lb3_flag = reinterpret_cast<uint32_t*>(base + L.lb3_flag);
+    P.lb3_agg = reinterpret_cast<uint32_t*>(base + L.lb3_agg);
+    P.lb3_inc = reinterpret_cast<uint32_t*>(base + L.lb3_inc);
+    if (++c->epoch >= 0x3FFFFFFFu) {
+        HIP_TRY(hipMemsetAsync(c->d_lb, 0, c->lb_cap, c->stream));
+        c->epoch = 1;
+    }
+    P.epoch = c->epoch;
+    P.ticket = c->d_ticket;
+    P.ticket_base = c->ticket_next;
+    c->ticket_next += P.n_tiles;
+    P.result = c->d_result;
+    return FALX_OK;
+}
+
+hipError_t fill_u64(hipStream_t s, unsigned long long* p, uint64_t n, unsigned long long v) {
+    if (n == 0) return hipSuccess;
+    uint64_t blocks = (n + WG - 1) / WG;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(k_fill_u64, dim3((uint32_t)blocks), dim3(WG), 0, s, p, n, v);
+    return hipGetLastError();
+}
+
+falx_status begin(falx_ctx* c) {
+    HIP_TRY(hipSetDevice(c->device));
+    HIP_TRY(fill_u64(c->stream, c->d_result, R_SLOTS, 0ULL));
+    HIP_TRY(hipEventRecord(c->ev0, c->stream));
+    return FALX_OK;
+}
+
+// Record the end event, bring the result slots home, and synchronise.
+falx_status finish(falx_ctx* c) {
+    HIP_TRY(hipEventRecord(c->ev1, c->stream));
+    HIP_TRY(hipMemcpyAsync(c->h_result, c->d_result, R_SLOTS * 8, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    float ms = 0;
+    HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    c->last_ms = ms;
+    if (c->h_result[R_ERROR]) return FALX_E_DEVICE;
+    return FALX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+falx_status falx_ctx_create(int device, falx_ctx** out) {
+    if (!out) return FALX_E_INVALID;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0 || device < 0 || device >= n) return FALX_E_NO_DEVICE;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) return FALX_E_NO_DEVICE;
+    if (strncmp(prop.gcnArchName, "gfx950", 6) != 0) return FALX_E_NO_DEVICE;
+    falx_ctx* c = new (std::nothrow) falx_ctx();
+    if (!c) return FALX_E_DEVICE;
+    c->device = device;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
+        hipMalloc(&c->d_ticket, 256) != hipSuccess || hipMalloc(&c->d_result, R_SLOTS * 8) != hipSuccess ||
+        hipMalloc(&c->d_red, RED_BYTES) != hipSuccess ||
+        hipHostMalloc(&c->h_result, R_SLOTS * 8, hipHostMallocDefault) != hipSuccess ||
+        hipMemset(c->d_ticket, 0, 256) != hipSuccess || hipDeviceSynchronize() != hipSuccess) {
+        falx_ctx_destroy(c);
+        return FALX_E_DEVICE;
+    }
+    *out = c;
+    return FALX_OK;
+}
+
+void falx_ctx_destroy(falx_ctx* c) {
+    if (!c) return;
+    // Teardown is best effort: errors here have no caller to report to.
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->d_lb) (void)hipFree(c->d_lb);
+    if (c->d_part) (void)hipFree(c->d_part);
+    if (c->d_red) (void)hipFree(c->d_red);
+    if (c->d_ticket) (void)hipFree(c->d_ticket);
+    if (c->d_result) (void)hipFree(c->d_result);
+    if (c->h_result) (void)hipHostFree(c->h_result);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+void* falx_ctx_stream(falx_ctx* c) { return c ? (void*)c->stream : nullptr; }
+
+double falx_ctx_last_kernel_ms(falx_ctx* c) { return c ? (double)c->last_ms : 0.0; }
+
+falx_status falx_device_alloc(falx_ctx* c, uint64_t bytes, void** d_out) {
+    if (!c || !d_out) return FALX_E_INVALID;
+    HIP_TRY(hipSetDevice(c->device));
+    HIP_TRY(hipMalloc(d_out, bytes ? bytes : 1));
+    return FALX_OK;
+}
+
+falx_status falx_device_free(falx_ctx* c, void* p) {
+    if (!c) return FALX_E_INVALID;
+    HIP_TRY(hipSetDevice(c->device));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (p) HIP_TRY(hipFree(p));
+    return FALX_OK;
+}
+
+falx_status falx_host_alloc_pinned(falx_ctx* c, uint64_t bytes, void** h_out) {
+    if (!c || !h_out) return FALX_E_INVALID;
+    HIP_TRY(hipSetDevice(c->device));
+    HIP_TRY(hipHostMalloc(h_out, bytes ? bytes : 1, hipHostMallocDefault));
+    return FALX_OK;
+}
+
+falx_status falx_host_free_pinned(falx_ctx* c, void* p) {
+    if (!c) return FALX_E_INVALID;
+    if (p) HIP_TRY(hipHostFree(p));
+    return FALX_OK;
+}
+
+falx_status falx_upload(falx_ctx* c, void* d_dst, const void* h_src, uint64_t bytes) {
+    if (!c) return FALX_E_INVALID;
+    if (!bytes) return FALX_OK;
+    HIP_TRY(hipSetDevice(c->device));
+    HIP_TRY(hipMemcpyAsync(d_dst, h_src, bytes, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return FALX_OK;
+}
+
+falx_status falx_download(falx_ctx* c, void* h_dst, const void* d_src, uint64_t bytes) {
+    if (!c) return FALX_E_INVALID;
+    if (!bytes) return FALX_OK;
+    HIP_TRY(hipSetDevice(c->device));
+    HIP_TRY(hipMemcpyAsync(h_dst, d_src, bytes, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return FALX_OK;
+}
+
+falx_status falx_spec_validate(const falx_spec* s) {
+    if (!s) return FALX_E_INVALID;
+    if (s->n_structural == 0 || s->n_structural > FALX_MAX_STRUCTURAL) return FALX_E_INVALID;
+    if (s->quote < -1 || s->quote > 255 || s->escape < -1 || s->escape > 255 || s->comment < -1 || s->comment > 255)
+        return FALX_E_INVALID;
+    if (s->quote >= 0 && s->escape == s->quote) return FALX_E_INVALID;       // formats.rs:237-240
+    if (s->comment == '\n' || s->comment == '\r') return FALX_E_INVALID;      // spec.rs comment rules
+    if (s->comment >= 0 && s->comment == s->quote) return FALX_E_INVALID;
+    if (s->n_columns > FALX_MAX_COLUMNS) return FALX_E_INVALID;               // codegen.rs:198-205
+    for (uint32_t k = 0; k < s->n_columns; ++k)
+        if (s->columns[k].type > FALX_COL_BYTES) return FALX_E_INVALID;
+    return FALX_OK;
+}
+
+falx_status falx_index_structurals(falx_ctx* c, const falx_spec* spec, const uint8_t* d, uint64_t len,
+                                   uint32_t* d_out, uint64_t out_cap, uint64_t* n_out) {
+    falx_status st = check_input(c, d, len);
+    if (st != FALX_OK) return st;
+    if ((st = falx_spec_validate(spec)) != FALX_OK || !n_out) return st != FALX_OK ? st : FALX_E_INVALID;
+    *n_out = 0;
+    if (len == 0) return FALX_OK;
+    Params P;
+    if ((st = begin(c)) != FALX_OK) return st;
+    if ((st = prepare(c, spec, d, len, P)) != FALX_OK) return st;
+    P.out = d_out;
+    P.out_cap = d_out ? out_cap : 0;
+    HIP_TRY(launch_scan<M_INDEX>(P, dialect_kind(spec), c->stream));
+    if ((st = finish(c)) != FALX_OK) return st;
+    *n_out = c->h_result[R_N_STRUCT];
+    return *n_out > P.out_cap ? FALX_E_CAPACITY : FALX_OK;
+}
+
+falx_status falx_parse(falx_ctx* c, const falx_spec* spec, const uint8_t* d, uint64_t len, uint32_t* d_seps,
+                       uint64_t seps_cap, uint64_t* d_ends, uint64_t ends_cap, uint64_t* n_seps, uint64_t* n_ends) {
+    falx_status st = check_input(c, d, len);
+    if (st != FALX_OK) return st;
+    if ((st = falx_spec_validate(spec)) != FALX_OK) return st;
+    if (!n_seps || !n_ends) return FALX_E_INVALID;
+    *n_seps = *n_ends = 0;
+    if (len == 0) return FALX_OK;
+    Params P;
+    if ((st = begin(c)) != FALX_OK) return st;
+    if ((st = prepare(c, spec, d, len, P)) != FALX_OK) return st;
+    P.out = d_seps;
+    P.out_cap = d_seps ? seps_cap : 0;
+    P.ends = reinterpret_cast<unsigned long long*>(d_ends);
+    P.ends_cap = d_ends ? ends_cap : 0;
+    HIP_TRY(launch_scan<M_TAPE>(P, dialect_kind(spec), c->stream));
+    if ((st = finish(c)) != FALX_OK) return st;
+    *n_seps = c->h_result[R_N_SEPS];
+    *n_ends = c->h_result[R_N_TERMS];
+    return (*n_seps > P.out_cap || *n_ends > P.ends_cap) ? FALX_E_CAPACITY : FALX_OK;
+}
+
+falx_status falx_parse_columns(falx_ctx* c, const falx_spec* spec, const uint8_t* d, uint64_t len,
+                               falx_columns* out) {
+    falx_status st = check_input(c, d, len);
+    if (st != FALX_OK) return st;
+    if ((st = falx_spec_validate(spec)) != FALX_OK) return st;
+    if (!out) return FALX_E_INVALID;
+    for (uint32_t k = 0; k < spec->n_columns; ++k) {
+        const falx_column_buf& b = out->cols[k];
+        if (!b.values || (!b.valid && out->rows_cap)) return FALX_E_INVALID;
+        if (spec->columns[k].type == FALX_COL_STR && !b.data && b.data_cap) return FALX_E_INVALID;
+    }
+    out->rows = 0;
+    Params P;
+    if ((st = begin(c)) != FALX_OK) return st;
+    // Validity words are OR-ed by the kernel: clear them first; string
+    // offsets start at 0 even for empty input.
+    for (uint32_t k = 0; k < spec->n_columns; ++k) {
+        HIP_TRY(fill_u64(c->stream, reinterpret_cast<unsigned long long*>(out->cols[k].valid),
+                         (out->rows_cap + 63) / 64, 0ULL));
+        out->cols[k].data_len = 0;
+    }
+    if (len == 0) {
+        for (uint32_t k = 0; k < spec->n_columns; ++k)
+            if (spec->columns[k].type == FALX_COL_STR)
+                HIP_TRY(hipMemsetAsync(out->cols[k].values, 0, 4, c->stream));
+        return finish(c);
+    }
+    if ((st = prepare(c, spec, d, len, P)) != FALX_OK) return st;
+    P.rows_cap = out->rows_cap;
+    for (uint32_t k = 0; k < spec->n_columns; ++k) {
+        P.cols[k].values = out->cols[k].values;
+        P.cols[k].valid = reinterpret_cast<unsigned long long*>(out->cols[k].valid);
+        P.cols[k].data = out->cols[k].data;
+        P.cols[k].data_cap = out->cols[k].data_cap;
+    }
+    HIP_TRY(launch_scan<M_COLUMNS>(P, dialect_kind(spec), c->stream));
+    if ((st = finish(c)) != FALX_OK) return st;
+    out->rows = c->h_result[R_N_ROWS];
+    bool cap = out->rows > out->rows_cap;
+    for (uint32_t k = 0; k < spec->n_columns; ++k) {
+        if (spec->columns[k].type != FALX_COL_STR) continue;
+        uint64_t dl = c->h_result[R_STR_LEN + P.cols[k].str_slot];
+        out->cols[k].data_len = dl;
+        if (dl > out->cols[k].data_cap) cap = true;
+        if (dl > 0x7FFFFFFFULL) return FALX_E_TOO_LARGE;
+    }
+    return cap ? FALX_E_CAPACITY : FALX_OK;
+}
+
+falx_status falx_parse_csv_stats(falx_ctx* c, const falx_spec* spec, const uint8_t* d, uint64_t len,
+                                 falx_csv_stats* out) {
+    falx_status st = check_input(c, d, len);
+    if (st != FALX_OK) return st;
+    if ((st = falx_spec_validate(spec)) != FALX_OK) return st;
+    if (!out) return FALX_E_INVALID;
+    memset(out, 0, sizeof *out);
+    if (len == 0) return FALX_OK;
+    Params P;
+    if ((st = begin(c)) != FALX_OK) return st;
+    if ((st = prepare(c, spec, d, len, P)) != FALX_OK) return st;
+    uint32_t nc = spec->n_columns ? spec->n_columns : 1;
+    size_t part_bytes = (size_t)P.n_tiles * nc * sizeof(StatPart) + (size_t)P.n_tiles * 8;
+    if ((st = ensure_part(c, part_bytes)) != FALX_OK) return st;
+    P.partials = c->d_part;
+    P.n_cols = spec->n_columns;
+    uint32_t groups = spec->n_columns ? (spec->n_columns + STAT_GROUP - 1) / STAT_GROUP : 1;
+    for (uint32_t g = 0; g < groups; ++g) {
+        P.col_group = g * STAT_GROUP;
+        if (g > 0) {  // each group is a full pass: fresh tickets and epoch
+            if (++c->epoch >= 0x3FFFFFFFu) c->epoch = 1;
+            P.epoch = c->epoch;
+            P.ticket_base = c->ticket_next;
+            c->ticket_next += P.n_tiles;
+        }
+        HIP_TRY(launch_scan<M_STATS>(P, dialect_kind(spec), c->stream));
+    }
+    StatPart* red = static_cast<StatPart*>(c->d_red);
+    unsigned long long* rec = reinterpret_cast<unsigned long long*>(red + FALX_MAX_COLUMNS);
+    if (spec->n_columns)
+        hipLaunchKernelGGL(k_reduce_stats, dim3(spec->n_columns), dim3(WG), 0, c->stream,
+                           static_cast<const StatPart*>(c->d_part), P.n_tiles, spec->n_columns, red);
+    hipLaunchKernelGGL(k_reduce_u64, dim3(1), dim3(WG), 0, c->stream,
+                       reinterpret_cast<const unsigned long long*>(static_cast<StatPart*>(c->d_part) +
+                                                                   (size_t)P.n_tiles * spec->n_columns),
+                       P.n_tiles, 1u, rec);
+    HIP_TRY(hipGetLastError());
+    StatPart host[FALX_MAX_COLUMNS + 1];
+    if ((st = finish(c)) != FALX_OK) return st;
+    HIP_TRY(hipMemcpy(host, red, sizeof host, hipMemcpyDeviceToHost));
+    out->records = reinterpret_cast<unsigned long long*>(&host[FALX_MAX_COLUMNS])[0];
+    for (uint32_t k = 0; k < spec->n_columns; ++k) {
+        out->cols[k].values = host[k].values;
+        out->cols[k].checksum = host[k].checksum;
+        out->cols[k].bytes = host[k].bytes;
+        out->cols[k].fixed6_sum = host[k].fixed6_sum;
+        out->cols[k].fixed6_cells = host[k].fixed6_cells;
+        out->cols[k].sum = host[k].sum;
+    }
+    return FALX_OK;
+}
+
+falx_status falx_parse_field_bytes(falx_ctx* c, const falx_spec* spec, const uint8_t* d, uint64_t len,
+                                   falx_field_byte_stats* out) {
+    falx_status st = check_input(c, d, len);
+    if (st != FALX_OK) return st;
+    if ((st = falx_spec_validate(spec)) != FALX_OK) return st;
+    if (!out) return FALX_E_INVALID;
+    out->fields = out->bytes = 0;
+    if (len == 0) return FALX_OK;
+    Params P;
+    if ((st = begin(c)) != FALX_OK) return st;
+    if ((st = prepare(c, spec, d, len, P)) != FALX_OK) return st;
+    if ((st = ensure_part(c, (size_t)P.n_tiles * 16)) != FALX_OK) return st;
+    P.partials = c->d_part;
+    HIP_TRY(launch_scan<M_FIELDBYTES>(P, dialect_kind(spec), c->stream));
+    unsigned long long* red = static_cast<unsigned long long*>(c->d_red);
+    hipLaunchKernelGGL(k_reduce_u64, dim3(1), dim3(WG), 0, c->stream,
+                       static_cast<const unsigned long long*>(c->d_part), P.n_tiles, 2u, red);
+    HIP_TRY(hipGetLastError());
+    if ((st = finish(c)) != FALX_OK) return st;
+    unsigned long long h[2];
+    HIP_TRY(hipMemcpy(h, red, 16, hipMemcpyDeviceToHost));
+    out->fields = h[0];
+    out->bytes = h[1];
+    return FALX_OK;
+}
+
+static falx_spec fixed_spec(const char* structural, int quote, int escape) {
+    falx_spec s;
+    memset(&s, 0, sizeof s);
+    s.n_structural = (uint32_t)strlen(structural);
+    memcpy(s.structural, structural, s.n_structural);
+    s.quote = quote;
+    s.escape = escape;
+    s.comment = -1;
+    return s;
+}
+
+falx_status falx_parse_ndjson_lines(falx_ctx* c, const uint8_t* d, uint64_t len, uint64_t* records) {
+    falx_status st = check_input(c, d, len);
+    if (st != FALX_OK) return st;
+    if (!records) return FALX_E_INVALID;
+    *records = 0;
+    if (len == 0) return FALX_OK;
+    falx_spec spec = fixed_spec("\n", '"', '\\');
+    Params P;
+    if ((st = begin(c)) != FALX_OK) return st;
+    if ((st = prepare(c, &spec, d, len, P)) != FALX_OK) return st;
+    if ((st = ensure_part(c, (size_t)P.n_tiles * 8)) != FALX_OK) return st;
+    P.partials = c->d_part;
+    HIP_TRY(launch_scan_fixed3<M_TERMS>(P, c->stream));
+    unsigned long long* red = static_cast<unsigned long long*>(c->d_red);
+    hipLaunchKernelGGL(k_reduce_u64, dim3(1), dim3(WG), 0, c->stream,
+                       static_cast<const unsigned long long*>(c->d_part), P.n_tiles, 1u, red);
+    HIP_TRY(hipGetLastError());
+    if ((st = finish(c)) != FALX_OK) return st;
+    unsigned long long h = 0;
+    uint8_t last = 0;
+    HIP_TRY(hipMemcpy(&h, red, 8, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(&last, d + len - 1, 1, hipMemcpyDeviceToHost));
+    *records = h + (last != '\n' ? 1 : 0);  // ndjson_has_trailing_record (ndjson.rs:853-855)
+    return FALX_OK;
+}
+
+falx_status falx_parse_ndjson_id_score(falx_ctx* c, const uint8_t* d, uint64_t len,
+                                       falx_ndjson_id_score_stats* out) {
+    falx_status st = check_input(c, d, len);
+    if (st != FALX_OK) return st;
+    if (!out) return FALX_E_INVALID;
+    memset(out, 0, sizeof *out);
+    if (len == 0) return FALX_OK;
+    if ((st = begin(c)) != FALX_OK) return st;
+    uint32_t n_tiles = (uint32_t)((len + TILE - 1) / TILE);
+    if ((st = ensure_part(c, (size_t)n_tiles * sizeof(NdPart))) != FALX_OK) return st;
+    hipLaunchKernelGGL(k_ndjson_id_score, dim3(n_tiles), dim3(WG), 0, c->stream, d, (uint32_t)len, n_tiles,
+                       static_cast<NdPart*>(c->d_part));
+    NdPart* red = static_cast<NdPart*>(c->d_red);
+    hipLaunchKernelGGL(k_reduce_nd, dim3(1), dim3(WG), 0, c->stream, static_cast<const NdPart*>(c->d_part),
+                       n_tiles, red);
+    HIP_TRY(hipGetLastError());
+    if ((st = finish(c)) != FALX_OK) return st;
+    NdPart h;
+    HIP_TRY(hipMemcpy(&h, red, sizeof h, hipMemcpyDeviceToHost));
+    out->records = h.records;
+    out->invalid = h.invalid;
+    out->id_sum = h.id_sum;
+    out->score_fixed6_sum = h.fx_sum;
+    out->score_fixed6_cells = h.fx_cells;
+    out->score_sum = h.score_sum;
+    return out->invalid ? FALX_E_SCHEMA : FALX_OK;
+}
+
+falx_status falx_parse_logfmt_pairs(falx_ctx* c, const uint8_t* d, uint64_t len, falx_logfmt_stats* out) {
+    falx_status st = check_input(c, d, len);
+    if (st != FALX_OK) return st;
+    if (!out) return FALX_E_INVALID;
+    memset(out, 0, sizeof *out);
+    if (len == 0) return FALX_OK;
+    falx_spec spec = fixed_spec(" =\n", '"', '\\');
+    Params P;
+    if ((st = begin(c)) != FALX_OK) return st;
+    if ((st = prepare(c, &spec, d, len, P)) != FALX_OK) return st;
+    if ((st = ensure_part(c, (size_t)P.n_tiles * 32)) != FALX_OK) return st;
+    P.partials = c->d_part;
+    HIP_TRY(launch_scan_fixed3<M_LOGFMT>(P, c->stream));
+    unsigned long long* red = static_cast<unsigned long long*>(c->d_red);
+    hipLaunchKernelGGL(k_reduce_u64, dim3(1), dim3(WG), 0, c->stream,
+                       static_cast<const unsigned long long*>(c->d_part), P.n_tiles, 4u, red);
+    HIP_TRY(hipGetLastError());
+    if ((st = finish(c)) != FALX_OK) return st;
+    unsigned long long h[4];
+    HIP_TRY(hipMemcpy(h, red, 32, hipMemcpyDeviceToHost));
+    out->pairs = h[0];
+    out->key_bytes = h[1];
+    out->value_bytes = h[2];
+    out->checksum = h[3];
+    return FALX_OK;
+}
+
+}  // extern "C"

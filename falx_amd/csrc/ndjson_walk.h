@@ -1,0 +1,225 @@
+// ndjson_walk.h — per-record walker of the NDJSON `sum(id + nested.score)`
+// sink, shared by the gfx950 kernel and its host build (for CPU sweeps).
+//
+// The reference's sink (ndjson.rs:679-723) splits records at raw '\n',
+// reads a leading `{"id":` integer and a `],"nested":{"score":` integer, and
+// panics on any other shape. This walker keeps the record split and the
+// wrapping i64 id sum, and reads any JSON object layout: top-level key `id`
+// (integer `-?[0-9]+`) and the `score` key of the top-level `nested` object
+// (a JSON number, correctly rounded to f64), with whitespace anywhere JSON
+// allows it and backslash escapes inside strings. The first occurrence of a
+// key wins; keys are compared as raw bytes.
+#pragma once
+
+#include "cells.h"
+
+namespace fx {
+
+FX_HD bool jws(int b) { return b == ' ' || b == '\t' || b == '\r' || b == '\n'; }
+
+template <class Src>
+struct JCur {
+    const Src* s;
+    uint32_t i, n;
+    FX_HD int peek() const { return i < n ? (int)s->at(i) : -1; }
+    FX_HD void ws() { while (i < n && jws(s->at(i))) ++i; }
+    FX_HD bool delim() const {
+        int c = peek();
+        return c < 0 || c == ',' || c == '}' || c == ']' || jws(c);
+    }
+};
+
+template <class Src>
+FX_HD bool j_skip_string(JCur<Src>& c) {  // at the opening quote
+    ++c.i;
+    while (c.i < c.n) {
+        uint8_t b = c.s->at(c.i);
+        if (b == '\\') { c.i += 2; continue; }
+        ++c.i;
+        if (b == '"') return true;
+    }
+    return false;
+}
+
+template <class Src>
+FX_HD bool j_skip_value(JCur<Src>& c) {
+    c.ws();
+    int b = c.peek();
+    if (b < 0) return false;
+    if (b == '"') return j_skip_string(c);
+    if (b == '{' || b == '[') {
+        int depth = 0;
+        while (c.i < c.n) {
+            uint8_t x = c.s->at(c.i);
+            if (x == '"') {
+                if (!j_skip_string(c)) return false;
+                continue;
+            }
+            if (x == '{' || x == '[') ++depth;
+            else if (x == '}' || x == ']') {
+                --depth;
+                if (depth == 0) { ++c.i; return true; }
+            }
+            ++c.i;
+        }
+        return false;
+    }
+    uint32_t s0 = c.i;
+    while (!c.delim()) ++c.i;
+    return c.i > s0;
+}
+
+// Key at '"': compares the raw key bytes with `lit` (len n).
+template <class Src>
+FX_HD bool j_key(JCur<Src>& c, uint32_t& ks, uint32_t& ke) {
+    if (c.peek() != '"') return false;
+    ks = c.i + 1;
+    if (!j_skip_string(c)) return false;
+    ke = c.i - 1;
+    return true;
+}
+
+template <class Src>
+FX_HD bool key_eq(const JCur<Src>& c, uint32_t ks, uint32_t ke, const char* lit, uint32_t n) {
+    if (ke - ks != n) return false;
+    for (uint32_t k = 0; k < n; ++k)
+        if (c.s->at(ks + k) != (uint8_t)lit[k]) return false;
+    return true;
+}
+
+template <class Src>
+FX_HD bool j_int(JCur<Src>& c, int64_t& v) {
+    bool neg = false;
+    if (c.peek() == '-') { neg = true; ++c.i; }
+    if (!is_digit(c.peek())) return false;
+    uint64_t acc = 0;
+    while (is_digit(c.peek())) { acc = acc * 10 + (uint64_t)(c.s->at(c.i) - '0'); ++c.i; }
+    if (!c.delim()) return false;
+    v = neg ? (int64_t)(0 - acc) : (int64_t)acc;
+    return true;
+}
+
+// JSON number: -?(0|[1-9]d*)(.d+)?([eE][+-]?d+)? -> correctly rounded f64,
+// plus the exact 1e-6 mantissa when it has no exponent, <= 6 decimals and
+// <= 12 integer digits.
+template <class Src>
+FX_HD bool j_number(JCur<Src>& c, double& v, bool& fx6, int64_t& mant) {
+    uint32_t s0 = c.i;
+    bool neg = false;
+    if (c.peek() == '-') { neg = true; ++c.i; }
+    if (!is_digit(c.peek())) return false;
+    uint32_t ip = c.i;
+    if (c.peek() == '0') ++c.i;
+    else while (is_digit(c.peek())) ++c.i;
+    uint32_t int_digits = c.i - ip;
+    uint32_t frac_digits = 0, fs = 0;
+    if (c.peek() == '.') {
+        ++c.i;
+        fs = c.i;
+        while (is_digit(c.peek())) ++c.i;
+        frac_digits = c.i - fs;
+        if (frac_digits == 0) return false;
+    }
+    bool has_exp = false;
+    int e = c.peek();
+    if (e == 'e' || e == 'E') {
+        has_exp = true;
+        ++c.i;
+        if (c.peek() == '+' || c.peek() == '-') ++c.i;
+        uint32_t es = c.i;
+        while (is_digit(c.peek())) ++c.i;
+        if (c.i == es) return false;
+    }
+    if (!c.delim()) return false;
+    F64Result r = parse_f64_stream(RawStream<Src>{c.s, s0, c.i});
+    if (!r.ok) return false;
+    v = r.value;
+    fx6 = false;
+    if (!has_exp && frac_digits <= 6 && int_digits <= 12) {
+        int64_t m = 0;
+        for (uint32_t k = ip; k < ip + int_digits; ++k) m = m * 10 + (c.s->at(k) - '0');
+        for (uint32_t k = 0; k < 6; ++k)
+            m = m * 10 + (k < frac_digits ? (c.s->at(fs + k) - '0') : 0);
+        mant = neg ? -m : m;
+        fx6 = true;
+    }
+    return true;
+}
+
+struct NdRecord {
+    int64_t id;
+    double score;
+    bool fx6;
+    int64_t mant;
+};
+
+// One record data[from, to) (its '\n' excluded). 1 = valid, 0 = invalid.
+template <class Src>
+FX_HD int ndjson_record(const Src& s, uint32_t from, uint32_t to, NdRecord& out) {
+    JCur<Src> c{&s, from, to};
+    out.id = 0;
+    out.score = 0;
+    out.fx6 = false;
+    out.mant = 0;
+    c.ws();
+    if (c.peek() != '{') return 0;
+    ++c.i;
+    bool have_id = false, have_score = false;
+    c.ws();
+    if (c.peek() == '}') return 0;
+    while (true) {
+        c.ws();
+        uint32_t ks, ke;
+        if (!j_key(c, ks, ke)) return 0;
+        c.ws();
+        if (c.peek() != ':') return 0;
+        ++c.i;
+        c.ws();
+        if (!have_id && key_eq(c, ks, ke, "id", 2)) {
+            if (!j_int(c, out.id)) return 0;
+            have_id = true;
+        } else if (!have_score && key_eq(c, ks, ke, "nested", 6) && c.peek() == '{') {
+            ++c.i;
+            c.ws();
+            if (c.peek() == '}') {
+                ++c.i;
+            } else {
+                while (true) {
+                    c.ws();
+                    uint32_t ks2, ke2;
+                    if (!j_key(c, ks2, ke2)) return 0;
+                    c.ws();
+                    if (c.peek() != ':') return 0;
+                    ++c.i;
+                    c.ws();
+                    if (!have_score && key_eq(c, ks2, ke2, "score", 5)) {
+                        if (!j_number(c, out.score, out.fx6, out.mant)) return 0;
+                        have_score = true;
+                    } else if (!j_skip_value(c)) {
+                        return 0;
+                    }
+                    c.ws();
+                    if (c.peek() == ',') { ++c.i; continue; }
+                    if (c.peek() == '}') { ++c.i; break; }
+                    return 0;
+                }
+            }
+        } else if (!j_skip_value(c)) {
+            return 0;
+        }
+        c.ws();
+        if (c.peek() == ',') { ++c.i; continue; }
+        if (c.peek() == '}') { ++c.i; break; }
+        return 0;
+    }
+    return (have_id && have_score) ? 1 : 0;
+}
+
+template <class Src>
+FX_HD bool blank_span(const Src& s, uint32_t from, uint32_t to) {
+    for (uint32_t p = from; p < to; ++p)
+        if (!jws(s.at(p))) return false;
+    return true;
+}
+
+}  // namespace fx

@@ -1,0 +1,1173 @@
+// scan.hip — gfx950 kernels of the structural-indexing / column-parsing path.
+//
+// One workgroup (4 wave64s) owns one 8 KiB tile of the input. A tile is read
+// from HBM once (coalesced 16-byte loads into LDS) and then:
+//   pass 1  each wave walks its 32 blocks of 64 bytes: byte-per-lane compares
+//           and __ballot give the reference's per-block class masks (quote,
+//           separator, newline, escape, comment) directly as 64-bit values;
+//           the wave composes the blocks' transfer functions of the dialect
+//           state (quote parity / escape add-carry / NORMAL-QUOTE-COMMENT
+//           region) into one function;
+//   LB1     decoupled look-back over tile transfer functions (dynamic tile
+//           ids, agent-scope release/acquire) gives the tile's entry state;
+//   pass 2  the same blocks, now with known carries, yield the structural
+//           and terminator masks (csv_geo.rs:2116-2127, csv_hash.rs:2124-2138,
+//           ndjson.rs:1016-1032 semantics); set bits are compacted into an
+//           LDS list (popcount/mbcnt stream compaction) and summarised into
+//           record/field counters;
+//   LB2     decoupled look-back over the counters gives output offsets, row
+//           numbers, the open field's ordinal and the open record's start;
+//   emit    the whole workgroup walks the compacted list: index/tape
+//           offsets, typed cells (parsed in place), fused statistics;
+//   LB3     (string columns only) a third look-back places the tile's cleaned
+//           bytes in the Arrow arena, then the cells are copied.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string.h>
+
+#include "../../include/falx_gpu.h"
+#include "cells.h"
+#include "ndjson_walk.h"
+
+#define DEV __device__ __forceinline__
+
+namespace fx {
+
+constexpr int WG = 256;
+constexpr int NWAVE = WG / 64;
+constexpr int BPW = 32;                   // 64-byte blocks per wave
+constexpr int TILE = NWAVE * BPW * 64;    // 8192 bytes per workgroup
+constexpr uint64_t EVEN_BITS = 0x5555555555555555ULL;
+constexpr uint32_t SPIN_LIMIT = 1u << 22;
+constexpr int STAT_GROUP = 4;             // stat columns accumulated per pass
+
+enum Mode : int { M_INDEX = 0, M_TAPE, M_COLUMNS, M_STATS, M_FIELDBYTES, M_LOGFMT, M_TERMS };
+
+// Record/field counters of a byte range; `compose(a, b)` = a then b.
+struct Counts {
+    uint32_t n_struct, n_seps, n_terms, n_rows;
+    uint32_t has_term, ord, last_struct, prev_struct;  // positions stored +1, 0 = none
+    uint32_t last_term, pad0, pad1, pad2;
+};
+
+struct ColDesc {
+    void* values;
+    unsigned long long* valid;
+    uint8_t* data;
+    uint64_t data_cap;
+    uint32_t index, type, str_slot, pad;
+};
+
+struct StatPart {
+    unsigned long long values, checksum, bytes;
+    long long fixed6_sum;
+    unsigned long long fixed6_cells;
+    double sum;
+};
+
+// result[] slots written by the last tile / reduce kernels
+enum : int {
+    R_N_STRUCT = 0, R_N_SEPS, R_N_TERMS, R_N_ROWS, R_ERROR, R_RECORDS, R_F0, R_F1, R_F2, R_F3,
+    R_STR_LEN = 16,   // + str_slot (32)
+    R_SLOTS = 64
+};
+
+struct Params {
+    const uint8_t* data;
+    uint32_t len;
+    uint32_t n_tiles;
+    uint32_t cand[8];
+    int32_t quote, escape, comment;
+    uint32_t n_cols, n_str;
+    uint32_t col_group;       // STATS: first column of this launch's group
+    uint64_t rows_cap;
+    ColDesc cols[FALX_MAX_COLUMNS];
+    unsigned long long* ticket;
+    unsigned long long ticket_base;
+    uint32_t epoch;
+    unsigned long long* lb1;
+    uint32_t* lb2_flag;
+    Counts* lb2_agg;
+    Counts* lb2_inc;
+    uint32_t* lb3_flag;
+    uint32_t* lb3_agg;
+    uint32_t* lb3_inc;
+    uint32_t* out;
+    uint64_t out_cap;
+    unsigned long long* ends;
+    uint64_t ends_cap;
+    void* partials;
+    unsigned long long* result;
+};
+
+// --------------------------------------------------------------------------
+// wave / bit helpers
+// --------------------------------------------------------------------------
+DEV uint64_t ballot(bool p) { return (uint64_t)__ballot(p); }
+
+DEV uint32_t mbcnt(uint64_t m) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+DEV uint32_t uni(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+
+DEV uint32_t popc(uint64_t m) { return (uint32_t)__popcll(m); }
+
+DEV uint32_t hibit(uint64_t m) { return 63u - (uint32_t)__clzll(m); }
+
+DEV uint64_t prefix_xor(uint64_t x) {
+    x ^= x << 1;
+    x ^= x << 2;
+    x ^= x << 4;
+    x ^= x << 8;
+    x ^= x << 16;
+    x ^= x << 32;
+    return x;
+}
+
+DEV uint64_t range_mask(uint32_t from, uint32_t to) {
+    uint64_t hi = to >= 64 ? ~0ULL : ((1ULL << to) - 1);
+    return hi & ~((1ULL << from) - 1);
+}
+
+// Three-state region resolver (csv_hash.rs:2149-2220; interp.rs:163-203).
+DEV uint64_t resolve_regions(uint64_t q, uint64_t s, uint64_t n, uint32_t& state) {
+    if (state == 0 && (q | s) == 0) return 0;
+    if (state != 2 && s == 0) {
+        uint64_t inert = prefix_xor(q);
+        if (state == 1) inert = ~inert;
+        state = (uint32_t)(inert >> 63);
+        return inert;
+    }
+    uint64_t inert = 0;
+    uint32_t run_start = 0;
+    uint64_t events = q | s | n;
+    while (events) {
+        uint32_t p = (uint32_t)__ffsll((long long)events) - 1;
+        uint64_t bit = 1ULL << p;
+        if (state == 1) {
+            if (q & bit) { inert |= range_mask(run_start, p); state = 0; }
+        } else if (state == 2) {
+            if (n & bit) { inert |= range_mask(run_start, p); state = 0; }
+        } else {
+            if (q & bit) { state = 1; run_start = p; }
+            else if (s & bit) { state = 2; run_start = p; }
+        }
+        events &= events - 1;
+    }
+    if (state != 0) inert |= range_mask(run_start, 64);
+    return inert;
+}
+
+// Transfer functions on <= 8 states, 4 bits per entry.
+DEV uint32_t fn_apply(uint32_t f, uint32_t s) { return (f >> (4 * s)) & 15u; }
+
+template <int NS>
+DEV uint32_t fn_then(uint32_t f, uint32_t g) {  // g after f
+    uint32_t h = 0;
+#pragma unroll
+    for (int s = 0; s < NS; ++s) h |= fn_apply(g, fn_apply(f, (uint32_t)s)) << (4 * s);
+    return h;
+}
+
+template <int NS>
+DEV uint32_t fn_ident() {
+    uint32_t h = 0;
+#pragma unroll
+    for (int s = 0; s < NS; ++s) h |= (uint32_t)s << (4 * s);
+    return h;
+}
+
+DEV Counts counts_zero() {
+    Counts c;
+    c.n_struct = c.n_seps = c.n_terms = c.n_rows = 0;
+    c.has_term = c.ord = c.last_struct = c.prev_struct = 0;
+    c.last_term = c.pad0 = c.pad1 = c.pad2 = 0;
+    return c;
+}
+
+DEV Counts compose(const Counts& a, const Counts& b) {
+    Counts r;
+    r.n_struct = a.n_struct + b.n_struct;
+    r.n_seps = a.n_seps + b.n_seps;
+    r.n_terms = a.n_terms + b.n_terms;
+    r.n_rows = a.n_rows + b.n_rows;
+    r.has_term = a.has_term | b.has_term;
+    r.ord = b.has_term ? b.ord : a.ord + b.ord;
+    r.last_struct = b.last_struct ? b.last_struct : a.last_struct;
+    r.prev_struct = b.n_struct >= 2 ? b.prev_struct : (b.n_struct == 1 ? a.last_struct : a.prev_struct);
+    r.last_term = b.last_term ? b.last_term : a.last_term;
+    r.pad0 = r.pad1 = r.pad2 = 0;
+    return r;
+}
+
+// Byte source for cells.h: LDS inside the tile, HBM outside it.
+struct TileSrc {
+    const uint8_t* g;
+    const uint8_t* lds;
+    uint32_t lo, hi;
+    DEV uint8_t at(uint32_t p) const { return (p - lo) < (hi - lo) ? lds[p - lo] : g[p]; }
+};
+
+// --------------------------------------------------------------------------
+// look-back primitives
+// --------------------------------------------------------------------------
+DEV unsigned long long lb1_word(uint32_t status, uint32_t epoch, uint32_t value) {
+    return ((unsigned long long)status << 62) | ((unsigned long long)(epoch & 0x3FFFFFFFu) << 32) | value;
+}
+
+DEV bool lb1_wait(const Params& P, uint32_t j, unsigned long long& v) {
+    for (uint32_t it = 0; it < SPIN_LIMIT; ++it) {
+        v = __hip_atomic_load(&P.lb1[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if ((v >> 62) != 0 && (uint32_t)((v >> 32) & 0x3FFFFFFFu) == (P.epoch & 0x3FFFFFFFu)) return true;
+        __builtin_amdgcn_s_sleep(1);
+    }
+    atomicOr(&P.result[R_ERROR], 1ULL);
+    return false;
+}
+
+DEV uint32_t lb_flag_wait(const Params& P, uint32_t* flag, uint32_t j) {
+    for (uint32_t it = 0; it < SPIN_LIMIT; ++it) {
+        uint32_t f = __hip_atomic_load(&flag[j], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+        if ((f >> 2) == (P.epoch & 0x3FFFFFFFu) && (f & 3u)) return f & 3u;
+        __builtin_amdgcn_s_sleep(1);
+    }
+    atomicOr(&P.result[R_ERROR], 2ULL);
+    return 2;
+}
+
+DEV void lb_flag_set(const Params& P, uint32_t* flag, uint32_t j, uint32_t status) {
+    __hip_atomic_store(&flag[j], ((P.epoch & 0x3FFFFFFFu) << 2) | status, __ATOMIC_RELEASE,
+                       __HIP_MEMORY_SCOPE_AGENT);
+}
+
+DEV void store_counts(Counts* dst, const Counts& c) {
+    uint4* d = reinterpret_cast<uint4*>(dst);
+    d[0] = make_uint4(c.n_struct, c.n_seps, c.n_terms, c.n_rows);
+    d[1] = make_uint4(c.has_term, c.ord, c.last_struct, c.prev_struct);
+    d[2] = make_uint4(c.last_term, 0u, 0u, 0u);
+}
+
+DEV Counts load_counts(const Counts* src) {
+    const uint4* s = reinterpret_cast<const uint4*>(src);
+    uint4 a = s[0], b = s[1], c = s[2];
+    Counts r;
+    r.n_struct = a.x; r.n_seps = a.y; r.n_terms = a.z; r.n_rows = a.w;
+    r.has_term = b.x; r.ord = b.y; r.last_struct = b.z; r.prev_struct = b.w;
+    r.last_term = c.x; r.pad0 = r.pad1 = r.pad2 = 0;
+    return r;
+}
+
+// --------------------------------------------------------------------------
+// block-wide scans over 256 threads
+// --------------------------------------------------------------------------
+struct Scan3 {
+    uint32_t a, b;
+    int32_t c;  // max
+};
+
+DEV Scan3 wave_incl(Scan3 v) {
+    const uint32_t ln = __lane_id();
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        uint32_t ta = __shfl_up(v.a, d);
+        uint32_t tb = __shfl_up(v.b, d);
+        int32_t tc = __shfl_up(v.c, d);
+        if (ln >= (uint32_t)d) { v.a += ta; v.b += tb; v.c = tc > v.c ? tc : v.c; }
+    }
+    return v;
+}
+
+// Exclusive scan (sum a, sum b, max c) across the workgroup; `tot` gets the
+// inclusive total. `sh` is 4-wave scratch.
+DEV Scan3 block_excl(Scan3 v, Scan3* sh, Scan3& tot) {
+    const uint32_t ln = __lane_id(), w = threadIdx.x >> 6;
+    Scan3 inc = wave_incl(v);
+    Scan3 ex;
+    ex.a = __shfl_up(inc.a, 1);
+    ex.b = __shfl_up(inc.b, 1);
+    ex.c = __shfl_up(inc.c, 1);
+    if (ln == 0) { ex.a = 0; ex.b = 0; ex.c = -1; }
+    if (ln == 63) sh[w] = inc;
+    __syncthreads();
+    Scan3 base{0, 0, -1};
+    for (uint32_t k = 0; k < w; ++k) {
+        base.a += sh[k].a;
+        base.b += sh[k].b;
+        base.c = sh[k].c > base.c ? sh[k].c : base.c;
+    }
+    tot = Scan3{0, 0, -1};
+    for (int k = 0; k < NWAVE; ++k) {
+        tot.a += sh[k].a;
+        tot.b += sh[k].b;
+        tot.c = sh[k].c > tot.c ? sh[k].c : tot.c;
+    }
+    __syncthreads();
+    ex.a += base.a;
+    ex.b += base.b;
+    ex.c = base.c > ex.c ? base.c : ex.c;
+    return ex;
+}
+
+DEV uint32_t block_excl_sum(uint32_t v, uint32_t* sh, uint32_t& tot) {
+    const uint32_t ln = __lane_id(), w = threadIdx.x >> 6;
+    uint32_t inc = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        uint32_t t = __shfl_up(inc, d);
+        if (ln >= (uint32_t)d) inc += t;
+    }
+    if (ln == 63) sh[w] = inc;
+    __syncthreads();
+    uint32_t base = 0;
+    for (uint32_t k = 0; k < w; ++k) base += sh[k];
+    tot = 0;
+    for (int k = 0; k < NWAVE; ++k) tot += sh[k];
+    __syncthreads();
+    return base + inc - v;
+}
+
+template <class T>
+DEV T block_reduce_sum(T v, T* sh) {
+    // fixed-order tree: wave butterfly, then waves in order
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d);
+    const uint32_t ln = __lane_id(), w = threadIdx.x >> 6;
+    if (ln == 0) sh[w] = v;
+    __syncthreads();
+    T t = sh[0];
+    for (int k = 1; k < NWAVE; ++k) t += sh[k];
+    __syncthreads();
+    return t;
+}
+
+// --------------------------------------------------------------------------
+// dialect step (one 64-byte block, wave-uniform)
+// --------------------------------------------------------------------------
+struct BM {
+    uint64_t cand, nl, q, esc, cm;
+};
+
+template <bool HQ, bool ESC, bool CMT>
+struct Dial {
+    static constexpr int NC = ESC ? 2 : 1;
+    static constexpr int NR = CMT ? 3 : (HQ ? 2 : 1);
+    static constexpr int NS = NC * NR;
+};
+
+struct StepOut {
+    uint64_t S, T, F;
+    uint32_t s_out;
+};
+
+// Apply one block with known entry state s_in (state = c + NC * r).
+template <bool HQ, bool ESC, bool CMT>
+DEV StepOut block_step(const BM& m, uint32_t s_in, uint64_t ls) {
+    constexpr int NC = Dial<HQ, ESC, CMT>::NC;
+    uint32_t c = ESC ? (s_in % NC) : 0u;
+    uint32_t r = s_in / NC;
+    uint64_t rq = m.q;
+    uint32_t c_out = 0;
+    if (ESC) {
+        // escaped_positions (formats.rs:334-343) with the Add carry.
+        uint64_t phase = EVEN_BITS ^ m.esc;
+        uint64_t partial = EVEN_BITS + phase;
+        uint32_t c1 = partial < EVEN_BITS;
+        uint64_t sum = partial + c;
+        uint32_t c2 = sum < partial;
+        c_out = c1 | c2;
+        rq &= EVEN_BITS ^ sum;
+    }
+    StepOut o;
+    o.F = 0;
+    if (CMT) {
+        uint64_t starts = m.cm & ((m.nl << 1) | ls);
+        uint32_t rr = r;
+        uint64_t inert = resolve_regions(HQ ? rq : 0ULL, starts, m.nl, rr);
+        o.S = HQ ? (m.cand & ~inert) : m.cand;
+        o.T = o.S & m.nl;
+        uint64_t inert_prev = (inert << 1) | (uint64_t)(r != 0);
+        o.F = (o.T & inert_prev) | (o.S & ~m.nl & inert);
+        r = rr;
+    } else if (HQ) {
+        uint64_t inside = prefix_xor(rq) ^ (r ? ~0ULL : 0ULL);
+        o.S = m.cand & ~inside;
+        o.T = o.S & m.nl;
+        r = (uint32_t)(inside >> 63);
+    } else {
+        o.S = m.cand;
+        o.T = o.S & m.nl;
+    }
+    o.s_out = c_out + NC * r;
+    return o;
+}
+
+// Transfer function of one block over all entry states.
+template <bool HQ, bool ESC, bool CMT>
+DEV uint32_t block_fn(const BM& m, uint64_t ls) {
+    constexpr int NS = Dial<HQ, ESC, CMT>::NS;
+    if (!HQ && !CMT) return 0u;
+    if (HQ && !ESC && !CMT) {
+        uint32_t p = popc(m.q) & 1u;
+        return p | ((1u ^ p) << 4);
+    }
+    if (HQ && ESC && !CMT) {
+        uint32_t f = 0;
+#pragma unroll
+        for (uint32_t c = 0; c < 2; ++c) {
+            uint64_t phase = EVEN_BITS ^ m.esc;
+            uint64_t partial = EVEN_BITS + phase;
+            uint32_t c1 = partial < EVEN_BITS;
+            uint64_t sum = partial + c;
+            uint32_t c2 = sum < partial;
+            uint32_t p = popc(m.q & (EVEN_BITS ^ sum)) & 1u;
+            uint32_t co = c1 | c2;
+            f |= (co + 2 * (0u ^ p)) << (4 * (c + 0));
+            f |= (co + 2 * (1u ^ p)) << (4 * (c + 2));
+        }
+        return f;
+    }
+    uint32_t f = 0;
+#pragma unroll
+    for (int s = 0; s < NS; ++s) f |= block_step<HQ, ESC, CMT>(m, (uint32_t)s, ls).s_out << (4 * s);
+    return f;
+}
+
+// --------------------------------------------------------------------------
+// the tile kernel
+// --------------------------------------------------------------------------
+struct Shared {
+    uint8_t bytes[TILE];
+    uint16_t list[TILE];
+    uint8_t cls[256];
+    uint32_t wfn[NWAVE];
+    uint32_t wentry[NWAVE];
+    uint32_t wcount[NWAVE];
+    uint32_t wbase[NWAVE + 1];
+    Counts wcnt[NWAVE];
+    Counts pre;       // tile exclusive prefix
+    Counts inc;       // tile inclusive
+    uint32_t tile;
+    Scan3 scan_sh[NWAVE];
+    uint32_t sum_sh[NWAVE];
+    double dsum_sh[NWAVE];
+    unsigned long long usum_sh[NWAVE];
+    long long lsum_sh[NWAVE];
+    uint32_t str_len[FALX_MAX_COLUMNS];   // tile cleaned bytes per str slot
+    uint32_t str_base[FALX_MAX_COLUMNS];  // arena base per str slot
+};
+
+DEV uint32_t list_at(const Shared& sh, uint32_t k) {
+    uint32_t w = 0;
+#pragma unroll
+    for (int i = 1; i < NWAVE; ++i) w += k >= sh.wbase[i];
+    return sh.list[w * BPW * 64 + (k - sh.wbase[w])];
+}
+
+struct Entry {
+    bool valid, is_term, flag;
+    uint32_t pos, fs, rs, ord, row, k;
+    int32_t lt;
+};
+
+template <bool HQ, bool ESC, bool CMT, int MODE>
+__global__ __launch_bounds__(WG) void k_scan(Params P) {
+    __shared__ Shared sh;
+    constexpr int NS = Dial<HQ, ESC, CMT>::NS;
+    const uint32_t tid = threadIdx.x, ln = tid & 63u, w = tid >> 6;
+
+    if (tid == 0) sh.tile = (uint32_t)(atomicAdd(P.ticket, 1ULL) - P.ticket_base);
+    {
+        uint32_t b = tid;
+        sh.cls[b] = (uint8_t)((P.cand[b >> 5] >> (b & 31)) & 1u);
+    }
+    __syncthreads();
+    const uint32_t tile = sh.tile;
+    const uint32_t tile_start = tile * (uint32_t)TILE;
+    const uint32_t tile_len = min((uint32_t)TILE, P.len - tile_start);
+    const bool last_tile = tile + 1 == P.n_tiles;
+
+    // ---- load: 16-byte coalesced reads into LDS, zero padded --------------
+    for (uint32_t i = tid * 16; i < (uint32_t)TILE; i += WG * 16) {
+        uint4 v = make_uint4(0, 0, 0, 0);
+        if (i + 16 <= tile_len) {
+            v = *reinterpret_cast<const uint4*>(P.data + tile_start + i);
+        } else if (i < tile_len) {
+            uint8_t tmp[16];
+            for (int k = 0; k < 16; ++k) tmp[k] = (i + k < tile_len) ? P.data[tile_start + i + k] : 0;
+            memcpy(&v, tmp, 16);
+        }
+        *reinterpret_cast<uint4*>(&sh.bytes[i]) = v;
+    }
+    __syncthreads();
+
+    // line-start carry entering this wave's first block
+    uint64_t ls0;
+    {
+        uint32_t wb = w * BPW * 64;
+        uint32_t v;
+        if (wb > 0) v = sh.bytes[wb - 1] == '\n';
+        else v = tile_start == 0 ? 1u : (uint32_t)(P.data[tile_start - 1] == '\n');
+        ls0 = uni(v);
+    }
+    const uint8_t qb = (uint8_t)P.quote, eb = (uint8_t)P.escape, cb = (uint8_t)P.comment;
+    auto classify = [&](uint32_t blk) {
+        uint8_t c = sh.bytes[blk * 64 + ln];
+        BM m;
+        m.cand = ballot(sh.cls[c] != 0);
+        m.nl = ballot(c == '\n');
+        m.q = HQ ? ballot(c == qb) : 0ULL;
+        m.esc = ESC ? ballot(c == eb) : 0ULL;
+        m.cm = CMT ? ballot(c == cb) : 0ULL;
+        return m;
+    };
+
+    // ---- pass 1 + LB1: entry state -----------------------------------------
+    if (NS > 1) {
+        uint32_t fn = fn_ident<NS>();
+        uint64_t ls = ls0;
+        for (int b = 0; b < BPW; ++b) {
+            BM m = classify(w * BPW + b);
+            fn = fn_then<NS>(fn, block_fn<HQ, ESC, CMT>(m, ls));
+            ls = m.nl >> 63;
+        }
+        if (ln == 0) sh.wfn[w] = fn;
+        __syncthreads();
+        if (tid == 0) {
+            uint32_t agg = sh.wfn[0];
+            for (int k = 1; k < NWAVE; ++k) agg = fn_then<NS>(agg, sh.wfn[k]);
+            uint32_t entry = 0;
+            if (tile > 0) {
+                __hip_atomic_store(&P.lb1[tile], lb1_word(1, P.epoch, agg), __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+                uint32_t g = fn_ident<NS>();
+                int32_t j = (int32_t)tile - 1;
+                while (true) {
+                    unsigned long long v;
+                    if (!lb1_wait(P, (uint32_t)j, v)) { entry = 0; break; }
+                    uint32_t val = (uint32_t)v;
+                    if ((v >> 62) == 2) { entry = fn_apply(g, val); break; }
+                    g = fn_then<NS>(val, g);
+                    --j;
+                }
+            }
+            __hip_atomic_store(&P.lb1[tile], lb1_word(2, P.epoch, fn_apply(agg, entry)), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+            uint32_t s = entry;
+            for (int k = 0; k < NWAVE; ++k) { sh.wentry[k] = s; s = fn_apply(sh.wfn[k], s); }
+        }
+        __syncthreads();
+    } else {
+        if (tid < NWAVE) sh.wentry[tid] = 0;
+        __syncthreads();
+    }
+
+    // ---- pass 2: masks, compaction, counters -------------------------------
+    {
+        uint32_t st = uni(sh.wentry[w]);
+        uint64_t ls = ls0;
+        Counts wc = counts_zero();
+        uint32_t cnt = 0;
+        const uint32_t seg = w * BPW * 64;
+        for (int b = 0; b < BPW; ++b) {
+            const uint32_t blk = w * BPW + b;
+            const uint32_t off = blk * 64;
+            BM m = classify(blk);
+            StepOut o = block_step<HQ, ESC, CMT>(m, st, ls);
+            ls = m.nl >> 63;
+            st = o.s_out;
+            uint64_t live = off + 64 <= tile_len ? ~0ULL : (off < tile_len ? ((1ULL << (tile_len - off)) - 1) : 0ULL);
+            uint64_t S = o.S & live, T = o.T & live, F = o.F & live;
+            if ((S >> ln) & 1ULL) {
+                uint16_t e = (uint16_t)((off + ln) | (((T >> ln) & 1ULL) << 14) | (((F >> ln) & 1ULL) << 15));
+                sh.list[seg + cnt + mbcnt(S)] = e;
+            }
+            uint32_t nS = popc(S);
+            cnt += nS;
+            if (nS) {
+                const uint32_t base = tile_start + off;
+                uint64_t seps = S & ~T;
+                wc.n_struct += nS;
+                wc.n_terms += popc(T);
+                wc.n_seps += popc(seps);
+                wc.n_rows += popc(T & ~F);
+                if (T) {
+                    uint32_t hb = hibit(T);
+                    wc.has_term = 1;
+                    uint64_t above = hb == 63 ? 0ULL : (seps & ~((2ULL << hb) - 1));
+                    wc.ord = popc(above);
+                    wc.last_term = base + hb + 1;
+                } else {
+                    wc.ord += popc(seps);
+                }
+                uint32_t h = hibit(S);
+                uint64_t S2 = S & ~(1ULL << h);
+                wc.prev_struct = S2 ? base + hibit(S2) + 1 : wc.last_struct;
+                wc.last_struct = base + h + 1;
+            }
+        }
+        if (ln == 0) { sh.wcnt[w] = wc; sh.wcount[w] = cnt; }
+    }
+    __syncthreads();
+
+    if (MODE == M_TERMS) {
+        if (tid == 0) {
+            unsigned long long t = 0;
+            for (int k = 0; k < NWAVE; ++k) t += sh.wcnt[k].n_terms;
+            reinterpret_cast<unsigned long long*>(P.partials)[tile] = t;
+        }
+        return;
+    }
+
+    // ---- LB2: counters prefix ---------------------------------------------
+    if (tid == 0) {
+        sh.wbase[0] = 0;
+        for (int k = 0; k < NWAVE; ++k) sh.wbase[k + 1] = sh.wbase[k] + sh.wcount[k];
+        Counts agg = sh.wcnt[0];
+        for (int k = 1; k < NWAVE; ++k) agg = compose(agg, sh.wcnt[k]);
+        Counts pre = counts_zero();
+        if (tile > 0) {
+            store_counts(&P.lb2_agg[tile], agg);
+            lb_flag_set(P, P.lb2_flag, tile, 1);
+            Counts acc = counts_zero();
+            int32_t j = (int32_t)tile - 1;
+            while (true) {
+                uint32_t f = lb_flag_wait(P, P.lb2_flag, (uint32_t)j);
+                if (f == 2) { pre = compose(load_counts(&P.lb2_inc[j]), acc); break; }
+                acc = compose(load_counts(&P.lb2_agg[j]), acc);
+                --j;
+            }
+        }
+        Counts inc = compose(pre, agg);
+        store_counts(&P.lb2_inc[tile], inc);
+        lb_flag_set(P, P.lb2_flag, tile, 2);
+        sh.pre = pre;
+        sh.inc = inc;
+        if (last_tile) {
+            P.result[R_N_STRUCT] = inc.n_struct;
+            P.result[R_N_SEPS] = inc.n_seps;
+            P.result[R_N_TERMS] = inc.n_terms;
+        }
+    }
+    __syncthreads();
+
+    const uint32_t n_list = sh.wbase[NWAVE];
+    const Counts pre = sh.pre;
+
+    if (MODE == M_INDEX) {
+        for (uint32_t k = tid; k < n_list; k += WG) {
+            uint64_t idx = (uint64_t)pre.n_struct + k;
+            if (idx < P.out_cap) P.out[idx] = tile_start + (list_at(sh, k) & 0x3FFFu);
+        }
+        return;
+    }
+
+    TileSrc src{P.data, sh.bytes, tile_start, tile_start + tile_len};
+    const CellDialect cd{P.quote, HQ ? P.escape : -1};
+
+    // Walk the compacted list in chunks of 256 with record context.
+    auto walk = [&](auto&& visit) {
+        uint32_t terms_before = 0, rows_before = 0;
+        int32_t lt_before = -1;
+        for (uint32_t base = 0; base < n_list; base += WG) {
+            Entry e;
+            e.k = base + tid;
+            e.valid = e.k < n_list;
+            uint16_t v = e.valid ? list_at(sh, e.k) : 0;
+            e.is_term = e.valid && (v & 0x4000u);
+            e.flag = e.valid && (v & 0x8000u);
+            e.pos = tile_start + (v & 0x3FFFu);
+            Scan3 in{e.is_term ? 1u : 0u, (e.is_term && !e.flag) ? 1u : 0u, e.is_term ? (int32_t)e.k : -1};
+            Scan3 tot;
+            Scan3 ex = block_excl(in, sh.scan_sh, tot);
+            int32_t lt = ex.c > lt_before ? ex.c : lt_before;
+            e.lt = lt;
+            if (e.valid) {
+                e.fs = e.k > 0 ? tile_start + (list_at(sh, e.k - 1) & 0x3FFFu) + 1 : pre.last_struct;
+                e.rs = lt >= 0 ? tile_start + (list_at(sh, (uint32_t)lt) & 0x3FFFu) + 1 : pre.last_term;
+                e.ord = lt >= 0 ? (e.k - (uint32_t)lt - 1) : (pre.ord + e.k);
+                e.row = pre.n_rows + rows_before + ex.b;
+            }
+            visit(e, terms_before + ex.a);
+            terms_before += tot.a;
+            rows_before += tot.b;
+            lt_before = tot.c > lt_before ? tot.c : lt_before;
+        }
+    };
+
+    const Counts inc = sh.inc;
+    const bool trailer = last_tile && inc.last_term < P.len;
+    const bool trailer_comment = trailer && CMT && P.data[inc.last_term] == cb;
+
+    if (MODE == M_TAPE) {
+        walk([&](const Entry& e, uint32_t terms_excl) {
+            if (!e.valid) return;
+            uint32_t seps_before = e.k - terms_excl;
+            if (e.is_term) {
+                uint64_t idx = (uint64_t)pre.n_terms + terms_excl;
+                if (idx < P.ends_cap)
+                    P.ends[idx] = ((unsigned long long)(pre.n_seps + seps_before) << 32) | e.pos;
+            } else {
+                uint64_t idx = (uint64_t)pre.n_seps + seps_before;
+                if (idx < P.out_cap) P.out[idx] = e.pos;
+            }
+        });
+        return;
+    }
+
+    if (MODE == M_FIELDBYTES) {
+        unsigned long long fields = 0, bytes = 0;
+        walk([&](const Entry& e, uint32_t) {
+            if (!e.valid || e.flag) return;
+            uint32_t to = e.pos;
+            if (e.is_term && e.pos > e.rs && src.at(e.pos - 1) == '\r') to = e.pos - 1;
+            fields += 1;
+            bytes += field_byte_len(src, e.fs, to, P.quote);
+        });
+        if (tid == 0 && trailer && !trailer_comment) {
+            uint32_t to = (P.len > inc.last_term && P.data[P.len - 1] == '\r') ? P.len - 1 : P.len;
+            fields += 1;
+            bytes += field_byte_len(src, inc.last_struct, to, P.quote);
+        }
+        fields = block_reduce_sum(fields, sh.usum_sh);
+        bytes = block_reduce_sum(bytes, sh.usum_sh);
+        if (tid == 0) {
+            unsigned long long* pp = reinterpret_cast<unsigned long long*>(P.partials) + 2 * (size_t)tile;
+            pp[0] = fields;
+            pp[1] = bytes;
+        }
+        return;
+    }
+
+    if (MODE == M_LOGFMT) {
+        unsigned long long pairs = 0, kbytes = 0, vbytes = 0, csum = 0;
+        auto pair = [&](uint32_t ks, uint32_t ke, uint32_t vs, uint32_t ve) {
+            pairs += 1;
+            kbytes += ke - ks;
+            for (uint32_t p = ks; p < ke; ++p) csum += src.at(p);
+            // add_logfmt_value (logfmt.rs:826-858)
+            if (ve == vs || src.at(vs) != '"') {
+                vbytes += ve - vs;
+                for (uint32_t p = vs; p < ve; ++p) csum += src.at(p);
+                return;
+            }
+            uint32_t bs = vs, be = ve;
+            if (ve - vs >= 2 && src.at(ve - 1) == '"') { bs = vs + 1; be = ve - 1; }
+            uint32_t i = bs;
+            while (i < be) {
+                uint8_t b = src.at(i);
+                if (b == '\\' && i + 1 < be) { csum += src.at(i + 1); i += 2; }
+                else { csum += b; i += 1; }
+                vbytes += 1;
+            }
+        };
+        walk([&](const Entry& e, uint32_t) {
+            if (!e.valid || (e.ord & 1u) == 0) return;
+            uint32_t to = e.pos;
+            if (e.is_term && e.pos > e.rs && src.at(e.pos - 1) == '\r') to = e.pos - 1;
+            uint32_t prev1 = e.fs - 1;  // structural that ended the key field
+            uint32_t ks;
+            if (e.ord == 1) ks = e.rs;
+            else if (e.k >= 2) ks = tile_start + (list_at(sh, e.k - 2) & 0x3FFFu) + 1;
+            else if (e.k == 1) ks = pre.last_struct;
+            else ks = pre.prev_struct;
+            pair(ks, prev1, e.fs, to);
+        });
+        if (tid == 0 && last_tile) {
+            uint32_t fs = inc.last_struct;
+            if (fs < P.len || (P.len > 0 && P.data[P.len - 1] != '\n')) {
+                if (inc.ord & 1u) {
+                    uint32_t e_ = (P.len > inc.last_term && P.data[P.len - 1] == '\r') ? P.len - 1 : P.len;
+                    uint32_t ks = inc.ord == 1 ? inc.last_term : inc.prev_struct;
+                    pair(ks, fs - 1, fs, e_);
+                }
+            }
+        }
+        pairs = block_reduce_sum(pairs, sh.usum_sh);
+        kbytes = block_reduce_sum(kbytes, sh.usum_sh);
+        vbytes = block_reduce_sum(vbytes, sh.usum_sh);
+        csum = block_reduce_sum(csum, sh.usum_sh);
+        if (tid == 0) {
+            unsigned long long* pp = reinterpret_cast<unsigned long long*>(P.partials) + 4 * (size_t)tile;
+            pp[0] = pairs; pp[1] = kbytes; pp[2] = vbytes; pp[3] = csum;
+        }
+        return;
+    }
+
+    if (MODE == M_STATS) {
+        const uint32_t c0 = P.col_group;
+        const uint32_t nc = min((uint32_t)STAT_GROUP, P.n_cols - c0);
+        unsigned long long values[STAT_GROUP] = {}, csum[STAT_GROUP] = {}, bytes[STAT_GROUP] = {},
+                           fx_cells[STAT_GROUP] = {};
+        long long fx_sum[STAT_GROUP] = {};
+        double dsum[STAT_GROUP] = {};
+        unsigned long long records = 0;
+        auto cell = [&](uint32_t j, uint32_t from, uint32_t to) {
+            const ColDesc& c = P.cols[c0 + j];
+            if (c.type == FALX_COL_F64) {
+                F64Result r = parse_f64_field(src, from, to, cd);
+                if (r.ok) {
+                    values[j] += 1;
+                    csum[j] += f64_to_bits(r.value);
+                    dsum[j] += r.value;
+                    if (r.fixed6) { fx_sum[j] += r.mantissa; fx_cells[j] += 1; }
+                }
+            } else if (c.type == FALX_COL_I64) {
+                int64_t v;
+                if (parse_i64_field(src, from, to, cd, v)) {
+                    values[j] += 1;
+                    csum[j] += (unsigned long long)v;
+                    dsum[j] += (double)v;
+                }
+            } else if (c.type == FALX_COL_STR) {
+                values[j] += 1;
+                unsigned long long cs = 0;
+                bytes[j] += clean_cell(src, from, to, cd, [&](uint8_t b) { cs += b; });
+                csum[j] += cs;
+            } else {
+                if (to > from) {
+                    values[j] += 1;
+                    bytes[j] += to - from;
+                    for (uint32_t p = from; p < to; ++p) csum[j] += src.at(p);
+                }
+            }
+        };
+        auto on_fields = [&](uint32_t ord, uint32_t from, uint32_t to) {
+#pragma unroll
+            for (uint32_t j = 0; j < (uint32_t)STAT_GROUP; ++j)
+                if (j < nc && P.cols[c0 + j].index == ord) cell(j, from, to);
+        };
+        walk([&](const Entry& e, uint32_t) {
+            if (!e.valid || e.flag) return;
+            if (!e.is_term) { on_fields(e.ord, e.fs, e.pos); return; }
+            uint32_t to = (e.pos > e.rs && src.at(e.pos - 1) == '\r') ? e.pos - 1 : e.pos;
+            on_fields(e.ord, e.fs, to);
+            records += 1;
+        });
+        if (tid == 0 && trailer && !trailer_comment) {
+            uint32_t to = (P.len > inc.last_term && P.data[P.len - 1] == '\r') ? P.len - 1 : P.len;
+            on_fields(inc.ord, inc.last_struct, to);
+            records += 1;
+        }
+        StatPart* parts = reinterpret_cast<StatPart*>(P.partials);
+        records = block_reduce_sum(records, sh.usum_sh);
+        for (uint32_t j = 0; j < nc; ++j) {
+            unsigned long long v = block_reduce_sum(values[j], sh.usum_sh);
+            unsigned long long cs = block_reduce_sum(csum[j], sh.usum_sh);
+            unsigned long long by = block_reduce_sum(bytes[j], sh.usum_sh);
+            long long fs = block_reduce_sum(fx_sum[j], sh.lsum_sh);
+            unsigned long long fc = block_reduce_sum(fx_cells[j], sh.usum_sh);
+            double ds = block_reduce_sum(dsum[j], sh.dsum_sh);
+            if (tid == 0) {
+                StatPart& sp = parts[(size_t)tile * P.n_cols + c0 + j];
+                sp.values = v; sp.checksum = cs; sp.bytes = by;
+                sp.fixed6_sum = fs; sp.fixed6_cells = fc; sp.sum = ds;
+            }
+        }
+        if (tid == 0 && c0 == 0) {
+            unsigned long long* rec = reinterpret_cast<unsigned long long*>(parts + (size_t)P.n_tiles * P.n_cols);
+            rec[tile] = records;
+        }
+        return;
+    }
+
+    // ---- M_COLUMNS --------------------------------------------------------
+    if (MODE == M_COLUMNS) {
+        const uint64_t rows_cap = P.rows_cap;
+        auto set_valid = [&](const ColDesc& c, uint32_t row) {
+            atomicOr(&c.valid[row >> 6], 1ULL << (row & 63));
+        };
+        // Numeric / bytes cell of column j at `row`; missing = !found.
+        auto emit_cell = [&](const ColDesc& c, uint32_t row, bool found, uint32_t from, uint32_t to) {
+            if (row >= rows_cap) return;
+            if (c.type == FALX_COL_F64) {
+                double v = 0.0;
+                bool ok = false;
+                if (found) {
+                    F64Result r = parse_f64_field(src, from, to, cd);
+                    ok = r.ok;
+                    if (ok) v = r.value;
+                }
+                reinterpret_cast<double*>(c.values)[row] = v;
+                if (ok) set_valid(c, row);
+            } else if (c.type == FALX_COL_I64) {
+                int64_t v = 0;
+                bool ok = found && parse_i64_field(src, from, to, cd, v);
+                reinterpret_cast<long long*>(c.values)[row] = ok ? v : 0;
+                if (ok) set_valid(c, row);
+            } else if (c.type == FALX_COL_BYTES) {
+                bool ok = found && from != to;
+                reinterpret_cast<uint2*>(c.values)[row] = ok ? make_uint2(from, to) : make_uint2(0, 0);
+                if (ok) set_valid(c, row);
+            }
+        };
+        const bool have_str = P.n_str > 0;
+        // E1: numeric cells + string lengths.
+        if (tid < FALX_MAX_COLUMNS) sh.str_len[tid] = 0;
+        __syncthreads();
+        auto fields = [&](uint32_t ord, uint32_t row, uint32_t from, uint32_t to, bool term) {
+            for (uint32_t j = 0; j < P.n_cols; ++j) {
+                const ColDesc& c = P.cols[j];
+                if (c.index == ord) {
+                    if (c.type == FALX_COL_STR) {
+                        uint32_t n = clean_cell(src, from, to, cd, [](uint8_t) {});
+                        atomicAdd(&sh.str_len[c.str_slot], n);
+                    } else {
+                        emit_cell(c, row, true, from, to);
+                    }
+                } else if (term && c.index > ord && c.type != FALX_COL_STR) {
+                    emit_cell(c, row, false, 0, 0);
+                }
+            }
+        };
+        walk([&](const Entry& e, uint32_t) {
+            if (!e.valid || e.flag) return;
+            if (!e.is_term) { fields(e.ord, e.row, e.fs, e.pos, false); return; }
+            uint32_t to = (e.pos > e.rs && src.at(e.pos - 1) == '\r') ? e.pos - 1 : e.pos;
+            fields(e.ord, e.row, e.fs, to, true);
+        });
+        const uint32_t trailer_row = inc.n_rows;
+        const bool trailer_row_live = trailer && !trailer_comment;
+        uint32_t trailer_to = (P.len > inc.last_term && P.len > 0 && P.data[P.len - 1] == '\r') ? P.len - 1 : P.len;
+        if (tid == 0 && trailer_row_live) fields(inc.ord, trailer_row, inc.last_struct, trailer_to, true);
+        if (tid == 0 && last_tile) P.result[R_N_ROWS] = inc.n_rows + (trailer_row_live ? 1 : 0);
+        __syncthreads();
+        if (!have_str) return;
+
+        // LB3: arena bases per string column.
+        if (tid == 0) {
+            const uint32_t ns = P.n_str;
+            uint32_t* agg = P.lb3_agg + (size_t)tile * FALX_MAX_COLUMNS;
+            uint32_t* incp = P.lb3_inc + (size_t)tile * FALX_MAX_COLUMNS;
+            uint32_t base[FALX_MAX_COLUMNS];
+            for (uint32_t s = 0; s < ns; ++s) base[s] = 0;
+            if (tile > 0) {
+                for (uint32_t s = 0; s < ns; ++s) agg[s] = sh.str_len[s];
+                lb_flag_set(P, P.lb3_flag, tile, 1);
+                int32_t j = (int32_t)tile - 1;
+                while (true) {
+                    uint32_t f = lb_flag_wait(P, P.lb3_flag, (uint32_t)j);
+                    const uint32_t* src3 = (f == 2 ? P.lb3_inc : P.lb3_agg) + (size_t)j * FALX_MAX_COLUMNS;
+                    for (uint32_t s = 0; s < ns; ++s) base[s] += src3[s];
+                    if (f == 2) break;
+                    --j;
+                }
+            }
+            for (uint32_t s = 0; s < ns; ++s) {
+                incp[s] = base[s] + sh.str_len[s];
+                sh.str_base[s] = base[s];
+            }
+            lb_flag_set(P, P.lb3_flag, tile, 2);
+            if (last_tile)
+                for (uint32_t s = 0; s < ns; ++s) P.result[R_STR_LEN + s] = incp[s];
+        }
+        __syncthreads();
+
+        // E2: ordered arena placement per string column.
+        for (uint32_t j = 0; j < P.n_cols; ++j) {
+            const ColDesc& c = P.cols[j];
+            if (c.type != FALX_COL_STR) continue;
+            int32_t* offs = reinterpret_cast<int32_t*>(c.values);
+            uint32_t running = sh.str_base[c.str_slot];
+            if (tile == 0 && tid == 0) offs[0] = 0;
+            walk([&](const Entry& e, uint32_t) {
+                bool cellhere = e.valid && !e.flag && c.index == e.ord;
+                uint32_t from = e.fs, to = e.pos;
+                if (cellhere && e.is_term && e.pos > e.rs && src.at(e.pos - 1) == '\r') to = e.pos - 1;
+                uint32_t n = cellhere ? clean_cell(src, from, to, cd, [](uint8_t) {}) : 0u;
+                uint32_t tot;
+                uint32_t ex = block_excl_sum(n, sh.sum_sh, tot);
+                uint32_t at = running + ex;
+                if (cellhere && e.row < rows_cap) {
+                    if ((uint64_t)at + n <= c.data_cap) {
+                        uint32_t q = at;
+                        clean_cell(src, from, to, cd, [&](uint8_t b) { c.data[q++] = b; });
+                    }
+                    set_valid(c, e.row);
+                }
+                if (e.valid && e.is_term && !e.flag && e.row < rows_cap) offs[e.row + 1] = (int32_t)(at + n);
+                running += tot;
+            });
+            if (tid == 0 && trailer_row_live && trailer_row < rows_cap) {
+                uint32_t at = running;
+                uint32_t n = 0;
+                if (c.index == inc.ord) {
+                    n = clean_cell(src, inc.last_struct, trailer_to, cd, [](uint8_t) {});
+                    if ((uint64_t)at + n <= c.data_cap) {
+                        uint32_t q = at;
+                        clean_cell(src, inc.last_struct, trailer_to, cd, [&](uint8_t b) { c.data[q++] = b; });
+                    }
+                    set_valid(c, trailer_row);
+                }
+                offs[trailer_row + 1] = (int32_t)(at + n);
+            }
+            __syncthreads();
+        }
+    }
+}
+
+// --------------------------------------------------------------------------
+// reductions, fills, NDJSON id+score
+// --------------------------------------------------------------------------
+__global__ __launch_bounds__(WG) void k_fill_u64(unsigned long long* p, uint64_t n, unsigned long long v) {
+    for (uint64_t i = (uint64_t)blockIdx.x * WG + threadIdx.x; i < n; i += (uint64_t)gridDim.x * WG) p[i] = v;
+}
+
+// Sums `width` u64 lanes over `n` rows (row-major) into out[0..width).
+__global__ __launch_bounds__(WG) void k_reduce_u64(const unsigned long long* parts, uint32_t n, uint32_t width,
+                                                   unsigned long long* out) {
+    __shared__ unsigned long long sh[NWAVE];
+    for (uint32_t c = 0; c < width; ++c) {
+        unsigned long long v = 0;
+        for (uint32_t i = threadIdx.x; i < n; i += WG) v += parts[(size_t)i * width + c];
+        v = block_reduce_sum(v, sh);
+        if (threadIdx.x == 0) out[c] = v;
+    }
+}
+
+// Per-column stat reduction: one workgroup per column, fixed-order FP64 tree
+// (each thread folds a strided run of tiles, then the block tree).
+__global__ __launch_bounds__(WG) void k_reduce_stats(const StatPart* parts, uint32_t n_tiles, uint32_t n_cols,
+                                                     StatPart* out) {
+    __shared__ unsigned long long ush[NWAVE];
+    __shared__ long long lsh[NWAVE];
+    __shared__ double dsh[NWAVE];
+    const uint32_t c = blockIdx.x;
+    unsigned long long v = 0, cs = 0, by = 0, fc = 0;
+    long long fs = 0;
+    double ds = 0;
+    for (uint32_t i = threadIdx.x; i < n_tiles; i += WG) {
+        const StatPart& p = parts[(size_t)i * n_cols + c];
+        v += p.values; cs += p.checksum; by += p.bytes; fs += p.fixed6_sum; fc += p.fixed6_cells; ds += p.sum;
+    }
+    v = block_reduce_sum(v, ush);
+    cs = block_reduce_sum(cs, ush);
+    by = block_reduce_sum(by, ush);
+    fc = block_reduce_sum(fc, ush);
+    fs = block_reduce_sum(fs, lsh);
+    ds = block_reduce_sum(ds, dsh);
+    if (threadIdx.x == 0) {
+        out[c].values = v; out[c].checksum = cs; out[c].bytes = by;
+        out[c].fixed6_sum = fs; out[c].fixed6_cells = fc; out[c].sum = ds;
+    }
+}
+
+struct NdPart {
+    unsigned long long records, invalid;
+    long long id_sum, fx_sum;
+    unsigned long long fx_cells;
+    double score_sum;
+    unsigned long long pad[2];
+};
+
+// NDJSON id+score: records are split at raw '\n' (ndjson.rs:679-723); a
+// record belongs to the tile holding its '\n' (the data's trailing record to
+// the last tile). Each thread walks whole records; no cross-tile state.
+__global__ __launch_bounds__(WG) void k_ndjson_id_score(const uint8_t* data, uint32_t len, uint32_t n_tiles,
+                                                        NdPart* parts) {
+    __shared__ uint8_t bytes[TILE];
+    __shared__ uint16_t nls[TILE];
+    __shared__ uint32_t wcnt[NWAVE], wb[NWAVE + 1];
+    __shared__ int64_t prev_nl;
+    __shared__ unsigned long long ush[NWAVE];
+    __shared__ long long lsh[NWAVE];
+    __shared__ double dsh[NWAVE];
+    const uint32_t tile = blockIdx.x, tid = threadIdx.x, ln = tid & 63u, w = tid >> 6;
+    const uint32_t tile_start = tile * (uint32_t)TILE;
+    const uint32_t tile_len = min((uint32_t)TILE, len - tile_start);
+    for (uint32_t i = tid * 16; i < (uint32_t)TILE; i += WG * 16) {
+        uint4 v = make_uint4(0, 0, 0, 0);
+        if (i + 16 <= tile_len) {
+            v = *reinterpret_cast<const uint4*>(data + tile_start + i);
+        } else if (i < tile_len) {
+            uint8_t tmp[16];
+            for (int k = 0; k < 16; ++k) tmp[k] = (i + k < tile_len) ? data[tile_start + i + k] : 0;
+            memcpy(&v, tmp, 16);
+        }
+        *reinterpret_cast<uint4*>(&bytes[i]) = v;
+    }
+    if (tid == 0) {
+        int64_t p = (int64_t)tile_start - 1;
+        while (p >= 0 && data[p] != '\n') --p;
+        prev_nl = p;
+    }
+    __syncthreads();
+    uint32_t cnt = 0;
+    const uint32_t seg = w * BPW * 64;
+    for (int b = 0; b < BPW; ++b) {
+        uint32_t off = (w * BPW + b) * 64;
+        bool isnl = off + ln < tile_len && bytes[off + ln] == '\n';
+        uint64_t m = ballot(isnl);
+        if (isnl) nls[seg + cnt + mbcnt(m)] = (uint16_t)(off + ln);
+        cnt += popc(m);
+    }
+    if (ln == 0) wcnt[w] = cnt;
+    __syncthreads();
+    if (tid == 0) {
+        wb[0] = 0;
+        for (int k = 0; k < NWAVE; ++k) wb[k + 1] = wb[k] + wcnt[k];
+    }
+    __syncthreads();
+    const uint32_t n = wb[NWAVE];
+    const bool tail = tile + 1 == n_tiles && len > 0 && data[len - 1] != '\n';
+    TileSrc src{data, bytes, tile_start, tile_start + tile_len};
+    auto nl_at = [&](uint32_t k) -> uint32_t {
+        uint32_t ww = 0;
+        for (int i = 1; i < NWAVE; ++i) ww += k >= wb[i];
+        return tile_start + nls[ww * BPW * 64 + (k - wb[ww])];
+    };
+    unsigned long long recs = 0, inval = 0, fxc = 0, ids = 0;
+    long long fxs = 0;
+    double ss = 0;
+    const uint32_t total = n + (tail ? 1u : 0u);
+    for (uint32_t k = tid; k < total; k += WG) {
+        uint32_t end = k < n ? nl_at(k) : len;
+        int64_t pn = k == 0 ? prev_nl : (int64_t)nl_at(k - 1);
+        uint32_t from = (uint32_t)(pn + 1);
+        if (blank_span(src, from, end)) continue;
+        NdRecord r;
+        if (ndjson_record(src, from, end, r)) {
+            recs += 1;
+            ids += (unsigned long long)r.id;
+            ss += r.score;
+            if (r.fx6) { fxs += r.mant; fxc += 1; }
+        } else {
+            inval += 1;
+        }
+    }
+    recs = block_reduce_sum(recs, ush);
+    inval = block_reduce_sum(inval, ush);
+    fxc = block_reduce_sum(fxc, ush);
+    ids = block_reduce_sum(ids, ush);
+    fxs = block_reduce_sum(fxs, lsh);
+    ss = block_reduce_sum(ss, dsh);
+    if (tid == 0) {
+        NdPart& p = parts[tile];
+        p.records = recs; p.invalid = inval; p.id_sum = (long long)ids; p.fx_sum = fxs; p.fx_cells = fxc; p.score_sum = ss;
+    }
+}
+
+__global__ __launch_bounds__(WG) void k_reduce_nd(const NdPart* parts, uint32_t n, NdPart* out) {
+    __shared__ unsigned long long ush[NWAVE];
+    __shared__ long long lsh[NWAVE];
+    __shared__ double dsh[NWAVE];
+    unsigned long long r = 0, iv = 0, fc = 0, id = 0;
+    long long fs = 0;
+    double ss = 0;
+    for (uint32_t i = threadIdx.x; i < n; i += WG) {
+        r += parts[i].records; iv += parts[i].invalid; fc += parts[i].fx_cells;
+        id += (unsigned long long)parts[i].id_sum;
+        fs += parts[i].fx_sum; ss += parts[i].score_sum;
+    }
+    r = block_reduce_sum(r, ush);
+    iv = block_reduce_sum(iv, ush);
+    fc = block_reduce_sum(fc, ush);
+    id = block_reduce_sum(id, ush);
+    fs = block_reduce_sum(fs, lsh);
+    ss = block_reduce_sum(ss, dsh);
+    if (threadIdx.x == 0) {
+        out->records = r; out->invalid = iv; out->fx_cells = fc; out->id_sum = (long long)id; out->fx_sum = fs; out->score_sum = ss;
+    }
+}
+
+}  // namespace fx

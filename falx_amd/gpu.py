@@ -1,0 +1,266 @@
+"""Host-side mirror of the reference's generated-parser API over the C ABI of
+libfalx_gpu.so (include/falx_gpu.h).
+
+The reference's functions take a `&[u8]` and return owned vectors
+(src/kernels/<fmt>.rs); here a :class:`Context` owns a device and a stream,
+inputs live in HBM as :class:`DeviceBuffer` objects, and each method returns
+numpy arrays copied back from the device (or leaves results on the device
+when ``download=False``). The native library is required: importing this
+module without a built library raises, there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Dict, Optional
+
+import numpy as np
+
+from . import build as _build
+from . import spec as S
+
+_lib = None
+
+# (name, restype, argtypes) for every entry point of include/falx_gpu.h
+_VP, _U64, _U64P = C.c_void_p, C.c_uint64, C.POINTER(C.c_uint64)
+_SIGS = [
+    ("falx_ctx_create", C.c_int, [C.c_int, C.POINTER(C.c_void_p)]),
+    ("falx_ctx_destroy", None, [_VP]),
+    ("falx_ctx_stream", _VP, [_VP]),
+    ("falx_ctx_last_kernel_ms", C.c_double, [_VP]),
+    ("falx_device_alloc", C.c_int, [_VP, _U64, C.POINTER(C.c_void_p)]),
+    ("falx_device_free", C.c_int, [_VP, _VP]),
+    ("falx_host_alloc_pinned", C.c_int, [_VP, _U64, C.POINTER(C.c_void_p)]),
+    ("falx_host_free_pinned", C.c_int, [_VP, _VP]),
+    ("falx_upload", C.c_int, [_VP, _VP, _VP, _U64]),
+    ("falx_download", C.c_int, [_VP, _VP, _VP, _U64]),
+    ("falx_spec_validate", C.c_int, [C.POINTER(S.CSpec)]),
+    ("falx_index_structurals", C.c_int, [_VP, C.POINTER(S.CSpec), _VP, _U64, _VP, _U64, _U64P]),
+    ("falx_parse", C.c_int, [_VP, C.POINTER(S.CSpec), _VP, _U64, _VP, _U64, _VP, _U64, _U64P, _U64P]),
+    ("falx_parse_columns", C.c_int, [_VP, C.POINTER(S.CSpec), _VP, _U64, C.POINTER(S.CColumns)]),
+    ("falx_parse_csv_stats", C.c_int, [_VP, C.POINTER(S.CSpec), _VP, _U64, C.POINTER(S.CCsvStats)]),
+    ("falx_parse_field_bytes", C.c_int, [_VP, C.POINTER(S.CSpec), _VP, _U64, C.POINTER(S.CFieldByteStats)]),
+    ("falx_parse_ndjson_lines", C.c_int, [_VP, _VP, _U64, _U64P]),
+    ("falx_parse_ndjson_id_score", C.c_int, [_VP, _VP, _U64, C.POINTER(S.CNdjsonIdScore)]),
+    ("falx_parse_logfmt_pairs", C.c_int, [_VP, _VP, _U64, C.POINTER(S.CLogfmtStats)]),
+]
+EXPORTED = [name for name, _, _ in _SIGS]
+
+
+class FalxError(RuntimeError):
+    def __init__(self, fn: str, status: int):
+        super().__init__(f"{fn}: {S.STATUS_NAMES.get(status, status)}")
+        self.status = status
+
+
+def lib_path() -> str:
+    return _build.gpu_lib_path()
+
+
+def load():
+    """Load libfalx_gpu.so (built in-tree by __graft_entry__.build())."""
+    global _lib
+    if _lib is None:
+        path = lib_path()
+        if not os.path.exists(path):
+            raise ImportError(f"libfalx_gpu.so not built ({path}); run __graft_entry__.build()")
+        L = C.CDLL(path)
+        for name, res, args in _SIGS:
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(fn: str, st: int, allow=(S.OK,)):
+    if st not in allow:
+        raise FalxError(fn, st)
+    return st
+
+
+class DeviceBuffer:
+    """A device allocation owned by a Context."""
+
+    def __init__(self, ctx: "Context", nbytes: int):
+        self.ctx = ctx
+        self.nbytes = int(nbytes)
+        p = C.c_void_p()
+        _check("falx_device_alloc", load().falx_device_alloc(ctx.h, max(1, self.nbytes), C.byref(p)))
+        self.ptr = p.value
+
+    def free(self):
+        if self.ptr:
+            load().falx_device_free(self.ctx.h, self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            if self.ptr and self.ctx.h:
+                self.free()
+        except Exception:
+            pass
+
+    def download(self, dtype=np.uint8, count: Optional[int] = None, offset: int = 0) -> np.ndarray:
+        dt = np.dtype(dtype)
+        n = (self.nbytes - offset) // dt.itemsize if count is None else int(count)
+        out = np.empty(n, dtype=dt)
+        if n:
+            _check("falx_download", load().falx_download(self.ctx.h, out.ctypes.data,
+                                                         self.ptr + offset, n * dt.itemsize))
+        return out
+
+
+class Context:
+    def __init__(self, device: int = 0):
+        L = load()
+        h = C.c_void_p()
+        _check("falx_ctx_create", L.falx_ctx_create(device, C.byref(h)))
+        self.h = h.value
+
+    def close(self):
+        if self.h:
+            load().falx_ctx_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def last_kernel_ms(self) -> float:
+        return load().falx_ctx_last_kernel_ms(self.h)
+
+    def alloc(self, nbytes: int) -> DeviceBuffer:
+        return DeviceBuffer(self, nbytes)
+
+    def upload(self, data) -> DeviceBuffer:
+        arr = np.ascontiguousarray(data if isinstance(data, np.ndarray) else
+                                   np.frombuffer(bytes(data), dtype=np.uint8))
+        buf = DeviceBuffer(self, arr.nbytes)
+        if arr.nbytes:
+            _check("falx_upload", load().falx_upload(self.h, buf.ptr, arr.ctypes.data, arr.nbytes))
+        return buf
+
+    # -- structural index -------------------------------------------------
+    def index_structurals(self, spec: S.Spec, data: DeviceBuffer, length: Optional[int] = None,
+                          out: Optional[DeviceBuffer] = None, download: bool = True):
+        n = data.nbytes if length is None else length
+        cs = spec.to_c()
+        if out is None:
+            out = self.alloc(4 * max(1, n))
+        cnt = C.c_uint64()
+        _check("falx_index_structurals",
+               load().falx_index_structurals(self.h, C.byref(cs), data.ptr, n, out.ptr, out.nbytes // 4,
+                                             C.byref(cnt)))
+        return out.download(np.uint32, cnt.value) if download else (out, cnt.value)
+
+    def parse(self, spec: S.Spec, data: DeviceBuffer, length: Optional[int] = None):
+        n = data.nbytes if length is None else length
+        cs = spec.to_c()
+        seps = self.alloc(4 * max(1, n))
+        ends = self.alloc(8 * max(1, n))
+        ns, ne = C.c_uint64(), C.c_uint64()
+        _check("falx_parse", load().falx_parse(self.h, C.byref(cs), data.ptr, n, seps.ptr, max(1, n), ends.ptr,
+                                               max(1, n), C.byref(ns), C.byref(ne)))
+        return seps.download(np.uint32, ns.value), ends.download(np.uint64, ne.value)
+
+    # -- columns ----------------------------------------------------------
+    def alloc_columns(self, spec: S.Spec, rows_cap: int, data_cap: int):
+        cols = S.CColumns()
+        cols.rows_cap = rows_cap
+        keep = []
+        words = (rows_cap + 63) // 64
+        for k, c in enumerate(spec.columns):
+            if c.type in (S.I64, S.F64, S.BYTES):
+                vals = self.alloc(8 * max(1, rows_cap))
+                data = None
+            else:
+                vals = self.alloc(4 * (rows_cap + 1))
+                data = self.alloc(max(1, data_cap))
+            valid = self.alloc(8 * max(1, words))
+            cols.cols[k].values = vals.ptr
+            cols.cols[k].valid = valid.ptr
+            if data is not None:
+                cols.cols[k].data = data.ptr
+                cols.cols[k].data_cap = data_cap
+            keep.append((vals, valid, data))
+        return cols, keep
+
+    def parse_columns_into(self, spec: S.Spec, data: DeviceBuffer, length: int, cols: S.CColumns) -> int:
+        cs = spec.to_c()
+        return _check("falx_parse_columns",
+                      load().falx_parse_columns(self.h, C.byref(cs), data.ptr, length, C.byref(cols)),
+                      allow=(S.OK, S.E_CAPACITY))
+
+    def parse_columns(self, spec: S.Spec, data: DeviceBuffer, length: Optional[int] = None,
+                      rows_cap: Optional[int] = None, data_cap: Optional[int] = None) -> Dict:
+        n = data.nbytes if length is None else length
+        rows_cap = n + 1 if rows_cap is None else rows_cap
+        data_cap = max(16, n) if data_cap is None else data_cap
+        cols, keep = self.alloc_columns(spec, rows_cap, data_cap)
+        st = self.parse_columns_into(spec, data, n, cols)
+        if st == S.E_CAPACITY:
+            raise FalxError("falx_parse_columns", st)
+        rows = cols.rows
+        words = (rows + 63) // 64
+        res = {"rows": rows, "cols": []}
+        for k, c in enumerate(spec.columns):
+            vals, valid, dbuf = keep[k]
+            entry = {"valid": valid.download(np.uint64, words)}
+            if c.type == S.I64:
+                entry["values"] = vals.download(np.int64, rows)
+            elif c.type == S.F64:
+                entry["values"] = vals.download(np.float64, rows)
+            elif c.type == S.BYTES:
+                entry["values"] = vals.download(np.uint32, 2 * rows).reshape(-1, 2)
+            else:
+                entry["offsets"] = vals.download(np.int32, rows + 1)
+                entry["data"] = dbuf.download(np.uint8, cols.cols[k].data_len)
+            res["cols"].append(entry)
+        return res
+
+    # -- fused sinks ------------------------------------------------------
+    def csv_stats(self, spec: S.Spec, data: DeviceBuffer, length: Optional[int] = None) -> Dict:
+        n = data.nbytes if length is None else length
+        cs = spec.to_c()
+        st = S.CCsvStats()
+        _check("falx_parse_csv_stats", load().falx_parse_csv_stats(self.h, C.byref(cs), data.ptr, n, C.byref(st)))
+        out = {"records": st.records, "cols": []}
+        for k in range(len(spec.columns)):
+            c = st.cols[k]
+            out["cols"].append({"values": c.values, "checksum": c.checksum, "bytes": c.bytes,
+                                "fixed6_sum": c.fixed6_sum, "fixed6_cells": c.fixed6_cells, "sum": c.sum})
+        return out
+
+    def field_bytes(self, spec: S.Spec, data: DeviceBuffer, length: Optional[int] = None) -> Dict:
+        n = data.nbytes if length is None else length
+        cs = spec.to_c()
+        st = S.CFieldByteStats()
+        _check("falx_parse_field_bytes",
+               load().falx_parse_field_bytes(self.h, C.byref(cs), data.ptr, n, C.byref(st)))
+        return {"fields": st.fields, "bytes": st.bytes}
+
+    def ndjson_lines(self, data: DeviceBuffer, length: Optional[int] = None) -> int:
+        n = data.nbytes if length is None else length
+        r = C.c_uint64()
+        _check("falx_parse_ndjson_lines", load().falx_parse_ndjson_lines(self.h, data.ptr, n, C.byref(r)))
+        return r.value
+
+    def ndjson_id_score(self, data: DeviceBuffer, length: Optional[int] = None) -> Dict:
+        n = data.nbytes if length is None else length
+        st = S.CNdjsonIdScore()
+        _check("falx_parse_ndjson_id_score",
+               load().falx_parse_ndjson_id_score(self.h, data.ptr, n, C.byref(st)), allow=(S.OK, S.E_SCHEMA))
+        return {"records": st.records, "invalid": st.invalid, "id_sum": st.id_sum,
+                "score_fixed6_sum": st.score_fixed6_sum, "score_fixed6_cells": st.score_fixed6_cells,
+                "score_sum": st.score_sum}
+
+    def logfmt_pairs(self, data: DeviceBuffer, length: Optional[int] = None) -> Dict:
+        n = data.nbytes if length is None else length
+        st = S.CLogfmtStats()
+        _check("falx_parse_logfmt_pairs", load().falx_parse_logfmt_pairs(self.h, data.ptr, n, C.byref(st)))
+        return {"pairs": st.pairs, "key_bytes": st.key_bytes, "value_bytes": st.value_bytes,
+                "checksum": st.checksum}

@@ -1,0 +1,246 @@
+/*
+ * falx_gpu.h — C ABI of the MI355X (gfx950) structural-indexing and
+ * column-parsing library.
+ *
+ * The reference exposes this path as the public functions of a generated
+ * Rust module (one module per format spec, src/kernels/<fmt>.rs). This
+ * header is the device backend a `falx::gpu` shim binds with `extern "C"`:
+ * plain pointers and sizes, status codes instead of panics, no unwinding.
+ *
+ * Conventions
+ *  - Every `d_` pointer is device memory (HBM). Inputs stay read-only.
+ *  - Entry points run on the context's stream and return after the result
+ *    counts are known (they synchronise that stream once per call).
+ *  - Offsets are u32, so inputs must be shorter than 2^32 bytes; longer
+ *    inputs return FALX_E_TOO_LARGE (the reference casts `len as u32`
+ *    unchecked, src/kernels/csv_geo.rs:1113).
+ *  - Output capacities are caller-provided. When a result does not fit,
+ *    the call returns FALX_E_CAPACITY and still reports the required size
+ *    in the count out-parameters, so the caller can grow and retry.
+ *  - A context is bound to one device and one stream; calls on one
+ *    context must not overlap (use one context per host thread).
+ */
+#ifndef FALX_GPU_H
+#define FALX_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum falx_status {
+    FALX_OK = 0,
+    FALX_E_INVALID = 1,    /* bad argument or spec (spec.rs:44-276 rules) */
+    FALX_E_TOO_LARGE = 2,  /* input >= 2^32 bytes, or a string column
+                              beyond the 2 GiB Arrow i32-offset limit
+                              (reference panics, csv_hash.rs:1405-1408) */
+    FALX_E_CAPACITY = 3,   /* an output buffer is too small; counts report
+                              the required sizes */
+    FALX_E_DEVICE = 4,     /* HIP runtime error */
+    FALX_E_NO_DEVICE = 5,  /* no usable gfx950 device */
+    FALX_E_SCHEMA = 6,     /* an NDJSON record lacks the schema keys
+                              (reference panics, ndjson.rs:720-723) */
+    FALX_E_BUSY = 7        /* admission refused: HBM budget exhausted */
+} falx_status;
+
+#define FALX_MAX_STRUCTURAL 16
+#define FALX_MAX_COLUMNS 32
+
+/* Column cell types (codegen.rs:78-94). */
+typedef enum falx_column_type {
+    FALX_COL_I64 = 0,   /* exactly `str::parse::<i64>` */
+    FALX_COL_F64 = 1,   /* exactly `str::parse::<f64>` (correctly rounded) */
+    FALX_COL_STR = 2,   /* cleaned bytes, Arrow varbinary (offsets + data) */
+    FALX_COL_BYTES = 3  /* raw (start, end) span into the input */
+} falx_column_type;
+
+typedef struct falx_column {
+    uint32_t index;     /* zero-based field ordinal within a record */
+    uint32_t type;      /* falx_column_type */
+} falx_column;
+
+/*
+ * Runtime dialect + projection descriptor. It carries what the reference
+ * bakes into a generated module at build time: formats.rs:30-53 (Dialect)
+ * and codegen.rs:99-110 (Column). `structural` lists field separators and,
+ * normally, '\n' (the record terminator class, formats.rs:206-224).
+ */
+typedef struct falx_spec {
+    uint8_t structural[FALX_MAX_STRUCTURAL];
+    uint32_t n_structural;
+    int32_t quote;      /* -1 = no quote convention */
+    int32_t escape;     /* -1 = none / doubled quotes; else backslash-style
+                           escape byte (formats.rs:15-25) */
+    int32_t comment;    /* -1 = none; else line-start comment byte */
+    uint32_t n_columns;
+    falx_column columns[FALX_MAX_COLUMNS];
+} falx_spec;
+
+/* One projected column's device buffers (Arrow layouts, csv_typed.rs:920-977). */
+typedef struct falx_column_buf {
+    void* values;       /* I64: int64[rows_cap]; F64: double[rows_cap];
+                           BYTES: uint32[2*rows_cap] (start, end);
+                           STR: int32 offsets[rows_cap + 1] */
+    uint64_t* valid;    /* LSB-first validity, ceil(rows_cap / 64) words */
+    uint8_t* data;      /* STR only: cleaned bytes arena */
+    uint64_t data_cap;  /* STR only: arena capacity (bytes) */
+    uint64_t data_len;  /* STR only, out: arena bytes used */
+} falx_column_buf;
+
+typedef struct falx_columns {
+    uint64_t rows_cap;  /* in: row capacity of every column buffer */
+    uint64_t rows;      /* out: rows produced (required rows on CAPACITY) */
+    falx_column_buf cols[FALX_MAX_COLUMNS];
+} falx_columns;
+
+/*
+ * Fused per-column statistics. `values`/`checksum`/`bytes` are the
+ * reference's fused counters generalised to any column set:
+ *  F64: values = parsed cells, checksum = wrapping sum of f64 bit patterns
+ *       (csv_geo.rs:1735-1750);
+ *  STR: values = present cells, bytes = cleaned length, checksum = wrapping
+ *       sum of cleaned bytes (csv_geo_text.rs:1870-1878);
+ *  I64: values = parsed cells, checksum = wrapping sum of the values;
+ *  BYTES: values = non-empty cells, bytes = raw length, checksum = byte sum.
+ * Added on top: `sum` (F64/I64 values, blocked FP64 tree) and, for F64
+ * cells of the fixed `[-]d{1,3}.d{6}` shape, the exact int64 sum of their
+ * signed 1e-6 mantissas.
+ */
+typedef struct falx_column_stats {
+    uint64_t values;
+    uint64_t checksum;
+    uint64_t bytes;
+    int64_t fixed6_sum;
+    uint64_t fixed6_cells;
+    double sum;
+} falx_column_stats;
+
+typedef struct falx_csv_stats {
+    uint64_t records;   /* every non-comment record, header included */
+    falx_column_stats cols[FALX_MAX_COLUMNS];
+} falx_csv_stats;
+
+typedef struct falx_field_byte_stats {
+    uint64_t fields;
+    uint64_t bytes;
+} falx_field_byte_stats;
+
+typedef struct falx_logfmt_stats {
+    uint64_t pairs;
+    uint64_t key_bytes;
+    uint64_t value_bytes;
+    uint64_t checksum;
+} falx_logfmt_stats;
+
+/*
+ * NDJSON `sum(id + nested.score)`. `id_sum` is the wrapping i64 sum of the
+ * top-level integer `id`; scores are JSON numbers parsed correctly rounded
+ * to f64 and summed with a blocked FP64 tree (`score_sum`); scores with at
+ * most six decimals and no exponent are also summed exactly in 1e-6 units
+ * (`score_fixed6_sum`, over `score_fixed6_cells` records). `invalid` counts
+ * non-blank records without both keys (the reference panics on those).
+ */
+typedef struct falx_ndjson_id_score_stats {
+    uint64_t records;
+    uint64_t invalid;
+    int64_t id_sum;
+    int64_t score_fixed6_sum;
+    uint64_t score_fixed6_cells;
+    double score_sum;
+} falx_ndjson_id_score_stats;
+
+typedef struct falx_ctx falx_ctx;
+
+/* ---- context, memory ---------------------------------------------------- */
+
+/* Create a context on `device` (HIP ordinal) with its own stream. */
+falx_status falx_ctx_create(int device, falx_ctx** out);
+void falx_ctx_destroy(falx_ctx* ctx);
+/* The context's hipStream_t, for callers that order their own work. */
+void* falx_ctx_stream(falx_ctx* ctx);
+/* Milliseconds the last entry point's kernels took on the device. */
+double falx_ctx_last_kernel_ms(falx_ctx* ctx);
+
+falx_status falx_device_alloc(falx_ctx* ctx, uint64_t bytes, void** d_out);
+falx_status falx_device_free(falx_ctx* ctx, void* d_ptr);
+falx_status falx_host_alloc_pinned(falx_ctx* ctx, uint64_t bytes, void** h_out);
+falx_status falx_host_free_pinned(falx_ctx* ctx, void* h_ptr);
+falx_status falx_upload(falx_ctx* ctx, void* d_dst, const void* h_src, uint64_t bytes);
+falx_status falx_download(falx_ctx* ctx, void* h_dst, const void* d_src, uint64_t bytes);
+
+/* spec.rs:44-276 validation rules applied to a runtime spec. */
+falx_status falx_spec_validate(const falx_spec* spec);
+
+/* ---- structural index ----------------------------------------------------- */
+
+/*
+ * Byte offsets of every structural byte (src/kernels/csv_geo.rs:24-52
+ * `index_structurals`, and `index_structurals_par` 95-149 / csv_hash.rs:345-400,
+ * which are byte-identical). Writes n_out offsets to d_out.
+ */
+falx_status falx_index_structurals(falx_ctx* ctx, const falx_spec* spec,
+                                   const uint8_t* d_data, uint64_t len,
+                                   uint32_t* d_out, uint64_t out_cap,
+                                   uint64_t* n_out);
+
+/* ---- tape ----------------------------------------------------------------- */
+
+/*
+ * The record tape of `parse` / `parse_par` (csv_geo.rs:394-421, 215-334,
+ * push_tape 1838-1849): separator offsets, and per terminated record an end
+ * entry `(separators before it << 32) | terminator offset`.
+ */
+falx_status falx_parse(falx_ctx* ctx, const falx_spec* spec,
+                       const uint8_t* d_data, uint64_t len,
+                       uint32_t* d_seps, uint64_t seps_cap,
+                       uint64_t* d_ends, uint64_t ends_cap,
+                       uint64_t* n_seps, uint64_t* n_ends);
+
+/* ---- typed columns -------------------------------------------------------- */
+
+/*
+ * Arrow-layout typed columns of the spec's projection: `parse_columns` /
+ * `parse_columns_par` (csv_geo.rs:1112-1116, 1216-1284; csv_hash.rs:1570-1663;
+ * csv_typed.rs:979-1152). Column buffers must be sized for `rows_cap` rows;
+ * validity words are fully written (bits past `rows` are zero).
+ */
+falx_status falx_parse_columns(falx_ctx* ctx, const falx_spec* spec,
+                               const uint8_t* d_data, uint64_t len,
+                               falx_columns* out);
+
+/* ---- fused sinks ---------------------------------------------------------- */
+
+/*
+ * Fused count/checksum/sum over the spec's columns without materialising
+ * them: `parse_csv_geo_stats[_par]` (csv_geo.rs:1505-1770) and
+ * `parse_csv_geo_text_stats[_par]` (csv_geo_text.rs:1609-1913), with the
+ * column ordinals taken from the spec instead of hardcoded.
+ */
+falx_status falx_parse_csv_stats(falx_ctx* ctx, const falx_spec* spec,
+                                 const uint8_t* d_data, uint64_t len,
+                                 falx_csv_stats* out);
+
+/* `parse_field_bytes[_par]` (csv_hash.rs:965-1227; tsv.rs:847-866). */
+falx_status falx_parse_field_bytes(falx_ctx* ctx, const falx_spec* spec,
+                                   const uint8_t* d_data, uint64_t len,
+                                   falx_field_byte_stats* out);
+
+/* `parse_ndjson_lines[_par]` (ndjson.rs:618-657): escape-aware line count. */
+falx_status falx_parse_ndjson_lines(falx_ctx* ctx, const uint8_t* d_data,
+                                    uint64_t len, uint64_t* records);
+
+/* `parse_ndjson_id_score[_par]` (ndjson.rs:679-749), schema-aware form. */
+falx_status falx_parse_ndjson_id_score(falx_ctx* ctx, const uint8_t* d_data,
+                                       uint64_t len,
+                                       falx_ndjson_id_score_stats* out);
+
+/* `parse_logfmt_pairs[_par]` (logfmt.rs:612-857). */
+falx_status falx_parse_logfmt_pairs(falx_ctx* ctx, const uint8_t* d_data,
+                                    uint64_t len, falx_logfmt_stats* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FALX_GPU_H */

@@ -1,0 +1,132 @@
+// ndjson_check.cpp — host build of falx_amd/csrc/ndjson_walk.h (the record
+// walker the gfx950 NDJSON kernel runs) swept against the CPU oracle on
+// randomized records: valid shapes, decoy keys inside strings, escapes,
+// whitespace, missing keys and malformed JSON.
+// usage: ndjson_check <seed> <count>
+#include <cinttypes>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+
+#include "../../falx_amd/csrc/ndjson_walk.h"
+#include "../../oracle/oracle.h"
+
+namespace {
+
+struct Rng {
+    uint64_t s;
+    uint64_t next() {
+        s ^= s >> 12;
+        s ^= s << 25;
+        s ^= s >> 27;
+        return s * 0x2545F4914F6CDD1DULL;
+    }
+    uint64_t below(uint64_t n) { return next() % n; }
+};
+
+struct BufSrc {
+    const uint8_t* p;
+    uint8_t at(uint32_t i) const { return p[i]; }
+};
+
+std::string ws(Rng& r) {
+    const char* w[] = {"", "", "", " ", "  ", "\t", " \r "};
+    return w[r.below(7)];
+}
+
+std::string jstring(Rng& r) {
+    std::string s = "\"";
+    int n = (int)r.below(12);
+    for (int i = 0; i < n; ++i) {
+        uint64_t k = r.below(14);
+        if (k == 0) s += "\\\"";
+        else if (k == 1) s += "\\\\";
+        else if (k == 2) s += "id\\\":9";
+        else if (k == 3) s += "{[";
+        else if (k == 4) s += "}]";
+        else s.push_back((char)('a' + r.below(26)));
+    }
+    return s + "\"";
+}
+
+std::string jnumber(Rng& r) {
+    std::string s;
+    if (r.below(3) == 0) s += "-";
+    uint64_t k = r.below(8);
+    if (k == 0) s += "0";
+    else s += std::to_string(r.below(1000000));
+    if (r.below(2)) { s += "."; int d = 1 + (int)r.below(8); for (int i = 0; i < d; ++i) s.push_back((char)('0' + r.below(10))); }
+    if (r.below(6) == 0) { s += r.below(2) ? "e" : "E"; if (r.below(2)) s += "-"; s += std::to_string(r.below(30)); }
+    if (r.below(40) == 0) s += "x";  // malformed
+    return s;
+}
+
+std::string jvalue(Rng& r, int depth);
+
+std::string jobject(Rng& r, int depth, bool with_keys) {
+    std::string s = "{" + ws(r);
+    int n = (int)r.below(4);
+    bool id_first = with_keys && r.below(2);
+    std::string id = "\"id\"" + ws(r) + ":" + ws(r) + (r.below(20) ? std::to_string((int64_t)r.below(1ULL << 40) - (r.below(4) == 0 ? 5000 : 0)) : jnumber(r));
+    std::string nested = "\"nested\"" + ws(r) + ":" + ws(r) + "{" + ws(r) +
+                         (r.below(3) ? "\"tags\":[" + jstring(r) + "]," + ws(r) : "") +
+                         (r.below(15) ? "\"score\"" + ws(r) + ":" + ws(r) + jnumber(r) : "\"scor\":1") + ws(r) + "}";
+    bool first = true;
+    auto add = [&](const std::string& kv) {
+        if (!first) s += "," + ws(r);
+        s += kv;
+        first = false;
+    };
+    if (id_first && r.below(10)) add(id);
+    for (int i = 0; i < n; ++i) add(jstring(r) + ws(r) + ":" + ws(r) + jvalue(r, depth + 1));
+    if (with_keys && r.below(10)) add(nested);
+    if (!id_first && with_keys && r.below(10)) add(id);
+    s += ws(r) + "}";
+    return s;
+}
+
+std::string jvalue(Rng& r, int depth) {
+    uint64_t k = r.below(depth > 2 ? 3 : 5);
+    if (k == 0) return jstring(r);
+    if (k == 1) return jnumber(r);
+    if (k == 2) return r.below(2) ? "true" : "null";
+    if (k == 3) return "[" + jvalue(r, depth + 1) + "," + ws(r) + jvalue(r, depth + 1) + "]";
+    return jobject(r, depth + 1, false);
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+    uint64_t seed = argc > 1 ? std::strtoull(argv[1], nullptr, 0) : 1;
+    uint64_t count = argc > 2 ? std::strtoull(argv[2], nullptr, 0) : 100000;
+    Rng r{seed | 1};
+    uint64_t valid = 0;
+    for (uint64_t i = 0; i < count; ++i) {
+        std::string line = jobject(r, 0, true);
+        if (r.below(50) == 0) line = line.substr(0, r.below(line.size() + 1));  // truncated record
+        std::string withnl = line + "\n";
+        falx_ndjson_id_score_stats want;
+        ora_ndjson_id_score((const uint8_t*)withnl.data(), withnl.size(), 1, &want);
+        BufSrc src{(const uint8_t*)line.data()};
+        bool blank = fx::blank_span(src, 0, (uint32_t)line.size());
+        fx::NdRecord rec;
+        int ok = blank ? -1 : fx::ndjson_record(src, 0, (uint32_t)line.size(), rec);
+        uint64_t g_rec = ok == 1, g_inv = ok == 0;
+        int64_t g_id = ok == 1 ? rec.id : 0;
+        int64_t g_fx = ok == 1 && rec.fx6 ? rec.mant : 0;
+        uint64_t g_fc = ok == 1 && rec.fx6;
+        double g_s = ok == 1 ? rec.score : 0.0;
+        if (g_rec != want.records || g_inv != want.invalid || g_id != want.id_sum ||
+            g_fx != want.score_fixed6_sum || g_fc != want.score_fixed6_cells ||
+            fx::f64_to_bits(0.0 + g_s) != fx::f64_to_bits(want.score_sum)) {
+            std::fprintf(stderr, "MISMATCH on record [%s]: got rec=%" PRIu64 " inv=%" PRIu64 " id=%" PRId64
+                                 " want rec=%" PRIu64 " inv=%" PRIu64 " id=%" PRId64 "\n",
+                         line.c_str(), g_rec, g_inv, g_id, want.records, want.invalid, want.id_sum);
+            return 1;
+        }
+        valid += g_rec;
+    }
+    std::printf("ndjson_check: %" PRIu64 " records agree (%" PRIu64 " valid)\n", count, valid);
+    return 0;
+}

@@ -1,0 +1,57 @@
+"""The C ABI library loads and exports every entry point include/falx_gpu.h
+declares (no device calls: this runs without a GPU)."""
+import ctypes
+import os
+import re
+
+import pytest
+
+from falx_amd import build, gpu, spec as S
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    text = open(os.path.join(ROOT, "include", "falx_gpu.h")).read()
+    return sorted(set(re.findall(r"^\s*(?:falx_status|void\*?|double)\s+(falx_\w+)\s*\(", text, re.M)))
+
+
+def test_header_declares_the_mirror_surface():
+    syms = declared_symbols()
+    assert set(syms) == set(gpu.EXPORTED), set(syms) ^ set(gpu.EXPORTED)
+
+
+def test_library_exports_every_declared_symbol():
+    path = build.gpu_lib_path()
+    if not os.path.exists(path):
+        pytest.skip("libfalx_gpu.so not built")
+    lib = ctypes.CDLL(path)
+    for name in declared_symbols():
+        assert hasattr(lib, name), name
+
+
+def test_struct_layouts_match_header():
+    # sizes the C compiler gives these structs (x86-64 / gfx950 share them)
+    assert ctypes.sizeof(S.CSpec) == 16 + 4 * 5 + 8 * 32
+    assert ctypes.sizeof(S.CColumnBuf) == 40
+    assert ctypes.sizeof(S.CColumns) == 16 + 40 * 32
+    assert ctypes.sizeof(S.CColumnStats) == 48
+    assert ctypes.sizeof(S.CNdjsonIdScore) == 48
+
+
+def test_spec_toml_front_end():
+    """docs/spec.md examples through the spec.rs validation rules."""
+    csv_hash = S.parse_spec_toml('name = "csv_hash"\nstructural = [",", "\\n"]\nquote = \'"\'\ncomment = "#"\n'
+                                 '[[columns]]\nindex = 0\nname = "key"\ntype = "string"\n'
+                                 '[[columns]]\nindex = 1\nname = "amount"\ntype = "i64"\n')
+    assert csv_hash.structural == b",\n" and csv_hash.quote == 34 and csv_hash.comment == 35
+    assert [(c.index, c.type) for c in csv_hash.columns] == [(0, S.STR), (1, S.I64)]
+    nd = S.parse_spec_toml('name = "ndjson"\nstructural = [",", ":", "\\n"]\nquote = \'"\'\n'
+                           'escape = "backslash"\nnesting = ["{}", "[]"]\n')
+    assert nd.escape == 92 and set(b"{}[]") <= set(nd.structural)
+    for bad in ['name = "x"\nstructural = [","]\ncomment = "\\n"\n',
+                'name = "x"\nstructural = [","]\nquote = "#"\ncomment = "#"\n',
+                'name = "x"\nstructural = [",,"]\n', 'structural = [","]\n',
+                'name = "x"\nstructural = [","]\nescape = "weird"\n']:
+        with pytest.raises(ValueError):
+            S.parse_spec_toml(bad)

@@ -1,0 +1,35 @@
+"""CPU sweeps of the device-side code paths that are written once for host
+and device (falx_amd/csrc/cells.h, ndjson_walk.h): their host build is run
+against the oracle on randomized cells and records."""
+import os
+import subprocess
+
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+OUT = os.path.join(HERE, "_build")
+
+
+def _build(name: str, ora) -> str:
+    os.makedirs(OUT, exist_ok=True)
+    exe = os.path.join(OUT, name)
+    lib = os.path.join(ROOT, "oracle", "_build")
+    subprocess.run(["g++", "-O2", "-std=c++17", "-o", exe, os.path.join(HERE, "native", name + ".cpp"),
+                    os.path.join(lib, "liboracle.so"), f"-Wl,-rpath,{lib}"], check=True)
+    return exe
+
+
+@pytest.mark.parametrize("seed", [1, 0xDEADBEEF, 20261203])
+def test_cell_parsers_match_oracle(ora, seed):
+    exe = _build("cells_check", ora)
+    r = subprocess.run([exe, str(seed), "150000"], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    assert "agree" in r.stdout
+
+
+@pytest.mark.parametrize("seed", [1, 77])
+def test_ndjson_walker_matches_oracle(ora, seed):
+    exe = _build("ndjson_check", ora)
+    r = subprocess.run([exe, str(seed), "100000"], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
