@@ -8,10 +8,15 @@
 //  - f64: exactly `str::parse::<f64>` (codegen.rs FLOAT_CELL_TPL 7566-7744,
 //    csv_geo.rs:1313-1486): grammar `[+-]?(inf|infinity|nan|(d+.?d*|.d+)
 //    ([eE][+-]?d+)?)` (inf/nan case-insensitive), correctly rounded, ties to
-//    even. Three paths, all exact: the fixed `[-]d{1,3}.d{6}` shape (m/1e6),
-//    Clinger's fast path (exact mantissa <= 2^53, |e10| <= 22), and an exact
-//    big-decimal conversion (shift the decimal by powers of two into
-//    [1/2, 1), then round the 53-bit significand half-to-even).
+//    even. Paths, all exact: the fixed `[-]d{1,3}.d{6}` shape (m/1e6);
+//    Clinger's fast path (exact mantissa <= 2^53, |e10| <= 22); Eisel-Lemire
+//    (Lemire 2021) with a 128-bit 5^q table, exact integer rounding where a
+//    tie is possible and a margin test elsewhere; and, for the cells that
+//    leaves undecided (subnormal results, >19-digit mantissas whose
+//    truncation bounds disagree, products within 2 units of a rounding
+//    boundary), an exact big-decimal conversion (shift the decimal by powers
+//    of two into [1/2, 1), then round the 53-bit significand half-to-even).
+//    The kernels hand those rare cells to a separate fixup pass.
 //  - cleaning: doubled-quote collapse or backslash unescape (codegen.rs
 //    append_clean_fn / clean_fn 8008-8185).
 //
@@ -21,12 +26,12 @@
 
 #include <stdint.h>
 
+#include "pow5_table.h"
+
 #if defined(__HIPCC__)
 #define FX_HD __host__ __device__ __forceinline__
-#define FX_HD_NOINLINE __host__ __device__ __attribute__((noinline))
 #else
 #define FX_HD inline
-#define FX_HD_NOINLINE __attribute__((noinline))
 #endif
 
 namespace fx {
@@ -215,7 +220,7 @@ FX_HD int pow2_shift(int k) {
 }
 
 // Convert a BigDec (already loaded, sign aside) to IEEE double bits.
-FX_HD_NOINLINE uint64_t bigdec_to_bits(BigDec& b) {
+FX_HD uint64_t bigdec_to_bits(BigDec& b) {
     b.trim();
     if (b.nd == 0) return 0;
     if (b.dp > 310) return 0x7FF0000000000000ULL;
@@ -254,11 +259,113 @@ FX_HD_NOINLINE uint64_t bigdec_to_bits(BigDec& b) {
 }
 
 // ---------------------------------------------------------------------------
+// Eisel-Lemire with exact tie handling
+// ---------------------------------------------------------------------------
+#if defined(__HIPCC__)
+__device__ static const uint64_t kPow5Dev[FX_POW5_QMAX - FX_POW5_QMIN + 1][2] = FX_POW5_TABLE_INIT;
+#endif
+static const uint64_t kPow5Host[FX_POW5_QMAX - FX_POW5_QMIN + 1][2] = FX_POW5_TABLE_INIT;
+
+FX_HD void pow5_entry(int32_t q, uint64_t& hi, uint64_t& lo) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    hi = kPow5Dev[q - FX_POW5_QMIN][0];
+    lo = kPow5Dev[q - FX_POW5_QMIN][1];
+#else
+    hi = kPow5Host[q - FX_POW5_QMIN][0];
+    lo = kPow5Host[q - FX_POW5_QMIN][1];
+#endif
+}
+
+FX_HD int clz64(uint64_t v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __clzll((long long)v);
+#else
+    return __builtin_clzll(v);
+#endif
+}
+
+constexpr uint64_t F64_INF_BITS = 0x7FF0000000000000ULL;
+
+struct ElOut {
+    uint64_t bits;  // positive IEEE bits
+    bool ok;        // false: undecided, use the exact big-decimal path
+};
+
+// value = v * 2^e2 for an integer v >= 1, rounded half-to-even to 53 bits.
+// The caller keeps the result in the normal range.
+FX_HD uint64_t round_dyadic(uint64_t v, int32_t e2) {
+    int nbits = 64 - clz64(v);
+    uint64_t M;
+    int32_t e = e2;
+    if (nbits <= 53) {
+        M = v << (53 - nbits);
+        e -= 53 - nbits;
+    } else {
+        int sh = nbits - 53;
+        uint64_t rem = v & ((1ULL << sh) - 1), half = 1ULL << (sh - 1);
+        M = v >> sh;
+        e += sh;
+        if (rem > half || (rem == half && (M & 1))) ++M;
+        if (M == (1ULL << 53)) { M >>= 1; ++e; }
+    }
+    return ((uint64_t)(e + 52 + 1023) << 52) | (M & ((1ULL << 52) - 1));
+}
+
+// w * 10^q (w exact, w != 0) -> double bits, or undecided.
+FX_HD ElOut eisel_lemire(uint64_t w, int32_t q) {
+    if (w == 0 || q < FX_POW5_QMIN) return ElOut{0, true};
+    if (q > FX_POW5_QMAX) return ElOut{F64_INF_BITS, true};
+    if (q < 0 && q >= -27) {
+        // w * 10^q is a dyadic rational iff 5^-q divides w: round exactly.
+        uint64_t p = 1;
+        for (int32_t i = 0; i < -q; ++i) p *= 5;
+        if (w % p == 0) return ElOut{round_dyadic(w / p, q), true};
+    }
+    const int lz = clz64(w);
+    const uint64_t wn = w << lz;
+    uint64_t th, tl;
+    pow5_entry(q, th, tl);
+    typedef unsigned __int128 u128;
+    const u128 a = (u128)wn * th, b = (u128)wn * tl;
+    const uint64_t p0 = (uint64_t)b;
+    const u128 mid = (u128)(uint64_t)a + (uint64_t)(b >> 64);
+    const uint64_t p1 = (uint64_t)mid;
+    const uint64_t p2 = (uint64_t)(a >> 64) + (uint64_t)(mid >> 64);
+    const int upper = (int)(p2 >> 63);
+    const int sh = upper + 9;
+    const uint64_t m54 = p2 >> sh;                    // 53 bits + rounding bit
+    const uint64_t rtop = p2 & ((1ULL << sh) - 1);    // discarded bits above p1
+    int32_t biased = ((q * 217706) >> 16) + 63 + upper - lz + 1023;
+    if (biased <= 0) return ElOut{0, false};          // subnormal: exact path
+    uint64_t M;
+    if (q >= 0 && q <= 55) {
+        // 5^q is exact in the table: the 192-bit product is exact.
+        if ((m54 & 1) && rtop == 0 && p1 == 0 && p0 == 0) {
+            M = m54 >> 1;
+            M += M & 1;                               // tie: to even
+        } else {
+            M = (m54 + (m54 & 1)) >> 1;
+        }
+    } else {
+        // No tie is possible here; the product (p0 dropped) is within 2
+        // units of p1's last place of the true value.
+        const bool near_low = rtop == 0 && p1 <= 2;
+        const bool near_high = rtop == ((1ULL << sh) - 1) && p1 >= ~0ULL - 2;
+        if (((m54 & 1) && near_low) || (!(m54 & 1) && near_high)) return ElOut{0, false};
+        M = (m54 + (m54 & 1)) >> 1;
+    }
+    if (M == (1ULL << 53)) { M >>= 1; ++biased; }
+    if (biased >= 2047) return ElOut{F64_INF_BITS, true};
+    return ElOut{((uint64_t)biased << 52) | (M & ((1ULL << 52) - 1)), true};
+}
+
+// ---------------------------------------------------------------------------
 // f64 scan + fast paths
 // ---------------------------------------------------------------------------
 struct F64Result {
     bool ok;
     bool fixed6;
+    bool deferred;      // valid cell whose value needs the exact slow path
     int64_t mantissa;   // signed 1e-6 units when fixed6
     double value;
 };
@@ -269,10 +376,10 @@ FX_HD double pow10_exact(int e) {
     return t[e];
 }
 
-// Slow path: reload the digits of an already-validated numeric cell.
+// Slow path: reload the digits of an already-validated numeric cell into
+// `b` and convert exactly. Returns positive bits (sign applied by caller).
 template <class St>
-FX_HD_NOINLINE uint64_t slow_f64_bits(St st) {
-    BigDec b;
+FX_HD uint64_t slow_f64_bits_into(BigDec& b, St st) {
     b.nd = 0;
     b.dp = 0;
     b.trunc = false;
@@ -318,10 +425,11 @@ FX_HD_NOINLINE uint64_t slow_f64_bits(St st) {
     return bigdec_to_bits(b);
 }
 
+// Scan + fast paths. A cell the fast paths cannot decide comes back with
+// ok && deferred (its value unset); see parse_f64_exact.
 template <class St>
 FX_HD F64Result parse_f64_stream(St st) {
-    F64Result r{false, false, 0, 0.0};
-    St start = st;
+    F64Result r{false, false, false, 0, 0.0};
     int c = st.next();
     bool neg = false, plus = false;
     if (c == '-' || c == '+') { neg = c == '-'; plus = c == '+'; c = st.next(); }
@@ -337,8 +445,8 @@ FX_HD F64Result parse_f64_stream(St st) {
         if (c >= 0) return r;
         const uint64_t INF = 0x666E69ULL, NAN_ = 0x6E616EULL, INFINITY_ = 0x7974696E69666E69ULL;
         uint64_t bits;
-        if (n == 3 && w == INF) bits = 0x7FF0000000000000ULL;
-        else if (n == 8 && w == INFINITY_) bits = 0x7FF0000000000000ULL;
+        if (n == 3 && w == INF) bits = F64_INF_BITS;
+        else if (n == 8 && w == INFINITY_) bits = F64_INF_BITS;
         else if (n == 3 && w == NAN_) bits = 0x7FF8000000000000ULL;
         else return r;
         r.ok = true;
@@ -349,7 +457,7 @@ FX_HD F64Result parse_f64_stream(St st) {
     int sig = 0;            // significant digits folded into mant
     int int_digits = 0, frac_digits = 0;
     int32_t exp10 = 0;
-    bool dropped = false;   // significant digits beyond 19 were dropped
+    bool dropped = false;   // nonzero significant digits beyond 19 were dropped
     bool dot = false;
     while (is_digit(c)) {
         ++int_digits;
@@ -402,8 +510,8 @@ FX_HD F64Result parse_f64_stream(St st) {
         r.mantissa = neg ? -(int64_t)mant : (int64_t)mant;
         return r;
     }
-    uint64_t sign = (uint64_t)neg << 63;
-    if (mant == 0 && !dropped) {
+    const uint64_t sign = (uint64_t)neg << 63;
+    if (mant == 0) {
         r.value = bits_to_f64(sign);
         return r;
     }
@@ -413,7 +521,17 @@ FX_HD F64Result parse_f64_stream(St st) {
         r.value = bits_to_f64(f64_to_bits(v) | sign);
         return r;
     }
-    r.value = bits_to_f64(slow_f64_bits(start) | sign);
+    ElOut e = eisel_lemire(mant, exp10);
+    if (e.ok && dropped) {
+        // The true value lies strictly between mant and mant + 1 (times 10^q).
+        ElOut e2 = eisel_lemire(mant + 1, exp10);
+        if (!e2.ok || e2.bits != e.bits) e.ok = false;
+    }
+    if (!e.ok) {
+        r.deferred = true;
+        return r;
+    }
+    r.value = bits_to_f64(e.bits | sign);
     return r;
 }
 
@@ -431,19 +549,41 @@ FX_HD bool quoted_both(const Src& s, uint32_t from, uint32_t to, uint8_t q) {
     return to - from >= 2 && s.at(from) == q && s.at(to - 1) == q;
 }
 
-template <class Src>
-FX_HD F64Result parse_f64_field(const Src& s, uint32_t from, uint32_t to, CellDialect d) {
+// Calls fn(stream) with the stream that cleans the cell as the dialect
+// says; numeric cells are cleaned only when they start with the quote byte.
+template <class Src, class Fn>
+FX_HD auto with_cell_stream(const Src& s, uint32_t from, uint32_t to, CellDialect d, Fn&& fn) {
     if (d.quote >= 0 && to > from && s.at(from) == (uint8_t)d.quote) {
         uint8_t q = (uint8_t)d.quote;
         bool both = quoted_both(s, from, to, q);
         if (d.escape < 0) {
-            if (both) return parse_f64_stream(DoubledStream<Src>{&s, from + 1, to - 1, q});
-            return parse_f64_stream(RawStream<Src>{&s, from, to});
+            if (both) return fn(DoubledStream<Src>{&s, from + 1, to - 1, q});
+            return fn(RawStream<Src>{&s, from, to});
         }
-        if (both) return parse_f64_stream(BackslashStream<Src>{&s, from + 1, to - 1, (uint8_t)d.escape});
-        return parse_f64_stream(BackslashStream<Src>{&s, from, to, (uint8_t)d.escape});
+        if (both) return fn(BackslashStream<Src>{&s, from + 1, to - 1, (uint8_t)d.escape});
+        return fn(BackslashStream<Src>{&s, from, to, (uint8_t)d.escape});
     }
-    return parse_f64_stream(RawStream<Src>{&s, from, to});
+    return fn(RawStream<Src>{&s, from, to});
+}
+
+// Fast paths only: a result with `deferred` set is a valid cell whose value
+// comes from f64_field_exact_bits.
+template <class Src>
+FX_HD F64Result parse_f64_field(const Src& s, uint32_t from, uint32_t to, CellDialect d) {
+    return with_cell_stream(s, from, to, d, [](auto st) { return parse_f64_stream(st); });
+}
+
+// Signed IEEE bits of a valid numeric stream by the exact big-decimal path.
+template <class St>
+FX_HD uint64_t exact_f64_bits(St st, BigDec& b) {
+    St first = st;
+    const uint64_t sign = (uint64_t)(first.next() == '-') << 63;
+    return slow_f64_bits_into(b, st) | sign;
+}
+
+template <class Src>
+FX_HD uint64_t f64_field_exact_bits(const Src& s, uint32_t from, uint32_t to, CellDialect d, BigDec& b) {
+    return with_cell_stream(s, from, to, d, [&](auto st) { return exact_f64_bits(st, b); });
 }
 
 template <class Src>
@@ -499,6 +639,17 @@ FX_HD uint32_t field_byte_len(const Src& s, uint32_t from, uint32_t to, int32_t 
     uint32_t n = 0;
     while (st.next() >= 0) ++n;
     return n;
+}
+
+// Complete parse with deferred cells resolved in `b` (host sweeps, fixups).
+template <class Src>
+FX_HD F64Result parse_f64_field_exact(const Src& s, uint32_t from, uint32_t to, CellDialect d, BigDec& b) {
+    F64Result r = parse_f64_field(s, from, to, d);
+    if (r.ok && r.deferred) {
+        r.value = bits_to_f64(f64_field_exact_bits(s, from, to, d, b));
+        r.deferred = false;
+    }
+    return r;
 }
 
 }  // namespace fx

@@ -3,6 +3,7 @@
 // (look-back state, per-tile partials, result slots).
 #include "scan.hip"
 
+#include <algorithm>
 #include <mutex>
 #include <new>
 
@@ -23,42 +24,48 @@ struct falx_ctx {
     void* d_part = nullptr;
     size_t part_cap = 0;
     void* d_red = nullptr;       // reduction outputs (StatPart[32] + extras)
+    bool timed = false;
+    uint64_t err_flags = 0, err_tile = 0;  // kernel-side detail of the last E_DEVICE
 };
 
 namespace {
 
-constexpr size_t RED_BYTES = 64 * 1024;
+constexpr size_t RED_BYTES = 4 << 20;   // reduction scratch: level-1 partials, then results
+constexpr size_t RED_OUT = 2 << 20;     // offset of the level-2 results inside it
 
 inline falx_status hip_status(hipError_t e) { return e == hipSuccess ? FALX_OK : FALX_E_DEVICE; }
+
+// HIP error of the last failing runtime call on this thread.
+thread_local int32_t tl_hip_error = 0;
 
 #define HIP_TRY(x)                                     \
     do {                                               \
         hipError_t _e = (x);                           \
-        if (_e != hipSuccess) return FALX_E_DEVICE;    \
+        if (_e != hipSuccess) {                        \
+            tl_hip_error = (int32_t)_e;                \
+            return FALX_E_DEVICE;                      \
+        }                                              \
     } while (0)
 
 inline size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 
 struct LbLayout {
-    size_t lb1, lb2_flag, lb2_agg, lb2_inc, lb3_flag, lb3_agg, lb3_inc, total;
+    size_t lb1, lb2, lb3, total;
 };
 
-LbLayout lb_layout(size_t n) {
+// Look-back words per tile: LB1 1, LB2 LB2_STRIDE, LB3 2 per string column.
+LbLayout lb_layout(size_t n, uint32_t n_str) {
     LbLayout L;
     size_t o = 0;
     L.lb1 = o; o = align_up(o + 8 * n, 256);
-    L.lb2_flag = o; o = align_up(o + 4 * n, 256);
-    L.lb2_agg = o; o = align_up(o + sizeof(Counts) * n, 256);
-    L.lb2_inc = o; o = align_up(o + sizeof(Counts) * n, 256);
-    L.lb3_flag = o; o = align_up(o + 4 * n, 256);
-    L.lb3_agg = o; o = align_up(o + 4 * FALX_MAX_COLUMNS * n, 256);
-    L.lb3_inc = o; o = align_up(o + 4 * FALX_MAX_COLUMNS * n, 256);
+    L.lb2 = o; o = align_up(o + 8 * LB2_STRIDE * n, 256);
+    L.lb3 = o; o = align_up(o + 16 * (size_t)n_str * n, 256);
     L.total = o;
     return L;
 }
 
-falx_status ensure_lb(falx_ctx* c, uint32_t n_tiles) {
-    LbLayout L = lb_layout(n_tiles);
+falx_status ensure_lb(falx_ctx* c, uint32_t n_tiles, uint32_t n_str) {
+    LbLayout L = lb_layout(n_tiles, n_str);
     if (L.total <= c->lb_cap) return FALX_OK;
     if (c->d_lb) HIP_TRY(hipFree(c->d_lb));
     c->d_lb = nullptr;
@@ -80,6 +87,23 @@ falx_status ensure_part(falx_ctx* c, size_t bytes) {
     HIP_TRY(hipMalloc(&c->d_part, cap));
     c->part_cap = cap;
     return FALX_OK;
+}
+
+// Tags of look-back words: one epoch per launch. Before the counter would
+// reach tags still present in the buffer, every word is cleared (a fill
+// kernel, so the step is also valid inside a captured stream).
+hipError_t next_epoch(falx_ctx* c, uint32_t& out) {
+    if (++c->epoch >= 0x3FFFFFFFu) {
+        const uint64_t n = c->lb_cap / 8;
+        const uint64_t blocks = std::min<uint64_t>((n + WG - 1) / WG, 65536);
+        if (n) hipLaunchKernelGGL(k_fill_u64, dim3((uint32_t)blocks), dim3(WG), 0, c->stream,
+                                  static_cast<unsigned long long*>(c->d_lb), n, 0ULL);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+        c->epoch = 1;
+    }
+    out = c->epoch;
+    return hipSuccess;
 }
 
 int dialect_kind(const falx_spec* s) {
@@ -143,22 +167,14 @@ falx_status prepare(falx_ctx* c, const falx_spec* spec, const uint8_t* d, uint64
         }
         P.n_str = ns;
     }
-    falx_status st = ensure_lb(c, P.n_tiles);
+    falx_status st = ensure_lb(c, P.n_tiles, P.n_str);
     if (st != FALX_OK) return st;
-    LbLayout L = lb_layout(P.n_tiles);
+    LbLayout L = lb_layout(P.n_tiles, P.n_str);
     char* base = static_cast<char*>(c->d_lb);
     P.lb1 = reinterpret_cast<unsigned long long*>(base + L.lb1);
-    P.lb2_flag = reinterpret_cast<uint32_t*>(base + L.lb2_flag);
-    P.lb2_agg = reinterpret_cast<Counts*>(base + L.lb2_agg);
-    P.lb2_inc = reinterpret_cast<Counts*>(base + L.lb2_inc);
-    P.lb3_flag = reinterpret_cast<uint32_t*>(base + L.lb3_flag);
-    P.lb3_agg = reinterpret_cast<uint32_t*>(base + L.lb3_agg);
-    P.lb3_inc = reinterpret_cast<uint32_t*>(base + L.lb3_inc);
-    if (++c->epoch >= 0x3FFFFFFFu) {
-        HIP_TRY(hipMemsetAsync(c->d_lb, 0, c->lb_cap, c->stream));
-        c->epoch = 1;
-    }
-    P.epoch = c->epoch;
+    P.lb2 = reinterpret_cast<unsigned long long*>(base + L.lb2);
+    P.lb3 = reinterpret_cast<unsigned long long*>(base + L.lb3);
+    HIP_TRY(next_epoch(c, P.epoch));
     P.ticket = c->d_ticket;
     P.ticket_base = c->ticket_next;
     c->ticket_next += P.n_tiles;
@@ -175,22 +191,63 @@ hipError_t fill_u64(hipStream_t s, unsigned long long* p, uint64_t n, unsigned l
 }
 
 falx_status begin(falx_ctx* c) {
+    tl_hip_error = 0;
+    c->err_flags = c->err_tile = 0;
+    (void)hipGetLastError();  // launches below report their own errors only
     HIP_TRY(hipSetDevice(c->device));
     HIP_TRY(fill_u64(c->stream, c->d_result, R_SLOTS, 0ULL));
-    HIP_TRY(hipEventRecord(c->ev0, c->stream));
+    c->timed = false;
     return FALX_OK;
 }
 
-// Record the end event, bring the result slots home, and synchronise.
+// Events bracket the call's main kernel(s) only (fills and the final
+// reductions excluded); falx_ctx_last_kernel_ms reports that span.
+hipError_t mark_start(falx_ctx* c) { return hipEventRecord(c->ev0, c->stream); }
+
+hipError_t mark_end(falx_ctx* c) {
+    c->timed = true;
+    return hipEventRecord(c->ev1, c->stream);
+}
+
+// Bring the result slots home and synchronise.
 falx_status finish(falx_ctx* c) {
-    HIP_TRY(hipEventRecord(c->ev1, c->stream));
+    if (!c->timed) {
+        HIP_TRY(hipEventRecord(c->ev0, c->stream));
+        HIP_TRY(hipEventRecord(c->ev1, c->stream));
+    }
     HIP_TRY(hipMemcpyAsync(c->h_result, c->d_result, R_SLOTS * 8, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
     float ms = 0;
     HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
     c->last_ms = ms;
-    if (c->h_result[R_ERROR]) return FALX_E_DEVICE;
+    if (c->h_result[R_ERROR]) {
+        c->err_flags = c->h_result[R_ERROR];
+        c->err_tile = c->h_result[R_ERR_TILE];
+        return FALX_E_DEVICE;
+    }
     return FALX_OK;
+}
+
+// Two-level fixed-order reductions into c->d_red + RED_OUT.
+hipError_t reduce_u64_2level(falx_ctx* c, const unsigned long long* parts, uint32_t n, uint32_t width,
+                             size_t out_off = 0) {
+    unsigned long long* tmp = static_cast<unsigned long long*>(c->d_red);
+    unsigned long long* out = reinterpret_cast<unsigned long long*>(static_cast<char*>(c->d_red) + RED_OUT + out_off);
+    uint32_t blocks = n < RED_BLOCKS * 4 ? 1u : RED_BLOCKS;
+    hipLaunchKernelGGL(k_reduce_u64, dim3(blocks), dim3(WG), 0, c->stream, parts, n, width, blocks == 1 ? out : tmp);
+    if (blocks > 1) hipLaunchKernelGGL(k_reduce_u64, dim3(1), dim3(WG), 0, c->stream, tmp, blocks, width, out);
+    return hipGetLastError();
+}
+
+hipError_t reduce_stats_2level(falx_ctx* c, const StatPart* parts, uint32_t n, uint32_t n_cols) {
+    StatPart* tmp = static_cast<StatPart*>(c->d_red);
+    StatPart* out = reinterpret_cast<StatPart*>(static_cast<char*>(c->d_red) + RED_OUT);
+    uint32_t blocks = n < RED_BLOCKS * 4 ? 1u : RED_BLOCKS;
+    hipLaunchKernelGGL(k_reduce_stats, dim3(n_cols, blocks), dim3(WG), 0, c->stream, parts, n, n_cols,
+                       blocks == 1 ? out : tmp);
+    if (blocks > 1)
+        hipLaunchKernelGGL(k_reduce_stats, dim3(n_cols, 1), dim3(WG), 0, c->stream, tmp, blocks, n_cols, out);
+    return hipGetLastError();
 }
 
 }  // namespace
@@ -241,6 +298,14 @@ void falx_ctx_destroy(falx_ctx* c) {
 void* falx_ctx_stream(falx_ctx* c) { return c ? (void*)c->stream : nullptr; }
 
 double falx_ctx_last_kernel_ms(falx_ctx* c) { return c ? (double)c->last_ms : 0.0; }
+
+falx_status falx_ctx_last_error(const falx_ctx* c, int32_t* hip_error, uint64_t* device_flags, uint64_t* tile) {
+    if (!c) return FALX_E_INVALID;
+    if (hip_error) *hip_error = tl_hip_error;
+    if (device_flags) *device_flags = c->err_flags;
+    if (tile) *tile = c->err_tile;
+    return FALX_OK;
+}
 
 falx_status falx_device_alloc(falx_ctx* c, uint64_t bytes, void** d_out) {
     if (!c || !d_out) return FALX_E_INVALID;
@@ -314,7 +379,9 @@ falx_status falx_index_structurals(falx_ctx* c, const falx_spec* spec, const uin
     if ((st = prepare(c, spec, d, len, P)) != FALX_OK) return st;
     P.out = d_out;
     P.out_cap = d_out ? out_cap : 0;
+    HIP_TRY(mark_start(c));
     HIP_TRY(launch_scan<M_INDEX>(P, dialect_kind(spec), c->stream));
+    HIP_TRY(mark_end(c));
     if ((st = finish(c)) != FALX_OK) return st;
     *n_out = c->h_result[R_N_STRUCT];
     return *n_out > P.out_cap ? FALX_E_CAPACITY : FALX_OK;
@@ -335,15 +402,19 @@ falx_status falx_parse(falx_ctx* c, const falx_spec* spec, const uint8_t* d, uin
     P.out_cap = d_seps ? seps_cap : 0;
     P.ends = reinterpret_cast<unsigned long long*>(d_ends);
     P.ends_cap = d_ends ? ends_cap : 0;
+    HIP_TRY(mark_start(c));
     HIP_TRY(launch_scan<M_TAPE>(P, dialect_kind(spec), c->stream));
+    HIP_TRY(mark_end(c));
     if ((st = finish(c)) != FALX_OK) return st;
     *n_seps = c->h_result[R_N_SEPS];
     *n_ends = c->h_result[R_N_TERMS];
     return (*n_seps > P.out_cap || *n_ends > P.ends_cap) ? FALX_E_CAPACITY : FALX_OK;
 }
 
-falx_status falx_parse_columns(falx_ctx* c, const falx_spec* spec, const uint8_t* d, uint64_t len,
-                               falx_columns* out) {
+static falx_status reduce_stats(falx_ctx* c, const falx_spec* spec, uint32_t n_tiles, falx_csv_stats* out);
+
+static falx_status columns_impl(falx_ctx* c, const falx_spec* spec, const uint8_t* d, uint64_t len,
+                                falx_columns* out, falx_csv_stats* stats) {
     falx_status st = check_input(c, d, len);
     if (st != FALX_OK) return st;
     if ((st = falx_spec_validate(spec)) != FALX_OK) return st;
@@ -354,6 +425,7 @@ falx_status falx_parse_columns(falx_ctx* c, const falx_spec* spec, const uint8_t
         if (spec->columns[k].type == FALX_COL_STR && !b.data && b.data_cap) return FALX_E_INVALID;
     }
     out->rows = 0;
+    if (stats) memset(stats, 0, sizeof *stats);
     Params P;
     if ((st = begin(c)) != FALX_OK) return st;
     // Validity words are OR-ed by the kernel: clear them first; string
@@ -377,8 +449,31 @@ falx_status falx_parse_columns(falx_ctx* c, const falx_spec* spec, const uint8_t
         P.cols[k].data = out->cols[k].data;
         P.cols[k].data_cap = out->cols[k].data_cap;
     }
+    if (stats) {
+        uint32_t nc = spec->n_columns ? spec->n_columns : 1;
+        size_t part_bytes = (size_t)P.n_tiles * nc * sizeof(StatPart) + (size_t)P.n_tiles * 8;
+        if ((st = ensure_part(c, part_bytes)) != FALX_OK) return st;
+        P.partials = c->d_part;
+    }
+    HIP_TRY(mark_start(c));
     HIP_TRY(launch_scan<M_COLUMNS>(P, dialect_kind(spec), c->stream));
-    if ((st = finish(c)) != FALX_OK) return st;
+    // Statistics of columns past the first group come from extra passes.
+    if (stats) {
+        Params Q = P;
+        for (uint32_t g = STAT_GROUP; g < spec->n_columns; g += STAT_GROUP) {
+            HIP_TRY(next_epoch(c, Q.epoch));
+            Q.ticket_base = c->ticket_next;
+            c->ticket_next += Q.n_tiles;
+            Q.col_group = g;
+            HIP_TRY(launch_scan<M_STATS>(Q, dialect_kind(spec), c->stream));
+        }
+    }
+    HIP_TRY(mark_end(c));
+    if (stats) {
+        if ((st = reduce_stats(c, spec, P.n_tiles, stats)) != FALX_OK) return st;
+    } else if ((st = finish(c)) != FALX_OK) {
+        return st;
+    }
     out->rows = c->h_result[R_N_ROWS];
     bool cap = out->rows > out->rows_cap;
     for (uint32_t k = 0; k < spec->n_columns; ++k) {
@@ -389,6 +484,45 @@ falx_status falx_parse_columns(falx_ctx* c, const falx_spec* spec, const uint8_t
         if (dl > 0x7FFFFFFFULL) return FALX_E_TOO_LARGE;
     }
     return cap ? FALX_E_CAPACITY : FALX_OK;
+}
+
+// Reduce the per-tile StatParts + record counts in c->d_part, finish the
+// call and fill `out`.
+static falx_status reduce_stats(falx_ctx* c, const falx_spec* spec, uint32_t n_tiles, falx_csv_stats* out) {
+    const StatPart* parts = static_cast<const StatPart*>(c->d_part);
+    if (spec->n_columns) HIP_TRY(reduce_stats_2level(c, parts, n_tiles, spec->n_columns));
+    // record counts: a u64 per tile after the StatParts, summed the same way
+    const size_t rec_off = 64 * 1024;
+    HIP_TRY(reduce_u64_2level(c, reinterpret_cast<const unsigned long long*>(parts + (size_t)n_tiles * spec->n_columns),
+                              n_tiles, 1u, rec_off));
+    falx_status st = finish(c);
+    if (st != FALX_OK) return st;
+    StatPart host[FALX_MAX_COLUMNS];
+    const char* red = static_cast<const char*>(c->d_red) + RED_OUT;
+    if (spec->n_columns) HIP_TRY(hipMemcpy(host, red, sizeof(StatPart) * spec->n_columns, hipMemcpyDeviceToHost));
+    unsigned long long records = 0;
+    HIP_TRY(hipMemcpy(&records, red + rec_off, 8, hipMemcpyDeviceToHost));
+    out->records = records;
+    for (uint32_t k = 0; k < spec->n_columns; ++k) {
+        out->cols[k].values = host[k].values;
+        out->cols[k].checksum = host[k].checksum;
+        out->cols[k].bytes = host[k].bytes;
+        out->cols[k].fixed6_sum = host[k].fixed6_sum;
+        out->cols[k].fixed6_cells = host[k].fixed6_cells;
+        out->cols[k].sum = host[k].sum;
+    }
+    return FALX_OK;
+}
+
+falx_status falx_parse_columns(falx_ctx* c, const falx_spec* spec, const uint8_t* d, uint64_t len,
+                               falx_columns* out) {
+    return columns_impl(c, spec, d, len, out, nullptr);
+}
+
+falx_status falx_parse_columns_with_stats(falx_ctx* c, const falx_spec* spec, const uint8_t* d, uint64_t len,
+                                          falx_columns* out, falx_csv_stats* stats) {
+    if (!stats) return FALX_E_INVALID;
+    return columns_impl(c, spec, d, len, out, stats);
 }
 
 falx_status falx_parse_csv_stats(falx_ctx* c, const falx_spec* spec, const uint8_t* d, uint64_t len,
@@ -408,39 +542,18 @@ falx_status falx_parse_csv_stats(falx_ctx* c, const falx_spec* spec, const uint8
     P.partials = c->d_part;
     P.n_cols = spec->n_columns;
     uint32_t groups = spec->n_columns ? (spec->n_columns + STAT_GROUP - 1) / STAT_GROUP : 1;
+    HIP_TRY(mark_start(c));
     for (uint32_t g = 0; g < groups; ++g) {
         P.col_group = g * STAT_GROUP;
         if (g > 0) {  // each group is a full pass: fresh tickets and epoch
-            if (++c->epoch >= 0x3FFFFFFFu) c->epoch = 1;
-            P.epoch = c->epoch;
+            HIP_TRY(next_epoch(c, P.epoch));
             P.ticket_base = c->ticket_next;
             c->ticket_next += P.n_tiles;
         }
         HIP_TRY(launch_scan<M_STATS>(P, dialect_kind(spec), c->stream));
     }
-    StatPart* red = static_cast<StatPart*>(c->d_red);
-    unsigned long long* rec = reinterpret_cast<unsigned long long*>(red + FALX_MAX_COLUMNS);
-    if (spec->n_columns)
-        hipLaunchKernelGGL(k_reduce_stats, dim3(spec->n_columns), dim3(WG), 0, c->stream,
-                           static_cast<const StatPart*>(c->d_part), P.n_tiles, spec->n_columns, red);
-    hipLaunchKernelGGL(k_reduce_u64, dim3(1), dim3(WG), 0, c->stream,
-                       reinterpret_cast<const unsigned long long*>(static_cast<StatPart*>(c->d_part) +
-                                                                   (size_t)P.n_tiles * spec->n_columns),
-                       P.n_tiles, 1u, rec);
-    HIP_TRY(hipGetLastError());
-    StatPart host[FALX_MAX_COLUMNS + 1];
-    if ((st = finish(c)) != FALX_OK) return st;
-    HIP_TRY(hipMemcpy(host, red, sizeof host, hipMemcpyDeviceToHost));
-    out->records = reinterpret_cast<unsigned long long*>(&host[FALX_MAX_COLUMNS])[0];
-    for (uint32_t k = 0; k < spec->n_columns; ++k) {
-        out->cols[k].values = host[k].values;
-        out->cols[k].checksum = host[k].checksum;
-        out->cols[k].bytes = host[k].bytes;
-        out->cols[k].fixed6_sum = host[k].fixed6_sum;
-        out->cols[k].fixed6_cells = host[k].fixed6_cells;
-        out->cols[k].sum = host[k].sum;
-    }
-    return FALX_OK;
+    HIP_TRY(mark_end(c));
+    return reduce_stats(c, spec, P.n_tiles, out);
 }
 
 falx_status falx_parse_field_bytes(falx_ctx* c, const falx_spec* spec, const uint8_t* d, uint64_t len,
@@ -456,11 +569,11 @@ falx_status falx_parse_field_bytes(falx_ctx* c, const falx_spec* spec, const uin
     if ((st = prepare(c, spec, d, len, P)) != FALX_OK) return st;
     if ((st = ensure_part(c, (size_t)P.n_tiles * 16)) != FALX_OK) return st;
     P.partials = c->d_part;
+    HIP_TRY(mark_start(c));
     HIP_TRY(launch_scan<M_FIELDBYTES>(P, dialect_kind(spec), c->stream));
-    unsigned long long* red = static_cast<unsigned long long*>(c->d_red);
-    hipLaunchKernelGGL(k_reduce_u64, dim3(1), dim3(WG), 0, c->stream,
-                       static_cast<const unsigned long long*>(c->d_part), P.n_tiles, 2u, red);
-    HIP_TRY(hipGetLastError());
+    HIP_TRY(mark_end(c));
+    HIP_TRY(reduce_u64_2level(c, static_cast<const unsigned long long*>(c->d_part), P.n_tiles, 2u));
+    const char* red = static_cast<const char*>(c->d_red) + RED_OUT;
     if ((st = finish(c)) != FALX_OK) return st;
     unsigned long long h[2];
     HIP_TRY(hipMemcpy(h, red, 16, hipMemcpyDeviceToHost));
@@ -492,11 +605,11 @@ falx_status falx_parse_ndjson_lines(falx_ctx* c, const uint8_t* d, uint64_t len,
     if ((st = prepare(c, &spec, d, len, P)) != FALX_OK) return st;
     if ((st = ensure_part(c, (size_t)P.n_tiles * 8)) != FALX_OK) return st;
     P.partials = c->d_part;
+    HIP_TRY(mark_start(c));
     HIP_TRY(launch_scan_fixed3<M_TERMS>(P, c->stream));
-    unsigned long long* red = static_cast<unsigned long long*>(c->d_red);
-    hipLaunchKernelGGL(k_reduce_u64, dim3(1), dim3(WG), 0, c->stream,
-                       static_cast<const unsigned long long*>(c->d_part), P.n_tiles, 1u, red);
-    HIP_TRY(hipGetLastError());
+    HIP_TRY(mark_end(c));
+    HIP_TRY(reduce_u64_2level(c, static_cast<const unsigned long long*>(c->d_part), P.n_tiles, 1u));
+    const char* red = static_cast<const char*>(c->d_red) + RED_OUT;
     if ((st = finish(c)) != FALX_OK) return st;
     unsigned long long h = 0;
     uint8_t last = 0;
@@ -516,12 +629,20 @@ falx_status falx_parse_ndjson_id_score(falx_ctx* c, const uint8_t* d, uint64_t l
     if ((st = begin(c)) != FALX_OK) return st;
     uint32_t n_tiles = (uint32_t)((len + TILE - 1) / TILE);
     if ((st = ensure_part(c, (size_t)n_tiles * sizeof(NdPart))) != FALX_OK) return st;
+    HIP_TRY(mark_start(c));
     hipLaunchKernelGGL(k_ndjson_id_score, dim3(n_tiles), dim3(WG), 0, c->stream, d, (uint32_t)len, n_tiles,
                        static_cast<NdPart*>(c->d_part));
-    NdPart* red = static_cast<NdPart*>(c->d_red);
-    hipLaunchKernelGGL(k_reduce_nd, dim3(1), dim3(WG), 0, c->stream, static_cast<const NdPart*>(c->d_part),
-                       n_tiles, red);
-    HIP_TRY(hipGetLastError());
+    HIP_TRY(mark_end(c));
+    {
+        NdPart* tmp = static_cast<NdPart*>(c->d_red);
+        NdPart* fin = reinterpret_cast<NdPart*>(static_cast<char*>(c->d_red) + RED_OUT);
+        uint32_t blocks = n_tiles < RED_BLOCKS * 4 ? 1u : RED_BLOCKS;
+        hipLaunchKernelGGL(k_reduce_nd, dim3(blocks), dim3(WG), 0, c->stream, static_cast<const NdPart*>(c->d_part),
+                           n_tiles, blocks == 1 ? fin : tmp);
+        if (blocks > 1) hipLaunchKernelGGL(k_reduce_nd, dim3(1), dim3(WG), 0, c->stream, tmp, blocks, fin);
+        HIP_TRY(hipGetLastError());
+    }
+    const NdPart* red = reinterpret_cast<const NdPart*>(static_cast<const char*>(c->d_red) + RED_OUT);
     if ((st = finish(c)) != FALX_OK) return st;
     NdPart h;
     HIP_TRY(hipMemcpy(&h, red, sizeof h, hipMemcpyDeviceToHost));
@@ -546,11 +667,11 @@ falx_status falx_parse_logfmt_pairs(falx_ctx* c, const uint8_t* d, uint64_t len,
     if ((st = prepare(c, &spec, d, len, P)) != FALX_OK) return st;
     if ((st = ensure_part(c, (size_t)P.n_tiles * 32)) != FALX_OK) return st;
     P.partials = c->d_part;
+    HIP_TRY(mark_start(c));
     HIP_TRY(launch_scan_fixed3<M_LOGFMT>(P, c->stream));
-    unsigned long long* red = static_cast<unsigned long long*>(c->d_red);
-    hipLaunchKernelGGL(k_reduce_u64, dim3(1), dim3(WG), 0, c->stream,
-                       static_cast<const unsigned long long*>(c->d_part), P.n_tiles, 4u, red);
-    HIP_TRY(hipGetLastError());
+    HIP_TRY(mark_end(c));
+    HIP_TRY(reduce_u64_2level(c, static_cast<const unsigned long long*>(c->d_part), P.n_tiles, 4u));
+    const char* red = static_cast<const char*>(c->d_red) + RED_OUT;
     if ((st = finish(c)) != FALX_OK) return st;
     unsigned long long h[4];
     HIP_TRY(hipMemcpy(h, red, 32, hipMemcpyDeviceToHost));

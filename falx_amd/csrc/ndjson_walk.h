@@ -103,7 +103,8 @@ FX_HD bool j_int(JCur<Src>& c, int64_t& v) {
 // plus the exact 1e-6 mantissa when it has no exponent, <= 6 decimals and
 // <= 12 integer digits.
 template <class Src>
-FX_HD bool j_number(JCur<Src>& c, double& v, bool& fx6, int64_t& mant) {
+FX_HD bool j_number(JCur<Src>& c, double& v, bool& fx6, int64_t& mant, bool& deferred, uint32_t& from,
+                    uint32_t& to) {
     uint32_t s0 = c.i;
     bool neg = false;
     if (c.peek() == '-') { neg = true; ++c.i; }
@@ -134,6 +135,9 @@ FX_HD bool j_number(JCur<Src>& c, double& v, bool& fx6, int64_t& mant) {
     F64Result r = parse_f64_stream(RawStream<Src>{c.s, s0, c.i});
     if (!r.ok) return false;
     v = r.value;
+    deferred = r.deferred;
+    from = s0;
+    to = c.i;
     fx6 = false;
     if (!has_exp && frac_digits <= 6 && int_digits <= 12) {
         int64_t m = 0;
@@ -150,6 +154,8 @@ struct NdRecord {
     int64_t id;
     double score;
     bool fx6;
+    bool score_deferred;          // score needs the exact path over [sfrom, sto)
+    uint32_t sfrom, sto;
     int64_t mant;
 };
 
@@ -160,6 +166,8 @@ FX_HD int ndjson_record(const Src& s, uint32_t from, uint32_t to, NdRecord& out)
     out.id = 0;
     out.score = 0;
     out.fx6 = false;
+    out.score_deferred = false;
+    out.sfrom = out.sto = 0;
     out.mant = 0;
     c.ws();
     if (c.peek() != '{') return 0;
@@ -193,7 +201,7 @@ FX_HD int ndjson_record(const Src& s, uint32_t from, uint32_t to, NdRecord& out)
                     ++c.i;
                     c.ws();
                     if (!have_score && key_eq(c, ks2, ke2, "score", 5)) {
-                        if (!j_number(c, out.score, out.fx6, out.mant)) return 0;
+                        if (!j_number(c, out.score, out.fx6, out.mant, out.score_deferred, out.sfrom, out.sto)) return 0;
                         have_score = true;
                     } else if (!j_skip_value(c)) {
                         return 0;
@@ -213,6 +221,14 @@ FX_HD int ndjson_record(const Src& s, uint32_t from, uint32_t to, NdRecord& out)
         return 0;
     }
     return (have_id && have_score) ? 1 : 0;
+}
+
+// Resolves a deferred score with the exact path (storage in `b`).
+template <class Src>
+FX_HD void ndjson_resolve(const Src& s, NdRecord& r, BigDec& b) {
+    if (!r.score_deferred) return;
+    r.score = bits_to_f64(exact_f64_bits(RawStream<Src>{&s, r.sfrom, r.sto}, b));
+    r.score_deferred = false;
 }
 
 template <class Src>

@@ -68,6 +68,7 @@ struct StatPart {
 // result[] slots written by the last tile / reduce kernels
 enum : int {
     R_N_STRUCT = 0, R_N_SEPS, R_N_TERMS, R_N_ROWS, R_ERROR, R_RECORDS, R_F0, R_F1, R_F2, R_F3,
+    R_ERR_TILE,       // first tile whose look-back gave up (+1)
     R_STR_LEN = 16,   // + str_slot (32)
     R_SLOTS = 64
 };
@@ -86,12 +87,8 @@ struct Params {
     unsigned long long ticket_base;
     uint32_t epoch;
     unsigned long long* lb1;
-    uint32_t* lb2_flag;
-    Counts* lb2_agg;
-    Counts* lb2_inc;
-    uint32_t* lb3_flag;
-    uint32_t* lb3_agg;
-    uint32_t* lb3_inc;
+    unsigned long long* lb2;  // per tile LB2_STRIDE tagged words: aggregate at +0, inclusive at +8
+    unsigned long long* lb3;  // per tile 2 * n_str tagged words: aggregate, then inclusive
     uint32_t* out;
     uint64_t out_cap;
     unsigned long long* ends;
@@ -216,46 +213,236 @@ DEV unsigned long long lb1_word(uint32_t status, uint32_t epoch, uint32_t value)
     return ((unsigned long long)status << 62) | ((unsigned long long)(epoch & 0x3FFFFFFFu) << 32) | value;
 }
 
-DEV bool lb1_wait(const Params& P, uint32_t j, unsigned long long& v) {
-    for (uint32_t it = 0; it < SPIN_LIMIT; ++it) {
-        v = __hip_atomic_load(&P.lb1[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if ((v >> 62) != 0 && (uint32_t)((v >> 32) & 0x3FFFFFFFu) == (P.epoch & 0x3FFFFFFFu)) return true;
-        __builtin_amdgcn_s_sleep(1);
-    }
-    atomicOr(&P.result[R_ERROR], 1ULL);
-    return false;
+// LB2/LB3 records are words (epoch << 32 | value) written and read with
+// relaxed agent-scope atomics: each word proves by its tag that it belongs
+// to this launch, so no flag and no acquire/release fence (an L2 write-back
+// or invalidate on this chip) is needed. Words of earlier launches carry
+// other tags; the host clears all of them when the epoch counter wraps.
+constexpr int LB2_W = 7;        // n_struct n_terms n_rows ord last_struct prev_struct last_term
+constexpr int LB2_STRIDE = 16;
+
+DEV unsigned long long tag_word(uint32_t epoch, uint32_t v) { return ((unsigned long long)epoch << 32) | v; }
+
+DEV void lb_store(unsigned long long* p, unsigned long long v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-DEV uint32_t lb_flag_wait(const Params& P, uint32_t* flag, uint32_t j) {
-    for (uint32_t it = 0; it < SPIN_LIMIT; ++it) {
-        uint32_t f = __hip_atomic_load(&flag[j], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-        if ((f >> 2) == (P.epoch & 0x3FFFFFFFu) && (f & 3u)) return f & 3u;
-        __builtin_amdgcn_s_sleep(1);
-    }
-    atomicOr(&P.result[R_ERROR], 2ULL);
-    return 2;
+DEV unsigned long long lb_load(const unsigned long long* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-DEV void lb_flag_set(const Params& P, uint32_t* flag, uint32_t j, uint32_t status) {
-    __hip_atomic_store(&flag[j], ((P.epoch & 0x3FFFFFFFu) << 2) | status, __ATOMIC_RELEASE,
-                       __HIP_MEMORY_SCOPE_AGENT);
+// Publishes `c` as tile j's aggregate (which = 0) or inclusive prefix (1).
+DEV void lb2_publish(const Params& P, uint32_t j, int which, const Counts& c) {
+    unsigned long long* d = P.lb2 + (size_t)j * LB2_STRIDE + 8 * which;
+    const uint32_t e = P.epoch;
+    lb_store(d + 0, tag_word(e, c.n_struct));
+    lb_store(d + 1, tag_word(e, c.n_terms));
+    lb_store(d + 2, tag_word(e, c.n_rows));
+    lb_store(d + 3, tag_word(e, c.ord));
+    lb_store(d + 4, tag_word(e, c.last_struct));
+    lb_store(d + 5, tag_word(e, c.prev_struct));
+    lb_store(d + 6, tag_word(e, c.last_term));
 }
 
-DEV void store_counts(Counts* dst, const Counts& c) {
-    uint4* d = reinterpret_cast<uint4*>(dst);
-    d[0] = make_uint4(c.n_struct, c.n_seps, c.n_terms, c.n_rows);
-    d[1] = make_uint4(c.has_term, c.ord, c.last_struct, c.prev_struct);
-    d[2] = make_uint4(c.last_term, 0u, 0u, 0u);
-}
-
-DEV Counts load_counts(const Counts* src) {
-    const uint4* s = reinterpret_cast<const uint4*>(src);
-    uint4 a = s[0], b = s[1], c = s[2];
+DEV Counts counts_from_words(const unsigned long long* w) {
     Counts r;
-    r.n_struct = a.x; r.n_seps = a.y; r.n_terms = a.z; r.n_rows = a.w;
-    r.has_term = b.x; r.ord = b.y; r.last_struct = b.z; r.prev_struct = b.w;
-    r.last_term = c.x; r.pad0 = r.pad1 = r.pad2 = 0;
+    r.n_struct = (uint32_t)w[0];
+    r.n_terms = (uint32_t)w[1];
+    r.n_rows = (uint32_t)w[2];
+    r.ord = (uint32_t)w[3];
+    r.last_struct = (uint32_t)w[4];
+    r.prev_struct = (uint32_t)w[5];
+    r.last_term = (uint32_t)w[6];
+    r.n_seps = r.n_struct - r.n_terms;
+    r.has_term = r.last_term != 0;
+    r.pad0 = r.pad1 = r.pad2 = 0;
     return r;
+}
+
+DEV Counts shfl_counts(const Counts& c, int src_lane) {
+    Counts r;
+    r.n_struct = __shfl(c.n_struct, src_lane);
+    r.n_seps = __shfl(c.n_seps, src_lane);
+    r.n_terms = __shfl(c.n_terms, src_lane);
+    r.n_rows = __shfl(c.n_rows, src_lane);
+    r.has_term = __shfl(c.has_term, src_lane);
+    r.ord = __shfl(c.ord, src_lane);
+    r.last_struct = __shfl(c.last_struct, src_lane);
+    r.prev_struct = __shfl(c.prev_struct, src_lane);
+    r.last_term = __shfl(c.last_term, src_lane);
+    r.pad0 = r.pad1 = r.pad2 = 0;
+    return r;
+}
+
+DEV Counts shfl_down_counts(const Counts& c, int d) {
+    Counts r;
+    r.n_struct = __shfl_down(c.n_struct, d);
+    r.n_seps = __shfl_down(c.n_seps, d);
+    r.n_terms = __shfl_down(c.n_terms, d);
+    r.n_rows = __shfl_down(c.n_rows, d);
+    r.has_term = __shfl_down(c.has_term, d);
+    r.ord = __shfl_down(c.ord, d);
+    r.last_struct = __shfl_down(c.last_struct, d);
+    r.prev_struct = __shfl_down(c.prev_struct, d);
+    r.last_term = __shfl_down(c.last_term, d);
+    r.pad0 = r.pad1 = r.pad2 = 0;
+    return r;
+}
+
+// Window of the wave-parallel look-back: lane i inspects tile base - i.
+// Returns the lowest lane holding an inclusive prefix (64 = none) once every
+// lane below it holds at least an aggregate; `status` is this lane's flag.
+struct LbWindow {
+    uint32_t k;
+    bool ok;
+};
+
+DEV LbWindow lb_window(uint32_t status, bool in_range) {
+    const uint64_t inc_mask = ballot(in_range && status == 2);
+    const uint64_t ready = ballot(!in_range || status != 0);
+    const uint32_t k = inc_mask ? (uint32_t)__ffsll((long long)inc_mask) - 1 : 64u;
+    const uint64_t need = k >= 64 ? ~0ULL : ((1ULL << k) - 1);
+    return LbWindow{k, (ready & need) == need};
+}
+
+DEV uint32_t tag_of(unsigned long long w) { return (uint32_t)(w >> 32); }
+
+// A look-back whose window stays incomplete for SPIN_LIMIT polls reports its
+// kind and tile and returns, so the kernel always ends.
+DEV void lb_give_up(const Params& P, uint32_t tile, unsigned long long kind) {
+    if (__lane_id() == 0) {
+        atomicOr(&P.result[R_ERROR], kind);
+        atomicCAS(&P.result[R_ERR_TILE], 0ULL, (unsigned long long)tile + 1ULL);
+    }
+}
+
+// LB1: entry state of `tile` from the transfer functions of its
+// predecessors. Whole wave; composes 64 tiles per round trip.
+template <int NS>
+DEV uint32_t lookback_fn(const Params& P, uint32_t tile) {
+    const uint32_t ln = __lane_id();
+    const uint32_t epoch = P.epoch & 0x3FFFFFFFu;
+    uint32_t g = fn_ident<NS>();
+    int32_t base = (int32_t)tile - 1;
+    for (uint32_t spins = 0;;) {
+        const int32_t j = base - (int32_t)ln;
+        unsigned long long v = 0;
+        if (j >= 0) v = __hip_atomic_load(&P.lb1[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const uint32_t status = (uint32_t)((v >> 32) & 0x3FFFFFFFu) == epoch ? (uint32_t)(v >> 62) : 0u;
+        LbWindow W = lb_window(status, j >= 0);
+        if (!W.ok) {
+            if (++spins > SPIN_LIMIT) {
+                lb_give_up(P, tile, 1ULL);
+                return 0;
+            }
+            __builtin_amdgcn_s_sleep(2);
+            continue;
+        }
+        uint32_t f = (ln < W.k && j >= 0) ? (uint32_t)v : fn_ident<NS>();
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            uint32_t hi = __shfl_down(f, d);
+            if ((ln & (2 * d - 1)) == 0 && ln + d < 64) f = fn_then<NS>(hi, f);
+        }
+        const uint32_t T = __shfl(f, 0);
+        if (W.k < 64) return fn_apply(g, fn_apply(T, (uint32_t)__shfl((uint32_t)v, (int)W.k)));
+        g = fn_then<NS>(T, g);
+        base -= 64;
+        spins = 0;
+    }
+}
+
+// LB2: exclusive counter prefix of `tile`. Whole wave. Each lane polls only
+// the last word of its tile's two records; the window's records are read in
+// full once it is complete (and polled again if a word is not yet visible).
+DEV Counts lookback_counts(const Params& P, uint32_t tile) {
+    const uint32_t ln = __lane_id();
+    const uint32_t epoch = P.epoch;
+    Counts acc = counts_zero();
+    int32_t base = (int32_t)tile - 1;
+    for (uint32_t spins = 0;; ++spins) {
+        if (spins > SPIN_LIMIT) {
+            lb_give_up(P, tile, 2ULL);
+            return counts_zero();
+        }
+        const int32_t j = base - (int32_t)ln;
+        const unsigned long long* d = P.lb2 + (size_t)(j >= 0 ? j : 0) * LB2_STRIDE;
+        uint32_t status = 0;
+        if (j >= 0) {
+            const unsigned long long si = lb_load(d + 8 + LB2_W - 1), sa = lb_load(d + LB2_W - 1);
+            status = tag_of(si) == epoch ? 2u : (tag_of(sa) == epoch ? 1u : 0u);
+        }
+        const LbWindow W = lb_window(status, j >= 0);
+        if (!W.ok) {
+            __builtin_amdgcn_s_sleep(2);
+            continue;
+        }
+        unsigned long long wv[LB2_W];
+        bool good = true;
+        const bool take = j >= 0 && ln <= W.k;
+        if (take) {
+            const unsigned long long* r = d + (ln == W.k ? 8 : 0);
+#pragma unroll
+            for (int i = 0; i < LB2_W; ++i) wv[i] = lb_load(r + i);
+#pragma unroll
+            for (int i = 0; i < LB2_W; ++i) good &= tag_of(wv[i]) == epoch;
+        }
+        if (ballot(!good)) continue;
+        Counts c = take ? counts_from_words(wv) : counts_zero();
+#pragma unroll
+        for (int dd = 1; dd < 64; dd <<= 1) {
+            Counts hi = shfl_down_counts(c, dd);
+            if ((ln & (2 * dd - 1)) == 0 && ln + dd < 64) c = compose(hi, c);
+        }
+        acc = compose(shfl_counts(c, 0), acc);
+        if (W.k < 64) return acc;
+        base -= 64;
+        spins = 0;
+    }
+}
+
+// LB3: exclusive arena prefix (sum per string column) into base_out[0..ns),
+// which lane 0 writes (LDS). Whole wave; same polling as LB2 with the last
+// word of each record as the sentinel.
+DEV void lookback_str(const Params& P, uint32_t tile, uint32_t ns, uint32_t* base_out) {
+    const uint32_t ln = __lane_id();
+    const uint32_t epoch = P.epoch;
+    if (ln == 0)
+        for (uint32_t s = 0; s < ns; ++s) base_out[s] = 0;
+    int32_t base = (int32_t)tile - 1;
+    for (uint32_t spins = 0;; ++spins) {
+        if (spins > SPIN_LIMIT) {
+            lb_give_up(P, tile, 4ULL);
+            return;
+        }
+        const int32_t j = base - (int32_t)ln;
+        const unsigned long long* d = P.lb3 + (size_t)(j >= 0 ? j : 0) * 2 * ns;
+        uint32_t status = 0;
+        if (j >= 0) {
+            const unsigned long long si = lb_load(d + 2 * ns - 1), sa = lb_load(d + ns - 1);
+            status = tag_of(si) == epoch ? 2u : (tag_of(sa) == epoch ? 1u : 0u);
+        }
+        const LbWindow W = lb_window(status, j >= 0);
+        if (!W.ok) {
+            __builtin_amdgcn_s_sleep(2);
+            continue;
+        }
+        const bool take = j >= 0 && ln <= W.k;
+        const unsigned long long* r = d + (ln == W.k ? ns : 0);
+        bool good = true;
+        if (take)
+            for (uint32_t s = 0; s < ns; ++s) good &= tag_of(lb_load(r + s)) == epoch;
+        if (ballot(!good)) continue;
+        for (uint32_t s = 0; s < ns; ++s) {
+            uint32_t v = take ? (uint32_t)lb_load(r + s) : 0u;
+#pragma unroll
+            for (int dd = 32; dd >= 1; dd >>= 1) v += __shfl_xor(v, dd);
+            if (ln == 0) base_out[s] += v;
+        }
+        if (W.k < 64) return;
+        base -= 64;
+        spins = 0;
+    }
 }
 
 // --------------------------------------------------------------------------
@@ -436,6 +623,55 @@ DEV uint32_t block_fn(const BM& m, uint64_t ls) {
 // --------------------------------------------------------------------------
 // the tile kernel
 // --------------------------------------------------------------------------
+// Exact conversion of the f64 cells the fast paths leave undecided (see
+// cells.h): one big decimal per workgroup in LDS, used by one lane at a
+// time. Lanes of a wave take turns by ballot; waves take an LDS lock.
+struct SlowF64 {
+    BigDec big;
+    uint32_t lock;
+};
+
+DEV void slow_lock(SlowF64& s) {
+    while (atomicCAS(&s.lock, 0u, 1u) != 0u) __builtin_amdgcn_s_sleep(1);
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+DEV void slow_unlock(SlowF64& s) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    atomicExch(&s.lock, 0u);
+}
+
+// Called by exactly the lanes whose cell is deferred.
+template <class Src>
+DEV double resolve_f64(SlowF64& s, const Src& src, uint32_t from, uint32_t to, CellDialect cd) {
+    uint64_t m = ballot(true);
+    double v = 0.0;
+    while (m) {
+        const uint32_t l = (uint32_t)__ffsll((long long)m) - 1u;
+        if (__lane_id() == l) {
+            slow_lock(s);
+            v = bits_to_f64(f64_field_exact_bits(src, from, to, cd, s.big));
+            slow_unlock(s);
+        }
+        m &= m - 1;
+    }
+    return v;
+}
+
+template <class Src>
+DEV void resolve_score(SlowF64& s, const Src& src, NdRecord& r) {
+    uint64_t m = ballot(true);
+    while (m) {
+        const uint32_t l = (uint32_t)__ffsll((long long)m) - 1u;
+        if (__lane_id() == l) {
+            slow_lock(s);
+            ndjson_resolve(src, r, s.big);
+            slow_unlock(s);
+        }
+        m &= m - 1;
+    }
+}
+
 struct Shared {
     uint8_t bytes[TILE];
     uint16_t list[TILE];
@@ -455,6 +691,7 @@ struct Shared {
     long long lsum_sh[NWAVE];
     uint32_t str_len[FALX_MAX_COLUMNS];   // tile cleaned bytes per str slot
     uint32_t str_base[FALX_MAX_COLUMNS];  // arena base per str slot
+    SlowF64 slow;                         // exact path for undecided f64 cells
 };
 
 DEV uint32_t list_at(const Shared& sh, uint32_t k) {
@@ -476,7 +713,10 @@ __global__ __launch_bounds__(WG) void k_scan(Params P) {
     constexpr int NS = Dial<HQ, ESC, CMT>::NS;
     const uint32_t tid = threadIdx.x, ln = tid & 63u, w = tid >> 6;
 
-    if (tid == 0) sh.tile = (uint32_t)(atomicAdd(P.ticket, 1ULL) - P.ticket_base);
+    if (tid == 0) {
+        sh.tile = (uint32_t)(atomicAdd(P.ticket, 1ULL) - P.ticket_base);
+        sh.slow.lock = 0;
+    }
     {
         uint32_t b = tid;
         sh.cls[b] = (uint8_t)((P.cand[b >> 5] >> (b & 31)) & 1u);
@@ -533,28 +773,22 @@ __global__ __launch_bounds__(WG) void k_scan(Params P) {
         }
         if (ln == 0) sh.wfn[w] = fn;
         __syncthreads();
-        if (tid == 0) {
+        if (w == 0) {
             uint32_t agg = sh.wfn[0];
             for (int k = 1; k < NWAVE; ++k) agg = fn_then<NS>(agg, sh.wfn[k]);
             uint32_t entry = 0;
             if (tile > 0) {
-                __hip_atomic_store(&P.lb1[tile], lb1_word(1, P.epoch, agg), __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_AGENT);
-                uint32_t g = fn_ident<NS>();
-                int32_t j = (int32_t)tile - 1;
-                while (true) {
-                    unsigned long long v;
-                    if (!lb1_wait(P, (uint32_t)j, v)) { entry = 0; break; }
-                    uint32_t val = (uint32_t)v;
-                    if ((v >> 62) == 2) { entry = fn_apply(g, val); break; }
-                    g = fn_then<NS>(val, g);
-                    --j;
-                }
+                if (ln == 0)
+                    __hip_atomic_store(&P.lb1[tile], lb1_word(1, P.epoch, agg), __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+                entry = lookback_fn<NS>(P, tile);
             }
-            __hip_atomic_store(&P.lb1[tile], lb1_word(2, P.epoch, fn_apply(agg, entry)), __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_AGENT);
-            uint32_t s = entry;
-            for (int k = 0; k < NWAVE; ++k) { sh.wentry[k] = s; s = fn_apply(sh.wfn[k], s); }
+            if (ln == 0) {
+                __hip_atomic_store(&P.lb1[tile], lb1_word(2, P.epoch, fn_apply(agg, entry)), __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+                uint32_t s = entry;
+                for (int k = 0; k < NWAVE; ++k) { sh.wentry[k] = s; s = fn_apply(sh.wfn[k], s); }
+            }
         }
         __syncthreads();
     } else {
@@ -620,33 +854,26 @@ __global__ __launch_bounds__(WG) void k_scan(Params P) {
     }
 
     // ---- LB2: counters prefix ---------------------------------------------
-    if (tid == 0) {
-        sh.wbase[0] = 0;
-        for (int k = 0; k < NWAVE; ++k) sh.wbase[k + 1] = sh.wbase[k] + sh.wcount[k];
+    if (w == 0) {
         Counts agg = sh.wcnt[0];
         for (int k = 1; k < NWAVE; ++k) agg = compose(agg, sh.wcnt[k]);
         Counts pre = counts_zero();
         if (tile > 0) {
-            store_counts(&P.lb2_agg[tile], agg);
-            lb_flag_set(P, P.lb2_flag, tile, 1);
-            Counts acc = counts_zero();
-            int32_t j = (int32_t)tile - 1;
-            while (true) {
-                uint32_t f = lb_flag_wait(P, P.lb2_flag, (uint32_t)j);
-                if (f == 2) { pre = compose(load_counts(&P.lb2_inc[j]), acc); break; }
-                acc = compose(load_counts(&P.lb2_agg[j]), acc);
-                --j;
-            }
+            if (ln == 0) lb2_publish(P, tile, 0, agg);
+            pre = lookback_counts(P, tile);
         }
-        Counts inc = compose(pre, agg);
-        store_counts(&P.lb2_inc[tile], inc);
-        lb_flag_set(P, P.lb2_flag, tile, 2);
-        sh.pre = pre;
-        sh.inc = inc;
-        if (last_tile) {
-            P.result[R_N_STRUCT] = inc.n_struct;
-            P.result[R_N_SEPS] = inc.n_seps;
-            P.result[R_N_TERMS] = inc.n_terms;
+        if (ln == 0) {
+            Counts inc = compose(pre, agg);
+            lb2_publish(P, tile, 1, inc);
+            sh.pre = pre;
+            sh.inc = inc;
+            sh.wbase[0] = 0;
+            for (int k = 0; k < NWAVE; ++k) sh.wbase[k + 1] = sh.wbase[k] + sh.wcount[k];
+            if (last_tile) {
+                P.result[R_N_STRUCT] = inc.n_struct;
+                P.result[R_N_SEPS] = inc.n_seps;
+                P.result[R_N_TERMS] = inc.n_terms;
+            }
         }
     }
     __syncthreads();
@@ -666,14 +893,17 @@ __global__ __launch_bounds__(WG) void k_scan(Params P) {
     const CellDialect cd{P.quote, HQ ? P.escape : -1};
 
     // Walk the compacted list in chunks of 256 with record context.
-    auto walk = [&](auto&& visit) {
+    // `vword` != 0 appends one virtual entry after the list: the terminator
+    // at `len` that closes an unterminated last record (last tile only).
+    auto walk = [&](auto&& visit, uint32_t vword = 0u) __attribute__((always_inline)) {
         uint32_t terms_before = 0, rows_before = 0;
         int32_t lt_before = -1;
-        for (uint32_t base = 0; base < n_list; base += WG) {
+        const uint32_t n_walk = n_list + (vword ? 1u : 0u);
+        for (uint32_t base = 0; base < n_walk; base += WG) {
             Entry e;
             e.k = base + tid;
-            e.valid = e.k < n_list;
-            uint16_t v = e.valid ? list_at(sh, e.k) : 0;
+            e.valid = e.k < n_walk;
+            uint16_t v = !e.valid ? 0 : (e.k < n_list ? list_at(sh, e.k) : (uint16_t)vword);
             e.is_term = e.valid && (v & 0x4000u);
             e.flag = e.valid && (v & 0x8000u);
             e.pos = tile_start + (v & 0x3FFFu);
@@ -698,6 +928,9 @@ __global__ __launch_bounds__(WG) void k_scan(Params P) {
     const Counts inc = sh.inc;
     const bool trailer = last_tile && inc.last_term < P.len;
     const bool trailer_comment = trailer && CMT && P.data[inc.last_term] == cb;
+    // Virtual terminator word (local position, term bit, comment flag).
+    const uint32_t trailer_word =
+        trailer ? ((P.len - tile_start) | 0x4000u | (trailer_comment ? 0x8000u : 0u)) : 0u;
 
     if (MODE == M_TAPE) {
         walk([&](const Entry& e, uint32_t terms_excl) {
@@ -794,80 +1027,99 @@ __global__ __launch_bounds__(WG) void k_scan(Params P) {
         return;
     }
 
-    if (MODE == M_STATS) {
-        const uint32_t c0 = P.col_group;
-        const uint32_t nc = min((uint32_t)STAT_GROUP, P.n_cols - c0);
-        unsigned long long values[STAT_GROUP] = {}, csum[STAT_GROUP] = {}, bytes[STAT_GROUP] = {},
-                           fx_cells[STAT_GROUP] = {};
-        long long fx_sum[STAT_GROUP] = {};
-        double dsum[STAT_GROUP] = {};
-        unsigned long long records = 0;
-        auto cell = [&](uint32_t j, uint32_t from, uint32_t to) {
-            const ColDesc& c = P.cols[c0 + j];
-            if (c.type == FALX_COL_F64) {
-                F64Result r = parse_f64_field(src, from, to, cd);
-                if (r.ok) {
-                    values[j] += 1;
-                    csum[j] += f64_to_bits(r.value);
-                    dsum[j] += r.value;
-                    if (r.fixed6) { fx_sum[j] += r.mantissa; fx_cells[j] += 1; }
-                }
-            } else if (c.type == FALX_COL_I64) {
-                int64_t v;
-                if (parse_i64_field(src, from, to, cd, v)) {
-                    values[j] += 1;
-                    csum[j] += (unsigned long long)v;
-                    dsum[j] += (double)v;
-                }
-            } else if (c.type == FALX_COL_STR) {
-                values[j] += 1;
-                unsigned long long cs = 0;
-                bytes[j] += clean_cell(src, from, to, cd, [&](uint8_t b) { cs += b; });
-                csum[j] += cs;
-            } else {
-                if (to > from) {
-                    values[j] += 1;
-                    bytes[j] += to - from;
-                    for (uint32_t p = from; p < to; ++p) csum[j] += src.at(p);
-                }
-            }
-        };
-        auto on_fields = [&](uint32_t ord, uint32_t from, uint32_t to) {
-#pragma unroll
-            for (uint32_t j = 0; j < (uint32_t)STAT_GROUP; ++j)
-                if (j < nc && P.cols[c0 + j].index == ord) cell(j, from, to);
-        };
-        walk([&](const Entry& e, uint32_t) {
-            if (!e.valid || e.flag) return;
-            if (!e.is_term) { on_fields(e.ord, e.fs, e.pos); return; }
-            uint32_t to = (e.pos > e.rs && src.at(e.pos - 1) == '\r') ? e.pos - 1 : e.pos;
-            on_fields(e.ord, e.fs, to);
-            records += 1;
-        });
-        if (tid == 0 && trailer && !trailer_comment) {
-            uint32_t to = (P.len > inc.last_term && P.data[P.len - 1] == '\r') ? P.len - 1 : P.len;
-            on_fields(inc.ord, inc.last_struct, to);
-            records += 1;
+    // Per-cell outcome folded into the column statistics.
+    struct CellStat {
+        bool ok, fx;
+        unsigned long long csum, bytes;
+        long long fxm;
+        double d;
+    };
+    struct Acc {
+        unsigned long long values, csum, bytes, fx_cells;
+        long long fx_sum;
+        double dsum;
+        DEV void add(const CellStat& o) {
+            if (!o.ok) return;
+            values += 1; csum += o.csum; bytes += o.bytes; dsum += o.d;
+            if (o.fx) { fx_sum += o.fxm; fx_cells += 1; }
         }
+    };
+    // acc[j] += o for a uniform j < STAT_GROUP, without indexing registers.
+    auto acc_add = [](Acc* acc, uint32_t j, const CellStat& o) __attribute__((always_inline)) {
+#pragma unroll
+        for (uint32_t jj = 0; jj < (uint32_t)STAT_GROUP; ++jj)
+            if (jj == j) acc[jj].add(o);
+    };
+    auto acc_publish = [&](Acc* acc, uint32_t first, uint32_t nc, unsigned long long records) {
         StatPart* parts = reinterpret_cast<StatPart*>(P.partials);
-        records = block_reduce_sum(records, sh.usum_sh);
-        for (uint32_t j = 0; j < nc; ++j) {
-            unsigned long long v = block_reduce_sum(values[j], sh.usum_sh);
-            unsigned long long cs = block_reduce_sum(csum[j], sh.usum_sh);
-            unsigned long long by = block_reduce_sum(bytes[j], sh.usum_sh);
-            long long fs = block_reduce_sum(fx_sum[j], sh.lsum_sh);
-            unsigned long long fc = block_reduce_sum(fx_cells[j], sh.usum_sh);
-            double ds = block_reduce_sum(dsum[j], sh.dsum_sh);
+#pragma unroll
+        for (uint32_t j = 0; j < (uint32_t)STAT_GROUP; ++j) {
+            if (j >= nc) break;
+            unsigned long long v = block_reduce_sum(acc[j].values, sh.usum_sh);
+            unsigned long long cs = block_reduce_sum(acc[j].csum, sh.usum_sh);
+            unsigned long long by = block_reduce_sum(acc[j].bytes, sh.usum_sh);
+            long long fs = block_reduce_sum(acc[j].fx_sum, sh.lsum_sh);
+            unsigned long long fc = block_reduce_sum(acc[j].fx_cells, sh.usum_sh);
+            double ds = block_reduce_sum(acc[j].dsum, sh.dsum_sh);
             if (tid == 0) {
-                StatPart& sp = parts[(size_t)tile * P.n_cols + c0 + j];
+                StatPart& sp = parts[(size_t)tile * P.n_cols + first + j];
                 sp.values = v; sp.checksum = cs; sp.bytes = by;
                 sp.fixed6_sum = fs; sp.fixed6_cells = fc; sp.sum = ds;
             }
         }
-        if (tid == 0 && c0 == 0) {
+        records = block_reduce_sum(records, sh.usum_sh);
+        if (tid == 0 && first == 0) {
             unsigned long long* rec = reinterpret_cast<unsigned long long*>(parts + (size_t)P.n_tiles * P.n_cols);
             rec[tile] = records;
         }
+    };
+    // Typed value of a present cell; resolves undecided f64 cells exactly.
+    auto f64_cell = [&](uint32_t from, uint32_t to) __attribute__((always_inline)) {
+        F64Result r = parse_f64_field(src, from, to, cd);
+        if (r.deferred) r.value = resolve_f64(sh.slow, src, from, to, cd);
+        return r;
+    };
+
+    if (MODE == M_STATS) {
+        const uint32_t c0 = P.col_group;
+        const uint32_t nc = min((uint32_t)STAT_GROUP, P.n_cols - c0);
+        Acc acc[STAT_GROUP];
+#pragma unroll
+        for (int j = 0; j < STAT_GROUP; ++j) acc[j] = Acc{0, 0, 0, 0, 0, 0.0};
+        unsigned long long records = 0;
+        walk([&](const Entry& e, uint32_t) {
+            if (!e.valid || e.flag) return;
+            uint32_t to = e.pos;
+            if (e.is_term) {
+                if (e.pos > e.rs && src.at(e.pos - 1) == '\r') to = e.pos - 1;
+                records += 1;
+            }
+            for (uint32_t j = 0; j < nc; ++j) {
+                const ColDesc& c = P.cols[c0 + j];
+                if (c.index != e.ord) continue;
+                CellStat o{false, false, 0, 0, 0, 0.0};
+                if (c.type == FALX_COL_F64) {
+                    F64Result r = f64_cell(e.fs, to);
+                    o = CellStat{r.ok, r.fixed6, f64_to_bits(r.value), 0, r.mantissa, r.value};
+                } else if (c.type == FALX_COL_I64) {
+                    int64_t v;
+                    o.ok = parse_i64_field(src, e.fs, to, cd, v);
+                    o.csum = (unsigned long long)v;
+                    o.d = (double)v;
+                } else if (c.type == FALX_COL_STR) {
+                    unsigned long long cs = 0;
+                    o.bytes = clean_cell(src, e.fs, to, cd, [&](uint8_t b) { cs += b; });
+                    o.ok = true;
+                    o.csum = cs;
+                } else if (to > e.fs) {
+                    o.ok = true;
+                    o.bytes = to - e.fs;
+                    for (uint32_t p = e.fs; p < to; ++p) o.csum += src.at(p);
+                }
+                acc_add(acc, j, o);
+            }
+        }, trailer_word);
+        acc_publish(acc, c0, nc, records);
         return;
     }
 
@@ -877,89 +1129,94 @@ __global__ __launch_bounds__(WG) void k_scan(Params P) {
         auto set_valid = [&](const ColDesc& c, uint32_t row) {
             atomicOr(&c.valid[row >> 6], 1ULL << (row & 63));
         };
-        // Numeric / bytes cell of column j at `row`; missing = !found.
-        auto emit_cell = [&](const ColDesc& c, uint32_t row, bool found, uint32_t from, uint32_t to) {
-            if (row >= rows_cap) return;
-            if (c.type == FALX_COL_F64) {
-                double v = 0.0;
-                bool ok = false;
-                if (found) {
-                    F64Result r = parse_f64_field(src, from, to, cd);
-                    ok = r.ok;
-                    if (ok) v = r.value;
-                }
-                reinterpret_cast<double*>(c.values)[row] = v;
-                if (ok) set_valid(c, row);
-            } else if (c.type == FALX_COL_I64) {
-                int64_t v = 0;
-                bool ok = found && parse_i64_field(src, from, to, cd, v);
-                reinterpret_cast<long long*>(c.values)[row] = ok ? v : 0;
-                if (ok) set_valid(c, row);
-            } else if (c.type == FALX_COL_BYTES) {
-                bool ok = found && from != to;
-                reinterpret_cast<uint2*>(c.values)[row] = ok ? make_uint2(from, to) : make_uint2(0, 0);
-                if (ok) set_valid(c, row);
-            }
-        };
+        // Optional fused statistics of the first STAT_GROUP columns,
+        // accumulated in registers while their cells are written.
+        Acc acc[STAT_GROUP];
+#pragma unroll
+        for (int j = 0; j < STAT_GROUP; ++j) acc[j] = Acc{0, 0, 0, 0, 0, 0.0};
+        unsigned long long records = 0;
+        const bool with_stats = P.partials != nullptr;
         const bool have_str = P.n_str > 0;
-        // E1: numeric cells + string lengths.
+        const uint32_t n_cols = P.n_cols;
+        const bool trailer_row_live = trailer && !trailer_comment;
+        // E1: typed cells (+ statistics) and cleaned string lengths. A record
+        // that ends before a column's field leaves that cell null.
         if (tid < FALX_MAX_COLUMNS) sh.str_len[tid] = 0;
         __syncthreads();
-        auto fields = [&](uint32_t ord, uint32_t row, uint32_t from, uint32_t to, bool term) {
-            for (uint32_t j = 0; j < P.n_cols; ++j) {
-                const ColDesc& c = P.cols[j];
-                if (c.index == ord) {
-                    if (c.type == FALX_COL_STR) {
-                        uint32_t n = clean_cell(src, from, to, cd, [](uint8_t) {});
-                        atomicAdd(&sh.str_len[c.str_slot], n);
-                    } else {
-                        emit_cell(c, row, true, from, to);
-                    }
-                } else if (term && c.index > ord && c.type != FALX_COL_STR) {
-                    emit_cell(c, row, false, 0, 0);
-                }
-            }
-        };
         walk([&](const Entry& e, uint32_t) {
             if (!e.valid || e.flag) return;
-            if (!e.is_term) { fields(e.ord, e.row, e.fs, e.pos, false); return; }
-            uint32_t to = (e.pos > e.rs && src.at(e.pos - 1) == '\r') ? e.pos - 1 : e.pos;
-            fields(e.ord, e.row, e.fs, to, true);
-        });
-        const uint32_t trailer_row = inc.n_rows;
-        const bool trailer_row_live = trailer && !trailer_comment;
-        uint32_t trailer_to = (P.len > inc.last_term && P.len > 0 && P.data[P.len - 1] == '\r') ? P.len - 1 : P.len;
-        if (tid == 0 && trailer_row_live) fields(inc.ord, trailer_row, inc.last_struct, trailer_to, true);
+            uint32_t to = e.pos;
+            if (e.is_term) {
+                if (e.pos > e.rs && src.at(e.pos - 1) == '\r') to = e.pos - 1;
+                records += 1;
+            }
+            const uint32_t row = e.row;
+            const bool live = row < rows_cap;
+            for (uint32_t j = 0; j < n_cols; ++j) {
+                const ColDesc& c = P.cols[j];
+                const bool found = c.index == e.ord;
+                if (!found && !(e.is_term && c.index > e.ord)) continue;
+                CellStat o{false, false, 0, 0, 0, 0.0};
+                if (c.type == FALX_COL_STR) {
+                    if (!found) continue;
+                    unsigned long long cs = 0;
+                    uint32_t n = clean_cell(src, e.fs, to, cd, [&](uint8_t b) { cs += b; });
+                    atomicAdd(&sh.str_len[c.str_slot], n);
+                    o = CellStat{true, false, cs, n, 0, 0.0};
+                } else if (c.type == FALX_COL_F64) {
+                    F64Result r{false, false, false, 0, 0.0};
+                    if (found) r = f64_cell(e.fs, to);
+                    if (live) {
+                        reinterpret_cast<double*>(c.values)[row] = r.ok ? r.value : 0.0;
+                        if (r.ok) set_valid(c, row);
+                    }
+                    o = CellStat{r.ok, r.fixed6, f64_to_bits(r.value), 0, r.mantissa, r.value};
+                } else if (c.type == FALX_COL_I64) {
+                    int64_t v = 0;
+                    const bool ok = found && parse_i64_field(src, e.fs, to, cd, v);
+                    if (live) {
+                        reinterpret_cast<long long*>(c.values)[row] = ok ? v : 0;
+                        if (ok) set_valid(c, row);
+                    }
+                    o = CellStat{ok, false, (unsigned long long)v, 0, 0, (double)v};
+                } else {
+                    const bool ok = found && to != e.fs;
+                    if (live) {
+                        reinterpret_cast<uint2*>(c.values)[row] = ok ? make_uint2(e.fs, to) : make_uint2(0, 0);
+                        if (ok) set_valid(c, row);
+                    }
+                    if (ok) {
+                        o.ok = true;
+                        o.bytes = to - e.fs;
+                        for (uint32_t p = e.fs; p < to; ++p) o.csum += src.at(p);
+                    }
+                }
+                if (with_stats && j < (uint32_t)STAT_GROUP) acc_add(acc, j, o);
+            }
+        }, trailer_word);
+        if (with_stats) acc_publish(acc, 0, min((uint32_t)STAT_GROUP, n_cols), records);
         if (tid == 0 && last_tile) P.result[R_N_ROWS] = inc.n_rows + (trailer_row_live ? 1 : 0);
         __syncthreads();
         if (!have_str) return;
 
         // LB3: arena bases per string column.
-        if (tid == 0) {
+        if (w == 0) {
             const uint32_t ns = P.n_str;
-            uint32_t* agg = P.lb3_agg + (size_t)tile * FALX_MAX_COLUMNS;
-            uint32_t* incp = P.lb3_inc + (size_t)tile * FALX_MAX_COLUMNS;
-            uint32_t base[FALX_MAX_COLUMNS];
-            for (uint32_t s = 0; s < ns; ++s) base[s] = 0;
+            unsigned long long* d = P.lb3 + (size_t)tile * 2 * ns;
+            if (ln == 0)
+                for (uint32_t s = 0; s < ns; ++s) sh.str_base[s] = 0;
             if (tile > 0) {
-                for (uint32_t s = 0; s < ns; ++s) agg[s] = sh.str_len[s];
-                lb_flag_set(P, P.lb3_flag, tile, 1);
-                int32_t j = (int32_t)tile - 1;
-                while (true) {
-                    uint32_t f = lb_flag_wait(P, P.lb3_flag, (uint32_t)j);
-                    const uint32_t* src3 = (f == 2 ? P.lb3_inc : P.lb3_agg) + (size_t)j * FALX_MAX_COLUMNS;
-                    for (uint32_t s = 0; s < ns; ++s) base[s] += src3[s];
-                    if (f == 2) break;
-                    --j;
+                if (ln == 0)
+                    for (uint32_t s = 0; s < ns; ++s) lb_store(d + s, tag_word(P.epoch, sh.str_len[s]));
+                lookback_str(P, tile, ns, sh.str_base);
+            }
+            if (ln == 0) {
+                for (uint32_t s = 0; s < ns; ++s) {
+                    const uint32_t incv = sh.str_base[s] + sh.str_len[s];
+                    lb_store(d + ns + s, tag_word(P.epoch, incv));
+                    if (last_tile) P.result[R_STR_LEN + s] = incv;
                 }
             }
-            for (uint32_t s = 0; s < ns; ++s) {
-                incp[s] = base[s] + sh.str_len[s];
-                sh.str_base[s] = base[s];
-            }
-            lb_flag_set(P, P.lb3_flag, tile, 2);
-            if (last_tile)
-                for (uint32_t s = 0; s < ns; ++s) P.result[R_STR_LEN + s] = incp[s];
         }
         __syncthreads();
 
@@ -987,20 +1244,7 @@ __global__ __launch_bounds__(WG) void k_scan(Params P) {
                 }
                 if (e.valid && e.is_term && !e.flag && e.row < rows_cap) offs[e.row + 1] = (int32_t)(at + n);
                 running += tot;
-            });
-            if (tid == 0 && trailer_row_live && trailer_row < rows_cap) {
-                uint32_t at = running;
-                uint32_t n = 0;
-                if (c.index == inc.ord) {
-                    n = clean_cell(src, inc.last_struct, trailer_to, cd, [](uint8_t) {});
-                    if ((uint64_t)at + n <= c.data_cap) {
-                        uint32_t q = at;
-                        clean_cell(src, inc.last_struct, trailer_to, cd, [&](uint8_t b) { c.data[q++] = b; });
-                    }
-                    set_valid(c, trailer_row);
-                }
-                offs[trailer_row + 1] = (int32_t)(at + n);
-            }
+            }, trailer_word);
             __syncthreads();
         }
     }
@@ -1013,30 +1257,38 @@ __global__ __launch_bounds__(WG) void k_fill_u64(unsigned long long* p, uint64_t
     for (uint64_t i = (uint64_t)blockIdx.x * WG + threadIdx.x; i < n; i += (uint64_t)gridDim.x * WG) p[i] = v;
 }
 
+// Deterministic two-level reductions over per-tile partials: level 1 folds
+// RED_BLOCKS contiguous tile ranges in a fixed order, level 2 folds those.
+constexpr uint32_t RED_BLOCKS = 256;
+
 // Sums `width` u64 lanes over `n` rows (row-major) into out[0..width).
+// grid.x = number of row chunks (1 = single pass).
 __global__ __launch_bounds__(WG) void k_reduce_u64(const unsigned long long* parts, uint32_t n, uint32_t width,
                                                    unsigned long long* out) {
     __shared__ unsigned long long sh[NWAVE];
+    const uint32_t chunk = (n + gridDim.x - 1) / gridDim.x;
+    const uint32_t lo = blockIdx.x * chunk, hi = min(n, lo + chunk);
     for (uint32_t c = 0; c < width; ++c) {
         unsigned long long v = 0;
-        for (uint32_t i = threadIdx.x; i < n; i += WG) v += parts[(size_t)i * width + c];
+        for (uint32_t i = lo + threadIdx.x; i < hi; i += WG) v += parts[(size_t)i * width + c];
         v = block_reduce_sum(v, sh);
-        if (threadIdx.x == 0) out[c] = v;
+        if (threadIdx.x == 0) out[(size_t)blockIdx.x * width + c] = v;
     }
 }
 
-// Per-column stat reduction: one workgroup per column, fixed-order FP64 tree
-// (each thread folds a strided run of tiles, then the block tree).
+// Per-column stat reduction. grid = (n_cols, chunks); fixed-order FP64 folds.
 __global__ __launch_bounds__(WG) void k_reduce_stats(const StatPart* parts, uint32_t n_tiles, uint32_t n_cols,
                                                      StatPart* out) {
     __shared__ unsigned long long ush[NWAVE];
     __shared__ long long lsh[NWAVE];
     __shared__ double dsh[NWAVE];
     const uint32_t c = blockIdx.x;
+    const uint32_t chunk = (n_tiles + gridDim.y - 1) / gridDim.y;
+    const uint32_t lo = blockIdx.y * chunk, hi = min(n_tiles, lo + chunk);
     unsigned long long v = 0, cs = 0, by = 0, fc = 0;
     long long fs = 0;
     double ds = 0;
-    for (uint32_t i = threadIdx.x; i < n_tiles; i += WG) {
+    for (uint32_t i = lo + threadIdx.x; i < hi; i += WG) {
         const StatPart& p = parts[(size_t)i * n_cols + c];
         v += p.values; cs += p.checksum; by += p.bytes; fs += p.fixed6_sum; fc += p.fixed6_cells; ds += p.sum;
     }
@@ -1047,8 +1299,9 @@ __global__ __launch_bounds__(WG) void k_reduce_stats(const StatPart* parts, uint
     fs = block_reduce_sum(fs, lsh);
     ds = block_reduce_sum(ds, dsh);
     if (threadIdx.x == 0) {
-        out[c].values = v; out[c].checksum = cs; out[c].bytes = by;
-        out[c].fixed6_sum = fs; out[c].fixed6_cells = fc; out[c].sum = ds;
+        StatPart& o = out[(size_t)blockIdx.y * n_cols + c];
+        o.values = v; o.checksum = cs; o.bytes = by;
+        o.fixed6_sum = fs; o.fixed6_cells = fc; o.sum = ds;
     }
 }
 
@@ -1069,6 +1322,7 @@ __global__ __launch_bounds__(WG) void k_ndjson_id_score(const uint8_t* data, uin
     __shared__ uint16_t nls[TILE];
     __shared__ uint32_t wcnt[NWAVE], wb[NWAVE + 1];
     __shared__ int64_t prev_nl;
+    __shared__ SlowF64 slow;
     __shared__ unsigned long long ush[NWAVE];
     __shared__ long long lsh[NWAVE];
     __shared__ double dsh[NWAVE];
@@ -1087,6 +1341,7 @@ __global__ __launch_bounds__(WG) void k_ndjson_id_score(const uint8_t* data, uin
         *reinterpret_cast<uint4*>(&bytes[i]) = v;
     }
     if (tid == 0) {
+        slow.lock = 0;
         int64_t p = (int64_t)tile_start - 1;
         while (p >= 0 && data[p] != '\n') --p;
         prev_nl = p;
@@ -1127,6 +1382,7 @@ __global__ __launch_bounds__(WG) void k_ndjson_id_score(const uint8_t* data, uin
         if (blank_span(src, from, end)) continue;
         NdRecord r;
         if (ndjson_record(src, from, end, r)) {
+            if (r.score_deferred) resolve_score(slow, src, r);
             recs += 1;
             ids += (unsigned long long)r.id;
             ss += r.score;
@@ -1151,10 +1407,12 @@ __global__ __launch_bounds__(WG) void k_reduce_nd(const NdPart* parts, uint32_t 
     __shared__ unsigned long long ush[NWAVE];
     __shared__ long long lsh[NWAVE];
     __shared__ double dsh[NWAVE];
+    const uint32_t chunk = (n + gridDim.x - 1) / gridDim.x;
+    const uint32_t lo = blockIdx.x * chunk, hi = min(n, lo + chunk);
     unsigned long long r = 0, iv = 0, fc = 0, id = 0;
     long long fs = 0;
     double ss = 0;
-    for (uint32_t i = threadIdx.x; i < n; i += WG) {
+    for (uint32_t i = lo + threadIdx.x; i < hi; i += WG) {
         r += parts[i].records; iv += parts[i].invalid; fc += parts[i].fx_cells;
         id += (unsigned long long)parts[i].id_sum;
         fs += parts[i].fx_sum; ss += parts[i].score_sum;
@@ -1166,7 +1424,8 @@ __global__ __launch_bounds__(WG) void k_reduce_nd(const NdPart* parts, uint32_t 
     fs = block_reduce_sum(fs, lsh);
     ss = block_reduce_sum(ss, dsh);
     if (threadIdx.x == 0) {
-        out->records = r; out->invalid = iv; out->fx_cells = fc; out->id_sum = (long long)id; out->fx_sum = fs; out->score_sum = ss;
+        NdPart& o = out[blockIdx.x];
+        o.records = r; o.invalid = iv; o.fx_cells = fc; o.id_sum = (long long)id; o.fx_sum = fs; o.score_sum = ss;
     }
 }
 

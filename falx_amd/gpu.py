@@ -28,6 +28,7 @@ _SIGS = [
     ("falx_ctx_destroy", None, [_VP]),
     ("falx_ctx_stream", _VP, [_VP]),
     ("falx_ctx_last_kernel_ms", C.c_double, [_VP]),
+    ("falx_ctx_last_error", C.c_int, [_VP, C.POINTER(C.c_int32), _U64P, _U64P]),
     ("falx_device_alloc", C.c_int, [_VP, _U64, C.POINTER(C.c_void_p)]),
     ("falx_device_free", C.c_int, [_VP, _VP]),
     ("falx_host_alloc_pinned", C.c_int, [_VP, _U64, C.POINTER(C.c_void_p)]),
@@ -38,6 +39,8 @@ _SIGS = [
     ("falx_index_structurals", C.c_int, [_VP, C.POINTER(S.CSpec), _VP, _U64, _VP, _U64, _U64P]),
     ("falx_parse", C.c_int, [_VP, C.POINTER(S.CSpec), _VP, _U64, _VP, _U64, _VP, _U64, _U64P, _U64P]),
     ("falx_parse_columns", C.c_int, [_VP, C.POINTER(S.CSpec), _VP, _U64, C.POINTER(S.CColumns)]),
+    ("falx_parse_columns_with_stats", C.c_int,
+     [_VP, C.POINTER(S.CSpec), _VP, _U64, C.POINTER(S.CColumns), C.POINTER(S.CCsvStats)]),
     ("falx_parse_csv_stats", C.c_int, [_VP, C.POINTER(S.CSpec), _VP, _U64, C.POINTER(S.CCsvStats)]),
     ("falx_parse_field_bytes", C.c_int, [_VP, C.POINTER(S.CSpec), _VP, _U64, C.POINTER(S.CFieldByteStats)]),
     ("falx_parse_ndjson_lines", C.c_int, [_VP, _VP, _U64, _U64P]),
@@ -48,8 +51,8 @@ EXPORTED = [name for name, _, _ in _SIGS]
 
 
 class FalxError(RuntimeError):
-    def __init__(self, fn: str, status: int):
-        super().__init__(f"{fn}: {S.STATUS_NAMES.get(status, status)}")
+    def __init__(self, fn: str, status: int, detail: str = ""):
+        super().__init__(f"{fn}: {S.STATUS_NAMES.get(status, status)}{detail}")
         self.status = status
 
 
@@ -73,10 +76,24 @@ def load():
     return _lib
 
 
-def _check(fn: str, st: int, allow=(S.OK,)):
+def _check(fn: str, st: int, allow=(S.OK,), ctx=None):
     if st not in allow:
-        raise FalxError(fn, st)
+        detail = ""
+        if st == S.E_DEVICE and ctx is not None:
+            hip, flags, tile = C.c_int32(), C.c_uint64(), C.c_uint64()
+            if load().falx_ctx_last_error(ctx, C.byref(hip), C.byref(flags), C.byref(tile)) == S.OK:
+                detail = f" (hip error {hip.value}, look-back flags {flags.value:#x}, tile {tile.value})"
+        raise FalxError(fn, st, detail)
     return st
+
+
+def _stats_dict(spec: S.Spec, st: S.CCsvStats) -> Dict:
+    out = {"records": st.records, "cols": []}
+    for k in range(len(spec.columns)):
+        c = st.cols[k]
+        out["cols"].append({"values": c.values, "checksum": c.checksum, "bytes": c.bytes,
+                            "fixed6_sum": c.fixed6_sum, "fixed6_cells": c.fixed6_cells, "sum": c.sum})
+    return out
 
 
 class DeviceBuffer:
@@ -123,6 +140,9 @@ class Context:
             load().falx_ctx_destroy(self.h)
             self.h = None
 
+    def _ck(self, fn: str, st: int, allow=(S.OK,)):
+        return _check(fn, st, allow, self.h)
+
     def __enter__(self):
         return self
 
@@ -141,7 +161,7 @@ class Context:
                                    np.frombuffer(bytes(data), dtype=np.uint8))
         buf = DeviceBuffer(self, arr.nbytes)
         if arr.nbytes:
-            _check("falx_upload", load().falx_upload(self.h, buf.ptr, arr.ctypes.data, arr.nbytes))
+            self._ck("falx_upload", load().falx_upload(self.h, buf.ptr, arr.ctypes.data, arr.nbytes))
         return buf
 
     # -- structural index -------------------------------------------------
@@ -152,7 +172,7 @@ class Context:
         if out is None:
             out = self.alloc(4 * max(1, n))
         cnt = C.c_uint64()
-        _check("falx_index_structurals",
+        self._ck("falx_index_structurals",
                load().falx_index_structurals(self.h, C.byref(cs), data.ptr, n, out.ptr, out.nbytes // 4,
                                              C.byref(cnt)))
         return out.download(np.uint32, cnt.value) if download else (out, cnt.value)
@@ -163,7 +183,7 @@ class Context:
         seps = self.alloc(4 * max(1, n))
         ends = self.alloc(8 * max(1, n))
         ns, ne = C.c_uint64(), C.c_uint64()
-        _check("falx_parse", load().falx_parse(self.h, C.byref(cs), data.ptr, n, seps.ptr, max(1, n), ends.ptr,
+        self._ck("falx_parse", load().falx_parse(self.h, C.byref(cs), data.ptr, n, seps.ptr, max(1, n), ends.ptr,
                                                max(1, n), C.byref(ns), C.byref(ne)))
         return seps.download(np.uint32, ns.value), ends.download(np.uint64, ne.value)
 
@@ -191,9 +211,19 @@ class Context:
 
     def parse_columns_into(self, spec: S.Spec, data: DeviceBuffer, length: int, cols: S.CColumns) -> int:
         cs = spec.to_c()
-        return _check("falx_parse_columns",
+        return self._ck("falx_parse_columns",
                       load().falx_parse_columns(self.h, C.byref(cs), data.ptr, length, C.byref(cols)),
                       allow=(S.OK, S.E_CAPACITY))
+
+    def parse_columns_with_stats_into(self, spec: S.Spec, data: DeviceBuffer, length: int,
+                                      cols: S.CColumns) -> Dict:
+        """One pass: columns into `cols` and the fused statistics returned."""
+        cs = spec.to_c()
+        st = S.CCsvStats()
+        self._ck("falx_parse_columns_with_stats",
+               load().falx_parse_columns_with_stats(self.h, C.byref(cs), data.ptr, length, C.byref(cols),
+                                                    C.byref(st)), allow=(S.OK, S.E_CAPACITY))
+        return _stats_dict(spec, st)
 
     def parse_columns(self, spec: S.Spec, data: DeviceBuffer, length: Optional[int] = None,
                       rows_cap: Optional[int] = None, data_cap: Optional[int] = None) -> Dict:
@@ -227,32 +257,27 @@ class Context:
         n = data.nbytes if length is None else length
         cs = spec.to_c()
         st = S.CCsvStats()
-        _check("falx_parse_csv_stats", load().falx_parse_csv_stats(self.h, C.byref(cs), data.ptr, n, C.byref(st)))
-        out = {"records": st.records, "cols": []}
-        for k in range(len(spec.columns)):
-            c = st.cols[k]
-            out["cols"].append({"values": c.values, "checksum": c.checksum, "bytes": c.bytes,
-                                "fixed6_sum": c.fixed6_sum, "fixed6_cells": c.fixed6_cells, "sum": c.sum})
-        return out
+        self._ck("falx_parse_csv_stats", load().falx_parse_csv_stats(self.h, C.byref(cs), data.ptr, n, C.byref(st)))
+        return _stats_dict(spec, st)
 
     def field_bytes(self, spec: S.Spec, data: DeviceBuffer, length: Optional[int] = None) -> Dict:
         n = data.nbytes if length is None else length
         cs = spec.to_c()
         st = S.CFieldByteStats()
-        _check("falx_parse_field_bytes",
+        self._ck("falx_parse_field_bytes",
                load().falx_parse_field_bytes(self.h, C.byref(cs), data.ptr, n, C.byref(st)))
         return {"fields": st.fields, "bytes": st.bytes}
 
     def ndjson_lines(self, data: DeviceBuffer, length: Optional[int] = None) -> int:
         n = data.nbytes if length is None else length
         r = C.c_uint64()
-        _check("falx_parse_ndjson_lines", load().falx_parse_ndjson_lines(self.h, data.ptr, n, C.byref(r)))
+        self._ck("falx_parse_ndjson_lines", load().falx_parse_ndjson_lines(self.h, data.ptr, n, C.byref(r)))
         return r.value
 
     def ndjson_id_score(self, data: DeviceBuffer, length: Optional[int] = None) -> Dict:
         n = data.nbytes if length is None else length
         st = S.CNdjsonIdScore()
-        _check("falx_parse_ndjson_id_score",
+        self._ck("falx_parse_ndjson_id_score",
                load().falx_parse_ndjson_id_score(self.h, data.ptr, n, C.byref(st)), allow=(S.OK, S.E_SCHEMA))
         return {"records": st.records, "invalid": st.invalid, "id_sum": st.id_sum,
                 "score_fixed6_sum": st.score_fixed6_sum, "score_fixed6_cells": st.score_fixed6_cells,
@@ -261,6 +286,6 @@ class Context:
     def logfmt_pairs(self, data: DeviceBuffer, length: Optional[int] = None) -> Dict:
         n = data.nbytes if length is None else length
         st = S.CLogfmtStats()
-        _check("falx_parse_logfmt_pairs", load().falx_parse_logfmt_pairs(self.h, data.ptr, n, C.byref(st)))
+        self._ck("falx_parse_logfmt_pairs", load().falx_parse_logfmt_pairs(self.h, data.ptr, n, C.byref(st)))
         return {"pairs": st.pairs, "key_bytes": st.key_bytes, "value_bytes": st.value_bytes,
                 "checksum": st.checksum}

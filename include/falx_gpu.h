@@ -159,8 +159,17 @@ falx_status falx_ctx_create(int device, falx_ctx** out);
 void falx_ctx_destroy(falx_ctx* ctx);
 /* The context's hipStream_t, for callers that order their own work. */
 void* falx_ctx_stream(falx_ctx* ctx);
-/* Milliseconds the last entry point's kernels took on the device. */
+/* Device milliseconds of the last entry point's main kernel(s): HIP events
+ * on the context stream around the scan launches (fills and final
+ * reductions excluded). */
 double falx_ctx_last_kernel_ms(falx_ctx* ctx);
+/* Detail of the last FALX_E_DEVICE from an entry point on this context (no
+ * counterpart in the reference, whose CPU path has no device to fail): the
+ * HIP error code of the failing runtime call on the calling thread (0 =
+ * none), and the kernel-side look-back flags (1 = state, 2 = counters,
+ * 4 = string arena) with the first tile that gave up, plus one (0 = none). */
+falx_status falx_ctx_last_error(const falx_ctx* ctx, int32_t* hip_error, uint64_t* device_flags,
+                                uint64_t* tile);
 
 falx_status falx_device_alloc(falx_ctx* ctx, uint64_t bytes, void** d_out);
 falx_status falx_device_free(falx_ctx* ctx, void* d_ptr);
@@ -208,6 +217,17 @@ falx_status falx_parse(falx_ctx* ctx, const falx_spec* spec,
 falx_status falx_parse_columns(falx_ctx* ctx, const falx_spec* spec,
                                const uint8_t* d_data, uint64_t len,
                                falx_columns* out);
+
+/*
+ * Typed columns plus the fused statistics of the same columns from one pass
+ * over the input: the reference's column lane followed by the sums over the
+ * materialised columns (examples/bench_columns.rs:151-179,
+ * examples/csv_geo_aggregate.rs:42-63), with `stats` as falx_parse_csv_stats
+ * would return it.
+ */
+falx_status falx_parse_columns_with_stats(falx_ctx* ctx, const falx_spec* spec,
+                                          const uint8_t* d_data, uint64_t len,
+                                          falx_columns* out, falx_csv_stats* stats);
 
 /* ---- fused sinks ---------------------------------------------------------- */
 

@@ -2,12 +2,13 @@
 // gfx950 kernels run) swept against the CPU oracle on randomized cells.
 // Exit status 0 = every cell agreed bit-for-bit.
 //
-// usage: cells_check <seed> <count>
+// usage: cells_check <seed> <count> [near-halfway count]
 #include <cinttypes>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <vector>
 
 #include "../../falx_amd/csrc/cells.h"
 #include "../../oracle/oracle.h"
@@ -93,6 +94,70 @@ std::string random_int(Rng& r) {
     return s;
 }
 
+// Exact decimal digits of m * 2^k as (digit string D, exponent x) with
+// value = D * 10^x.
+std::pair<std::string, int> exact_decimal(uint64_t m, int k) {
+    std::vector<int> d;  // little-endian base-10
+    for (uint64_t v = m; v; v /= 10) d.push_back((int)(v % 10));
+    int mul = k >= 0 ? 2 : 5, reps = k >= 0 ? k : -k;
+    for (int i = 0; i < reps; ++i) {
+        int carry = 0;
+        for (int& x : d) {
+            int y = x * mul + carry;
+            x = y % 10;
+            carry = y / 10;
+        }
+        while (carry) { d.push_back(carry % 10); carry /= 10; }
+    }
+    std::string D;
+    for (size_t i = d.size(); i-- > 0;) D.push_back((char)('0' + d[i]));
+    return {D, k >= 0 ? 0 : k};
+}
+
+std::string with_exp(const std::string& D, int x) { return D + "e" + std::to_string(x); }
+
+// Cells at, just above, just below and truncated near the midpoint between
+// a random double and its successor (normal and subnormal).
+std::string halfway_cell(Rng& r) {
+    uint64_t M, E;
+    int kind = (int)r.below(4);
+    if (kind == 0) {  // subnormal
+        M = r.next() & ((1ULL << 52) - 1);
+        E = 0;
+    } else {
+        M = (1ULL << 52) | (r.next() & ((1ULL << 52) - 1));
+        E = kind == 1 ? r.below(2046) + 1 : 1023 + r.below(140) - 70;
+    }
+    int e2 = (E == 0 ? -1074 : (int)E - 1075);  // value = M * 2^e2
+    auto [D, x] = exact_decimal(2 * M + 1, e2 - 1);
+    std::string s;
+    switch (r.below(5)) {
+        case 0: s = with_exp(D, x); break;
+        case 1: s = with_exp(D + "0001", x - 4); break;
+        case 2: {  // just below: D*10 - 1
+            std::string B = D + "0";
+            int i = (int)B.size() - 1;
+            while (B[i] == '0') { B[i] = '9'; --i; }
+            B[i] -= 1;
+            s = with_exp(B, x - 1);
+            break;
+        }
+        case 3: {  // truncated to 17..40 significant digits
+            size_t n = 17 + r.below(24);
+            if (n < D.size()) s = with_exp(D.substr(0, n), x + (int)(D.size() - n));
+            else s = with_exp(D, x);
+            break;
+        }
+        default: {  // same value in positional form when short enough
+            if (x == 0) s = D;
+            else if ((int)D.size() + x > 0) s = D.substr(0, D.size() + x) + "." + D.substr(D.size() + x);
+            else s = "0." + std::string((size_t)(-x - (int)D.size()), '0') + D;
+        }
+    }
+    if (r.below(2)) s = "-" + s;
+    return s;
+}
+
 int fail(const char* what, const std::string& cell) {
     std::fprintf(stderr, "MISMATCH %s on cell [%s]\n", what, cell.c_str());
     return 1;
@@ -103,6 +168,7 @@ int fail(const char* what, const std::string& cell) {
 int main(int argc, char** argv) {
     uint64_t seed = argc > 1 ? std::strtoull(argv[1], nullptr, 0) : 1;
     uint64_t count = argc > 2 ? std::strtoull(argv[2], nullptr, 0) : 200000;
+    uint64_t n_halfway = argc > 3 ? std::strtoull(argv[3], nullptr, 0) : count / 10;
     Rng r{seed | 1};
     // Fixed known-answer cells first (columns.rs:408-451 hand cases and IEEE edges).
     const char* fixed[] = {"1.5", "-0.0", "0.1", "0.0", "1_000", "1.5e308", "5e-324", "NaN", "inf",
@@ -113,22 +179,28 @@ int main(int argc, char** argv) {
                            "9007199254740993", "9007199254740992.5", "0.30000000000000004",
                            "-12.345678", "+12.345678", "999.999999", "-0.000000",
                            "123456789012345678901234567890", "1e23", "8.98846567431158e307"};
-    uint64_t checked = 0;
+    uint64_t checked = 0, deferred_plain = 0, deferred_halfway = 0, plain = 0, halfway = 0;
+    static fx::BigDec big;
     falx_spec dq{};
     dq.quote = '"';
     dq.escape = -1;
     dq.comment = -1;
     falx_spec bs = dq;
     bs.escape = '\\';
-    for (uint64_t i = 0; i < count + sizeof fixed / sizeof fixed[0]; ++i) {
-        std::string cell = i < sizeof fixed / sizeof fixed[0] ? std::string(fixed[i]) : random_number(r);
+    const uint64_t n_fixed = sizeof fixed / sizeof fixed[0];
+    for (uint64_t i = 0; i < count + n_halfway + n_fixed; ++i) {
+        const bool hw = i >= count + n_fixed;
+        std::string cell = i < n_fixed ? std::string(fixed[i]) : hw ? halfway_cell(r) : random_number(r);
         const uint8_t* p = (const uint8_t*)cell.data();
         BufSrc src{p};
         double ov;
         int ofx;
         int64_t om;
         int ook = ora_parse_f64(p, cell.size(), &ov, &ofx, &om);
-        fx::F64Result g = fx::parse_f64_stream(fx::RawStream<BufSrc>{&src, 0, (uint32_t)cell.size()});
+        fx::F64Result fast = fx::parse_f64_stream(fx::RawStream<BufSrc>{&src, 0, (uint32_t)cell.size()});
+        if (fast.ok && fast.deferred) ++(hw ? deferred_halfway : deferred_plain);
+        ++(hw ? halfway : plain);
+        fx::F64Result g = fx::parse_f64_field_exact(src, 0, (uint32_t)cell.size(), fx::CellDialect{-1, -1}, big);
         if ((int)g.ok != ook) return fail("f64 ok", cell);
         if (ook) {
             if (fx::f64_to_bits(g.value) != fx::f64_to_bits(ov)) return fail("f64 bits", cell);
@@ -141,7 +213,7 @@ int main(int argc, char** argv) {
         BufSrc qs{qp};
         for (int dial = 0; dial < 2; ++dial) {
             fx::CellDialect cd{'"', dial ? '\\' : -1};
-            fx::F64Result gq = fx::parse_f64_field(qs, 0, (uint32_t)quoted.size(), cd);
+            fx::F64Result gq = fx::parse_f64_field_exact(qs, 0, (uint32_t)quoted.size(), cd, big);
             if ((int)gq.ok != ook) return fail("quoted f64 ok", quoted);
             if (ook && fx::f64_to_bits(gq.value) != fx::f64_to_bits(ov)) return fail("quoted f64", quoted);
         }
@@ -178,6 +250,8 @@ int main(int argc, char** argv) {
         }
         ++checked;
     }
-    std::printf("cells_check: %" PRIu64 " cells agree\n", checked);
+    std::printf("cells_check: %" PRIu64 " cells agree; deferred to the exact path: %" PRIu64 " of %" PRIu64
+                " ordinary f64 cells, %" PRIu64 " of %" PRIu64 " near-halfway cells\n",
+                checked, deferred_plain, plain, deferred_halfway, halfway);
     return 0;
 }

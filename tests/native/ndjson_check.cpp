@@ -53,6 +53,12 @@ std::string jstring(Rng& r) {
 std::string jnumber(Rng& r) {
     std::string s;
     if (r.below(3) == 0) s += "-";
+    if (r.below(25) == 0) {  // long mantissas near the subnormal range: exact path
+        s.push_back((char)('1' + r.below(9)));
+        for (int i = 0, n = 15 + (int)r.below(30); i < n; ++i) s.push_back((char)('0' + r.below(10)));
+        s += "e-" + std::to_string(280 + r.below(60));
+        return s;
+    }
     uint64_t k = r.below(8);
     if (k == 0) s += "0";
     else s += std::to_string(r.below(1000000));
@@ -112,6 +118,8 @@ int main(int argc, char** argv) {
         bool blank = fx::blank_span(src, 0, (uint32_t)line.size());
         fx::NdRecord rec;
         int ok = blank ? -1 : fx::ndjson_record(src, 0, (uint32_t)line.size(), rec);
+        static fx::BigDec big;
+        if (ok == 1) fx::ndjson_resolve(src, rec, big);
         uint64_t g_rec = ok == 1, g_inv = ok == 0;
         int64_t g_id = ok == 1 ? rec.id : 0;
         int64_t g_fx = ok == 1 && rec.fx6 ? rec.mant : 0;

@@ -277,3 +277,46 @@ def test_logfmt_pairs(ctx, ora):
         if not doc:
             continue
         assert ctx.logfmt_pairs(ctx.upload(doc), len(doc)) == ora.logfmt_pairs(doc), f"doc {i} {doc!r}"
+
+
+WIDE = S.GEO_ALL.with_columns([(0, S.I64), (1, S.STR), (2, S.F64), (3, S.F64), (2, S.F64), (0, S.BYTES)],
+                              name="geo_wide")
+
+
+@pytest.mark.parametrize("spec", [S.GEO_LATLON, S.GEO_CITY_LATLON, WIDE], ids=lambda s: s.name)
+def test_columns_with_stats_one_pass(ctx, ora, spec):
+    """falx_parse_columns_with_stats == parse_columns + csv_stats (oracle)."""
+    data, truth = datagen.geo_csv(1024 * 1024 + 99, seed=42)
+    buf = ctx.upload(data)
+    rows = truth["rows"]
+    cols, keep = ctx.alloc_columns(spec, rows, data.size)
+    st = ctx.parse_columns_with_stats_into(spec, buf, data.size, cols)
+    assert cols.rows == rows
+    assert_stats_equal(st, ora.csv_stats(spec, data), spec, spec.name)
+    want = ora.parse_columns(spec, data)
+    got = {"rows": rows, "cols": []}
+    for k, c in enumerate(spec.columns):
+        vals, valid, dbuf = keep[k]
+        e = {"valid": valid.download(np.uint64, (rows + 63) // 64)}
+        if c.type == S.STR:
+            e["offsets"] = vals.download(np.int32, rows + 1)
+            e["data"] = dbuf.download(np.uint8, cols.cols[k].data_len)
+        elif c.type == S.BYTES:
+            e["values"] = vals.download(np.uint32, 2 * rows).reshape(-1, 2)
+        else:
+            e["values"] = vals.download(np.int64 if c.type == S.I64 else np.float64, rows)
+        got["cols"].append(e)
+    assert_columns_equal(got, want, spec, spec.name)
+
+
+def test_capacity_errors(ctx, ora):
+    """Too-small outputs report CAPACITY with the required sizes."""
+    from falx_amd.gpu import FalxError
+    data, truth = datagen.geo_csv(200_000, seed=3)
+    buf = ctx.upload(data)
+    cols, _ = ctx.alloc_columns(S.GEO_CITY_LATLON, truth["rows"] - 5, 100)
+    assert ctx.parse_columns_into(S.GEO_CITY_LATLON, buf, data.size, cols) == S.E_CAPACITY
+    assert cols.rows == truth["rows"] and cols.cols[0].data_len == truth["city_bytes"]
+    small = ctx.alloc(4 * 10)
+    with pytest.raises(FalxError):
+        ctx.index_structurals(S.CSV, buf, data.size, out=small)

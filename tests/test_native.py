@@ -23,7 +23,7 @@ def _build(name: str, ora) -> str:
 @pytest.mark.parametrize("seed", [1, 0xDEADBEEF, 20261203])
 def test_cell_parsers_match_oracle(ora, seed):
     exe = _build("cells_check", ora)
-    r = subprocess.run([exe, str(seed), "150000"], capture_output=True, text=True)
+    r = subprocess.run([exe, str(seed), "150000", "15000"], capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
     assert "agree" in r.stdout
 
