@@ -147,6 +147,7 @@ falx_status prepare(falx_ctx* c, const falx_spec* spec, const uint8_t* d, uint64
     P.data = d;
     P.len = (uint32_t)len;
     P.n_tiles = (uint32_t)((len + TILE - 1) / TILE);
+    for (int k = 0; k < 4; ++k) P.sepb[k] = 0x100;
     if (spec) {
         for (uint32_t i = 0; i < spec->n_structural; ++i) {
             uint8_t b = spec->structural[i];
@@ -155,6 +156,12 @@ falx_status prepare(falx_ctx* c, const falx_spec* spec, const uint8_t* d, uint64
         bool nl = false, sep = false;
         for (uint32_t i = 0; i < spec->n_structural; ++i) (spec->structural[i] == '\n' ? nl : sep) = true;
         if (nl || !sep) P.cand['\n' >> 5] |= 1u << ('\n' & 31);
+        // Up to four candidate bytes are tested by compares instead of the table.
+        for (uint32_t b = 0; b < 256; ++b)
+            if ((P.cand[b >> 5] >> (b & 31)) & 1u) {
+                if (P.n_sepb < 4) P.sepb[P.n_sepb] = b;
+                ++P.n_sepb;
+            }
         P.quote = spec->quote;
         P.escape = spec->quote >= 0 ? spec->escape : -1;
         P.comment = spec->comment;

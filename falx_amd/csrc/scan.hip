@@ -78,6 +78,8 @@ struct Params {
     uint32_t len;
     uint32_t n_tiles;
     uint32_t cand[8];
+    uint32_t sepb[4];         // the candidate bytes when there are at most 4 (0x100 = unused)
+    uint32_t n_sepb;          // number of candidate bytes; more than 4 uses the cand[] table
     int32_t quote, escape, comment;
     uint32_t n_cols, n_str;
     uint32_t col_group;       // STATS: first column of this launch's group
@@ -468,7 +470,7 @@ DEV Scan3 wave_incl(Scan3 v) {
 // Exclusive scan (sum a, sum b, max c) across the workgroup; `tot` gets the
 // inclusive total. `sh` is 4-wave scratch.
 DEV Scan3 block_excl(Scan3 v, Scan3* sh, Scan3& tot) {
-    const uint32_t ln = __lane_id(), w = threadIdx.x >> 6;
+    const uint32_t ln = __lane_id(), w = uni(threadIdx.x >> 6);
     Scan3 inc = wave_incl(v);
     Scan3 ex;
     ex.a = __shfl_up(inc.a, 1);
@@ -497,7 +499,7 @@ DEV Scan3 block_excl(Scan3 v, Scan3* sh, Scan3& tot) {
 }
 
 DEV uint32_t block_excl_sum(uint32_t v, uint32_t* sh, uint32_t& tot) {
-    const uint32_t ln = __lane_id(), w = threadIdx.x >> 6;
+    const uint32_t ln = __lane_id(), w = uni(threadIdx.x >> 6);
     uint32_t inc = v;
 #pragma unroll
     for (int d = 1; d < 64; d <<= 1) {
@@ -519,7 +521,7 @@ DEV T block_reduce_sum(T v, T* sh) {
     // fixed-order tree: wave butterfly, then waves in order
 #pragma unroll
     for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d);
-    const uint32_t ln = __lane_id(), w = threadIdx.x >> 6;
+    const uint32_t ln = __lane_id(), w = uni(threadIdx.x >> 6);
     if (ln == 0) sh[w] = v;
     __syncthreads();
     T t = sh[0];
@@ -711,7 +713,7 @@ template <bool HQ, bool ESC, bool CMT, int MODE>
 __global__ __launch_bounds__(WG) void k_scan(Params P) {
     __shared__ Shared sh;
     constexpr int NS = Dial<HQ, ESC, CMT>::NS;
-    const uint32_t tid = threadIdx.x, ln = tid & 63u, w = tid >> 6;
+    const uint32_t tid = threadIdx.x, ln = tid & 63u, w = uni(tid >> 6);  // w is wave-uniform
 
     if (tid == 0) {
         sh.tile = (uint32_t)(atomicAdd(P.ticket, 1ULL) - P.ticket_base);
@@ -722,7 +724,7 @@ __global__ __launch_bounds__(WG) void k_scan(Params P) {
         sh.cls[b] = (uint8_t)((P.cand[b >> 5] >> (b & 31)) & 1u);
     }
     __syncthreads();
-    const uint32_t tile = sh.tile;
+    const uint32_t tile = uni(sh.tile);
     const uint32_t tile_start = tile * (uint32_t)TILE;
     const uint32_t tile_len = min((uint32_t)TILE, P.len - tile_start);
     const bool last_tile = tile + 1 == P.n_tiles;
@@ -754,7 +756,10 @@ __global__ __launch_bounds__(WG) void k_scan(Params P) {
     auto classify = [&](uint32_t blk) {
         uint8_t c = sh.bytes[blk * 64 + ln];
         BM m;
-        m.cand = ballot(sh.cls[c] != 0);
+        if (P.n_sepb <= 4)
+            m.cand = ballot(c == P.sepb[0] || c == P.sepb[1] || c == P.sepb[2] || c == P.sepb[3]);
+        else
+            m.cand = ballot(sh.cls[c] != 0);
         m.nl = ballot(c == '\n');
         m.q = HQ ? ballot(c == qb) : 0ULL;
         m.esc = ESC ? ballot(c == eb) : 0ULL;
@@ -766,6 +771,7 @@ __global__ __launch_bounds__(WG) void k_scan(Params P) {
     if (NS > 1) {
         uint32_t fn = fn_ident<NS>();
         uint64_t ls = ls0;
+#pragma unroll 4
         for (int b = 0; b < BPW; ++b) {
             BM m = classify(w * BPW + b);
             fn = fn_then<NS>(fn, block_fn<HQ, ESC, CMT>(m, ls));
@@ -803,6 +809,7 @@ __global__ __launch_bounds__(WG) void k_scan(Params P) {
         Counts wc = counts_zero();
         uint32_t cnt = 0;
         const uint32_t seg = w * BPW * 64;
+#pragma unroll 2
         for (int b = 0; b < BPW; ++b) {
             const uint32_t blk = w * BPW + b;
             const uint32_t off = blk * 64;
@@ -878,7 +885,7 @@ __global__ __launch_bounds__(WG) void k_scan(Params P) {
     }
     __syncthreads();
 
-    const uint32_t n_list = sh.wbase[NWAVE];
+    const uint32_t n_list = uni(sh.wbase[NWAVE]);
     const Counts pre = sh.pre;
 
     if (MODE == M_INDEX) {
@@ -1034,13 +1041,15 @@ __global__ __launch_bounds__(WG) void k_scan(Params P) {
         long long fxm;
         double d;
     };
+    // One thread's share of one tile: counts and byte totals fit 32 bits.
     struct Acc {
-        unsigned long long values, csum, bytes, fx_cells;
+        uint32_t values, bytes, fx_cells;
+        unsigned long long csum;
         long long fx_sum;
         double dsum;
         DEV void add(const CellStat& o) {
             if (!o.ok) return;
-            values += 1; csum += o.csum; bytes += o.bytes; dsum += o.d;
+            values += 1; csum += o.csum; bytes += (uint32_t)o.bytes; dsum += o.d;
             if (o.fx) { fx_sum += o.fxm; fx_cells += 1; }
         }
     };
@@ -1055,11 +1064,11 @@ __global__ __launch_bounds__(WG) void k_scan(Params P) {
 #pragma unroll
         for (uint32_t j = 0; j < (uint32_t)STAT_GROUP; ++j) {
             if (j >= nc) break;
-            unsigned long long v = block_reduce_sum(acc[j].values, sh.usum_sh);
+            unsigned long long v = block_reduce_sum((unsigned long long)acc[j].values, sh.usum_sh);
             unsigned long long cs = block_reduce_sum(acc[j].csum, sh.usum_sh);
-            unsigned long long by = block_reduce_sum(acc[j].bytes, sh.usum_sh);
+            unsigned long long by = block_reduce_sum((unsigned long long)acc[j].bytes, sh.usum_sh);
             long long fs = block_reduce_sum(acc[j].fx_sum, sh.lsum_sh);
-            unsigned long long fc = block_reduce_sum(acc[j].fx_cells, sh.usum_sh);
+            unsigned long long fc = block_reduce_sum((unsigned long long)acc[j].fx_cells, sh.usum_sh);
             double ds = block_reduce_sum(acc[j].dsum, sh.dsum_sh);
             if (tid == 0) {
                 StatPart& sp = parts[(size_t)tile * P.n_cols + first + j];
@@ -1126,8 +1135,25 @@ __global__ __launch_bounds__(WG) void k_scan(Params P) {
     // ---- M_COLUMNS --------------------------------------------------------
     if (MODE == M_COLUMNS) {
         const uint64_t rows_cap = P.rows_cap;
-        auto set_valid = [&](const ColDesc& c, uint32_t row) {
-            atomicOr(&c.valid[row >> 6], 1ULL << (row & 63));
+        // Validity bits, one wave at a time. Every non-comment record has
+        // exactly one event per column (its cell, or the terminator of a
+        // record too short to have it), and events appear in byte order, so
+        // the event lanes of a wave hold consecutive rows: their bits form
+        // one run that costs at most two atomics instead of one per cell.
+        // Called by every lane of the wave.
+        auto flush_valid = [&](const ColDesc& c, bool event, bool vbit, uint32_t row) {
+            const uint64_t E = ballot(event);
+            if (E == 0) return;
+            const uint32_t first = (uint32_t)__ffsll((long long)E) - 1u;
+            const uint32_t r0 = __shfl(row, (int)first);
+            unsigned long long bits = (event && vbit) ? (1ULL << mbcnt(E)) : 0ULL;
+#pragma unroll
+            for (int d = 32; d >= 1; d >>= 1) bits |= __shfl_xor(bits, d);
+            if (ln == first && bits) {
+                const uint32_t sh0 = r0 & 63u;
+                atomicOr(&c.valid[r0 >> 6], bits << sh0);
+                if (sh0 && (bits >> (64 - sh0))) atomicOr(&c.valid[(r0 >> 6) + 1], bits >> (64 - sh0));
+            }
         };
         // Optional fused statistics of the first STAT_GROUP columns,
         // accumulated in registers while their cells are written.
@@ -1144,9 +1170,9 @@ __global__ __launch_bounds__(WG) void k_scan(Params P) {
         if (tid < FALX_MAX_COLUMNS) sh.str_len[tid] = 0;
         __syncthreads();
         walk([&](const Entry& e, uint32_t) {
-            if (!e.valid || e.flag) return;
+            const bool act = e.valid && !e.flag;
             uint32_t to = e.pos;
-            if (e.is_term) {
+            if (act && e.is_term) {
                 if (e.pos > e.rs && src.at(e.pos - 1) == '\r') to = e.pos - 1;
                 records += 1;
             }
@@ -1154,44 +1180,44 @@ __global__ __launch_bounds__(WG) void k_scan(Params P) {
             const bool live = row < rows_cap;
             for (uint32_t j = 0; j < n_cols; ++j) {
                 const ColDesc& c = P.cols[j];
-                const bool found = c.index == e.ord;
-                if (!found && !(e.is_term && c.index > e.ord)) continue;
-                CellStat o{false, false, 0, 0, 0, 0.0};
-                if (c.type == FALX_COL_STR) {
-                    if (!found) continue;
-                    unsigned long long cs = 0;
-                    uint32_t n = clean_cell(src, e.fs, to, cd, [&](uint8_t b) { cs += b; });
-                    atomicAdd(&sh.str_len[c.str_slot], n);
-                    o = CellStat{true, false, cs, n, 0, 0.0};
-                } else if (c.type == FALX_COL_F64) {
-                    F64Result r{false, false, false, 0, 0.0};
-                    if (found) r = f64_cell(e.fs, to);
-                    if (live) {
-                        reinterpret_cast<double*>(c.values)[row] = r.ok ? r.value : 0.0;
-                        if (r.ok) set_valid(c, row);
+                const bool found = act && c.index == e.ord;
+                const bool event = found || (act && e.is_term && c.index > e.ord);
+                bool vbit = false;
+                if (event) {
+                    CellStat o{false, false, 0, 0, 0, 0.0};
+                    if (c.type == FALX_COL_STR) {
+                        if (found) {
+                            unsigned long long cs = 0;
+                            uint32_t n = clean_cell(src, e.fs, to, cd, [&](uint8_t b) { cs += b; });
+                            atomicAdd(&sh.str_len[c.str_slot], n);
+                            o = CellStat{true, false, cs, n, 0, 0.0};
+                        }
+                    } else if (c.type == FALX_COL_F64) {
+                        F64Result r{false, false, false, 0, 0.0};
+                        if (found) r = f64_cell(e.fs, to);
+                        if (live) reinterpret_cast<double*>(c.values)[row] = r.ok ? r.value : 0.0;
+                        vbit = r.ok && live;
+                        o = CellStat{r.ok, r.fixed6, f64_to_bits(r.value), 0, r.mantissa, r.value};
+                    } else if (c.type == FALX_COL_I64) {
+                        int64_t v = 0;
+                        const bool ok = found && parse_i64_field(src, e.fs, to, cd, v);
+                        if (live) reinterpret_cast<long long*>(c.values)[row] = ok ? v : 0;
+                        vbit = ok && live;
+                        o = CellStat{ok, false, (unsigned long long)v, 0, 0, (double)v};
+                    } else {
+                        const bool ok = found && to != e.fs;
+                        if (live)
+                            reinterpret_cast<uint2*>(c.values)[row] = ok ? make_uint2(e.fs, to) : make_uint2(0, 0);
+                        vbit = ok && live;
+                        if (ok) {
+                            o.ok = true;
+                            o.bytes = to - e.fs;
+                            for (uint32_t p = e.fs; p < to; ++p) o.csum += src.at(p);
+                        }
                     }
-                    o = CellStat{r.ok, r.fixed6, f64_to_bits(r.value), 0, r.mantissa, r.value};
-                } else if (c.type == FALX_COL_I64) {
-                    int64_t v = 0;
-                    const bool ok = found && parse_i64_field(src, e.fs, to, cd, v);
-                    if (live) {
-                        reinterpret_cast<long long*>(c.values)[row] = ok ? v : 0;
-                        if (ok) set_valid(c, row);
-                    }
-                    o = CellStat{ok, false, (unsigned long long)v, 0, 0, (double)v};
-                } else {
-                    const bool ok = found && to != e.fs;
-                    if (live) {
-                        reinterpret_cast<uint2*>(c.values)[row] = ok ? make_uint2(e.fs, to) : make_uint2(0, 0);
-                        if (ok) set_valid(c, row);
-                    }
-                    if (ok) {
-                        o.ok = true;
-                        o.bytes = to - e.fs;
-                        for (uint32_t p = e.fs; p < to; ++p) o.csum += src.at(p);
-                    }
+                    if (with_stats && j < (uint32_t)STAT_GROUP) acc_add(acc, j, o);
                 }
-                if (with_stats && j < (uint32_t)STAT_GROUP) acc_add(acc, j, o);
+                if (c.type != FALX_COL_STR) flush_valid(c, event, vbit, row);
             }
         }, trailer_word);
         if (with_stats) acc_publish(acc, 0, min((uint32_t)STAT_GROUP, n_cols), records);
@@ -1228,21 +1254,22 @@ __global__ __launch_bounds__(WG) void k_scan(Params P) {
             uint32_t running = sh.str_base[c.str_slot];
             if (tile == 0 && tid == 0) offs[0] = 0;
             walk([&](const Entry& e, uint32_t) {
-                bool cellhere = e.valid && !e.flag && c.index == e.ord;
+                const bool act = e.valid && !e.flag;
+                const bool cellhere = act && c.index == e.ord;
+                const bool event = cellhere || (act && e.is_term && c.index > e.ord);
                 uint32_t from = e.fs, to = e.pos;
                 if (cellhere && e.is_term && e.pos > e.rs && src.at(e.pos - 1) == '\r') to = e.pos - 1;
                 uint32_t n = cellhere ? clean_cell(src, from, to, cd, [](uint8_t) {}) : 0u;
                 uint32_t tot;
                 uint32_t ex = block_excl_sum(n, sh.sum_sh, tot);
                 uint32_t at = running + ex;
-                if (cellhere && e.row < rows_cap) {
-                    if ((uint64_t)at + n <= c.data_cap) {
-                        uint32_t q = at;
-                        clean_cell(src, from, to, cd, [&](uint8_t b) { c.data[q++] = b; });
-                    }
-                    set_valid(c, e.row);
+                const bool live = e.row < rows_cap;
+                if (cellhere && live && (uint64_t)at + n <= c.data_cap) {
+                    uint32_t q = at;
+                    clean_cell(src, from, to, cd, [&](uint8_t b) { c.data[q++] = b; });
                 }
-                if (e.valid && e.is_term && !e.flag && e.row < rows_cap) offs[e.row + 1] = (int32_t)(at + n);
+                flush_valid(c, event, cellhere && live, e.row);
+                if (act && e.is_term && live) offs[e.row + 1] = (int32_t)(at + n);
                 running += tot;
             }, trailer_word);
             __syncthreads();
@@ -1326,7 +1353,7 @@ __global__ __launch_bounds__(WG) void k_ndjson_id_score(const uint8_t* data, uin
     __shared__ unsigned long long ush[NWAVE];
     __shared__ long long lsh[NWAVE];
     __shared__ double dsh[NWAVE];
-    const uint32_t tile = blockIdx.x, tid = threadIdx.x, ln = tid & 63u, w = tid >> 6;
+    const uint32_t tile = blockIdx.x, tid = threadIdx.x, ln = tid & 63u, w = uni(tid >> 6);
     const uint32_t tile_start = tile * (uint32_t)TILE;
     const uint32_t tile_len = min((uint32_t)TILE, len - tile_start);
     for (uint32_t i = tid * 16; i < (uint32_t)TILE; i += WG * 16) {
@@ -1363,7 +1390,7 @@ __global__ __launch_bounds__(WG) void k_ndjson_id_score(const uint8_t* data, uin
         for (int k = 0; k < NWAVE; ++k) wb[k + 1] = wb[k] + wcnt[k];
     }
     __syncthreads();
-    const uint32_t n = wb[NWAVE];
+    const uint32_t n = uni(wb[NWAVE]);
     const bool tail = tile + 1 == n_tiles && len > 0 && data[len - 1] != '\n';
     TileSrc src{data, bytes, tile_start, tile_start + tile_len};
     auto nl_at = [&](uint32_t k) -> uint32_t {
