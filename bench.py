@@ -6,9 +6,14 @@ one pass over the HBM-resident input, on one MI355X.
 A step is one `falx_parse_columns_with_stats` call over the whole 1 GiB
 (inputs already in HBM, outputs preallocated). Prints one JSON line with the
 input GiB/s, the roofline of the scan kernel (HIP events on its stream) and
-the CPU oracle's parallel baseline timed on this host. Extra lanes (City +
-lat/lon columns, fused sum alone, structural index, pinned-H2D end-to-end)
-are reported under "lanes".
+the CPU oracle's parallel baseline timed on this host. Extra lanes are
+reported under "lanes", each with the oracle's parallel CPU path on the same
+input beside it:
+  geo CSV 1 GiB: lat/lon columns, fused count/sum, City + lat/lon columns,
+    structural index, pinned-H2D end-to-end;
+  NDJSON 1 GiB: sum(id + nested.score);
+  csv_hash 1 GiB: structural index, typed columns (i64, f64, string),
+    field bytes.
 
 usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -56,6 +61,33 @@ def time_steps(fn, steps: int, warmup: int, ctx):
         kms.append(ctx.last_kernel_ms)
     t1 = time.perf_counter()
     return (t1 - t0) * 1000.0 / steps, sum(kms) / len(kms)
+
+
+def cpu_time(fn, runs: int):
+    """Median seconds of `runs` calls of fn (one untimed warm-up)."""
+    fn()
+    times = []
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        fn()
+        times.append(time.perf_counter() - t0)
+    times.sort()
+    return times[len(times) // 2]
+
+
+def cpu_columns(spec, data, rows: int, data_cap: int, threads: int):
+    """Closure running the oracle's parse_columns_par into preallocated
+    columns (allocation outside the timed call, as on the GPU)."""
+    from oracle import oracle as O
+    cols, keep = O.alloc_columns(spec, rows + 1, data_cap)
+    cs = spec.to_c()
+    lib = O.lib()
+
+    def run():
+        st = lib.ora_parse_columns(C.byref(cs), C.c_void_p(data.ctypes.data), data.size, threads, C.byref(cols))
+        assert st == 0, st
+        return keep
+    return run
 
 
 def cpu_baseline(data, rows: int, runs: int, threads: int):
@@ -178,6 +210,84 @@ def main() -> int:
         ems, _ = time_steps(e2e, max(3, args.steps // 2), 1, ctx)
         lanes["e2e_pinned_h2d"] = {"gib_s": round(n_in / GIB / (ems / 1000.0), 2), "ms_per_step": round(ems, 3)}
         gpu.load().falx_host_free_pinned(ctx.h, hp)
+        if not args.no_cpu:
+            from oracle import oracle as O
+            thr = args.cpu_threads or min(os.cpu_count() or 1, 16)
+            cpu_lane = {
+                "latlon_columns": cpu_columns(spec, data, rows, 16, thr),
+                "fused_count_sum": lambda: O.csv_stats(spec, data, thr),
+                "city_latlon_columns": cpu_columns(S.GEO_CITY_LATLON, data, rows, n_in, thr),
+                "index_structurals": lambda: O.index_structurals(spec, data, thr),
+            }
+            for name, fn in cpu_lane.items():
+                log(f"timing CPU oracle lane {name}")
+                lanes[name]["cpu_gib_s"] = round(n_in / GIB / cpu_time(fn, args.cpu_runs), 3)
+                lanes[name]["cpu_threads"] = thr
+        del cols3, _k3, idx_out
+
+        # NDJSON sum(id + nested.score), 1 GiB
+        log(f"generating {args.size} B NDJSON")
+        nd, nd_truth = datagen.ndjson(args.size)
+        nd_n = int(nd.size)
+        nd_buf = ctx.upload(nd)
+        nd_res = [None]
+
+        def nd_step():
+            nd_res[0] = ctx.ndjson_id_score(nd_buf, nd_n)
+        log("timing lane ndjson_id_score_sum")
+        lms, lkms = time_steps(nd_step, max(3, args.steps // 2), 1, ctx)
+        r = nd_res[0]
+        lanes["ndjson_id_score_sum"] = {
+            "gib_s": round(nd_n / GIB / (lms / 1000.0), 2), "ms_per_step": round(lms, 3),
+            "kernel_ms": round(lkms, 3), "hbm_frac": round(nd_n / (lkms / 1000.0) / 1e9 / HBM_PEAK_GBS, 3),
+            "bytes": nd_n, "records": r["records"],
+            "verified": (r["records"] == nd_truth["records"] and r["id_sum"] == nd_truth["id_sum"]
+                         and r["score_fixed6_sum"] == nd_truth["score_fixed6_sum"] and r["invalid"] == 0)}
+        if not args.no_cpu:
+            log("timing CPU oracle lane ndjson_id_score_sum")
+            lanes["ndjson_id_score_sum"]["cpu_gib_s"] = round(
+                nd_n / GIB / cpu_time(lambda: O.ndjson_id_score(nd, thr), args.cpu_runs), 3)
+            lanes["ndjson_id_score_sum"]["cpu_threads"] = thr
+        nd_buf.free()
+        del nd
+
+        # csv_hash 1 GiB: index, typed columns, field bytes
+        log(f"generating {args.size} B csv_hash")
+        hs, hs_truth = datagen.hash_csv(args.size)
+        hs_n = int(hs.size)
+        hs_buf = ctx.upload(hs)
+        hspec = S.HASH_TYPED
+        hs_rows = int(hs_truth["records"]) + 1
+        hcols, _hk = ctx.alloc_columns(hspec, hs_rows, hs_n)
+        hidx = ctx.alloc(4 * (hs_n // 4 + 64))
+        hs_struct = [0]
+
+        def hs_index():
+            _, hs_struct[0] = ctx.index_structurals(hspec, hs_buf, hs_n, out=hidx, download=False)
+        hash_lanes = [
+            ("csv_hash_index_structurals", hs_index, None,
+             lambda: O.index_structurals(hspec, hs, thr)),
+            ("csv_hash_typed_columns", lambda: ctx.parse_columns_into(hspec, hs_buf, hs_n, hcols), None,
+             (cpu_columns(hspec, hs, hs_rows, hs_n, thr) if not args.no_cpu else None)),
+            ("csv_hash_field_bytes", lambda: ctx.field_bytes(hspec, hs_buf, hs_n), hs_n,
+             lambda: O.field_bytes(hspec, hs, thr)),
+        ]
+        for name, fn, ab, cfn in hash_lanes:
+            log(f"timing lane {name}")
+            lms, lkms = time_steps(fn, max(3, args.steps // 2), 1, ctx)
+            if name == "csv_hash_index_structurals":
+                ab = hs_n + 4 * hs_struct[0]
+            elif ab is None:
+                ab = hs_n + 16 * hcols.rows + 4 * (hcols.rows + 1) + 3 * 8 * ((hcols.rows + 63) // 64)
+            lanes[name] = {"gib_s": round(hs_n / GIB / (lms / 1000.0), 2), "ms_per_step": round(lms, 3),
+                           "kernel_ms": round(lkms, 3),
+                           "hbm_frac": round(ab / (lkms / 1000.0) / 1e9 / HBM_PEAK_GBS, 3), "bytes": hs_n}
+            if not args.no_cpu:
+                log(f"timing CPU oracle lane {name}")
+                lanes[name]["cpu_gib_s"] = round(hs_n / GIB / cpu_time(cfn, args.cpu_runs), 3)
+                lanes[name]["cpu_threads"] = thr
+        lanes["csv_hash_typed_columns"]["rows"] = int(hcols.rows)
+        lanes["csv_hash_typed_columns"]["verified"] = int(hcols.rows) == hs_rows - 1 or int(hcols.rows) == hs_rows
 
     cpu = None
     sums = {}
@@ -185,9 +295,16 @@ def main() -> int:
         threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
         log(f"timing CPU oracle baseline at {threads} threads")
         gibs, med, times, (s_lat, s_lon) = cpu_baseline(data, rows, args.cpu_runs, threads)
+        # one thread on a 128 MiB prefix of the same input
+        pre = data[: min(n_in, 128 << 20)]
+        from oracle import oracle as O1
+        one = pre.size / GIB / cpu_time(cpu_columns(spec, pre, rows, 16, 1), 1)
         cpu = {"value": round(gibs, 3), "unit": "GiB/s", "cores": threads, "kind": "port",
                "sample": f"the same {n_in} B input, {args.cpu_runs} runs of parse_columns_par "
-                         f"(lat/lon) + sequential column sums, median {med * 1000:.0f} ms"}
+                         f"(lat/lon) + sequential column sums, median {med * 1000:.0f} ms",
+               "one_thread_gib_s": round(one, 3),
+               "one_thread_sample": f"first {pre.size} B of the same input, parse_columns at 1 thread",
+               "isa": O1.isa()}
         exact_lat = Fraction(truth["lat_fixed6_sum"], 10**6)
         sums = {
             "lat_sum_gpu_tree": st["cols"][0]["sum"], "lat_sum_ref_fold": s_lat,
