@@ -49,23 +49,34 @@ thread_local int32_t tl_hip_error = 0;
 
 inline size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 
+// Entry states of each dialect kind (see dialect_kind): quote x2, backslash
+// escape x2, comment regions x3.
+constexpr uint32_t kKindStates[6] = {1, 3, 2, 4, 3, 6};
+
+uint32_t summary_stride(uint32_t ns) { return (uint32_t)TS_COUNTS + (uint32_t)C7 * ns; }
+
+// Per-context scratch for a launch over n tiles: tile summaries, group
+// summaries, group and tile prefixes, and the string-arena look-back words.
 struct LbLayout {
-    size_t lb1, lb2, lb3, total;
+    size_t tsum, gsum, gpre, tpre, lb3, total;
 };
 
-// Look-back words per tile: LB1 1, LB2 LB2_STRIDE, LB3 2 per string column.
-LbLayout lb_layout(size_t n, uint32_t n_str) {
+LbLayout lb_layout(size_t n, uint32_t n_str, uint32_t ns) {
+    const size_t stride = 4 * (size_t)summary_stride(ns);
+    const size_t groups = (n + SG - 1) / SG;
     LbLayout L;
     size_t o = 0;
-    L.lb1 = o; o = align_up(o + 8 * n, 256);
-    L.lb2 = o; o = align_up(o + 8 * LB2_STRIDE * n, 256);
+    L.tsum = o; o = align_up(o + stride * n, 256);
+    L.gsum = o; o = align_up(o + stride * groups, 256);
+    L.gpre = o; o = align_up(o + 32 * groups, 256);
+    L.tpre = o; o = align_up(o + 32 * n, 256);
     L.lb3 = o; o = align_up(o + 16 * (size_t)n_str * n, 256);
     L.total = o;
     return L;
 }
 
-falx_status ensure_lb(falx_ctx* c, uint32_t n_tiles, uint32_t n_str) {
-    LbLayout L = lb_layout(n_tiles, n_str);
+falx_status ensure_lb(falx_ctx* c, uint32_t n_tiles, uint32_t n_str, uint32_t ns) {
+    LbLayout L = lb_layout(n_tiles, n_str, ns);
     if (L.total <= c->lb_cap) return FALX_OK;
     if (c->d_lb) HIP_TRY(hipFree(c->d_lb));
     c->d_lb = nullptr;
@@ -113,8 +124,13 @@ int dialect_kind(const falx_spec* s) {
     return esc ? 5 : 4;
 }
 
+// Launches one k_scan pass over all tiles. Tiles take tickets in launch
+// order, so each pass gets its own range of the context's ticket counter.
 template <int MODE>
-hipError_t launch_scan(const Params& P, int kind, hipStream_t s) {
+hipError_t launch_scan(falx_ctx* c, Params& P, int kind) {
+    P.ticket_base = c->ticket_next;
+    c->ticket_next += P.n_tiles;
+    const hipStream_t s = c->stream;
     dim3 g(P.n_tiles), b(WG);
     switch (kind) {
     case 0: hipLaunchKernelGGL((k_scan<false, false, false, MODE>), g, b, 0, s, P); break;
@@ -127,10 +143,29 @@ hipError_t launch_scan(const Params& P, int kind, hipStream_t s) {
     return hipGetLastError();
 }
 
-// Logfmt and NDJSON-line counting use their fixed dialect (kind 3).
-template <int MODE>
-hipError_t launch_scan_fixed3(const Params& P, hipStream_t s) {
-    hipLaunchKernelGGL((k_scan<true, true, false, MODE>), dim3(P.n_tiles), dim3(WG), 0, s, P);
+// The tile-prefix pipeline: tile summaries (every entry state), then the
+// three-level scan to each tile's entry state and exclusive counters, and
+// the grand totals in the result slots. Emit passes read P.tpre.
+hipError_t run_prefix(falx_ctx* c, Params& P, int kind) {
+    const uint32_t ns = kKindStates[kind];
+    const uint32_t stride = summary_stride(ns);
+    const uint32_t groups = (P.n_tiles + SG - 1) / SG;
+    LbLayout L = lb_layout(P.n_tiles, P.n_str, ns);
+    char* base = static_cast<char*>(c->d_lb);
+    uint32_t* tsum = reinterpret_cast<uint32_t*>(base + L.tsum);
+    uint32_t* gsum = reinterpret_cast<uint32_t*>(base + L.gsum);
+    uint32_t* gpre = reinterpret_cast<uint32_t*>(base + L.gpre);
+    uint32_t* tpre = reinterpret_cast<uint32_t*>(base + L.tpre);
+    P.tsum = tsum;
+    P.ts_stride = stride;
+    P.tpre = nullptr;
+    hipError_t e = launch_scan<M_SUMMARY>(c, P, kind);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_scan_reduce, dim3(groups), dim3(SG), 0, c->stream, tsum, stride, P.n_tiles, ns, gsum);
+    hipLaunchKernelGGL(k_scan_groups, dim3(1), dim3(64), 0, c->stream, gsum, stride, groups, ns, gpre, P.result);
+    hipLaunchKernelGGL(k_scan_tiles, dim3(groups), dim3(SG), 0, c->stream, tsum, stride, P.n_tiles, ns, gpre,
+                       tpre);
+    P.tpre = tpre;
     return hipGetLastError();
 }
 
@@ -174,17 +209,13 @@ falx_status prepare(falx_ctx* c, const falx_spec* spec, const uint8_t* d, uint64
         }
         P.n_str = ns;
     }
-    falx_status st = ensure_lb(c, P.n_tiles, P.n_str);
+    const uint32_t ns = kKindStates[spec ? dialect_kind(spec) : 0];
+    falx_status st = ensure_lb(c, P.n_tiles, P.n_str, ns);
     if (st != FALX_OK) return st;
-    LbLayout L = lb_layout(P.n_tiles, P.n_str);
-    char* base = static_cast<char*>(c->d_lb);
-    P.lb1 = reinterpret_cast<unsigned long long*>(base + L.lb1);
-    P.lb2 = reinterpret_cast<unsigned long long*>(base + L.lb2);
-    P.lb3 = reinterpret_cast<unsigned long long*>(base + L.lb3);
+    LbLayout L = lb_layout(P.n_tiles, P.n_str, ns);
+    P.lb3 = reinterpret_cast<unsigned long long*>(static_cast<char*>(c->d_lb) + L.lb3);
     HIP_TRY(next_epoch(c, P.epoch));
     P.ticket = c->d_ticket;
-    P.ticket_base = c->ticket_next;
-    c->ticket_next += P.n_tiles;
     P.result = c->d_result;
     return FALX_OK;
 }
@@ -387,7 +418,8 @@ falx_status falx_index_structurals(falx_ctx* c, const falx_spec* spec, const uin
     P.out = d_out;
     P.out_cap = d_out ? out_cap : 0;
     HIP_TRY(mark_start(c));
-    HIP_TRY(launch_scan<M_INDEX>(P, dialect_kind(spec), c->stream));
+    HIP_TRY(run_prefix(c, P, dialect_kind(spec)));
+    HIP_TRY(launch_scan<M_INDEX>(c, P, dialect_kind(spec)));
     HIP_TRY(mark_end(c));
     if ((st = finish(c)) != FALX_OK) return st;
     *n_out = c->h_result[R_N_STRUCT];
@@ -410,7 +442,8 @@ falx_status falx_parse(falx_ctx* c, const falx_spec* spec, const uint8_t* d, uin
     P.ends = reinterpret_cast<unsigned long long*>(d_ends);
     P.ends_cap = d_ends ? ends_cap : 0;
     HIP_TRY(mark_start(c));
-    HIP_TRY(launch_scan<M_TAPE>(P, dialect_kind(spec), c->stream));
+    HIP_TRY(run_prefix(c, P, dialect_kind(spec)));
+    HIP_TRY(launch_scan<M_TAPE>(c, P, dialect_kind(spec)));
     HIP_TRY(mark_end(c));
     if ((st = finish(c)) != FALX_OK) return st;
     *n_seps = c->h_result[R_N_SEPS];
@@ -463,16 +496,14 @@ static falx_status columns_impl(falx_ctx* c, const falx_spec* spec, const uint8_
         P.partials = c->d_part;
     }
     HIP_TRY(mark_start(c));
-    HIP_TRY(launch_scan<M_COLUMNS>(P, dialect_kind(spec), c->stream));
+    HIP_TRY(run_prefix(c, P, dialect_kind(spec)));
+    HIP_TRY(launch_scan<M_COLUMNS>(c, P, dialect_kind(spec)));
     // Statistics of columns past the first group come from extra passes.
     if (stats) {
         Params Q = P;
         for (uint32_t g = STAT_GROUP; g < spec->n_columns; g += STAT_GROUP) {
-            HIP_TRY(next_epoch(c, Q.epoch));
-            Q.ticket_base = c->ticket_next;
-            c->ticket_next += Q.n_tiles;
             Q.col_group = g;
-            HIP_TRY(launch_scan<M_STATS>(Q, dialect_kind(spec), c->stream));
+            HIP_TRY(launch_scan<M_STATS>(c, Q, dialect_kind(spec)));
         }
     }
     HIP_TRY(mark_end(c));
@@ -550,14 +581,10 @@ falx_status falx_parse_csv_stats(falx_ctx* c, const falx_spec* spec, const uint8
     P.n_cols = spec->n_columns;
     uint32_t groups = spec->n_columns ? (spec->n_columns + STAT_GROUP - 1) / STAT_GROUP : 1;
     HIP_TRY(mark_start(c));
+    HIP_TRY(run_prefix(c, P, dialect_kind(spec)));
     for (uint32_t g = 0; g < groups; ++g) {
         P.col_group = g * STAT_GROUP;
-        if (g > 0) {  // each group is a full pass: fresh tickets and epoch
-            HIP_TRY(next_epoch(c, P.epoch));
-            P.ticket_base = c->ticket_next;
-            c->ticket_next += P.n_tiles;
-        }
-        HIP_TRY(launch_scan<M_STATS>(P, dialect_kind(spec), c->stream));
+        HIP_TRY(launch_scan<M_STATS>(c, P, dialect_kind(spec)));
     }
     HIP_TRY(mark_end(c));
     return reduce_stats(c, spec, P.n_tiles, out);
@@ -577,7 +604,8 @@ falx_status falx_parse_field_bytes(falx_ctx* c, const falx_spec* spec, const uin
     if ((st = ensure_part(c, (size_t)P.n_tiles * 16)) != FALX_OK) return st;
     P.partials = c->d_part;
     HIP_TRY(mark_start(c));
-    HIP_TRY(launch_scan<M_FIELDBYTES>(P, dialect_kind(spec), c->stream));
+    HIP_TRY(run_prefix(c, P, dialect_kind(spec)));
+    HIP_TRY(launch_scan<M_FIELDBYTES>(c, P, dialect_kind(spec)));
     HIP_TRY(mark_end(c));
     HIP_TRY(reduce_u64_2level(c, static_cast<const unsigned long long*>(c->d_part), P.n_tiles, 2u));
     const char* red = static_cast<const char*>(c->d_red) + RED_OUT;
@@ -610,19 +638,15 @@ falx_status falx_parse_ndjson_lines(falx_ctx* c, const uint8_t* d, uint64_t len,
     Params P;
     if ((st = begin(c)) != FALX_OK) return st;
     if ((st = prepare(c, &spec, d, len, P)) != FALX_OK) return st;
-    if ((st = ensure_part(c, (size_t)P.n_tiles * 8)) != FALX_OK) return st;
-    P.partials = c->d_part;
+    // The tile summaries already count the terminators: the scan's grand
+    // total is the answer (no emit pass).
     HIP_TRY(mark_start(c));
-    HIP_TRY(launch_scan_fixed3<M_TERMS>(P, c->stream));
+    HIP_TRY(run_prefix(c, P, dialect_kind(&spec)));
     HIP_TRY(mark_end(c));
-    HIP_TRY(reduce_u64_2level(c, static_cast<const unsigned long long*>(c->d_part), P.n_tiles, 1u));
-    const char* red = static_cast<const char*>(c->d_red) + RED_OUT;
     if ((st = finish(c)) != FALX_OK) return st;
-    unsigned long long h = 0;
     uint8_t last = 0;
-    HIP_TRY(hipMemcpy(&h, red, 8, hipMemcpyDeviceToHost));
     HIP_TRY(hipMemcpy(&last, d + len - 1, 1, hipMemcpyDeviceToHost));
-    *records = h + (last != '\n' ? 1 : 0);  // ndjson_has_trailing_record (ndjson.rs:853-855)
+    *records = c->h_result[R_N_TERMS] + (last != '\n' ? 1 : 0);  // ndjson_has_trailing_record (ndjson.rs:853-855)
     return FALX_OK;
 }
 
@@ -675,7 +699,8 @@ falx_status falx_parse_logfmt_pairs(falx_ctx* c, const uint8_t* d, uint64_t len,
     if ((st = ensure_part(c, (size_t)P.n_tiles * 32)) != FALX_OK) return st;
     P.partials = c->d_part;
     HIP_TRY(mark_start(c));
-    HIP_TRY(launch_scan_fixed3<M_LOGFMT>(P, c->stream));
+    HIP_TRY(run_prefix(c, P, dialect_kind(&spec)));
+    HIP_TRY(launch_scan<M_LOGFMT>(c, P, dialect_kind(&spec)));
     HIP_TRY(mark_end(c));
     HIP_TRY(reduce_u64_2level(c, static_cast<const unsigned long long*>(c->d_part), P.n_tiles, 4u));
     const char* red = static_cast<const char*>(c->d_red) + RED_OUT;

@@ -41,7 +41,7 @@ constexpr uint64_t EVEN_BITS = 0x5555555555555555ULL;
 constexpr uint32_t SPIN_LIMIT = 1u << 22;
 constexpr int STAT_GROUP = 4;             // stat columns accumulated per pass
 
-enum Mode : int { M_INDEX = 0, M_TAPE, M_COLUMNS, M_STATS, M_FIELDBYTES, M_LOGFMT, M_TERMS };
+enum Mode : int { M_INDEX = 0, M_TAPE, M_COLUMNS, M_STATS, M_FIELDBYTES, M_LOGFMT, M_SUMMARY };
 
 // Record/field counters of a byte range; `compose(a, b)` = a then b.
 struct Counts {
@@ -88,8 +88,9 @@ struct Params {
     unsigned long long* ticket;
     unsigned long long ticket_base;
     uint32_t epoch;
-    unsigned long long* lb1;
-    unsigned long long* lb2;  // per tile LB2_STRIDE tagged words: aggregate at +0, inclusive at +8
+    uint32_t* tsum;           // M_SUMMARY out / emit in: per tile ts_stride words (tile summary)
+    const uint32_t* tpre;     // emit modes: per tile 8 words, entry state + exclusive counters
+    uint32_t ts_stride;
     unsigned long long* lb3;  // per tile 2 * n_str tagged words: aggregate, then inclusive
     uint32_t* out;
     uint64_t out_cap;
@@ -185,17 +186,21 @@ DEV Counts counts_zero() {
     return c;
 }
 
-DEV Counts compose(const Counts& a, const Counts& b) {
+// Operands by value: conditional expressions over references into
+// different address spaces (LDS and registers) would force a private copy.
+DEV Counts compose(Counts a, Counts b) {
     Counts r;
     r.n_struct = a.n_struct + b.n_struct;
     r.n_seps = a.n_seps + b.n_seps;
     r.n_terms = a.n_terms + b.n_terms;
     r.n_rows = a.n_rows + b.n_rows;
     r.has_term = a.has_term | b.has_term;
-    r.ord = b.has_term ? b.ord : a.ord + b.ord;
-    r.last_struct = b.last_struct ? b.last_struct : a.last_struct;
-    r.prev_struct = b.n_struct >= 2 ? b.prev_struct : (b.n_struct == 1 ? a.last_struct : a.prev_struct);
-    r.last_term = b.last_term ? b.last_term : a.last_term;
+    const uint32_t a_ls = a.last_struct, a_ps = a.prev_struct, a_lt = a.last_term, a_ord = a.ord;
+    const uint32_t b_ls = b.last_struct, b_ps = b.prev_struct, b_lt = b.last_term, b_ord = b.ord;
+    r.ord = b.has_term ? b_ord : a_ord + b_ord;
+    r.last_struct = b_ls ? b_ls : a_ls;
+    r.prev_struct = b.n_struct >= 2 ? b_ps : (b.n_struct == 1 ? a_ls : a_ps);
+    r.last_term = b_lt ? b_lt : a_lt;
     r.pad0 = r.pad1 = r.pad2 = 0;
     return r;
 }
@@ -209,21 +214,93 @@ struct TileSrc {
 };
 
 // --------------------------------------------------------------------------
-// look-back primitives
+// counters and tile summaries
 // --------------------------------------------------------------------------
-DEV unsigned long long lb1_word(uint32_t status, uint32_t epoch, uint32_t value) {
-    return ((unsigned long long)status << 62) | ((unsigned long long)(epoch & 0x3FFFFFFFu) << 32) | value;
+// Counts travel between kernels as 7 words (n_seps and has_term follow from
+// them): n_struct n_terms n_rows ord last_struct prev_struct last_term.
+constexpr int C7 = 7;
+
+DEV Counts c7_load(const uint32_t* p) {
+    Counts r;
+    r.n_struct = p[0];
+    r.n_terms = p[1];
+    r.n_rows = p[2];
+    r.ord = p[3];
+    r.last_struct = p[4];
+    r.prev_struct = p[5];
+    r.last_term = p[6];
+    r.n_seps = r.n_struct - r.n_terms;
+    r.has_term = r.last_term != 0;
+    r.pad0 = r.pad1 = r.pad2 = 0;
+    return r;
 }
 
-// LB2/LB3 records are words (epoch << 32 | value) written and read with
-// relaxed agent-scope atomics: each word proves by its tag that it belongs
-// to this launch, so no flag and no acquire/release fence (an L2 write-back
-// or invalidate on this chip) is needed. Words of earlier launches carry
-// other tags; the host clears all of them when the epoch counter wraps.
-constexpr int LB2_W = 7;        // n_struct n_terms n_rows ord last_struct prev_struct last_term
-constexpr int LB2_STRIDE = 16;
+DEV void c7_store(uint32_t* p, const Counts& c) {
+    p[0] = c.n_struct;
+    p[1] = c.n_terms;
+    p[2] = c.n_rows;
+    p[3] = c.ord;
+    p[4] = c.last_struct;
+    p[5] = c.prev_struct;
+    p[6] = c.last_term;
+}
 
+// Adds one 64-byte block's structurals S (terminators T, comment-record
+// terminators F) at byte `base` to the running counters.
+DEV void count_block(Counts& wc, uint64_t S, uint64_t T, uint64_t F, uint32_t base) {
+    const uint32_t nS = popc(S);
+    if (!nS) return;
+    const uint64_t seps = S & ~T;
+    wc.n_struct += nS;
+    wc.n_terms += popc(T);
+    wc.n_seps += popc(seps);
+    wc.n_rows += popc(T & ~F);
+    if (T) {
+        const uint32_t hb = hibit(T);
+        wc.has_term = 1;
+        const uint64_t above = hb == 63 ? 0ULL : (seps & ~((2ULL << hb) - 1));
+        wc.ord = popc(above);
+        wc.last_term = base + hb + 1;
+    } else {
+        wc.ord += popc(seps);
+    }
+    const uint32_t h = hibit(S);
+    const uint64_t S2 = S & ~(1ULL << h);
+    wc.prev_struct = S2 ? base + hibit(S2) + 1 : wc.last_struct;
+    wc.last_struct = base + h + 1;
+}
+
+// Tile summary layout (u32 words, stride 8 + 7*NS): [0] tile transfer
+// function, [1..4] the four waves' functions, [8 + 7s ..] counters of the
+// tile entered in state s. Functions pack 4 bits per state.
+constexpr int TS_COUNTS = 8;
+constexpr int NS_MAX = 6;
+
+DEV uint32_t fn_ident_rt(uint32_t ns) {
+    uint32_t h = 0;
+    for (uint32_t s = 0; s < ns; ++s) h |= s << (4 * s);
+    return h;
+}
+
+
+// f then g, for `ns` states.
+DEV uint32_t fn_then_rt(uint32_t f, uint32_t g, uint32_t ns) {
+    uint32_t h = 0;
+    for (uint32_t s = 0; s < ns; ++s) h |= fn_apply(g, fn_apply(f, s)) << (4 * s);
+    return h;
+}
+
+// --------------------------------------------------------------------------
+// string-arena look-back (LB3)
+// --------------------------------------------------------------------------
+// LB3 records are words (epoch << 32 | value) written and read with relaxed
+// agent-scope atomics: each word proves by its tag that it belongs to this
+// launch, so no flag and no acquire/release fence (an L2 write-back or
+// invalidate on this chip) is needed. Words of earlier launches carry other
+// tags; the host clears all of them when the epoch counter wraps.
 DEV unsigned long long tag_word(uint32_t epoch, uint32_t v) { return ((unsigned long long)epoch << 32) | v; }
+
+DEV uint32_t tag_of(unsigned long long w) { return (uint32_t)(w >> 32); }
 
 DEV void lb_store(unsigned long long* p, unsigned long long v) {
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -231,64 +308,6 @@ DEV void lb_store(unsigned long long* p, unsigned long long v) {
 
 DEV unsigned long long lb_load(const unsigned long long* p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Publishes `c` as tile j's aggregate (which = 0) or inclusive prefix (1).
-DEV void lb2_publish(const Params& P, uint32_t j, int which, const Counts& c) {
-    unsigned long long* d = P.lb2 + (size_t)j * LB2_STRIDE + 8 * which;
-    const uint32_t e = P.epoch;
-    lb_store(d + 0, tag_word(e, c.n_struct));
-    lb_store(d + 1, tag_word(e, c.n_terms));
-    lb_store(d + 2, tag_word(e, c.n_rows));
-    lb_store(d + 3, tag_word(e, c.ord));
-    lb_store(d + 4, tag_word(e, c.last_struct));
-    lb_store(d + 5, tag_word(e, c.prev_struct));
-    lb_store(d + 6, tag_word(e, c.last_term));
-}
-
-DEV Counts counts_from_words(const unsigned long long* w) {
-    Counts r;
-    r.n_struct = (uint32_t)w[0];
-    r.n_terms = (uint32_t)w[1];
-    r.n_rows = (uint32_t)w[2];
-    r.ord = (uint32_t)w[3];
-    r.last_struct = (uint32_t)w[4];
-    r.prev_struct = (uint32_t)w[5];
-    r.last_term = (uint32_t)w[6];
-    r.n_seps = r.n_struct - r.n_terms;
-    r.has_term = r.last_term != 0;
-    r.pad0 = r.pad1 = r.pad2 = 0;
-    return r;
-}
-
-DEV Counts shfl_counts(const Counts& c, int src_lane) {
-    Counts r;
-    r.n_struct = __shfl(c.n_struct, src_lane);
-    r.n_seps = __shfl(c.n_seps, src_lane);
-    r.n_terms = __shfl(c.n_terms, src_lane);
-    r.n_rows = __shfl(c.n_rows, src_lane);
-    r.has_term = __shfl(c.has_term, src_lane);
-    r.ord = __shfl(c.ord, src_lane);
-    r.last_struct = __shfl(c.last_struct, src_lane);
-    r.prev_struct = __shfl(c.prev_struct, src_lane);
-    r.last_term = __shfl(c.last_term, src_lane);
-    r.pad0 = r.pad1 = r.pad2 = 0;
-    return r;
-}
-
-DEV Counts shfl_down_counts(const Counts& c, int d) {
-    Counts r;
-    r.n_struct = __shfl_down(c.n_struct, d);
-    r.n_seps = __shfl_down(c.n_seps, d);
-    r.n_terms = __shfl_down(c.n_terms, d);
-    r.n_rows = __shfl_down(c.n_rows, d);
-    r.has_term = __shfl_down(c.has_term, d);
-    r.ord = __shfl_down(c.ord, d);
-    r.last_struct = __shfl_down(c.last_struct, d);
-    r.prev_struct = __shfl_down(c.prev_struct, d);
-    r.last_term = __shfl_down(c.last_term, d);
-    r.pad0 = r.pad1 = r.pad2 = 0;
-    return r;
 }
 
 // Window of the wave-parallel look-back: lane i inspects tile base - i.
@@ -307,8 +326,6 @@ DEV LbWindow lb_window(uint32_t status, bool in_range) {
     return LbWindow{k, (ready & need) == need};
 }
 
-DEV uint32_t tag_of(unsigned long long w) { return (uint32_t)(w >> 32); }
-
 // A look-back whose window stays incomplete for SPIN_LIMIT polls reports its
 // kind and tile and returns, so the kernel always ends.
 DEV void lb_give_up(const Params& P, uint32_t tile, unsigned long long kind) {
@@ -318,94 +335,10 @@ DEV void lb_give_up(const Params& P, uint32_t tile, unsigned long long kind) {
     }
 }
 
-// LB1: entry state of `tile` from the transfer functions of its
-// predecessors. Whole wave; composes 64 tiles per round trip.
-template <int NS>
-DEV uint32_t lookback_fn(const Params& P, uint32_t tile) {
-    const uint32_t ln = __lane_id();
-    const uint32_t epoch = P.epoch & 0x3FFFFFFFu;
-    uint32_t g = fn_ident<NS>();
-    int32_t base = (int32_t)tile - 1;
-    for (uint32_t spins = 0;;) {
-        const int32_t j = base - (int32_t)ln;
-        unsigned long long v = 0;
-        if (j >= 0) v = __hip_atomic_load(&P.lb1[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        const uint32_t status = (uint32_t)((v >> 32) & 0x3FFFFFFFu) == epoch ? (uint32_t)(v >> 62) : 0u;
-        LbWindow W = lb_window(status, j >= 0);
-        if (!W.ok) {
-            if (++spins > SPIN_LIMIT) {
-                lb_give_up(P, tile, 1ULL);
-                return 0;
-            }
-            __builtin_amdgcn_s_sleep(2);
-            continue;
-        }
-        uint32_t f = (ln < W.k && j >= 0) ? (uint32_t)v : fn_ident<NS>();
-#pragma unroll
-        for (int d = 1; d < 64; d <<= 1) {
-            uint32_t hi = __shfl_down(f, d);
-            if ((ln & (2 * d - 1)) == 0 && ln + d < 64) f = fn_then<NS>(hi, f);
-        }
-        const uint32_t T = __shfl(f, 0);
-        if (W.k < 64) return fn_apply(g, fn_apply(T, (uint32_t)__shfl((uint32_t)v, (int)W.k)));
-        g = fn_then<NS>(T, g);
-        base -= 64;
-        spins = 0;
-    }
-}
-
-// LB2: exclusive counter prefix of `tile`. Whole wave. Each lane polls only
-// the last word of its tile's two records; the window's records are read in
-// full once it is complete (and polled again if a word is not yet visible).
-DEV Counts lookback_counts(const Params& P, uint32_t tile) {
-    const uint32_t ln = __lane_id();
-    const uint32_t epoch = P.epoch;
-    Counts acc = counts_zero();
-    int32_t base = (int32_t)tile - 1;
-    for (uint32_t spins = 0;; ++spins) {
-        if (spins > SPIN_LIMIT) {
-            lb_give_up(P, tile, 2ULL);
-            return counts_zero();
-        }
-        const int32_t j = base - (int32_t)ln;
-        const unsigned long long* d = P.lb2 + (size_t)(j >= 0 ? j : 0) * LB2_STRIDE;
-        uint32_t status = 0;
-        if (j >= 0) {
-            const unsigned long long si = lb_load(d + 8 + LB2_W - 1), sa = lb_load(d + LB2_W - 1);
-            status = tag_of(si) == epoch ? 2u : (tag_of(sa) == epoch ? 1u : 0u);
-        }
-        const LbWindow W = lb_window(status, j >= 0);
-        if (!W.ok) {
-            __builtin_amdgcn_s_sleep(2);
-            continue;
-        }
-        unsigned long long wv[LB2_W];
-        bool good = true;
-        const bool take = j >= 0 && ln <= W.k;
-        if (take) {
-            const unsigned long long* r = d + (ln == W.k ? 8 : 0);
-#pragma unroll
-            for (int i = 0; i < LB2_W; ++i) wv[i] = lb_load(r + i);
-#pragma unroll
-            for (int i = 0; i < LB2_W; ++i) good &= tag_of(wv[i]) == epoch;
-        }
-        if (ballot(!good)) continue;
-        Counts c = take ? counts_from_words(wv) : counts_zero();
-#pragma unroll
-        for (int dd = 1; dd < 64; dd <<= 1) {
-            Counts hi = shfl_down_counts(c, dd);
-            if ((ln & (2 * dd - 1)) == 0 && ln + dd < 64) c = compose(hi, c);
-        }
-        acc = compose(shfl_counts(c, 0), acc);
-        if (W.k < 64) return acc;
-        base -= 64;
-        spins = 0;
-    }
-}
-
-// LB3: exclusive arena prefix (sum per string column) into base_out[0..ns),
-// which lane 0 writes (LDS). Whole wave; same polling as LB2 with the last
-// word of each record as the sentinel.
+// Exclusive arena prefix (sum per string column) into base_out[0..ns),
+// which lane 0 writes (LDS). Whole wave. Each lane polls the last word of
+// its tile's two records and reads a window's records in full only once it
+// is complete (and polls again if a word is not yet visible).
 DEV void lookback_str(const Params& P, uint32_t tile, uint32_t ns, uint32_t* base_out) {
     const uint32_t ln = __lane_id();
     const uint32_t epoch = P.epoch;
@@ -536,6 +469,35 @@ DEV T block_reduce_sum(T v, T* sh) {
 struct BM {
     uint64_t cand, nl, q, esc, cm;
 };
+
+
+DEV Counts shfl_down_counts(const Counts& c, int d) {
+    Counts r;
+    r.n_struct = __shfl_down(c.n_struct, d);
+    r.n_seps = __shfl_down(c.n_seps, d);
+    r.n_terms = __shfl_down(c.n_terms, d);
+    r.n_rows = __shfl_down(c.n_rows, d);
+    r.has_term = __shfl_down(c.has_term, d);
+    r.ord = __shfl_down(c.ord, d);
+    r.last_struct = __shfl_down(c.last_struct, d);
+    r.prev_struct = __shfl_down(c.prev_struct, d);
+    r.last_term = __shfl_down(c.last_term, d);
+    r.pad0 = r.pad1 = r.pad2 = 0;
+    return r;
+}
+
+// Counters of lanes 0..n-1 composed in lane order (lane 0 = first bytes);
+// the result is in lane 0. n is a power of two.
+template <int N>
+DEV Counts wave_compose(Counts c) {
+    const uint32_t ln = __lane_id();
+#pragma unroll
+    for (int d = 1; d < N; d <<= 1) {
+        const Counts hi = shfl_down_counts(c, d);
+        if ((ln & (2 * d - 1)) == 0) c = compose(c, hi);
+    }
+    return c;
+}
 
 template <bool HQ, bool ESC, bool CMT>
 struct Dial {
@@ -767,120 +729,135 @@ __global__ __launch_bounds__(WG) void k_scan(Params P) {
         return m;
     };
 
-    // ---- pass 1 + LB1: entry state -----------------------------------------
-    if (NS > 1) {
-        uint32_t fn = fn_ident<NS>();
-        uint64_t ls = ls0;
+    const auto live_mask = [&](uint32_t off) -> uint64_t {
+        return off + 64 <= tile_len ? ~0ULL : (off < tile_len ? ((1ULL << (tile_len - off)) - 1) : 0ULL);
+    };
+
+    // ---- classification: ballots per block, then lane b owns block b ------
+    // Each wave classifies its BPW blocks with one byte per lane and
+    // __ballot, and hands block b's masks to lane b: the mask algebra, the
+    // counters and the compaction then run for all blocks at once, one per
+    // lane, instead of block by block on the scalar unit.
+    BM mine{0ULL, 0ULL, 0ULL, 0ULL, 0ULL};
 #pragma unroll 4
-        for (int b = 0; b < BPW; ++b) {
-            BM m = classify(w * BPW + b);
-            fn = fn_then<NS>(fn, block_fn<HQ, ESC, CMT>(m, ls));
-            ls = m.nl >> 63;
-        }
-        if (ln == 0) sh.wfn[w] = fn;
-        __syncthreads();
-        if (w == 0) {
-            uint32_t agg = sh.wfn[0];
-            for (int k = 1; k < NWAVE; ++k) agg = fn_then<NS>(agg, sh.wfn[k]);
-            uint32_t entry = 0;
-            if (tile > 0) {
-                if (ln == 0)
-                    __hip_atomic_store(&P.lb1[tile], lb1_word(1, P.epoch, agg), __ATOMIC_RELAXED,
-                                       __HIP_MEMORY_SCOPE_AGENT);
-                entry = lookback_fn<NS>(P, tile);
-            }
-            if (ln == 0) {
-                __hip_atomic_store(&P.lb1[tile], lb1_word(2, P.epoch, fn_apply(agg, entry)), __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_AGENT);
-                uint32_t s = entry;
-                for (int k = 0; k < NWAVE; ++k) { sh.wentry[k] = s; s = fn_apply(sh.wfn[k], s); }
-            }
-        }
-        __syncthreads();
-    } else {
-        if (tid < NWAVE) sh.wentry[tid] = 0;
-        __syncthreads();
+    for (int b = 0; b < BPW; ++b) {
+        const BM m = classify(w * BPW + b);
+        const bool me = ln == (uint32_t)b;
+        mine.cand = me ? m.cand : mine.cand;
+        mine.nl = me ? m.nl : mine.nl;
+        if (HQ) mine.q = me ? m.q : mine.q;
+        if (ESC) mine.esc = me ? m.esc : mine.esc;
+        if (CMT) mine.cm = me ? m.cm : mine.cm;
     }
-
-    // ---- pass 2: masks, compaction, counters -------------------------------
-    {
-        uint32_t st = uni(sh.wentry[w]);
-        uint64_t ls = ls0;
-        Counts wc = counts_zero();
-        uint32_t cnt = 0;
-        const uint32_t seg = w * BPW * 64;
-#pragma unroll 2
-        for (int b = 0; b < BPW; ++b) {
-            const uint32_t blk = w * BPW + b;
-            const uint32_t off = blk * 64;
-            BM m = classify(blk);
-            StepOut o = block_step<HQ, ESC, CMT>(m, st, ls);
-            ls = m.nl >> 63;
-            st = o.s_out;
-            uint64_t live = off + 64 <= tile_len ? ~0ULL : (off < tile_len ? ((1ULL << (tile_len - off)) - 1) : 0ULL);
-            uint64_t S = o.S & live, T = o.T & live, F = o.F & live;
-            if ((S >> ln) & 1ULL) {
-                uint16_t e = (uint16_t)((off + ln) | (((T >> ln) & 1ULL) << 14) | (((F >> ln) & 1ULL) << 15));
-                sh.list[seg + cnt + mbcnt(S)] = e;
-            }
-            uint32_t nS = popc(S);
-            cnt += nS;
-            if (nS) {
-                const uint32_t base = tile_start + off;
-                uint64_t seps = S & ~T;
-                wc.n_struct += nS;
-                wc.n_terms += popc(T);
-                wc.n_seps += popc(seps);
-                wc.n_rows += popc(T & ~F);
-                if (T) {
-                    uint32_t hb = hibit(T);
-                    wc.has_term = 1;
-                    uint64_t above = hb == 63 ? 0ULL : (seps & ~((2ULL << hb) - 1));
-                    wc.ord = popc(above);
-                    wc.last_term = base + hb + 1;
-                } else {
-                    wc.ord += popc(seps);
-                }
-                uint32_t h = hibit(S);
-                uint64_t S2 = S & ~(1ULL << h);
-                wc.prev_struct = S2 ? base + hibit(S2) + 1 : wc.last_struct;
-                wc.last_struct = base + h + 1;
-            }
-        }
-        if (ln == 0) { sh.wcnt[w] = wc; sh.wcount[w] = cnt; }
+    const bool own = ln < (uint32_t)BPW;
+    const uint32_t off = (w * BPW + (own ? ln : 0u)) * 64;      // this lane's block in the tile
+    const uint64_t live = own ? live_mask(off) : 0ULL;
+    // Line-start carry: the previous block ends in '\n' (ls0 for the first).
+    const uint32_t prev_hi = __shfl_up((uint32_t)(mine.nl >> 32), 1);
+    const uint64_t ls = ln == 0 ? ls0 : (uint64_t)(prev_hi >> 31);
+    // Exclusive scan of the blocks' transfer functions over the lanes.
+    uint32_t f_incl = own ? block_fn<HQ, ESC, CMT>(mine, ls) : fn_ident<NS>();
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t g = __shfl_up(f_incl, d);
+        if (ln >= (uint32_t)d) f_incl = fn_then<NS>(g, f_incl);
     }
-    __syncthreads();
+    uint32_t f_excl = __shfl_up(f_incl, 1);
+    if (ln == 0) f_excl = fn_ident<NS>();
 
-    if (MODE == M_TERMS) {
+    // ---- summary mode: every entry state's trajectory through the tile ----
+    if (MODE == M_SUMMARY) {
+        Counts* wsum = reinterpret_cast<Counts*>(sh.list);   // the list is unused here
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+            Counts c = counts_zero();
+            if (own) {
+                const StepOut o = block_step<HQ, ESC, CMT>(mine, fn_apply(f_excl, (uint32_t)s), ls);
+                count_block(c, o.S & live, o.T & live, o.F & live, tile_start + off);
+            }
+            c = wave_compose<BPW>(c);
+            if (ln == 0) wsum[w * NS + s] = c;
+        }
+        if (ln == 63) sh.wfn[w] = f_incl;
+        __syncthreads();
+        uint32_t* out = P.tsum + (size_t)tile * P.ts_stride;
+        if (tid < (uint32_t)NS) {
+            uint32_t s = tid;
+            Counts c = counts_zero();
+            for (int k = 0; k < NWAVE; ++k) {
+                c = compose(c, wsum[k * NS + s]);
+                s = fn_apply(sh.wfn[k], s);
+            }
+            c7_store(out + TS_COUNTS + C7 * tid, c);
+        }
         if (tid == 0) {
-            unsigned long long t = 0;
-            for (int k = 0; k < NWAVE; ++k) t += sh.wcnt[k].n_terms;
-            reinterpret_cast<unsigned long long*>(P.partials)[tile] = t;
+            uint32_t f = fn_ident<NS>();
+            for (int k = 0; k < NWAVE; ++k) {
+                f = fn_then<NS>(f, sh.wfn[k]);
+                out[1 + k] = sh.wfn[k];
+            }
+            out[0] = f;
         }
         return;
     }
 
-    // ---- LB2: counters prefix ---------------------------------------------
-    if (w == 0) {
+    // ---- entry states: tile prefix from the scan, then through the waves --
+    if (tid == 0) {
+        const uint32_t* tp = P.tpre + (size_t)tile * 8;
+        const uint32_t* ts = P.tsum + (size_t)tile * P.ts_stride;
+        uint32_t s = tp[0];
+        for (int k = 0; k < NWAVE; ++k) {
+            sh.wentry[k] = s;
+            s = fn_apply(ts[1 + k], s);
+        }
+        sh.pre = c7_load(tp + 1);
+    }
+    __syncthreads();
+
+    // ---- pass 2: masks, compaction, counters (one block per lane) ---------
+    {
+        const uint32_t entry = fn_apply(f_excl, uni(sh.wentry[w]));
+        uint64_t S = 0, T = 0, F = 0;
+        if (own) {
+            const StepOut o = block_step<HQ, ESC, CMT>(mine, entry, ls);
+            S = o.S & live;
+            T = o.T & live;
+            F = o.F & live;
+        }
+        const uint32_t n = popc(S);
+        uint32_t incl = n;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const uint32_t t = __shfl_up(incl, d);
+            if (ln >= (uint32_t)d) incl += t;
+        }
+        const uint32_t cnt = __shfl(incl, 63);
+        // This lane's structurals, in byte order, at its exclusive offset.
+        uint16_t* lst = sh.list + w * BPW * 64 + (incl - n);
+        for (uint64_t rem = S; rem; rem &= rem - 1) {
+            const uint32_t bit = (uint32_t)__ffsll((long long)rem) - 1u;
+            *lst++ = (uint16_t)((off + bit) | (uint32_t)(((T >> bit) & 1ULL) << 14) |
+                                (uint32_t)(((F >> bit) & 1ULL) << 15));
+        }
+        Counts wc = counts_zero();
+        count_block(wc, S, T, F, tile_start + off);
+        wc = wave_compose<BPW>(wc);
+        if (ln == 0) { sh.wcnt[w] = wc; sh.wcount[w] = cnt; }
+    }
+    __syncthreads();
+
+    // ---- tile counters: exclusive prefix (from the scan) and inclusive ------
+    if (tid == 0) {
         Counts agg = sh.wcnt[0];
         for (int k = 1; k < NWAVE; ++k) agg = compose(agg, sh.wcnt[k]);
-        Counts pre = counts_zero();
-        if (tile > 0) {
-            if (ln == 0) lb2_publish(P, tile, 0, agg);
-            pre = lookback_counts(P, tile);
-        }
-        if (ln == 0) {
-            Counts inc = compose(pre, agg);
-            lb2_publish(P, tile, 1, inc);
-            sh.pre = pre;
-            sh.inc = inc;
-            sh.wbase[0] = 0;
-            for (int k = 0; k < NWAVE; ++k) sh.wbase[k + 1] = sh.wbase[k] + sh.wcount[k];
-            if (last_tile) {
-                P.result[R_N_STRUCT] = inc.n_struct;
-                P.result[R_N_SEPS] = inc.n_seps;
-                P.result[R_N_TERMS] = inc.n_terms;
-            }
+        const Counts inc = compose(sh.pre, agg);
+        sh.inc = inc;
+        sh.wbase[0] = 0;
+        for (int k = 0; k < NWAVE; ++k) sh.wbase[k + 1] = sh.wbase[k] + sh.wcount[k];
+        if (last_tile) {
+            P.result[R_N_STRUCT] = inc.n_struct;
+            P.result[R_N_SEPS] = inc.n_seps;
+            P.result[R_N_TERMS] = inc.n_terms;
         }
     }
     __syncthreads();
@@ -1282,6 +1259,132 @@ __global__ __launch_bounds__(WG) void k_scan(Params P) {
 // --------------------------------------------------------------------------
 __global__ __launch_bounds__(WG) void k_fill_u64(unsigned long long* p, uint64_t n, unsigned long long v) {
     for (uint64_t i = (uint64_t)blockIdx.x * WG + threadIdx.x; i < n; i += (uint64_t)gridDim.x * WG) p[i] = v;
+}
+
+// --------------------------------------------------------------------------
+// tile-prefix scan: tile summaries -> each tile's entry state and exclusive
+// counters. Summaries compose like the reference's chunk transfer functions
+// (csv_hash.rs:184-191): (A then B).fn = B.fn o A.fn and
+// (A then B).counts[s] = A.counts[s] then B.counts[A.fn(s)].
+// --------------------------------------------------------------------------
+constexpr uint32_t SG = 128;   // tiles per scan group
+
+// Loads summary `i` (fn + ns counters) into LDS slots, or the identity.
+DEV void sum_to_lds(const uint32_t* src, bool have, uint32_t ns, uint32_t* fn, uint32_t* cs) {
+    *fn = have ? src[0] : fn_ident_rt(ns);
+    for (uint32_t k = 0; k < ns * C7; ++k) cs[k] = have ? src[TS_COUNTS + k] : 0u;
+}
+
+// Level 1: one summary per group of SG tiles (tree reduction in LDS).
+__global__ __launch_bounds__(SG) void k_scan_reduce(const uint32_t* ts, uint32_t stride, uint32_t n_tiles,
+                                                     uint32_t ns, uint32_t* gs) {
+    __shared__ uint32_t fn[SG];
+    __shared__ uint32_t cs[SG][NS_MAX * C7];
+    const uint32_t t = threadIdx.x, tile = blockIdx.x * SG + t;
+    sum_to_lds(ts + (size_t)tile * stride, tile < n_tiles, ns, &fn[t], cs[t]);
+    __syncthreads();
+    for (uint32_t d = 1; d < SG; d <<= 1) {
+        if ((t & (2 * d - 1)) == 0) {
+            const uint32_t fa = fn[t], fb = fn[t + d];
+            Counts out[NS_MAX];
+#pragma unroll
+            for (uint32_t s = 0; s < (uint32_t)NS_MAX; ++s)
+                if (s < ns) out[s] = compose(c7_load(&cs[t][C7 * s]), c7_load(&cs[t + d][C7 * fn_apply(fa, s)]));
+#pragma unroll
+            for (uint32_t s = 0; s < (uint32_t)NS_MAX; ++s)
+                if (s < ns) c7_store(&cs[t][C7 * s], out[s]);
+            fn[t] = fn_then_rt(fa, fb, ns);
+        }
+        __syncthreads();
+    }
+    if (t < ns * C7 + 1) {
+        uint32_t* dst = gs + (size_t)blockIdx.x * stride;
+        if (t == 0) dst[0] = fn[0];
+        else dst[TS_COUNTS + t - 1] = cs[0][t - 1];
+    }
+}
+
+// Level 2: one wave over the group summaries. Each lane folds a run of
+// groups, lane 0 chains the lanes from state 0, and each lane then walks its
+// groups with the concrete state, writing every group's entry (state + 7
+// counter words). The grand totals go to the result slots.
+__global__ __launch_bounds__(64) void k_scan_groups(const uint32_t* gs, uint32_t stride, uint32_t n_groups,
+                                                     uint32_t ns, uint32_t* gpre, unsigned long long* result) {
+    __shared__ uint32_t lfn[64];
+    __shared__ uint32_t lcs[64][NS_MAX * C7];
+    __shared__ uint32_t lent[65][8];
+    const uint32_t l = threadIdx.x;
+    const uint32_t per = (n_groups + 63) / 64;
+    const uint32_t g0 = min(n_groups, l * per), g1 = min(n_groups, g0 + per);
+    uint32_t f = fn_ident_rt(ns);
+    Counts c[NS_MAX];
+#pragma unroll
+    for (int s = 0; s < NS_MAX; ++s) c[s] = counts_zero();
+    for (uint32_t g = g0; g < g1; ++g) {
+        const uint32_t* p = gs + (size_t)g * stride;
+#pragma unroll
+        for (uint32_t s = 0; s < (uint32_t)NS_MAX; ++s)
+            if (s < ns) c[s] = compose(c[s], c7_load(p + TS_COUNTS + C7 * fn_apply(f, s)));
+        f = fn_then_rt(f, p[0], ns);
+    }
+    lfn[l] = f;
+#pragma unroll
+    for (uint32_t s = 0; s < (uint32_t)NS_MAX; ++s)
+        if (s < ns) c7_store(&lcs[l][C7 * s], c[s]);
+    __syncthreads();
+    if (l == 0) {
+        uint32_t s = 0;
+        Counts C = counts_zero();
+        for (uint32_t k = 0; k < 64; ++k) {
+            lent[k][0] = s;
+            c7_store(&lent[k][1], C);
+            C = compose(C, c7_load(&lcs[k][C7 * s]));
+            s = fn_apply(lfn[k], s);
+        }
+        lent[64][0] = s;
+        c7_store(&lent[64][1], C);
+        result[R_N_STRUCT] = C.n_struct;
+        result[R_N_SEPS] = C.n_seps;
+        result[R_N_TERMS] = C.n_terms;
+        result[R_N_ROWS] = C.n_rows;
+    }
+    __syncthreads();
+    uint32_t s = lent[l][0];
+    Counts C = c7_load(&lent[l][1]);
+    for (uint32_t g = g0; g < g1; ++g) {
+        const uint32_t* p = gs + (size_t)g * stride;
+        uint32_t* o = gpre + (size_t)g * 8;
+        o[0] = s;
+        c7_store(o + 1, C);
+        C = compose(C, c7_load(p + TS_COUNTS + C7 * s));
+        s = fn_apply(p[0], s);
+    }
+}
+
+// Level 3: per group, thread 0 walks its SG tiles from the group's entry.
+__global__ __launch_bounds__(SG) void k_scan_tiles(const uint32_t* ts, uint32_t stride, uint32_t n_tiles,
+                                                    uint32_t ns, const uint32_t* gpre, uint32_t* tpre) {
+    __shared__ uint32_t fn[SG];
+    __shared__ uint32_t cs[SG][NS_MAX * C7];
+    __shared__ uint32_t out[SG][8];
+    const uint32_t t = threadIdx.x, tile = blockIdx.x * SG + t;
+    const bool have = tile < n_tiles;
+    sum_to_lds(ts + (size_t)tile * stride, have, ns, &fn[t], cs[t]);
+    __syncthreads();
+    if (t == 0) {
+        const uint32_t* g = gpre + (size_t)blockIdx.x * 8;
+        uint32_t s = g[0];
+        Counts C = c7_load(g + 1);
+        for (uint32_t k = 0; k < SG; ++k) {
+            out[k][0] = s;
+            c7_store(&out[k][1], C);
+            C = compose(C, c7_load(&cs[k][C7 * s]));
+            s = fn_apply(fn[k], s);
+        }
+    }
+    __syncthreads();
+    if (have)
+        for (int i = 0; i < 8; ++i) tpre[(size_t)tile * 8 + i] = out[t][i];
 }
 
 // Deterministic two-level reductions over per-tile partials: level 1 folds
