@@ -384,7 +384,7 @@ DEV void lookback_str(const Params& P, uint32_t tile, uint32_t ns, uint32_t* bas
 // block-wide scans over 256 threads
 // --------------------------------------------------------------------------
 struct Scan3 {
-    uint32_t a, b;
+    uint32_t a, b, d;
     int32_t c;  // max
 };
 
@@ -394,13 +394,14 @@ DEV Scan3 wave_incl(Scan3 v) {
     for (int d = 1; d < 64; d <<= 1) {
         uint32_t ta = __shfl_up(v.a, d);
         uint32_t tb = __shfl_up(v.b, d);
+        uint32_t td = __shfl_up(v.d, d);
         int32_t tc = __shfl_up(v.c, d);
-        if (ln >= (uint32_t)d) { v.a += ta; v.b += tb; v.c = tc > v.c ? tc : v.c; }
+        if (ln >= (uint32_t)d) { v.a += ta; v.b += tb; v.d += td; v.c = tc > v.c ? tc : v.c; }
     }
     return v;
 }
 
-// Exclusive scan (sum a, sum b, max c) across the workgroup; `tot` gets the
+// Exclusive scan (sum a, b, d; max c) across the workgroup; `tot` gets the
 // inclusive total. `sh` is 4-wave scratch.
 DEV Scan3 block_excl(Scan3 v, Scan3* sh, Scan3& tot) {
     const uint32_t ln = __lane_id(), w = uni(threadIdx.x >> 6);
@@ -408,25 +409,29 @@ DEV Scan3 block_excl(Scan3 v, Scan3* sh, Scan3& tot) {
     Scan3 ex;
     ex.a = __shfl_up(inc.a, 1);
     ex.b = __shfl_up(inc.b, 1);
+    ex.d = __shfl_up(inc.d, 1);
     ex.c = __shfl_up(inc.c, 1);
-    if (ln == 0) { ex.a = 0; ex.b = 0; ex.c = -1; }
+    if (ln == 0) { ex.a = 0; ex.b = 0; ex.d = 0; ex.c = -1; }
     if (ln == 63) sh[w] = inc;
     __syncthreads();
-    Scan3 base{0, 0, -1};
+    Scan3 base{0, 0, 0, -1};
     for (uint32_t k = 0; k < w; ++k) {
         base.a += sh[k].a;
         base.b += sh[k].b;
+        base.d += sh[k].d;
         base.c = sh[k].c > base.c ? sh[k].c : base.c;
     }
-    tot = Scan3{0, 0, -1};
+    tot = Scan3{0, 0, 0, -1};
     for (int k = 0; k < NWAVE; ++k) {
         tot.a += sh[k].a;
         tot.b += sh[k].b;
+        tot.d += sh[k].d;
         tot.c = sh[k].c > tot.c ? sh[k].c : tot.c;
     }
     __syncthreads();
     ex.a += base.a;
     ex.b += base.b;
+    ex.d += base.d;
     ex.c = base.c > ex.c ? base.c : ex.c;
     return ex;
 }
@@ -658,18 +663,24 @@ struct Shared {
     SlowF64 slow;                         // exact path for undecided f64 cells
 };
 
-DEV uint32_t list_at(const Shared& sh, uint32_t k) {
-    uint32_t w = 0;
-#pragma unroll
-    for (int i = 1; i < NWAVE; ++i) w += k >= sh.wbase[i];
-    return sh.list[w * BPW * 64 + (k - sh.wbase[w])];
-}
+DEV uint32_t list_at(const Shared& sh, uint32_t k) { return sh.list[k]; }
 
+// One list entry with its record context.
 struct Entry {
     bool valid, is_term, flag;
-    uint32_t pos, fs, rs, ord, row, k;
-    int32_t lt;
+    uint32_t pos;   // byte offset of the structural (len for the virtual terminator)
+    uint32_t fs;    // first byte of the field it ends
+    uint32_t rs;    // first byte of its record
+    uint32_t ord;   // field ordinal within the record
+    uint32_t row;   // row of its record
+    uint32_t k;     // index in the tile's list
 };
+
+constexpr int ITEMS = 4;   // consecutive list entries per thread in a walk
+
+// Walk kinds: plain, or with a per-entry weight scanned before the visits.
+struct NoWeight { static constexpr bool value = false; };
+struct Weighted { static constexpr bool value = true; };
 
 template <bool HQ, bool ESC, bool CMT, int MODE>
 __global__ __launch_bounds__(WG) void k_scan(Params P) {
@@ -832,17 +843,21 @@ __global__ __launch_bounds__(WG) void k_scan(Params P) {
             if (ln >= (uint32_t)d) incl += t;
         }
         const uint32_t cnt = __shfl(incl, 63);
-        // This lane's structurals, in byte order, at its exclusive offset.
-        uint16_t* lst = sh.list + w * BPW * 64 + (incl - n);
+        Counts wc = counts_zero();
+        count_block(wc, S, T, F, tile_start + off);
+        wc = wave_compose<BPW>(wc);
+        if (ln == 0) { sh.wcnt[w] = wc; sh.wcount[w] = cnt; }
+        __syncthreads();
+        // One list for the tile, in byte order: this lane's structurals go
+        // after the earlier waves' and the earlier lanes' entries.
+        uint32_t wb = 0;
+        for (uint32_t k = 0; k < w; ++k) wb += sh.wcount[k];
+        uint16_t* lst = sh.list + uni(wb) + (incl - n);
         for (uint64_t rem = S; rem; rem &= rem - 1) {
             const uint32_t bit = (uint32_t)__ffsll((long long)rem) - 1u;
             *lst++ = (uint16_t)((off + bit) | (uint32_t)(((T >> bit) & 1ULL) << 14) |
                                 (uint32_t)(((F >> bit) & 1ULL) << 15));
         }
-        Counts wc = counts_zero();
-        count_block(wc, S, T, F, tile_start + off);
-        wc = wave_compose<BPW>(wc);
-        if (ln == 0) { sh.wcnt[w] = wc; sh.wcount[w] = cnt; }
     }
     __syncthreads();
 
@@ -876,37 +891,100 @@ __global__ __launch_bounds__(WG) void k_scan(Params P) {
     TileSrc src{P.data, sh.bytes, tile_start, tile_start + tile_len};
     const CellDialect cd{P.quote, HQ ? P.escape : -1};
 
-    // Walk the compacted list in chunks of 256 with record context.
+    // Walk the list with record context: each thread takes ITEMS
+    // consecutive entries (WG*ITEMS per round); one block scan per round
+    // gives every thread the terminators, rows and last terminator before
+    // its first entry. Then all threads visit their entries in step:
+    // visit(e, terms_before, weight_before) with counts over all earlier
+    // entries of the tile. A weighted walk (WEIGHTED) first evaluates
+    // weight(e) with context and scans those sums too, so each visit knows
+    // the weight of everything before it (string arena offsets).
     // `vword` != 0 appends one virtual entry after the list: the terminator
     // at `len` that closes an unterminated last record (last tile only).
-    auto walk = [&](auto&& visit, uint32_t vword = 0u) __attribute__((always_inline)) {
-        uint32_t terms_before = 0, rows_before = 0;
+    auto walk_w = [&](auto weighted, auto&& weight, auto&& visit, uint32_t vword) __attribute__((always_inline)) {
+        constexpr bool WEIGHTED = decltype(weighted)::value;
+        uint32_t terms_before = 0, rows_before = 0, w_before = 0;
         int32_t lt_before = -1;
         const uint32_t n_walk = n_list + (vword ? 1u : 0u);
-        for (uint32_t base = 0; base < n_walk; base += WG) {
+        const auto word_at = [&](uint32_t k) -> uint32_t { return k < n_list ? (uint32_t)sh.list[k] : vword; };
+        // Entry k without record context; `fs` is the end of entry k-1 plus 1.
+        const auto entry_at = [&](uint32_t k, uint32_t fs) -> Entry {
             Entry e;
-            e.k = base + tid;
-            e.valid = e.k < n_walk;
-            uint16_t v = !e.valid ? 0 : (e.k < n_list ? list_at(sh, e.k) : (uint16_t)vword);
+            e.k = k;
+            e.valid = k < n_walk;
+            const uint32_t v = e.valid ? word_at(k) : 0u;
             e.is_term = e.valid && (v & 0x4000u);
             e.flag = e.valid && (v & 0x8000u);
             e.pos = tile_start + (v & 0x3FFFu);
-            Scan3 in{e.is_term ? 1u : 0u, (e.is_term && !e.flag) ? 1u : 0u, e.is_term ? (int32_t)e.k : -1};
-            Scan3 tot;
-            Scan3 ex = block_excl(in, sh.scan_sh, tot);
-            int32_t lt = ex.c > lt_before ? ex.c : lt_before;
-            e.lt = lt;
-            if (e.valid) {
-                e.fs = e.k > 0 ? tile_start + (list_at(sh, e.k - 1) & 0x3FFFu) + 1 : pre.last_struct;
-                e.rs = lt >= 0 ? tile_start + (list_at(sh, (uint32_t)lt) & 0x3FFFu) + 1 : pre.last_term;
-                e.ord = lt >= 0 ? (e.k - (uint32_t)lt - 1) : (pre.ord + e.k);
-                e.row = pre.n_rows + rows_before + ex.b;
+            e.fs = fs;
+            e.rs = 0;
+            e.ord = 0;
+            e.row = 0;
+            return e;
+        };
+        for (uint32_t base = 0; base < n_walk; base += WG * ITEMS) {
+            const uint32_t k0 = base + tid * ITEMS;
+            const uint32_t fs0 = k0 == 0 ? pre.last_struct
+                                         : (k0 - 1 < n_walk ? tile_start + (word_at(k0 - 1) & 0x3FFFu) + 1 : 0u);
+            // This thread's terminators, rows and last terminator.
+            Scan3 agg{0, 0, 0, -1};
+            uint32_t fs = fs0;
+            for (int i = 0; i < ITEMS; ++i) {
+                const Entry e = entry_at(k0 + i, fs);
+                fs = e.pos + 1;
+                if (e.is_term) {
+                    agg.a += 1;
+                    agg.b += e.flag ? 0u : 1u;
+                    agg.c = (int32_t)e.k;
+                }
             }
-            visit(e, terms_before + ex.a);
+            Scan3 tot;
+            const Scan3 ex = block_excl(agg, sh.scan_sh, tot);
+            const int32_t lt0 = ex.c > lt_before ? ex.c : lt_before;
+            const uint32_t rs0 = lt0 >= 0 ? tile_start + (word_at((uint32_t)lt0) & 0x3FFFu) + 1 : pre.last_term;
+            const uint32_t row0_t = pre.n_rows + rows_before + ex.b;
+            // The thread's entries with record context, in order.
+            const auto each = [&](auto&& f) {
+                int32_t lt = lt0;
+                uint32_t rs = rs0, row = row0_t, terms = terms_before + ex.a, fs1 = fs0;
+                for (int i = 0; i < ITEMS; ++i) {
+                    Entry e = entry_at(k0 + i, fs1);
+                    fs1 = e.pos + 1;
+                    e.rs = rs;
+                    e.ord = lt >= 0 ? (e.k - (uint32_t)lt - 1) : (pre.ord + e.k);
+                    e.row = row;
+                    f(e, terms);
+                    if (e.is_term) {
+                        lt = (int32_t)e.k;
+                        rs = e.pos + 1;
+                        terms += 1;
+                        if (!e.flag) row += 1;
+                    }
+                }
+            };
+            uint32_t wsum = 0;
+            if (WEIGHTED) {
+                uint32_t mine = 0;
+                each([&](const Entry& e, uint32_t) { if (e.valid) mine += weight(e); });
+                uint32_t wt;
+                wsum = w_before + block_excl_sum(mine, sh.sum_sh, wt);
+                w_before += wt;
+            }
+            each([&](const Entry& e, uint32_t terms) {
+                visit(e, terms, wsum);
+                if (WEIGHTED && e.valid) wsum += weight(e);
+            });
             terms_before += tot.a;
             rows_before += tot.b;
             lt_before = tot.c > lt_before ? tot.c : lt_before;
         }
+    };
+    auto walk = [&](auto&& visit, uint32_t vword = 0u) __attribute__((always_inline)) {
+        walk_w(NoWeight{}, [](const Entry&) { return 0u; }, visit, vword);
+    };
+    // Field end without the record's trailing '\r' (csv_geo.rs:1056-1060).
+    auto field_end = [&](const Entry& e) -> uint32_t {
+        return (e.is_term && e.pos > e.rs && src.at(e.pos - 1) == '\r') ? e.pos - 1 : e.pos;
     };
 
     const Counts inc = sh.inc;
@@ -917,7 +995,7 @@ __global__ __launch_bounds__(WG) void k_scan(Params P) {
         trailer ? ((P.len - tile_start) | 0x4000u | (trailer_comment ? 0x8000u : 0u)) : 0u;
 
     if (MODE == M_TAPE) {
-        walk([&](const Entry& e, uint32_t terms_excl) {
+        walk([&](const Entry& e, uint32_t terms_excl, uint32_t) {
             if (!e.valid) return;
             uint32_t seps_before = e.k - terms_excl;
             if (e.is_term) {
@@ -934,12 +1012,10 @@ __global__ __launch_bounds__(WG) void k_scan(Params P) {
 
     if (MODE == M_FIELDBYTES) {
         unsigned long long fields = 0, bytes = 0;
-        walk([&](const Entry& e, uint32_t) {
+        walk([&](const Entry& e, uint32_t, uint32_t) {
             if (!e.valid || e.flag) return;
-            uint32_t to = e.pos;
-            if (e.is_term && e.pos > e.rs && src.at(e.pos - 1) == '\r') to = e.pos - 1;
             fields += 1;
-            bytes += field_byte_len(src, e.fs, to, P.quote);
+            bytes += field_byte_len(src, e.fs, field_end(e), P.quote);
         });
         if (tid == 0 && trailer && !trailer_comment) {
             uint32_t to = (P.len > inc.last_term && P.data[P.len - 1] == '\r') ? P.len - 1 : P.len;
@@ -978,10 +1054,9 @@ __global__ __launch_bounds__(WG) void k_scan(Params P) {
                 vbytes += 1;
             }
         };
-        walk([&](const Entry& e, uint32_t) {
+        walk([&](const Entry& e, uint32_t, uint32_t) {
             if (!e.valid || (e.ord & 1u) == 0) return;
-            uint32_t to = e.pos;
-            if (e.is_term && e.pos > e.rs && src.at(e.pos - 1) == '\r') to = e.pos - 1;
+            const uint32_t to = field_end(e);
             uint32_t prev1 = e.fs - 1;  // structural that ended the key field
             uint32_t ks;
             if (e.ord == 1) ks = e.rs;
@@ -1073,13 +1148,10 @@ __global__ __launch_bounds__(WG) void k_scan(Params P) {
 #pragma unroll
         for (int j = 0; j < STAT_GROUP; ++j) acc[j] = Acc{0, 0, 0, 0, 0, 0.0};
         unsigned long long records = 0;
-        walk([&](const Entry& e, uint32_t) {
+        walk([&](const Entry& e, uint32_t, uint32_t) {
             if (!e.valid || e.flag) return;
-            uint32_t to = e.pos;
-            if (e.is_term) {
-                if (e.pos > e.rs && src.at(e.pos - 1) == '\r') to = e.pos - 1;
-                records += 1;
-            }
+            const uint32_t to = field_end(e);
+            if (e.is_term) records += 1;
             for (uint32_t j = 0; j < nc; ++j) {
                 const ColDesc& c = P.cols[c0 + j];
                 if (c.index != e.ord) continue;
@@ -1112,24 +1184,46 @@ __global__ __launch_bounds__(WG) void k_scan(Params P) {
     // ---- M_COLUMNS --------------------------------------------------------
     if (MODE == M_COLUMNS) {
         const uint64_t rows_cap = P.rows_cap;
-        // Validity bits, one wave at a time. Every non-comment record has
-        // exactly one event per column (its cell, or the terminator of a
-        // record too short to have it), and events appear in byte order, so
-        // the event lanes of a wave hold consecutive rows: their bits form
-        // one run that costs at most two atomics instead of one per cell.
-        // Called by every lane of the wave.
-        auto flush_valid = [&](const ColDesc& c, bool event, bool vbit, uint32_t row) {
-            const uint64_t E = ballot(event);
-            if (E == 0) return;
-            const uint32_t first = (uint32_t)__ffsll((long long)E) - 1u;
-            const uint32_t r0 = __shfl(row, (int)first);
-            unsigned long long bits = (event && vbit) ? (1ULL << mbcnt(E)) : 0ULL;
-#pragma unroll
-            for (int d = 32; d >= 1; d >>= 1) bits |= __shfl_xor(bits, d);
-            if (ln == first && bits) {
-                const uint32_t sh0 = r0 & 63u;
-                atomicOr(&c.valid[r0 >> 6], bits << sh0);
-                if (sh0 && (bits >> (64 - sh0))) atomicOr(&c.valid[(r0 >> 6) + 1], bits >> (64 - sh0));
+        const uint32_t n_cols = P.n_cols;
+        const bool trailer_row_live = trailer && !trailer_comment;
+        // Validity bits of the rows this tile's cells can touch, [pre.n_rows,
+        // inc.n_rows], staged in LDS (bit i of column j's region is row
+        // row0 + i) and written out once: words wholly inside the tile with
+        // plain stores, the two boundary words (shared with the neighbouring
+        // tiles' records) with atomics. Tiles whose span does not fit the
+        // stage use global atomics per cell.
+        constexpr uint32_t VSTAGE = 512;
+        __shared__ uint32_t vbits[VSTAGE];
+        const uint32_t row0 = pre.n_rows & ~63u;
+        const uint32_t words64 = (inc.n_rows - row0) / 64 + 1;
+        const uint32_t wpc = 2 * words64;
+        const bool staged = n_cols * wpc <= VSTAGE;
+        const uint64_t vwords = (rows_cap + 63) / 64;
+        for (uint32_t i = tid; i < VSTAGE; i += WG) vbits[i] = 0;
+        auto set_valid = [&](uint32_t j, const ColDesc& c, uint32_t row) {
+            if (staged) {
+                const uint32_t r = row - row0;
+                atomicOr(&vbits[j * wpc + (r >> 5)], 1u << (r & 31));
+            } else {
+                atomicOr(&c.valid[row >> 6], 1ULL << (row & 63));
+            }
+        };
+        auto flush_valid = [&](bool strings) {
+            if (!staged) return;
+            for (uint32_t j = 0; j < n_cols; ++j) {
+                const ColDesc& c = P.cols[j];
+                if ((c.type == FALX_COL_STR) != strings) continue;
+                for (uint32_t wi = tid; wi < words64; wi += WG) {
+                    const uint64_t gw = (uint64_t)(row0 >> 6) + wi;
+                    if (gw >= vwords) continue;
+                    const unsigned long long v =
+                        vbits[j * wpc + 2 * wi] | ((unsigned long long)vbits[j * wpc + 2 * wi + 1] << 32);
+                    if (wi == 0 || wi == words64 - 1) {
+                        if (v) atomicOr(&c.valid[gw], v);
+                    } else {
+                        c.valid[gw] = v;
+                    }
+                }
             }
         };
         // Optional fused statistics of the first STAT_GROUP columns,
@@ -1140,66 +1234,62 @@ __global__ __launch_bounds__(WG) void k_scan(Params P) {
         unsigned long long records = 0;
         const bool with_stats = P.partials != nullptr;
         const bool have_str = P.n_str > 0;
-        const uint32_t n_cols = P.n_cols;
-        const bool trailer_row_live = trailer && !trailer_comment;
         // E1: typed cells (+ statistics) and cleaned string lengths. A record
         // that ends before a column's field leaves that cell null.
         if (tid < FALX_MAX_COLUMNS) sh.str_len[tid] = 0;
         __syncthreads();
-        walk([&](const Entry& e, uint32_t) {
-            const bool act = e.valid && !e.flag;
-            uint32_t to = e.pos;
-            if (act && e.is_term) {
-                if (e.pos > e.rs && src.at(e.pos - 1) == '\r') to = e.pos - 1;
-                records += 1;
-            }
+        walk([&](const Entry& e, uint32_t, uint32_t) {
+            if (!e.valid || e.flag) return;
+            const uint32_t to = field_end(e);
+            if (e.is_term) records += 1;
             const uint32_t row = e.row;
             const bool live = row < rows_cap;
             for (uint32_t j = 0; j < n_cols; ++j) {
                 const ColDesc& c = P.cols[j];
-                const bool found = act && c.index == e.ord;
-                const bool event = found || (act && e.is_term && c.index > e.ord);
-                bool vbit = false;
-                if (event) {
-                    CellStat o{false, false, 0, 0, 0, 0.0};
-                    if (c.type == FALX_COL_STR) {
-                        if (found) {
-                            unsigned long long cs = 0;
-                            uint32_t n = clean_cell(src, e.fs, to, cd, [&](uint8_t b) { cs += b; });
-                            atomicAdd(&sh.str_len[c.str_slot], n);
-                            o = CellStat{true, false, cs, n, 0, 0.0};
-                        }
-                    } else if (c.type == FALX_COL_F64) {
-                        F64Result r{false, false, false, 0, 0.0};
-                        if (found) r = f64_cell(e.fs, to);
-                        if (live) reinterpret_cast<double*>(c.values)[row] = r.ok ? r.value : 0.0;
-                        vbit = r.ok && live;
-                        o = CellStat{r.ok, r.fixed6, f64_to_bits(r.value), 0, r.mantissa, r.value};
-                    } else if (c.type == FALX_COL_I64) {
-                        int64_t v = 0;
-                        const bool ok = found && parse_i64_field(src, e.fs, to, cd, v);
-                        if (live) reinterpret_cast<long long*>(c.values)[row] = ok ? v : 0;
-                        vbit = ok && live;
-                        o = CellStat{ok, false, (unsigned long long)v, 0, 0, (double)v};
-                    } else {
-                        const bool ok = found && to != e.fs;
-                        if (live)
-                            reinterpret_cast<uint2*>(c.values)[row] = ok ? make_uint2(e.fs, to) : make_uint2(0, 0);
-                        vbit = ok && live;
-                        if (ok) {
-                            o.ok = true;
-                            o.bytes = to - e.fs;
-                            for (uint32_t p = e.fs; p < to; ++p) o.csum += src.at(p);
-                        }
+                const bool found = c.index == e.ord;
+                if (!found && !(e.is_term && c.index > e.ord)) continue;
+                CellStat o{false, false, 0, 0, 0, 0.0};
+                if (c.type == FALX_COL_STR) {
+                    if (!found) continue;
+                    unsigned long long cs = 0;
+                    uint32_t n = clean_cell(src, e.fs, to, cd, [&](uint8_t b) { cs += b; });
+                    atomicAdd(&sh.str_len[c.str_slot], n);
+                    o = CellStat{true, false, cs, n, 0, 0.0};
+                } else if (c.type == FALX_COL_F64) {
+                    F64Result r{false, false, false, 0, 0.0};
+                    if (found) r = f64_cell(e.fs, to);
+                    if (live) {
+                        reinterpret_cast<double*>(c.values)[row] = r.ok ? r.value : 0.0;
+                        if (r.ok) set_valid(j, c, row);
                     }
-                    if (with_stats && j < (uint32_t)STAT_GROUP) acc_add(acc, j, o);
+                    o = CellStat{r.ok, r.fixed6, f64_to_bits(r.value), 0, r.mantissa, r.value};
+                } else if (c.type == FALX_COL_I64) {
+                    int64_t v = 0;
+                    const bool ok = found && parse_i64_field(src, e.fs, to, cd, v);
+                    if (live) {
+                        reinterpret_cast<long long*>(c.values)[row] = ok ? v : 0;
+                        if (ok) set_valid(j, c, row);
+                    }
+                    o = CellStat{ok, false, (unsigned long long)v, 0, 0, (double)v};
+                } else {
+                    const bool ok = found && to != e.fs;
+                    if (live) {
+                        reinterpret_cast<uint2*>(c.values)[row] = ok ? make_uint2(e.fs, to) : make_uint2(0, 0);
+                        if (ok) set_valid(j, c, row);
+                    }
+                    if (ok) {
+                        o.ok = true;
+                        o.bytes = to - e.fs;
+                        for (uint32_t p = e.fs; p < to; ++p) o.csum += src.at(p);
+                    }
                 }
-                if (c.type != FALX_COL_STR) flush_valid(c, event, vbit, row);
+                if (with_stats && j < (uint32_t)STAT_GROUP) acc_add(acc, j, o);
             }
         }, trailer_word);
         if (with_stats) acc_publish(acc, 0, min((uint32_t)STAT_GROUP, n_cols), records);
         if (tid == 0 && last_tile) P.result[R_N_ROWS] = inc.n_rows + (trailer_row_live ? 1 : 0);
         __syncthreads();
+        flush_valid(false);
         if (!have_str) return;
 
         // LB3: arena bases per string column.
@@ -1223,34 +1313,40 @@ __global__ __launch_bounds__(WG) void k_scan(Params P) {
         }
         __syncthreads();
 
-        // E2: ordered arena placement per string column.
-        for (uint32_t j = 0; j < P.n_cols; ++j) {
+        // E2: ordered arena placement per string column: the walk's weight
+        // is the cell's cleaned length, so each cell learns its arena offset.
+        for (uint32_t j = 0; j < n_cols; ++j) {
             const ColDesc& c = P.cols[j];
             if (c.type != FALX_COL_STR) continue;
             int32_t* offs = reinterpret_cast<int32_t*>(c.values);
-            uint32_t running = sh.str_base[c.str_slot];
+            const uint32_t arena = sh.str_base[c.str_slot];
             if (tile == 0 && tid == 0) offs[0] = 0;
-            walk([&](const Entry& e, uint32_t) {
-                const bool act = e.valid && !e.flag;
-                const bool cellhere = act && c.index == e.ord;
-                const bool event = cellhere || (act && e.is_term && c.index > e.ord);
-                uint32_t from = e.fs, to = e.pos;
-                if (cellhere && e.is_term && e.pos > e.rs && src.at(e.pos - 1) == '\r') to = e.pos - 1;
-                uint32_t n = cellhere ? clean_cell(src, from, to, cd, [](uint8_t) {}) : 0u;
-                uint32_t tot;
-                uint32_t ex = block_excl_sum(n, sh.sum_sh, tot);
-                uint32_t at = running + ex;
-                const bool live = e.row < rows_cap;
-                if (cellhere && live && (uint64_t)at + n <= c.data_cap) {
-                    uint32_t q = at;
-                    clean_cell(src, from, to, cd, [&](uint8_t b) { c.data[q++] = b; });
-                }
-                flush_valid(c, event, cellhere && live, e.row);
-                if (act && e.is_term && live) offs[e.row + 1] = (int32_t)(at + n);
-                running += tot;
-            }, trailer_word);
-            __syncthreads();
+            walk_w(
+                Weighted{},
+                [&](const Entry& e) -> uint32_t {
+                    return (!e.flag && c.index == e.ord) ? clean_cell(src, e.fs, field_end(e), cd, [](uint8_t) {})
+                                                         : 0u;
+                },
+                [&](const Entry& e, uint32_t, uint32_t before) {
+                    if (!e.valid || e.flag) return;
+                    const bool cellhere = c.index == e.ord;
+                    const bool live = e.row < rows_cap;
+                    const uint32_t at = arena + before;
+                    uint32_t n = 0;
+                    if (cellhere) {
+                        uint64_t q = at;
+                        n = clean_cell(src, e.fs, field_end(e), cd, [&](uint8_t b) {
+                            if (live && q < c.data_cap) c.data[q] = b;
+                            ++q;
+                        });
+                        if (live) set_valid(j, c, e.row);
+                    }
+                    if (e.is_term && live) offs[e.row + 1] = (int32_t)(at + n);
+                },
+                trailer_word);
         }
+        __syncthreads();
+        flush_valid(true);
     }
 }
 
