@@ -109,11 +109,13 @@ FX_HD bool parse_i64_stream(St st, int64_t& out) {
     if (c == '-' || c == '+') { neg = c == '-'; c = st.next(); }
     if (c < 0) return false;
     uint64_t v = 0;
-    const uint64_t lim = neg ? 0x8000000000000000ULL : 0x7FFFFFFFFFFFFFFFULL;
+    // v * 10 + d must stay within 2^63 - 1 (2^63 when negative); both
+    // limits are 922337203685477580 * 10 + 7 (+ 8), so no division is needed.
+    const uint64_t lim10 = 922337203685477580ULL, lim_d = neg ? 8u : 7u;
     while (c >= 0) {
         if (!is_digit(c)) return false;
         uint64_t d = (uint64_t)(c - '0');
-        if (v > (lim - d) / 10) return false;
+        if (v > lim10 || (v == lim10 && d > lim_d)) return false;
         v = v * 10 + d;
         c = st.next();
     }
@@ -533,6 +535,39 @@ FX_HD F64Result parse_f64_stream(St st) {
     }
     r.value = bits_to_f64(e.bits | sign);
     return r;
+}
+
+// The fixed `[-]d{1,3}.d{6}` shape (csv_geo.rs:1414-1453) from its last 8
+// bytes `x` (low byte first) and the 0..3 bytes before them: the '.' and the
+// digits are checked eight bytes at a time and the value is m / 1e6 with the
+// sign bit ORed in, exactly as parse_f64_stream's fixed-6 path. False for
+// any other shape (the general parser then decides).
+FX_HD bool fixed6_tail(uint64_t x, const uint8_t* pre, uint32_t npre, F64Result& r) {
+    if (npre > 3 || ((x >> 8) & 0xFFu) != '.') return false;
+    const uint64_t y = (x & ~0xFF00ULL) | 0x3000ULL;    // '.' replaced by '0'
+    if ((y & 0xF0F0F0F0F0F0F0F0ULL) != 0x3030303030303030ULL) return false;
+    if (((y & 0x0F0F0F0F0F0F0F0FULL) + 0x0606060606060606ULL) & 0xF0F0F0F0F0F0F0F0ULL) return false;
+    uint32_t frac = 0;
+    for (int k = 2; k < 8; ++k) frac = frac * 10u + (uint32_t)((x >> (8 * k)) & 15u);
+    uint32_t p = 0;
+    bool neg = false;
+    if (p < npre && pre[p] == '-') { neg = true; ++p; }
+    if (npre - p > 2) return false;
+    uint32_t iv = 0;
+    for (; p < npre; ++p) {
+        const uint32_t c = (uint32_t)pre[p] - '0';
+        if (c > 9u) return false;
+        iv = iv * 10u + c;
+    }
+    iv = iv * 10u + (uint32_t)(x & 15u);
+    const uint64_t m = (uint64_t)iv * 1000000u + frac;
+    const double v = (double)m / 1000000.0;
+    r.ok = true;
+    r.fixed6 = true;
+    r.deferred = false;
+    r.mantissa = neg ? -(int64_t)m : (int64_t)m;
+    r.value = bits_to_f64(f64_to_bits(v) | ((uint64_t)neg << 63));
+    return true;
 }
 
 // ---------------------------------------------------------------------------

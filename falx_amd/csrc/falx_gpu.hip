@@ -16,7 +16,6 @@ struct falx_ctx {
     float last_ms = 0.f;
     unsigned long long* d_ticket = nullptr;
     unsigned long long ticket_next = 0;
-    uint32_t epoch = 0;
     unsigned long long* d_result = nullptr;
     unsigned long long* h_result = nullptr;
     void* d_lb = nullptr;
@@ -56,9 +55,10 @@ constexpr uint32_t kKindStates[6] = {1, 3, 2, 4, 3, 6};
 uint32_t summary_stride(uint32_t ns) { return (uint32_t)TS_COUNTS + (uint32_t)C7 * ns; }
 
 // Per-context scratch for a launch over n tiles: tile summaries, group
-// summaries, group and tile prefixes, and the string-arena look-back words.
+// summaries, group and tile prefixes, and per tile and string column the
+// cleaned byte count and arena offset.
 struct LbLayout {
-    size_t tsum, gsum, gpre, tpre, lb3, total;
+    size_t tsum, gsum, gpre, tpre, str_len, str_base, total;
 };
 
 LbLayout lb_layout(size_t n, uint32_t n_str, uint32_t ns) {
@@ -70,7 +70,8 @@ LbLayout lb_layout(size_t n, uint32_t n_str, uint32_t ns) {
     L.gsum = o; o = align_up(o + stride * groups, 256);
     L.gpre = o; o = align_up(o + 32 * groups, 256);
     L.tpre = o; o = align_up(o + 32 * n, 256);
-    L.lb3 = o; o = align_up(o + 16 * (size_t)n_str * n, 256);
+    L.str_len = o; o = align_up(o + 4 * (size_t)n_str * n, 256);
+    L.str_base = o; o = align_up(o + 4 * (size_t)n_str * n, 256);
     L.total = o;
     return L;
 }
@@ -98,23 +99,6 @@ falx_status ensure_part(falx_ctx* c, size_t bytes) {
     HIP_TRY(hipMalloc(&c->d_part, cap));
     c->part_cap = cap;
     return FALX_OK;
-}
-
-// Tags of look-back words: one epoch per launch. Before the counter would
-// reach tags still present in the buffer, every word is cleared (a fill
-// kernel, so the step is also valid inside a captured stream).
-hipError_t next_epoch(falx_ctx* c, uint32_t& out) {
-    if (++c->epoch >= 0x3FFFFFFFu) {
-        const uint64_t n = c->lb_cap / 8;
-        const uint64_t blocks = std::min<uint64_t>((n + WG - 1) / WG, 65536);
-        if (n) hipLaunchKernelGGL(k_fill_u64, dim3((uint32_t)blocks), dim3(WG), 0, c->stream,
-                                  static_cast<unsigned long long*>(c->d_lb), n, 0ULL);
-        hipError_t e = hipGetLastError();
-        if (e != hipSuccess) return e;
-        c->epoch = 1;
-    }
-    out = c->epoch;
-    return hipSuccess;
 }
 
 int dialect_kind(const falx_spec* s) {
@@ -213,8 +197,8 @@ falx_status prepare(falx_ctx* c, const falx_spec* spec, const uint8_t* d, uint64
     falx_status st = ensure_lb(c, P.n_tiles, P.n_str, ns);
     if (st != FALX_OK) return st;
     LbLayout L = lb_layout(P.n_tiles, P.n_str, ns);
-    P.lb3 = reinterpret_cast<unsigned long long*>(static_cast<char*>(c->d_lb) + L.lb3);
-    HIP_TRY(next_epoch(c, P.epoch));
+    P.str_len = reinterpret_cast<uint32_t*>(static_cast<char*>(c->d_lb) + L.str_len);
+    P.str_base = reinterpret_cast<const uint32_t*>(static_cast<char*>(c->d_lb) + L.str_base);
     P.ticket = c->d_ticket;
     P.result = c->d_result;
     return FALX_OK;
@@ -505,6 +489,14 @@ static falx_status columns_impl(falx_ctx* c, const falx_spec* spec, const uint8_
             Q.col_group = g;
             HIP_TRY(launch_scan<M_STATS>(c, Q, dialect_kind(spec)));
         }
+    }
+    // String columns: arena bases from the per-tile byte counts, then the
+    // placement pass.
+    if (P.n_str) {
+        hipLaunchKernelGGL(k_slot_scan, dim3(P.n_str), dim3(SLOT_WG), 0, c->stream, P.str_len, P.n_tiles, P.n_str,
+                           const_cast<uint32_t*>(P.str_base), P.result);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(launch_scan<M_STRINGS>(c, P, dialect_kind(spec)));
     }
     HIP_TRY(mark_end(c));
     if (stats) {

@@ -1,26 +1,27 @@
 // scan.hip — gfx950 kernels of the structural-indexing / column-parsing path.
 //
-// One workgroup (4 wave64s) owns one 8 KiB tile of the input. A tile is read
-// from HBM once (coalesced 16-byte loads into LDS) and then:
-//   pass 1  each wave walks its 32 blocks of 64 bytes: byte-per-lane compares
-//           and __ballot give the reference's per-block class masks (quote,
-//           separator, newline, escape, comment) directly as 64-bit values;
-//           the wave composes the blocks' transfer functions of the dialect
-//           state (quote parity / escape add-carry / NORMAL-QUOTE-COMMENT
-//           region) into one function;
-//   LB1     decoupled look-back over tile transfer functions (dynamic tile
-//           ids, agent-scope release/acquire) gives the tile's entry state;
-//   pass 2  the same blocks, now with known carries, yield the structural
-//           and terminator masks (csv_geo.rs:2116-2127, csv_hash.rs:2124-2138,
-//           ndjson.rs:1016-1032 semantics); set bits are compacted into an
-//           LDS list (popcount/mbcnt stream compaction) and summarised into
-//           record/field counters;
-//   LB2     decoupled look-back over the counters gives output offsets, row
-//           numbers, the open field's ordinal and the open record's start;
-//   emit    the whole workgroup walks the compacted list: index/tape
-//           offsets, typed cells (parsed in place), fused statistics;
-//   LB3     (string columns only) a third look-back places the tile's cleaned
-//           bytes in the Arrow arena, then the cells are copied.
+// The input is cut into 8 KiB tiles; one workgroup (4 wave64s) takes a tile,
+// reads it from HBM with coalesced 16-byte loads into LDS, and classifies
+// it: each wave compares one byte per lane and __ballot gives the
+// reference's per-block class masks (quote, separator, newline, escape,
+// comment) as 64-bit values; block b's masks then go to lane b, so the
+// dialect's mask algebra (quote parity / escape add-carry / NORMAL-QUOTE-
+// COMMENT regions, csv_geo.rs:2116-2127, csv_hash.rs:2124-2138,
+// ndjson.rs:1016-1032) runs for 32 blocks at once on the vector unit.
+//
+// A call is a short pipeline of passes over all tiles, with no waiting
+// between tiles:
+//   summary  per tile: the transfer function of its dialect state and, for
+//            every possible entry state, its record/field counters;
+//   scan     three small kernels compose the summaries (the algebra of the
+//            reference's chunk transfer functions, csv_hash.rs:184-191) into
+//            each tile's entry state and exclusive counters;
+//   emit     per tile, with its entry state known: structural/terminator
+//            masks, popcount/mbcnt compaction into an LDS list, then a walk
+//            of the list with record context: index/tape offsets, typed
+//            cells parsed in place, fused statistics;
+//   strings  (string columns only) after a scan of the tiles' cleaned byte
+//            counts, the cells are copied into the Arrow arena.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <string.h>
@@ -41,7 +42,7 @@ constexpr uint64_t EVEN_BITS = 0x5555555555555555ULL;
 constexpr uint32_t SPIN_LIMIT = 1u << 22;
 constexpr int STAT_GROUP = 4;             // stat columns accumulated per pass
 
-enum Mode : int { M_INDEX = 0, M_TAPE, M_COLUMNS, M_STATS, M_FIELDBYTES, M_LOGFMT, M_SUMMARY };
+enum Mode : int { M_INDEX = 0, M_TAPE, M_COLUMNS, M_STATS, M_FIELDBYTES, M_LOGFMT, M_SUMMARY, M_STRINGS };
 
 // Record/field counters of a byte range; `compose(a, b)` = a then b.
 struct Counts {
@@ -87,11 +88,11 @@ struct Params {
     ColDesc cols[FALX_MAX_COLUMNS];
     unsigned long long* ticket;
     unsigned long long ticket_base;
-    uint32_t epoch;
     uint32_t* tsum;           // M_SUMMARY out / emit in: per tile ts_stride words (tile summary)
     const uint32_t* tpre;     // emit modes: per tile 8 words, entry state + exclusive counters
     uint32_t ts_stride;
-    unsigned long long* lb3;  // per tile 2 * n_str tagged words: aggregate, then inclusive
+    uint32_t* str_len;        // COLUMNS out: per tile n_str cleaned byte counts
+    const uint32_t* str_base; // STRINGS in: per tile n_str arena offsets
     uint32_t* out;
     uint64_t out_cap;
     unsigned long long* ends;
@@ -288,96 +289,6 @@ DEV uint32_t fn_then_rt(uint32_t f, uint32_t g, uint32_t ns) {
     uint32_t h = 0;
     for (uint32_t s = 0; s < ns; ++s) h |= fn_apply(g, fn_apply(f, s)) << (4 * s);
     return h;
-}
-
-// --------------------------------------------------------------------------
-// string-arena look-back (LB3)
-// --------------------------------------------------------------------------
-// LB3 records are words (epoch << 32 | value) written and read with relaxed
-// agent-scope atomics: each word proves by its tag that it belongs to this
-// launch, so no flag and no acquire/release fence (an L2 write-back or
-// invalidate on this chip) is needed. Words of earlier launches carry other
-// tags; the host clears all of them when the epoch counter wraps.
-DEV unsigned long long tag_word(uint32_t epoch, uint32_t v) { return ((unsigned long long)epoch << 32) | v; }
-
-DEV uint32_t tag_of(unsigned long long w) { return (uint32_t)(w >> 32); }
-
-DEV void lb_store(unsigned long long* p, unsigned long long v) {
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-DEV unsigned long long lb_load(const unsigned long long* p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Window of the wave-parallel look-back: lane i inspects tile base - i.
-// Returns the lowest lane holding an inclusive prefix (64 = none) once every
-// lane below it holds at least an aggregate; `status` is this lane's flag.
-struct LbWindow {
-    uint32_t k;
-    bool ok;
-};
-
-DEV LbWindow lb_window(uint32_t status, bool in_range) {
-    const uint64_t inc_mask = ballot(in_range && status == 2);
-    const uint64_t ready = ballot(!in_range || status != 0);
-    const uint32_t k = inc_mask ? (uint32_t)__ffsll((long long)inc_mask) - 1 : 64u;
-    const uint64_t need = k >= 64 ? ~0ULL : ((1ULL << k) - 1);
-    return LbWindow{k, (ready & need) == need};
-}
-
-// A look-back whose window stays incomplete for SPIN_LIMIT polls reports its
-// kind and tile and returns, so the kernel always ends.
-DEV void lb_give_up(const Params& P, uint32_t tile, unsigned long long kind) {
-    if (__lane_id() == 0) {
-        atomicOr(&P.result[R_ERROR], kind);
-        atomicCAS(&P.result[R_ERR_TILE], 0ULL, (unsigned long long)tile + 1ULL);
-    }
-}
-
-// Exclusive arena prefix (sum per string column) into base_out[0..ns),
-// which lane 0 writes (LDS). Whole wave. Each lane polls the last word of
-// its tile's two records and reads a window's records in full only once it
-// is complete (and polls again if a word is not yet visible).
-DEV void lookback_str(const Params& P, uint32_t tile, uint32_t ns, uint32_t* base_out) {
-    const uint32_t ln = __lane_id();
-    const uint32_t epoch = P.epoch;
-    if (ln == 0)
-        for (uint32_t s = 0; s < ns; ++s) base_out[s] = 0;
-    int32_t base = (int32_t)tile - 1;
-    for (uint32_t spins = 0;; ++spins) {
-        if (spins > SPIN_LIMIT) {
-            lb_give_up(P, tile, 4ULL);
-            return;
-        }
-        const int32_t j = base - (int32_t)ln;
-        const unsigned long long* d = P.lb3 + (size_t)(j >= 0 ? j : 0) * 2 * ns;
-        uint32_t status = 0;
-        if (j >= 0) {
-            const unsigned long long si = lb_load(d + 2 * ns - 1), sa = lb_load(d + ns - 1);
-            status = tag_of(si) == epoch ? 2u : (tag_of(sa) == epoch ? 1u : 0u);
-        }
-        const LbWindow W = lb_window(status, j >= 0);
-        if (!W.ok) {
-            __builtin_amdgcn_s_sleep(2);
-            continue;
-        }
-        const bool take = j >= 0 && ln <= W.k;
-        const unsigned long long* r = d + (ln == W.k ? ns : 0);
-        bool good = true;
-        if (take)
-            for (uint32_t s = 0; s < ns; ++s) good &= tag_of(lb_load(r + s)) == epoch;
-        if (ballot(!good)) continue;
-        for (uint32_t s = 0; s < ns; ++s) {
-            uint32_t v = take ? (uint32_t)lb_load(r + s) : 0u;
-#pragma unroll
-            for (int dd = 32; dd >= 1; dd >>= 1) v += __shfl_xor(v, dd);
-            if (ln == 0) base_out[s] += v;
-        }
-        if (W.k < 64) return;
-        base -= 64;
-        spins = 0;
-    }
 }
 
 // --------------------------------------------------------------------------
@@ -659,11 +570,26 @@ struct Shared {
     unsigned long long usum_sh[NWAVE];
     long long lsum_sh[NWAVE];
     uint32_t str_len[FALX_MAX_COLUMNS];   // tile cleaned bytes per str slot
-    uint32_t str_base[FALX_MAX_COLUMNS];  // arena base per str slot
     SlowF64 slow;                         // exact path for undecided f64 cells
 };
 
 DEV uint32_t list_at(const Shared& sh, uint32_t k) { return sh.list[k]; }
+
+// Fixed-6 cells read straight from the tile in LDS: the last 8 bytes come
+// from three aligned words (fixed6_tail in cells.h decides). Bytes [lf, lt)
+// are local to the tile.
+DEV bool fixed6_lds(const uint8_t* lds, uint32_t lf, uint32_t lt, F64Result& r) {
+    const uint32_t len = lt - lf;
+    if (len < 8 || len > 11) return false;
+    const uint32_t s8 = lt - 8, a = s8 & ~3u, sh8 = (s8 & 3u) * 8u;
+    // Up to 3 bytes past the tile's last byte are read: still inside the
+    // workgroup's LDS (the list follows the tile) and never used.
+    const uint32_t* wp = reinterpret_cast<const uint32_t*>(lds + a);
+    const uint32_t w0 = wp[0], w1 = wp[1], w2 = wp[2];
+    const uint32_t lo = sh8 ? (w0 >> sh8) | (w1 << (32 - sh8)) : w0;
+    const uint32_t hi = sh8 ? (w1 >> sh8) | (w2 << (32 - sh8)) : w1;
+    return fixed6_tail(((uint64_t)hi << 32) | lo, lds + lf, len - 8, r);
+}
 
 // One list entry with its record context.
 struct Entry {
@@ -676,7 +602,10 @@ struct Entry {
     uint32_t k;     // index in the tile's list
 };
 
-constexpr int ITEMS = 4;   // consecutive list entries per thread in a walk
+// Consecutive list entries per thread in a walk: the tile's mean number of
+// fields per record, 1..ITEMS_MAX, so that with regular records every lane
+// is at the same field of its record at each step.
+constexpr uint32_t ITEMS_MAX = 8;
 
 // Walk kinds: plain, or with a per-entry weight scanned before the visits.
 struct NoWeight { static constexpr bool value = false; };
@@ -891,8 +820,8 @@ __global__ __launch_bounds__(WG) void k_scan(Params P) {
     TileSrc src{P.data, sh.bytes, tile_start, tile_start + tile_len};
     const CellDialect cd{P.quote, HQ ? P.escape : -1};
 
-    // Walk the list with record context: each thread takes ITEMS
-    // consecutive entries (WG*ITEMS per round); one block scan per round
+    // Walk the list with record context: each thread takes `items`
+    // consecutive entries (WG*items per round); one block scan per round
     // gives every thread the terminators, rows and last terminator before
     // its first entry. Then all threads visit their entries in step:
     // visit(e, terms_before, weight_before) with counts over all earlier
@@ -901,11 +830,15 @@ __global__ __launch_bounds__(WG) void k_scan(Params P) {
     // the weight of everything before it (string arena offsets).
     // `vword` != 0 appends one virtual entry after the list: the terminator
     // at `len` that closes an unterminated last record (last tile only).
-    auto walk_w = [&](auto weighted, auto&& weight, auto&& visit, uint32_t vword) __attribute__((always_inline)) {
+    auto walk_w = [&](auto weighted, auto&& weight, auto&& visit, uint32_t vword, auto&& after)
+        __attribute__((always_inline)) {
         constexpr bool WEIGHTED = decltype(weighted)::value;
         uint32_t terms_before = 0, rows_before = 0, w_before = 0;
         int32_t lt_before = -1;
         const uint32_t n_walk = n_list + (vword ? 1u : 0u);
+        const uint32_t t_terms = sh.inc.n_terms - pre.n_terms;
+        const uint32_t items =
+            uni(t_terms ? min(ITEMS_MAX, max(1u, (n_list + t_terms / 2) / t_terms)) : ITEMS_MAX);
         const auto word_at = [&](uint32_t k) -> uint32_t { return k < n_list ? (uint32_t)sh.list[k] : vword; };
         // Entry k without record context; `fs` is the end of entry k-1 plus 1.
         const auto entry_at = [&](uint32_t k, uint32_t fs) -> Entry {
@@ -922,14 +855,14 @@ __global__ __launch_bounds__(WG) void k_scan(Params P) {
             e.row = 0;
             return e;
         };
-        for (uint32_t base = 0; base < n_walk; base += WG * ITEMS) {
-            const uint32_t k0 = base + tid * ITEMS;
+        for (uint32_t base = 0; base < n_walk; base += WG * items) {
+            const uint32_t k0 = base + tid * items;
             const uint32_t fs0 = k0 == 0 ? pre.last_struct
                                          : (k0 - 1 < n_walk ? tile_start + (word_at(k0 - 1) & 0x3FFFu) + 1 : 0u);
             // This thread's terminators, rows and last terminator.
             Scan3 agg{0, 0, 0, -1};
             uint32_t fs = fs0;
-            for (int i = 0; i < ITEMS; ++i) {
+            for (uint32_t i = 0; i < items; ++i) {
                 const Entry e = entry_at(k0 + i, fs);
                 fs = e.pos + 1;
                 if (e.is_term) {
@@ -947,7 +880,7 @@ __global__ __launch_bounds__(WG) void k_scan(Params P) {
             const auto each = [&](auto&& f) {
                 int32_t lt = lt0;
                 uint32_t rs = rs0, row = row0_t, terms = terms_before + ex.a, fs1 = fs0;
-                for (int i = 0; i < ITEMS; ++i) {
+                for (uint32_t i = 0; i < items; ++i) {
                     Entry e = entry_at(k0 + i, fs1);
                     fs1 = e.pos + 1;
                     e.rs = rs;
@@ -974,15 +907,27 @@ __global__ __launch_bounds__(WG) void k_scan(Params P) {
                 visit(e, terms, wsum);
                 if (WEIGHTED && e.valid) wsum += weight(e);
             });
+            after();
             terms_before += tot.a;
             rows_before += tot.b;
             lt_before = tot.c > lt_before ? tot.c : lt_before;
         }
     };
     auto walk = [&](auto&& visit, uint32_t vword = 0u) __attribute__((always_inline)) {
-        walk_w(NoWeight{}, [](const Entry&) { return 0u; }, visit, vword);
+        walk_w(NoWeight{}, [](const Entry&) { return 0u; }, visit, vword, [] {});
     };
     // Field end without the record's trailing '\r' (csv_geo.rs:1056-1060).
+    // A string cell that cleans to itself: no quote convention, or doubled
+    // quotes and not quoted at both ends (backslash dialects unescape
+    // anywhere, so they always run the cleaning stream).
+    auto raw_cell = [&](uint32_t from, uint32_t to) -> bool {
+        if (cd.quote < 0) return true;
+        if (cd.escape >= 0) return false;
+        return !quoted_both(src, from, to, (uint8_t)cd.quote);
+    };
+    auto clean_len = [&](uint32_t from, uint32_t to) -> uint32_t {
+        return raw_cell(from, to) ? to - from : clean_cell(src, from, to, cd, [](uint8_t) {});
+    };
     auto field_end = [&](const Entry& e) -> uint32_t {
         return (e.is_term && e.pos > e.rs && src.at(e.pos - 1) == '\r') ? e.pos - 1 : e.pos;
     };
@@ -1111,35 +1056,68 @@ __global__ __launch_bounds__(WG) void k_scan(Params P) {
         for (uint32_t jj = 0; jj < (uint32_t)STAT_GROUP; ++jj)
             if (jj == j) acc[jj].add(o);
     };
+    // Per-tile statistics of up to STAT_GROUP columns plus the record count,
+    // reduced in a fixed order (wave butterfly, then waves in order) with one
+    // exchange through LDS: StatPart field f of column j is slot 6j + f.
+    __shared__ unsigned long long stat_red[NWAVE][STAT_GROUP * 6 + 1];
     auto acc_publish = [&](Acc* acc, uint32_t first, uint32_t nc, unsigned long long records) {
-        StatPart* parts = reinterpret_cast<StatPart*>(P.partials);
+        const auto wsum = [](unsigned long long v) {
+#pragma unroll
+            for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d);
+            return v;
+        };
+        const auto wsum_f = [](double v) {
+#pragma unroll
+            for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d);
+            return v;
+        };
 #pragma unroll
         for (uint32_t j = 0; j < (uint32_t)STAT_GROUP; ++j) {
             if (j >= nc) break;
-            unsigned long long v = block_reduce_sum((unsigned long long)acc[j].values, sh.usum_sh);
-            unsigned long long cs = block_reduce_sum(acc[j].csum, sh.usum_sh);
-            unsigned long long by = block_reduce_sum((unsigned long long)acc[j].bytes, sh.usum_sh);
-            long long fs = block_reduce_sum(acc[j].fx_sum, sh.lsum_sh);
-            unsigned long long fc = block_reduce_sum((unsigned long long)acc[j].fx_cells, sh.usum_sh);
-            double ds = block_reduce_sum(acc[j].dsum, sh.dsum_sh);
-            if (tid == 0) {
-                StatPart& sp = parts[(size_t)tile * P.n_cols + first + j];
-                sp.values = v; sp.checksum = cs; sp.bytes = by;
-                sp.fixed6_sum = fs; sp.fixed6_cells = fc; sp.sum = ds;
+            const unsigned long long v0 = wsum(acc[j].values), v1 = wsum(acc[j].csum), v2 = wsum(acc[j].bytes),
+                                     v3 = wsum((unsigned long long)acc[j].fx_sum), v4 = wsum(acc[j].fx_cells);
+            const double v5 = wsum_f(acc[j].dsum);
+            if (ln == 0) {
+                unsigned long long* r = &stat_red[w][6 * j];
+                r[0] = v0; r[1] = v1; r[2] = v2; r[3] = v3; r[4] = v4;
+                r[5] = (unsigned long long)__double_as_longlong(v5);
             }
         }
-        records = block_reduce_sum(records, sh.usum_sh);
+        const unsigned long long rec = wsum(records);
+        if (ln == 0) stat_red[w][STAT_GROUP * 6] = rec;
+        __syncthreads();
+        StatPart* parts = reinterpret_cast<StatPart*>(P.partials);
+        if (tid < nc * 6) {
+            const uint32_t j = tid / 6, f = tid % 6;
+            unsigned long long* dst = reinterpret_cast<unsigned long long*>(&parts[(size_t)tile * P.n_cols + first + j]);
+            if (f == 5) {
+                double t = __longlong_as_double((long long)stat_red[0][tid]);
+                for (int k = 1; k < NWAVE; ++k) t += __longlong_as_double((long long)stat_red[k][tid]);
+                dst[5] = (unsigned long long)__double_as_longlong(t);
+            } else {
+                unsigned long long t = stat_red[0][tid];
+                for (int k = 1; k < NWAVE; ++k) t += stat_red[k][tid];
+                dst[f] = t;
+            }
+        }
         if (tid == 0 && first == 0) {
-            unsigned long long* rec = reinterpret_cast<unsigned long long*>(parts + (size_t)P.n_tiles * P.n_cols);
-            rec[tile] = records;
+            unsigned long long t = 0;
+            for (int k = 0; k < NWAVE; ++k) t += stat_red[k][STAT_GROUP * 6];
+            reinterpret_cast<unsigned long long*>(parts + (size_t)P.n_tiles * P.n_cols)[tile] = t;
         }
     };
     // Typed value of a present cell; resolves undecided f64 cells exactly.
     auto f64_cell = [&](uint32_t from, uint32_t to) __attribute__((always_inline)) {
-        F64Result r = parse_f64_field(src, from, to, cd);
+        F64Result r;
+        if (from >= tile_start && to <= tile_start + tile_len &&
+            fixed6_lds(sh.bytes, from - tile_start, to - tile_start, r))
+            return r;
+        r = parse_f64_field(src, from, to, cd);
         if (r.deferred) r.value = resolve_f64(sh.slow, src, from, to, cd);
         return r;
     };
+
+    auto is_num = [](uint32_t type) { return type == FALX_COL_F64 || type == FALX_COL_I64; };
 
     if (MODE == M_STATS) {
         const uint32_t c0 = P.col_group;
@@ -1181,51 +1159,58 @@ __global__ __launch_bounds__(WG) void k_scan(Params P) {
         return;
     }
 
-    // ---- M_COLUMNS --------------------------------------------------------
-    if (MODE == M_COLUMNS) {
-        const uint64_t rows_cap = P.rows_cap;
-        const uint32_t n_cols = P.n_cols;
-        const bool trailer_row_live = trailer && !trailer_comment;
-        // Validity bits of the rows this tile's cells can touch, [pre.n_rows,
-        // inc.n_rows], staged in LDS (bit i of column j's region is row
-        // row0 + i) and written out once: words wholly inside the tile with
-        // plain stores, the two boundary words (shared with the neighbouring
-        // tiles' records) with atomics. Tiles whose span does not fit the
-        // stage use global atomics per cell.
-        constexpr uint32_t VSTAGE = 512;
-        __shared__ uint32_t vbits[VSTAGE];
-        const uint32_t row0 = pre.n_rows & ~63u;
-        const uint32_t words64 = (inc.n_rows - row0) / 64 + 1;
-        const uint32_t wpc = 2 * words64;
-        const bool staged = n_cols * wpc <= VSTAGE;
-        const uint64_t vwords = (rows_cap + 63) / 64;
+    // ---- validity bits for the column passes --------------------------------
+    // Bits of the rows this tile's cells can touch, [pre.n_rows, inc.n_rows],
+    // are staged in LDS (bit i of column j's region is row row0 + i) and
+    // written out once: words wholly inside the tile with plain stores, the
+    // two boundary words (shared with the neighbouring tiles' records) with
+    // atomics. Tiles whose span does not fit the stage use global atomics
+    // per cell.
+    constexpr uint32_t VSTAGE = 512;
+    __shared__ uint32_t vbits[VSTAGE];
+    const uint64_t rows_cap = P.rows_cap;
+    const uint32_t n_cols = P.n_cols;
+    const uint32_t row0 = pre.n_rows & ~63u;
+    const uint32_t words64 = (inc.n_rows - row0) / 64 + 1;
+    const uint32_t wpc = 2 * words64;
+    const bool staged = n_cols * wpc <= VSTAGE;
+    const uint64_t vwords = (rows_cap + 63) / 64;
+    auto vstage_clear = [&]() {
         for (uint32_t i = tid; i < VSTAGE; i += WG) vbits[i] = 0;
-        auto set_valid = [&](uint32_t j, const ColDesc& c, uint32_t row) {
-            if (staged) {
-                const uint32_t r = row - row0;
-                atomicOr(&vbits[j * wpc + (r >> 5)], 1u << (r & 31));
-            } else {
-                atomicOr(&c.valid[row >> 6], 1ULL << (row & 63));
-            }
-        };
-        auto flush_valid = [&](bool strings) {
-            if (!staged) return;
-            for (uint32_t j = 0; j < n_cols; ++j) {
-                const ColDesc& c = P.cols[j];
-                if ((c.type == FALX_COL_STR) != strings) continue;
-                for (uint32_t wi = tid; wi < words64; wi += WG) {
-                    const uint64_t gw = (uint64_t)(row0 >> 6) + wi;
-                    if (gw >= vwords) continue;
-                    const unsigned long long v =
-                        vbits[j * wpc + 2 * wi] | ((unsigned long long)vbits[j * wpc + 2 * wi + 1] << 32);
-                    if (wi == 0 || wi == words64 - 1) {
-                        if (v) atomicOr(&c.valid[gw], v);
-                    } else {
-                        c.valid[gw] = v;
-                    }
+    };
+    auto set_valid = [&](uint32_t j, const ColDesc& c, uint32_t row) {
+        if (staged) {
+            const uint32_t r = row - row0;
+            atomicOr(&vbits[j * wpc + (r >> 5)], 1u << (r & 31));
+        } else {
+            atomicOr(&c.valid[row >> 6], 1ULL << (row & 63));
+        }
+    };
+    // After a barrier: writes the staged words of the string columns
+    // (strings) or of the others.
+    auto flush_valid = [&](bool strings) {
+        if (!staged) return;
+        for (uint32_t j = 0; j < n_cols; ++j) {
+            const ColDesc& c = P.cols[j];
+            if ((c.type == FALX_COL_STR) != strings) continue;
+            for (uint32_t wi = tid; wi < words64; wi += WG) {
+                const uint64_t gw = (uint64_t)(row0 >> 6) + wi;
+                if (gw >= vwords) continue;
+                const unsigned long long v =
+                    vbits[j * wpc + 2 * wi] | ((unsigned long long)vbits[j * wpc + 2 * wi + 1] << 32);
+                if (wi == 0 || wi == words64 - 1) {
+                    if (v) atomicOr(&c.valid[gw], v);
+                } else {
+                    c.valid[gw] = v;
                 }
             }
-        };
+        }
+    };
+
+    // ---- M_COLUMNS --------------------------------------------------------
+    if (MODE == M_COLUMNS) {
+        const bool trailer_row_live = trailer && !trailer_comment;
+        vstage_clear();
         // Optional fused statistics of the first STAT_GROUP columns,
         // accumulated in registers while their cells are written.
         Acc acc[STAT_GROUP];
@@ -1238,6 +1223,30 @@ __global__ __launch_bounds__(WG) void k_scan(Params P) {
         // that ends before a column's field leaves that cell null.
         if (tid < FALX_MAX_COLUMNS) sh.str_len[tid] = 0;
         __syncthreads();
+        // A typed cell of column j: value, validity, statistics.
+        auto num_cell = [&](uint32_t j, const ColDesc c, bool found, uint32_t from, uint32_t to, uint32_t row)
+            __attribute__((always_inline)) {
+            const bool live = row < rows_cap;
+            CellStat o{false, false, 0, 0, 0, 0.0};
+            if (c.type == FALX_COL_F64) {
+                F64Result r{false, false, false, 0, 0.0};
+                if (found) r = f64_cell(from, to);
+                if (live) {
+                    reinterpret_cast<double*>(c.values)[row] = r.ok ? r.value : 0.0;
+                    if (r.ok) set_valid(j, c, row);
+                }
+                o = CellStat{r.ok, r.fixed6, f64_to_bits(r.value), 0, r.mantissa, r.value};
+            } else {
+                int64_t v = 0;
+                const bool ok = found && parse_i64_field(src, from, to, cd, v);
+                if (live) {
+                    reinterpret_cast<long long*>(c.values)[row] = ok ? v : 0;
+                    if (ok) set_valid(j, c, row);
+                }
+                o = CellStat{ok, false, (unsigned long long)v, 0, 0, (double)v};
+            }
+            if (with_stats && j < (uint32_t)STAT_GROUP) acc_add(acc, j, o);
+        };
         walk([&](const Entry& e, uint32_t, uint32_t) {
             if (!e.valid || e.flag) return;
             const uint32_t to = field_end(e);
@@ -1252,25 +1261,13 @@ __global__ __launch_bounds__(WG) void k_scan(Params P) {
                 if (c.type == FALX_COL_STR) {
                     if (!found) continue;
                     unsigned long long cs = 0;
-                    uint32_t n = clean_cell(src, e.fs, to, cd, [&](uint8_t b) { cs += b; });
+                    const uint32_t n = with_stats ? clean_cell(src, e.fs, to, cd, [&](uint8_t b) { cs += b; })
+                                                  : clean_len(e.fs, to);
                     atomicAdd(&sh.str_len[c.str_slot], n);
                     o = CellStat{true, false, cs, n, 0, 0.0};
-                } else if (c.type == FALX_COL_F64) {
-                    F64Result r{false, false, false, 0, 0.0};
-                    if (found) r = f64_cell(e.fs, to);
-                    if (live) {
-                        reinterpret_cast<double*>(c.values)[row] = r.ok ? r.value : 0.0;
-                        if (r.ok) set_valid(j, c, row);
-                    }
-                    o = CellStat{r.ok, r.fixed6, f64_to_bits(r.value), 0, r.mantissa, r.value};
-                } else if (c.type == FALX_COL_I64) {
-                    int64_t v = 0;
-                    const bool ok = found && parse_i64_field(src, e.fs, to, cd, v);
-                    if (live) {
-                        reinterpret_cast<long long*>(c.values)[row] = ok ? v : 0;
-                        if (ok) set_valid(j, c, row);
-                    }
-                    o = CellStat{ok, false, (unsigned long long)v, 0, 0, (double)v};
+                } else if (is_num(c.type)) {
+                    num_cell(j, c, found, e.fs, to, row);
+                    continue;
                 } else {
                     const bool ok = found && to != e.fs;
                     if (live) {
@@ -1292,58 +1289,54 @@ __global__ __launch_bounds__(WG) void k_scan(Params P) {
         flush_valid(false);
         if (!have_str) return;
 
-        // LB3: arena bases per string column.
-        if (w == 0) {
-            const uint32_t ns = P.n_str;
-            unsigned long long* d = P.lb3 + (size_t)tile * 2 * ns;
-            if (ln == 0)
-                for (uint32_t s = 0; s < ns; ++s) sh.str_base[s] = 0;
-            if (tile > 0) {
-                if (ln == 0)
-                    for (uint32_t s = 0; s < ns; ++s) lb_store(d + s, tag_word(P.epoch, sh.str_len[s]));
-                lookback_str(P, tile, ns, sh.str_base);
-            }
-            if (ln == 0) {
-                for (uint32_t s = 0; s < ns; ++s) {
-                    const uint32_t incv = sh.str_base[s] + sh.str_len[s];
-                    lb_store(d + ns + s, tag_word(P.epoch, incv));
-                    if (last_tile) P.result[R_STR_LEN + s] = incv;
-                }
-            }
-        }
-        __syncthreads();
+        // String columns: the tile's cleaned byte counts go out for the
+        // arena scan; the bytes themselves are placed by the STRINGS pass.
+        if (tid < P.n_str) P.str_len[(size_t)tile * P.n_str + tid] = sh.str_len[tid];
+    }
 
+    // ---- M_STRINGS: arena placement of the string columns ------------------
+    if (MODE == M_STRINGS) {
+        vstage_clear();
+        __syncthreads();
         // E2: ordered arena placement per string column: the walk's weight
         // is the cell's cleaned length, so each cell learns its arena offset.
         for (uint32_t j = 0; j < n_cols; ++j) {
             const ColDesc& c = P.cols[j];
             if (c.type != FALX_COL_STR) continue;
             int32_t* offs = reinterpret_cast<int32_t*>(c.values);
-            const uint32_t arena = sh.str_base[c.str_slot];
+            const uint32_t arena = P.str_base[(size_t)tile * P.n_str + c.str_slot];
             if (tile == 0 && tid == 0) offs[0] = 0;
             walk_w(
                 Weighted{},
                 [&](const Entry& e) -> uint32_t {
-                    return (!e.flag && c.index == e.ord) ? clean_cell(src, e.fs, field_end(e), cd, [](uint8_t) {})
-                                                         : 0u;
+                    return (!e.flag && c.index == e.ord) ? clean_len(e.fs, field_end(e)) : 0u;
                 },
                 [&](const Entry& e, uint32_t, uint32_t before) {
                     if (!e.valid || e.flag) return;
                     const bool cellhere = c.index == e.ord;
                     const bool live = e.row < rows_cap;
                     const uint32_t at = arena + before;
+                    const uint32_t to = field_end(e);
                     uint32_t n = 0;
                     if (cellhere) {
-                        uint64_t q = at;
-                        n = clean_cell(src, e.fs, field_end(e), cd, [&](uint8_t b) {
-                            if (live && q < c.data_cap) c.data[q] = b;
-                            ++q;
-                        });
+                        if (raw_cell(e.fs, to)) {
+                            n = to - e.fs;
+                            if (live) {
+                                const uint64_t end = min((uint64_t)at + n, c.data_cap);
+                                for (uint64_t q = at, p = e.fs; q < end; ++q, ++p) c.data[q] = src.at((uint32_t)p);
+                            }
+                        } else {
+                            uint64_t q = at;
+                            n = clean_cell(src, e.fs, to, cd, [&](uint8_t b) {
+                                if (live && q < c.data_cap) c.data[q] = b;
+                                ++q;
+                            });
+                        }
                         if (live) set_valid(j, c, e.row);
                     }
                     if (e.is_term && live) offs[e.row + 1] = (int32_t)(at + n);
                 },
-                trailer_word);
+                trailer_word, [] {});
         }
         __syncthreads();
         flush_valid(true);
@@ -1455,6 +1448,37 @@ __global__ __launch_bounds__(64) void k_scan_groups(const uint32_t* gs, uint32_t
         C = compose(C, c7_load(p + TS_COUNTS + C7 * s));
         s = fn_apply(p[0], s);
     }
+}
+
+// String arena bases: exclusive prefix over tiles of one string slot's
+// cleaned byte counts (blockIdx.x = slot). Each thread sums a run of tiles,
+// the block scans the runs, and each thread writes its tiles' offsets; the
+// total goes to result slot R_STR_LEN + slot.
+constexpr uint32_t SLOT_WG = 1024;
+
+__global__ __launch_bounds__(SLOT_WG) void k_slot_scan(const uint32_t* len, uint32_t n_tiles, uint32_t ns,
+                                                       uint32_t* base, unsigned long long* result) {
+    __shared__ unsigned long long wtot[SLOT_WG / 64];
+    const uint32_t slot = blockIdx.x, t = threadIdx.x, ln = __lane_id(), w = t >> 6;
+    const uint32_t per = (n_tiles + SLOT_WG - 1) / SLOT_WG;
+    const uint32_t t0 = min(n_tiles, t * per), t1 = min(n_tiles, t0 + per);
+    unsigned long long sum = 0;
+    for (uint32_t i = t0; i < t1; ++i) sum += len[(size_t)i * ns + slot];
+    unsigned long long incl = sum;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const unsigned long long u = __shfl_up(incl, d);
+        if (ln >= (uint32_t)d) incl += u;
+    }
+    if (ln == 63) wtot[w] = incl;
+    __syncthreads();
+    unsigned long long run = incl - sum;
+    for (uint32_t k = 0; k < w; ++k) run += wtot[k];
+    for (uint32_t i = t0; i < t1; ++i) {
+        base[(size_t)i * ns + slot] = (uint32_t)run;
+        run += len[(size_t)i * ns + slot];
+    }
+    if (t == SLOT_WG - 1) result[R_STR_LEN + slot] = run;
 }
 
 // Level 3: per group, thread 0 walks its SG tiles from the group's entry.

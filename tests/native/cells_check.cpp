@@ -158,6 +158,21 @@ std::string halfway_cell(Rng& r) {
     return s;
 }
 
+// Fixed-6 shapes with one random mutation (sign, digit count, separator,
+// stray byte) to probe the edges of the fast path.
+std::string near_fixed6(Rng& r) {
+    std::string s;
+    if (r.below(3) == 0) s.push_back("-+"[r.below(2)]);
+    digits(r, s, 1 + (int)r.below(4));
+    s.push_back('.');
+    digits(r, s, 5 + (int)r.below(3));
+    if (r.below(2)) {
+        const char alpha[] = "0123456789.-+eE x/:";
+        s[r.below(s.size())] = alpha[r.below(sizeof alpha - 1)];
+    }
+    return s;
+}
+
 int fail(const char* what, const std::string& cell) {
     std::fprintf(stderr, "MISMATCH %s on cell [%s]\n", what, cell.c_str());
     return 1;
@@ -230,6 +245,26 @@ int main(int argc, char** argv) {
         if (ook && gv != ov) return fail("i64 value", cell);
         ++checked;
     }
+    // The fixed-6 tail fast path accepts exactly the oracle's fixed-6 cells.
+    uint64_t fx_checked = 0;
+    for (uint64_t i = 0; i < count; ++i) {
+        std::string cell = (i & 1) ? near_fixed6(r) : random_number(r);
+        if (cell.size() < 8 || cell.size() > 11) continue;
+        const uint8_t* p = (const uint8_t*)cell.data();
+        uint64_t x = 0;
+        std::memcpy(&x, p + cell.size() - 8, 8);
+        fx::F64Result g{};
+        bool got = fx::fixed6_tail(x, p, (uint32_t)cell.size() - 8, g);
+        double ov;
+        int ofx;
+        int64_t om;
+        int ook = ora_parse_f64(p, cell.size(), &ov, &ofx, &om);
+        if (got != (ook && ofx)) return fail("fixed6 tail shape", cell);
+        if (got && (fx::f64_to_bits(g.value) != fx::f64_to_bits(ov) || g.mantissa != om))
+            return fail("fixed6 tail value", cell);
+        ++fx_checked;
+    }
+    checked += fx_checked;
     // Cleaning: random quote/escape soup.
     for (uint64_t i = 0; i < count; ++i) {
         std::string cell;

@@ -320,3 +320,30 @@ def test_capacity_errors(ctx, ora):
     small = ctx.alloc(4 * 10)
     with pytest.raises(FalxError):
         ctx.index_structurals(S.CSV, buf, data.size, out=small)
+
+
+def test_columns_dense_rows_and_fixed6_edges(ctx, ora):
+    """Tiles holding thousands of short records with six columns, so a tile's
+    row span does not fit the LDS validity stage (global atomics per cell),
+    mixed with fixed-6 edge cells at tile seams (fast path vs general path)."""
+    spec = S.CSV.with_columns([(0, S.F64), (1, S.I64), (2, S.STR), (0, S.F64), (3, S.BYTES), (5, S.F64)],
+                              "dense6")
+    rng = Rng(0xF1E6_0D5E)
+    edge = [b"1.234567", b"-1.234567", b"+1.234567", b"123.456789", b"-123.456789", b"1234.567890",
+            b"0.000000", b"-0.000000", b"1.23456", b"1.2345678", b"12.345e67", b"1.234567\r", b'"1.234567"',
+            b".1234567", b"-.123456", b"1..23456", b"999.999999", b"-999.999999"]
+    parts = []
+    for i in range(30000):
+        k = rng.below(10)
+        if k < 6:
+            parts.append(b"\n")                                  # empty record
+        elif k < 8:
+            parts.append(edge[rng.below(len(edge))] + b"\n")
+        else:
+            parts.append(edge[rng.below(len(edge))] + b"," + str(rng.below(1000)).encode() + b",x,y,z,"
+                         + edge[rng.below(len(edge))] + b"\n")
+    doc = b"".join(parts)
+    want = ora.parse_columns(spec, doc)
+    got = ctx.parse_columns(spec, ctx.upload(doc), len(doc))
+    assert_columns_equal(got, want, spec, "dense rows")
+    assert_stats_equal(ctx.csv_stats(spec, ctx.upload(doc), len(doc)), ora.csv_stats(spec, doc), spec, "dense")
