@@ -1300,11 +1300,23 @@ __global__ __launch_bounds__(WG) void k_scan(Params P) {
         __syncthreads();
         // E2: ordered arena placement per string column: the walk's weight
         // is the cell's cleaned length, so each cell learns its arena offset.
+        // A tile's cells of one column fill one contiguous arena range; when
+        // it fits, the bytes are gathered in LDS and written out with
+        // contiguous stores instead of per-cell byte stores.
+        constexpr uint32_t SSTAGE = 8192;
+        __shared__ uint8_t sstage[SSTAGE];
         for (uint32_t j = 0; j < n_cols; ++j) {
             const ColDesc& c = P.cols[j];
             if (c.type != FALX_COL_STR) continue;
             int32_t* offs = reinterpret_cast<int32_t*>(c.values);
             const uint32_t arena = P.str_base[(size_t)tile * P.n_str + c.str_slot];
+            const uint32_t total = P.str_len[(size_t)tile * P.n_str + c.str_slot];
+            const bool gather = total <= SSTAGE;
+            // Byte `b` of the tile's range: to the LDS stage or to the arena.
+            auto put = [&](uint64_t q, uint8_t b) {
+                if (gather) sstage[q - arena] = b;
+                else if (q < c.data_cap) c.data[q] = b;
+            };
             if (tile == 0 && tid == 0) offs[0] = 0;
             walk_w(
                 Weighted{},
@@ -1321,14 +1333,12 @@ __global__ __launch_bounds__(WG) void k_scan(Params P) {
                     if (cellhere) {
                         if (raw_cell(e.fs, to)) {
                             n = to - e.fs;
-                            if (live) {
-                                const uint64_t end = min((uint64_t)at + n, c.data_cap);
-                                for (uint64_t q = at, p = e.fs; q < end; ++q, ++p) c.data[q] = src.at((uint32_t)p);
-                            }
+                            if (live)
+                                for (uint32_t k = 0; k < n; ++k) put((uint64_t)at + k, src.at(e.fs + k));
                         } else {
                             uint64_t q = at;
                             n = clean_cell(src, e.fs, to, cd, [&](uint8_t b) {
-                                if (live && q < c.data_cap) c.data[q] = b;
+                                if (live) put(q, b);
                                 ++q;
                             });
                         }
@@ -1337,6 +1347,14 @@ __global__ __launch_bounds__(WG) void k_scan(Params P) {
                     if (e.is_term && live) offs[e.row + 1] = (int32_t)(at + n);
                 },
                 trailer_word, [] {});
+            if (gather) {
+                // Cells of rows past rows_cap were not staged; the call
+                // reports CAPACITY then, so those bytes need no value.
+                __syncthreads();
+                const uint64_t end = min((uint64_t)arena + total, c.data_cap);
+                for (uint64_t q = (uint64_t)arena + tid; q < end; q += WG) c.data[q] = sstage[q - arena];
+                __syncthreads();
+            }
         }
         __syncthreads();
         flush_valid(true);
