@@ -427,6 +427,9 @@ FX_HD uint64_t slow_f64_bits_into(BigDec& b, St st) {
     return bigdec_to_bits(b);
 }
 
+FX_HD F64Result f64_decide(bool neg, bool plus, bool dot, bool has_exp, int int_digits, int frac_digits,
+                           uint64_t mant, int32_t exp10, bool dropped);
+
 // Scan + fast paths. A cell the fast paths cannot decide comes back with
 // ok && deferred (its value unset); see parse_f64_exact.
 template <class St>
@@ -503,7 +506,16 @@ FX_HD F64Result parse_f64_stream(St st) {
         exp10 += eneg ? -e : e;
     }
     if (c >= 0) return r;
-    r.ok = true;
+    return f64_decide(neg, plus, dot, has_exp, int_digits, frac_digits, mant, exp10, dropped);
+}
+
+// The value of a cell that matched the decimal grammar: `mant` holds its
+// first 19 significant digits (`dropped` when nonzero digits followed),
+// scaled by 10^exp10. Shared by the byte stream above and the register
+// parser in num_swar.h, so both decide every cell the same way.
+FX_HD F64Result f64_decide(bool neg, bool plus, bool dot, bool has_exp, int int_digits, int frac_digits,
+                           uint64_t mant, int32_t exp10, bool dropped) {
+    F64Result r{true, false, false, 0, 0.0};
     // Fixed `[-]d{1,3}.d{6}` shape (csv_geo.rs:1414-1453): m / 1e6, sign ORed.
     if (!plus && !has_exp && dot && frac_digits == 6 && int_digits >= 1 && int_digits <= 3) {
         double v = (double)mant / 1000000.0;

@@ -485,7 +485,7 @@ static falx_status columns_impl(falx_ctx* c, const falx_spec* spec, const uint8_
     // Statistics of columns past the first group come from extra passes.
     if (stats) {
         Params Q = P;
-        for (uint32_t g = STAT_GROUP; g < spec->n_columns; g += STAT_GROUP) {
+        for (uint32_t g = COL_STAT_GROUP; g < spec->n_columns; g += STAT_GROUP) {
             Q.col_group = g;
             HIP_TRY(launch_scan<M_STATS>(c, Q, dialect_kind(spec)));
         }

@@ -29,6 +29,7 @@
 #include "../../include/falx_gpu.h"
 #include "cells.h"
 #include "ndjson_walk.h"
+#include "num_swar.h"
 
 #define DEV __device__ __forceinline__
 
@@ -40,7 +41,8 @@ constexpr int BPW = 32;                   // 64-byte blocks per wave
 constexpr int TILE = NWAVE * BPW * 64;    // 8192 bytes per workgroup
 constexpr uint64_t EVEN_BITS = 0x5555555555555555ULL;
 constexpr uint32_t SPIN_LIMIT = 1u << 22;
-constexpr int STAT_GROUP = 4;             // stat columns accumulated per pass
+constexpr int STAT_GROUP = 4;             // stat columns accumulated per STATS pass
+constexpr int COL_STAT_GROUP = 2;         // stat columns fused into a COLUMNS pass
 
 enum Mode : int { M_INDEX = 0, M_TAPE, M_COLUMNS, M_STATS, M_FIELDBYTES, M_LOGFMT, M_SUMMARY, M_STRINGS };
 
@@ -591,6 +593,33 @@ DEV bool fixed6_lds(const uint8_t* lds, uint32_t lf, uint32_t lt, F64Result& r) 
     return fixed6_tail(((uint64_t)hi << 32) | lo, lds + lf, len - 8, r);
 }
 
+// The 4 NW bytes of the tile that end at local byte lt, as NW words (the
+// window num_swar reads): NW + 1 aligned LDS words funnel-shifted by lt mod
+// 4. Words below the tile's first byte read as word 0 and only feed window
+// bytes below the cell, which num_swar masks off. The highest word read
+// ends at most 3 bytes past the tile, inside the workgroup's LDS (the list
+// follows the tile).
+template <int NW>
+DEV Win<NW> win_lds(const uint8_t* lds, uint32_t lt) {
+    const uint32_t* wp = reinterpret_cast<const uint32_t*>(lds);
+    const int32_t q = (int32_t)((lt + 32u) >> 2) - 8 - NW;   // word of local byte lt - 4 NW (may be < 0)
+    const uint32_t sh8 = (lt & 3u) * 8u;
+    uint32_t r[NW + 1];
+#pragma unroll
+    for (int k = 0; k <= NW; ++k) r[k] = wp[max(q + k, 0)];
+    Win<NW> W;
+#pragma unroll
+    for (int k = 0; k < NW; ++k) W.w[k] = (uint32_t)((((uint64_t)r[k + 1] << 32) | r[k]) >> sh8);
+    return W;
+}
+
+// A numeric cell read as f64 (f) or as i64 (iok, iv).
+struct NumVal {
+    F64Result f;
+    bool iok;
+    int64_t iv;
+};
+
 // One list entry with its record context.
 struct Entry {
     bool valid, is_term, flag;
@@ -1050,17 +1079,19 @@ __global__ __launch_bounds__(WG) void k_scan(Params P) {
             if (o.fx) { fx_sum += o.fxm; fx_cells += 1; }
         }
     };
-    // acc[j] += o for a uniform j < STAT_GROUP, without indexing registers.
-    auto acc_add = [](Acc* acc, uint32_t j, const CellStat& o) __attribute__((always_inline)) {
+    // acc[j] += o for a uniform j < N, without indexing registers.
+    auto acc_add = [](auto& acc, uint32_t j, const CellStat& o) __attribute__((always_inline)) {
+        constexpr uint32_t N = sizeof(acc) / sizeof(acc[0]);
 #pragma unroll
-        for (uint32_t jj = 0; jj < (uint32_t)STAT_GROUP; ++jj)
+        for (uint32_t jj = 0; jj < N; ++jj)
             if (jj == j) acc[jj].add(o);
     };
     // Per-tile statistics of up to STAT_GROUP columns plus the record count,
     // reduced in a fixed order (wave butterfly, then waves in order) with one
     // exchange through LDS: StatPart field f of column j is slot 6j + f.
     __shared__ unsigned long long stat_red[NWAVE][STAT_GROUP * 6 + 1];
-    auto acc_publish = [&](Acc* acc, uint32_t first, uint32_t nc, unsigned long long records) {
+    auto acc_publish = [&](auto& acc, uint32_t first, uint32_t nc, unsigned long long records) {
+        constexpr uint32_t N = sizeof(acc) / sizeof(acc[0]);
         const auto wsum = [](unsigned long long v) {
 #pragma unroll
             for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d);
@@ -1072,7 +1103,7 @@ __global__ __launch_bounds__(WG) void k_scan(Params P) {
             return v;
         };
 #pragma unroll
-        for (uint32_t j = 0; j < (uint32_t)STAT_GROUP; ++j) {
+        for (uint32_t j = 0; j < N; ++j) {
             if (j >= nc) break;
             const unsigned long long v0 = wsum(acc[j].values), v1 = wsum(acc[j].csum), v2 = wsum(acc[j].bytes),
                                      v3 = wsum((unsigned long long)acc[j].fx_sum), v4 = wsum(acc[j].fx_cells);
@@ -1106,15 +1137,51 @@ __global__ __launch_bounds__(WG) void k_scan(Params P) {
             reinterpret_cast<unsigned long long*>(parts + (size_t)P.n_tiles * P.n_cols)[tile] = t;
         }
     };
-    // Typed value of a present cell; resolves undecided f64 cells exactly.
-    auto f64_cell = [&](uint32_t from, uint32_t to) __attribute__((always_inline)) {
-        F64Result r;
-        if (from >= tile_start && to <= tile_start + tile_len &&
-            fixed6_lds(sh.bytes, from - tile_start, to - tile_start, r))
-            return r;
-        r = parse_f64_field(src, from, to, cd);
-        if (r.deferred) r.value = resolve_f64(sh.slow, src, from, to, cd);
-        return r;
+    // Typed value of a present cell as f64 (as_f64) or i64: for cells inside
+    // the tile the fixed-6 test, then the register parser; the stream parser
+    // for the cells those leave undecided; undecided f64 values are then
+    // resolved exactly.
+    const bool swar_ok = cd.quote < '0' || cd.quote > '9';   // num_swar's quote condition
+    auto num_value = [&](uint32_t from, uint32_t to, bool as_f64) __attribute__((always_inline)) {
+        NumVal v;
+        bool done = false;
+        // The fixed `[-]d{1,3}.d{6}` shape first: three word reads decide it.
+        if (as_f64 && from >= tile_start && to <= tile_start + tile_len &&
+            fixed6_lds(sh.bytes, from - tile_start, to - tile_start, v.f)) {
+            done = true;
+        } else if (swar_ok && from >= tile_start && to <= tile_start + tile_len && to - from <= NUM_WIN) {
+            const uint32_t lt = to - tile_start;
+            // The narrow window when every lane here can use it.
+            if (__all(to - from <= Win12::BYTES)) {
+                auto at = [&](uint32_t p) { return (uint32_t)sh.bytes[lt - Win12::BYTES + p]; };
+                done = num_swar(win_lds<3>(sh.bytes, lt), to - from, as_f64, cd.quote, at, v.f, v.iok, v.iv);
+            } else {
+                auto at = [&](uint32_t p) { return (uint32_t)sh.bytes[lt - Win24::BYTES + p]; };
+                done = num_swar(win_lds<6>(sh.bytes, lt), to - from, as_f64, cd.quote, at, v.f, v.iok, v.iv);
+            }
+        }
+        if (!done) {
+            if (as_f64) {
+                v.f = parse_f64_field(src, from, to, cd);
+            } else {
+                v.iok = parse_i64_field(src, from, to, cd, v.iv);
+                if (!v.iok) v.iv = 0;
+            }
+        }
+        if (as_f64 && v.f.ok && v.f.deferred) v.f.value = resolve_f64(sh.slow, src, from, to, cd);
+        return v;
+    };
+    // The f64 and i64 columns that read ordinal `ord`: one parse per entry
+    // serves all of them (an entry read both ways also parses as i64).
+    auto num_wants = [&](uint32_t c0, uint32_t nc, uint32_t ord, bool& want_f, bool& want_i)
+        __attribute__((always_inline)) {
+        want_f = want_i = false;
+        for (uint32_t j = c0; j < c0 + nc; ++j) {
+            const ColDesc& c = P.cols[j];
+            if (c.index != ord) continue;
+            want_f |= c.type == FALX_COL_F64;
+            want_i |= c.type == FALX_COL_I64;
+        }
     };
 
     auto is_num = [](uint32_t type) { return type == FALX_COL_F64 || type == FALX_COL_I64; };
@@ -1130,16 +1197,27 @@ __global__ __launch_bounds__(WG) void k_scan(Params P) {
             if (!e.valid || e.flag) return;
             const uint32_t to = field_end(e);
             if (e.is_term) records += 1;
+            bool want_f, want_i;
+            num_wants(c0, nc, e.ord, want_f, want_i);
+            // One parse for both kinds, so lanes at f64 and i64 columns run
+            // the same instructions; nf serves the f64 columns, ni the i64.
+            NumVal nf{}, ni{};
+            if (want_f || want_i) nf = num_value(e.fs, to, want_f);
+            ni = nf;
+            if (want_f && want_i) {
+                ni.iok = parse_i64_field(src, e.fs, to, cd, ni.iv);
+                if (!ni.iok) ni.iv = 0;
+            }
             for (uint32_t j = 0; j < nc; ++j) {
                 const ColDesc& c = P.cols[c0 + j];
                 if (c.index != e.ord) continue;
                 CellStat o{false, false, 0, 0, 0, 0.0};
                 if (c.type == FALX_COL_F64) {
-                    F64Result r = f64_cell(e.fs, to);
+                    const F64Result& r = nf.f;
                     o = CellStat{r.ok, r.fixed6, f64_to_bits(r.value), 0, r.mantissa, r.value};
                 } else if (c.type == FALX_COL_I64) {
-                    int64_t v;
-                    o.ok = parse_i64_field(src, e.fs, to, cd, v);
+                    const int64_t v = ni.iv;
+                    o.ok = ni.iok;
                     o.csum = (unsigned long long)v;
                     o.d = (double)v;
                 } else if (c.type == FALX_COL_STR) {
@@ -1211,11 +1289,13 @@ __global__ __launch_bounds__(WG) void k_scan(Params P) {
     if (MODE == M_COLUMNS) {
         const bool trailer_row_live = trailer && !trailer_comment;
         vstage_clear();
-        // Optional fused statistics of the first STAT_GROUP columns,
-        // accumulated in registers while their cells are written.
-        Acc acc[STAT_GROUP];
+        // Optional fused statistics of the first COL_STAT_GROUP columns,
+        // accumulated in registers while their cells are written (further
+        // columns get STATS passes: every accumulator costs registers here,
+        // next to the walk and the parsers).
+        Acc acc[COL_STAT_GROUP];
 #pragma unroll
-        for (int j = 0; j < STAT_GROUP; ++j) acc[j] = Acc{0, 0, 0, 0, 0, 0.0};
+        for (int j = 0; j < COL_STAT_GROUP; ++j) acc[j] = Acc{0, 0, 0, 0, 0, 0.0};
         unsigned long long records = 0;
         const bool with_stats = P.partials != nullptr;
         const bool have_str = P.n_str > 0;
@@ -1224,28 +1304,28 @@ __global__ __launch_bounds__(WG) void k_scan(Params P) {
         if (tid < FALX_MAX_COLUMNS) sh.str_len[tid] = 0;
         __syncthreads();
         // A typed cell of column j: value, validity, statistics.
-        auto num_cell = [&](uint32_t j, const ColDesc c, bool found, uint32_t from, uint32_t to, uint32_t row)
-            __attribute__((always_inline)) {
+        auto num_cell = [&](uint32_t j, const ColDesc c, bool found, const NumVal& nf, const NumVal& ni,
+                            uint32_t row) __attribute__((always_inline)) {
             const bool live = row < rows_cap;
             CellStat o{false, false, 0, 0, 0, 0.0};
             if (c.type == FALX_COL_F64) {
                 F64Result r{false, false, false, 0, 0.0};
-                if (found) r = f64_cell(from, to);
+                if (found) r = nf.f;
                 if (live) {
                     reinterpret_cast<double*>(c.values)[row] = r.ok ? r.value : 0.0;
                     if (r.ok) set_valid(j, c, row);
                 }
                 o = CellStat{r.ok, r.fixed6, f64_to_bits(r.value), 0, r.mantissa, r.value};
             } else {
-                int64_t v = 0;
-                const bool ok = found && parse_i64_field(src, from, to, cd, v);
+                const bool ok = found && ni.iok;
+                const int64_t v = ok ? ni.iv : 0;
                 if (live) {
-                    reinterpret_cast<long long*>(c.values)[row] = ok ? v : 0;
+                    reinterpret_cast<long long*>(c.values)[row] = v;
                     if (ok) set_valid(j, c, row);
                 }
                 o = CellStat{ok, false, (unsigned long long)v, 0, 0, (double)v};
             }
-            if (with_stats && j < (uint32_t)STAT_GROUP) acc_add(acc, j, o);
+            if (with_stats && j < (uint32_t)COL_STAT_GROUP) acc_add(acc, j, o);
         };
         walk([&](const Entry& e, uint32_t, uint32_t) {
             if (!e.valid || e.flag) return;
@@ -1253,6 +1333,17 @@ __global__ __launch_bounds__(WG) void k_scan(Params P) {
             if (e.is_term) records += 1;
             const uint32_t row = e.row;
             const bool live = row < rows_cap;
+            bool want_f, want_i;
+            num_wants(0, n_cols, e.ord, want_f, want_i);
+            // One parse for both kinds, so lanes at f64 and i64 columns run
+            // the same instructions; nf serves the f64 columns, ni the i64.
+            NumVal nf{}, ni{};
+            if (want_f || want_i) nf = num_value(e.fs, to, want_f);
+            ni = nf;
+            if (want_f && want_i) {
+                ni.iok = parse_i64_field(src, e.fs, to, cd, ni.iv);
+                if (!ni.iok) ni.iv = 0;
+            }
             for (uint32_t j = 0; j < n_cols; ++j) {
                 const ColDesc& c = P.cols[j];
                 const bool found = c.index == e.ord;
@@ -1266,7 +1357,7 @@ __global__ __launch_bounds__(WG) void k_scan(Params P) {
                     atomicAdd(&sh.str_len[c.str_slot], n);
                     o = CellStat{true, false, cs, n, 0, 0.0};
                 } else if (is_num(c.type)) {
-                    num_cell(j, c, found, e.fs, to, row);
+                    num_cell(j, c, found, nf, ni, row);
                     continue;
                 } else {
                     const bool ok = found && to != e.fs;
@@ -1280,10 +1371,10 @@ __global__ __launch_bounds__(WG) void k_scan(Params P) {
                         for (uint32_t p = e.fs; p < to; ++p) o.csum += src.at(p);
                     }
                 }
-                if (with_stats && j < (uint32_t)STAT_GROUP) acc_add(acc, j, o);
+                if (with_stats && j < (uint32_t)COL_STAT_GROUP) acc_add(acc, j, o);
             }
         }, trailer_word);
-        if (with_stats) acc_publish(acc, 0, min((uint32_t)STAT_GROUP, n_cols), records);
+        if (with_stats) acc_publish(acc, 0, min((uint32_t)COL_STAT_GROUP, n_cols), records);
         if (tid == 0 && last_tile) P.result[R_N_ROWS] = inc.n_rows + (trailer_row_live ? 1 : 0);
         __syncthreads();
         flush_valid(false);

@@ -63,6 +63,11 @@ def assert_stats_equal(got, want, spec, label=""):
             assert g[key] == w[key], f"{label}: col {k} {key} {g[key]} != {w[key]}"
         # FP64 sums: blocked tree (GPU) vs sequential fold (oracle) differ
         # only by rounding; the exact integer sums above pin the terms.
+        if not np.isfinite(w["sum"]) or not np.isfinite(g["sum"]):
+            # A NaN or infinite cell makes both sums NaN or the same infinity.
+            assert (np.isnan(g["sum"]) and np.isnan(w["sum"])) or g["sum"] == w["sum"], \
+                f"{label}: col {k} sum {g['sum']} vs {w['sum']}"
+            continue
         n = max(1, w["values"])
         tol = 1e-7 * max(1.0, abs(w["sum"])) + 1e-6 * n ** 0.5
         assert abs(g["sum"] - w["sum"]) <= tol, f"{label}: col {k} sum {g['sum']} vs {w['sum']}"

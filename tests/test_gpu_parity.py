@@ -347,3 +347,45 @@ def test_columns_dense_rows_and_fixed6_edges(ctx, ora):
     got = ctx.parse_columns(spec, ctx.upload(doc), len(doc))
     assert_columns_equal(got, want, spec, "dense rows")
     assert_stats_equal(ctx.csv_stats(spec, ctx.upload(doc), len(doc)), ora.csv_stats(spec, doc), spec, "dense")
+
+
+def _numeric_cell(rng: Rng) -> bytes:
+    """Numeric-looking cells around the register parser's limits: 12- and
+    24-byte windows, 19 digits, four non-digits, exponents of 0-6 digits,
+    signs, quotes and stray bytes (the shapes it must hand to the stream
+    parser)."""
+    dg = lambda n: bytes(48 + rng.below(10) for _ in range(n))
+    s = b"-" if rng.below(4) == 0 else (b"+" if rng.below(12) == 0 else b"")
+    k = rng.below(10)
+    if k < 3:
+        s += dg(rng.below(6)) + b"." + dg(rng.below(14))
+    elif k < 5:
+        s += dg(1 + rng.below(21))
+    elif k < 7:
+        s += dg(rng.below(9)) + (b"." + dg(rng.below(9)) if rng.below(2) else b"") + (b"e" if rng.below(2) else b"E")
+        s += [b"", b"-", b"+"][rng.below(3)] + dg(rng.below(7))
+    elif k < 8:
+        s += [b"9223372036854775807", b"9223372036854775808", b"00000000000000000000012", b"inf", b"nan"][rng.below(5)]
+    elif k < 9:
+        s = b'"' + s + dg(1 + rng.below(8)) + (b"." + dg(3) if rng.below(2) else b"") + b'"'
+    else:
+        s += bytes(b"0123456789.eE+-x "[rng.below(17)] for _ in range(rng.below(26)))
+    return s
+
+
+def test_columns_register_parser_edges(ctx, ora):
+    """i64 and f64 columns over the same cells, records of varying length so
+    the cells fall at every offset of a tile and across tile seams."""
+    spec = S.CSV.with_columns([(0, S.I64), (0, S.F64), (1, S.F64), (2, S.I64), (3, S.STR)], "numedges")
+    rng = Rng(0x5A4_C311)
+    lines = []
+    for _ in range(40000):
+        cells = [_numeric_cell(rng) for _ in range(1 + rng.below(3))]
+        if rng.below(4) == 0:
+            cells.append(b"pad" * rng.below(9))
+        lines.append(b",".join(cells) + (b"\r\n" if rng.below(20) == 0 else b"\n"))
+    doc = b"".join(lines)
+    want = ora.parse_columns(spec, doc)
+    got = ctx.parse_columns(spec, ctx.upload(doc), len(doc))
+    assert_columns_equal(got, want, spec, "numeric edges")
+    assert_stats_equal(ctx.csv_stats(spec, ctx.upload(doc), len(doc)), ora.csv_stats(spec, doc), spec, "numeric edges")

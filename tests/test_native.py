@@ -33,3 +33,11 @@ def test_ndjson_walker_matches_oracle(ora, seed):
     exe = _build("ndjson_check", ora)
     r = subprocess.run([exe, str(seed), "100000"], capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
+
+
+@pytest.mark.parametrize("seed", [3, 0xC0FFEE])
+def test_register_number_parser_matches_oracle(ora, seed):
+    exe = _build("swar_check", ora)
+    r = subprocess.run([exe, str(seed), "300000"], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    assert "agree" in r.stdout
