@@ -160,8 +160,8 @@ falx_status check_input(const falx_ctx* c, const uint8_t* d, uint64_t len) {
     return FALX_OK;
 }
 
-// Common Params for a launch over d_data[0, len).
-falx_status prepare(falx_ctx* c, const falx_spec* spec, const uint8_t* d, uint64_t len, Params& P) {
+// The spec's part of the Params of a launch over d_data[0, len) (no scratch).
+void spec_params(const falx_spec* spec, const uint8_t* d, uint64_t len, Params& P) {
     memset(&P, 0, sizeof P);
     P.data = d;
     P.len = (uint32_t)len;
@@ -193,6 +193,11 @@ falx_status prepare(falx_ctx* c, const falx_spec* spec, const uint8_t* d, uint64
         }
         P.n_str = ns;
     }
+}
+
+// Common Params for a launch over d_data[0, len), scratch in the context.
+falx_status prepare(falx_ctx* c, const falx_spec* spec, const uint8_t* d, uint64_t len, Params& P) {
+    spec_params(spec, d, len, P);
     const uint32_t ns = kKindStates[spec ? dialect_kind(spec) : 0];
     falx_status st = ensure_lb(c, P.n_tiles, P.n_str, ns);
     if (st != FALX_OK) return st;
@@ -707,3 +712,5 @@ falx_status falx_parse_logfmt_pairs(falx_ctx* c, const uint8_t* d, uint64_t len,
 }
 
 }  // extern "C"
+
+#include "batch.hip"

@@ -640,22 +640,20 @@ constexpr uint32_t ITEMS_MAX = 8;
 struct NoWeight { static constexpr bool value = false; };
 struct Weighted { static constexpr bool value = true; };
 
+// One tile of one pass (MODE) over the document P describes. Called by every
+// thread of the workgroup; returns with the workgroup converged (every
+// early return is uniform).
 template <bool HQ, bool ESC, bool CMT, int MODE>
-__global__ __launch_bounds__(WG) void k_scan(Params P) {
-    __shared__ Shared sh;
+__device__ __forceinline__ void scan_tile(const Params& P, const uint32_t tile, Shared& sh) {
     constexpr int NS = Dial<HQ, ESC, CMT>::NS;
     const uint32_t tid = threadIdx.x, ln = tid & 63u, w = uni(tid >> 6);  // w is wave-uniform
 
-    if (tid == 0) {
-        sh.tile = (uint32_t)(atomicAdd(P.ticket, 1ULL) - P.ticket_base);
-        sh.slow.lock = 0;
-    }
+    if (tid == 0) sh.slow.lock = 0;
     {
         uint32_t b = tid;
         sh.cls[b] = (uint8_t)((P.cand[b >> 5] >> (b & 31)) & 1u);
     }
     __syncthreads();
-    const uint32_t tile = uni(sh.tile);
     const uint32_t tile_start = tile * (uint32_t)TILE;
     const uint32_t tile_len = min((uint32_t)TILE, P.len - tile_start);
     const bool last_tile = tile + 1 == P.n_tiles;
@@ -1452,6 +1450,77 @@ __global__ __launch_bounds__(WG) void k_scan(Params P) {
     }
 }
 
+// One workgroup per tile, tiles in ticket order.
+template <bool HQ, bool ESC, bool CMT, int MODE>
+__global__ __launch_bounds__(WG) void k_scan(Params P) {
+    __shared__ Shared sh;
+    if (threadIdx.x == 0) sh.tile = (uint32_t)(atomicAdd(P.ticket, 1ULL) - P.ticket_base);
+    __syncthreads();
+    scan_tile<HQ, ESC, CMT, MODE>(P, uni(sh.tile), sh);
+}
+
+// --------------------------------------------------------------------------
+// batches: persistent workgroups over a queue of (document, chunk) units
+// --------------------------------------------------------------------------
+// A stage's work is a table of segments, one per document (or per document
+// and column): segment i covers units [start[i], start[i + 1]) of the
+// stage's unit range and names its Params (req[i]); for tile stages a unit
+// is a chunk of chunk[i] consecutive tiles. Tables live at fixed device
+// addresses, so a captured launch sequence can be replayed for every batch.
+constexpr uint32_t BATCH_MAX_SEG = 1024;
+
+struct SegTable {
+    uint32_t n, total;
+    uint32_t req[BATCH_MAX_SEG];
+    uint32_t chunk[BATCH_MAX_SEG];
+    uint32_t start[BATCH_MAX_SEG + 1];
+};
+
+// Segment of unit u < T.total: the last i with start[i] <= u.
+DEV uint32_t seg_find(const SegTable& T, uint32_t u) {
+    uint32_t lo = 0, hi = T.n;   // start[lo] <= u < start[hi]
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (T.start[mid] <= u) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+// Persistent tile pass: workgroups take chunk tickets until the table is
+// drained; each chunk runs scan_tile over its tiles with its document's
+// Params. Documents of different sizes and specs (of one dialect kind) mix
+// freely in one launch.
+template <bool HQ, bool ESC, bool CMT, int MODE>
+__global__ __launch_bounds__(WG) void k_scan_batch(const Params* params, const SegTable* T,
+                                                   unsigned long long* ticket) {
+    __shared__ Shared sh;
+    __shared__ uint32_t s_p, s_t0, s_t1;
+    for (;;) {
+        if (threadIdx.x == 0) {
+            const uint32_t u = (uint32_t)atomicAdd(ticket, 1ULL);
+            if (u >= T->total) {
+                s_p = ~0u;
+            } else {
+                const uint32_t i = seg_find(*T, u);
+                const uint32_t pi = T->req[i], c = T->chunk[i];
+                const uint32_t t0 = (u - T->start[i]) * c;
+                s_p = pi;
+                s_t0 = t0;
+                s_t1 = min(t0 + c, params[pi].n_tiles);
+            }
+        }
+        __syncthreads();
+        const uint32_t pi = uni(s_p), t0 = uni(s_t0), t1 = uni(s_t1);
+        __syncthreads();
+        if (pi == ~0u) return;
+        const Params& P = params[pi];
+        for (uint32_t t = t0; t < t1; ++t) {
+            scan_tile<HQ, ESC, CMT, MODE>(P, t, sh);
+            __syncthreads();
+        }
+    }
+}
+
 // --------------------------------------------------------------------------
 // reductions, fills, NDJSON id+score
 // --------------------------------------------------------------------------
@@ -1474,11 +1543,11 @@ DEV void sum_to_lds(const uint32_t* src, bool have, uint32_t ns, uint32_t* fn, u
 }
 
 // Level 1: one summary per group of SG tiles (tree reduction in LDS).
-__global__ __launch_bounds__(SG) void k_scan_reduce(const uint32_t* ts, uint32_t stride, uint32_t n_tiles,
-                                                     uint32_t ns, uint32_t* gs) {
+DEV void scan_reduce_body(const uint32_t* ts, uint32_t stride, uint32_t n_tiles, uint32_t ns, uint32_t* gs,
+                          uint32_t blk) {
     __shared__ uint32_t fn[SG];
     __shared__ uint32_t cs[SG][NS_MAX * C7];
-    const uint32_t t = threadIdx.x, tile = blockIdx.x * SG + t;
+    const uint32_t t = threadIdx.x, tile = blk * SG + t;
     sum_to_lds(ts + (size_t)tile * stride, tile < n_tiles, ns, &fn[t], cs[t]);
     __syncthreads();
     for (uint32_t d = 1; d < SG; d <<= 1) {
@@ -1496,18 +1565,23 @@ __global__ __launch_bounds__(SG) void k_scan_reduce(const uint32_t* ts, uint32_t
         __syncthreads();
     }
     if (t < ns * C7 + 1) {
-        uint32_t* dst = gs + (size_t)blockIdx.x * stride;
+        uint32_t* dst = gs + (size_t)blk * stride;
         if (t == 0) dst[0] = fn[0];
         else dst[TS_COUNTS + t - 1] = cs[0][t - 1];
     }
+}
+
+__global__ __launch_bounds__(SG) void k_scan_reduce(const uint32_t* ts, uint32_t stride, uint32_t n_tiles,
+                                                     uint32_t ns, uint32_t* gs) {
+    scan_reduce_body(ts, stride, n_tiles, ns, gs, blockIdx.x);
 }
 
 // Level 2: one wave over the group summaries. Each lane folds a run of
 // groups, lane 0 chains the lanes from state 0, and each lane then walks its
 // groups with the concrete state, writing every group's entry (state + 7
 // counter words). The grand totals go to the result slots.
-__global__ __launch_bounds__(64) void k_scan_groups(const uint32_t* gs, uint32_t stride, uint32_t n_groups,
-                                                     uint32_t ns, uint32_t* gpre, unsigned long long* result) {
+DEV void scan_groups_body(const uint32_t* gs, uint32_t stride, uint32_t n_groups, uint32_t ns, uint32_t* gpre,
+                          unsigned long long* result) {
     __shared__ uint32_t lfn[64];
     __shared__ uint32_t lcs[64][NS_MAX * C7];
     __shared__ uint32_t lent[65][8];
@@ -1559,16 +1633,21 @@ __global__ __launch_bounds__(64) void k_scan_groups(const uint32_t* gs, uint32_t
     }
 }
 
+__global__ __launch_bounds__(64) void k_scan_groups(const uint32_t* gs, uint32_t stride, uint32_t n_groups,
+                                                     uint32_t ns, uint32_t* gpre, unsigned long long* result) {
+    scan_groups_body(gs, stride, n_groups, ns, gpre, result);
+}
+
 // String arena bases: exclusive prefix over tiles of one string slot's
 // cleaned byte counts (blockIdx.x = slot). Each thread sums a run of tiles,
 // the block scans the runs, and each thread writes its tiles' offsets; the
 // total goes to result slot R_STR_LEN + slot.
 constexpr uint32_t SLOT_WG = 1024;
 
-__global__ __launch_bounds__(SLOT_WG) void k_slot_scan(const uint32_t* len, uint32_t n_tiles, uint32_t ns,
-                                                       uint32_t* base, unsigned long long* result) {
+DEV void slot_scan_body(const uint32_t* len, uint32_t n_tiles, uint32_t ns, uint32_t* base,
+                        unsigned long long* result, uint32_t slot) {
     __shared__ unsigned long long wtot[SLOT_WG / 64];
-    const uint32_t slot = blockIdx.x, t = threadIdx.x, ln = __lane_id(), w = t >> 6;
+    const uint32_t t = threadIdx.x, ln = __lane_id(), w = t >> 6;
     const uint32_t per = (n_tiles + SLOT_WG - 1) / SLOT_WG;
     const uint32_t t0 = min(n_tiles, t * per), t1 = min(n_tiles, t0 + per);
     unsigned long long sum = 0;
@@ -1590,18 +1669,23 @@ __global__ __launch_bounds__(SLOT_WG) void k_slot_scan(const uint32_t* len, uint
     if (t == SLOT_WG - 1) result[R_STR_LEN + slot] = run;
 }
 
+__global__ __launch_bounds__(SLOT_WG) void k_slot_scan(const uint32_t* len, uint32_t n_tiles, uint32_t ns,
+                                                       uint32_t* base, unsigned long long* result) {
+    slot_scan_body(len, n_tiles, ns, base, result, blockIdx.x);
+}
+
 // Level 3: per group, thread 0 walks its SG tiles from the group's entry.
-__global__ __launch_bounds__(SG) void k_scan_tiles(const uint32_t* ts, uint32_t stride, uint32_t n_tiles,
-                                                    uint32_t ns, const uint32_t* gpre, uint32_t* tpre) {
+DEV void scan_tiles_body(const uint32_t* ts, uint32_t stride, uint32_t n_tiles, uint32_t ns, const uint32_t* gpre,
+                         uint32_t* tpre, uint32_t blk) {
     __shared__ uint32_t fn[SG];
     __shared__ uint32_t cs[SG][NS_MAX * C7];
     __shared__ uint32_t out[SG][8];
-    const uint32_t t = threadIdx.x, tile = blockIdx.x * SG + t;
+    const uint32_t t = threadIdx.x, tile = blk * SG + t;
     const bool have = tile < n_tiles;
     sum_to_lds(ts + (size_t)tile * stride, have, ns, &fn[t], cs[t]);
     __syncthreads();
     if (t == 0) {
-        const uint32_t* g = gpre + (size_t)blockIdx.x * 8;
+        const uint32_t* g = gpre + (size_t)blk * 8;
         uint32_t s = g[0];
         Counts C = c7_load(g + 1);
         for (uint32_t k = 0; k < SG; ++k) {
@@ -1616,34 +1700,43 @@ __global__ __launch_bounds__(SG) void k_scan_tiles(const uint32_t* ts, uint32_t 
         for (int i = 0; i < 8; ++i) tpre[(size_t)tile * 8 + i] = out[t][i];
 }
 
+__global__ __launch_bounds__(SG) void k_scan_tiles(const uint32_t* ts, uint32_t stride, uint32_t n_tiles,
+                                                    uint32_t ns, const uint32_t* gpre, uint32_t* tpre) {
+    scan_tiles_body(ts, stride, n_tiles, ns, gpre, tpre, blockIdx.x);
+}
+
 // Deterministic two-level reductions over per-tile partials: level 1 folds
 // RED_BLOCKS contiguous tile ranges in a fixed order, level 2 folds those.
 constexpr uint32_t RED_BLOCKS = 256;
 
 // Sums `width` u64 lanes over `n` rows (row-major) into out[0..width).
 // grid.x = number of row chunks (1 = single pass).
-__global__ __launch_bounds__(WG) void k_reduce_u64(const unsigned long long* parts, uint32_t n, uint32_t width,
-                                                   unsigned long long* out) {
+DEV void reduce_u64_body(const unsigned long long* parts, uint32_t n, uint32_t width, unsigned long long* out,
+                         uint32_t blk, uint32_t nblk) {
     __shared__ unsigned long long sh[NWAVE];
-    const uint32_t chunk = (n + gridDim.x - 1) / gridDim.x;
-    const uint32_t lo = blockIdx.x * chunk, hi = min(n, lo + chunk);
+    const uint32_t chunk = (n + nblk - 1) / nblk;
+    const uint32_t lo = blk * chunk, hi = min(n, lo + chunk);
     for (uint32_t c = 0; c < width; ++c) {
         unsigned long long v = 0;
         for (uint32_t i = lo + threadIdx.x; i < hi; i += WG) v += parts[(size_t)i * width + c];
         v = block_reduce_sum(v, sh);
-        if (threadIdx.x == 0) out[(size_t)blockIdx.x * width + c] = v;
+        if (threadIdx.x == 0) out[(size_t)blk * width + c] = v;
     }
 }
 
+__global__ __launch_bounds__(WG) void k_reduce_u64(const unsigned long long* parts, uint32_t n, uint32_t width,
+                                                   unsigned long long* out) {
+    reduce_u64_body(parts, n, width, out, blockIdx.x, gridDim.x);
+}
+
 // Per-column stat reduction. grid = (n_cols, chunks); fixed-order FP64 folds.
-__global__ __launch_bounds__(WG) void k_reduce_stats(const StatPart* parts, uint32_t n_tiles, uint32_t n_cols,
-                                                     StatPart* out) {
+DEV void reduce_stats_body(const StatPart* parts, uint32_t n_tiles, uint32_t n_cols, StatPart* out, uint32_t c,
+                           uint32_t blk, uint32_t nblk) {
     __shared__ unsigned long long ush[NWAVE];
     __shared__ long long lsh[NWAVE];
     __shared__ double dsh[NWAVE];
-    const uint32_t c = blockIdx.x;
-    const uint32_t chunk = (n_tiles + gridDim.y - 1) / gridDim.y;
-    const uint32_t lo = blockIdx.y * chunk, hi = min(n_tiles, lo + chunk);
+    const uint32_t chunk = (n_tiles + nblk - 1) / nblk;
+    const uint32_t lo = blk * chunk, hi = min(n_tiles, lo + chunk);
     unsigned long long v = 0, cs = 0, by = 0, fc = 0;
     long long fs = 0;
     double ds = 0;
@@ -1658,10 +1751,15 @@ __global__ __launch_bounds__(WG) void k_reduce_stats(const StatPart* parts, uint
     fs = block_reduce_sum(fs, lsh);
     ds = block_reduce_sum(ds, dsh);
     if (threadIdx.x == 0) {
-        StatPart& o = out[(size_t)blockIdx.y * n_cols + c];
+        StatPart& o = out[(size_t)blk * n_cols + c];
         o.values = v; o.checksum = cs; o.bytes = by;
         o.fixed6_sum = fs; o.fixed6_cells = fc; o.sum = ds;
     }
+}
+
+__global__ __launch_bounds__(WG) void k_reduce_stats(const StatPart* parts, uint32_t n_tiles, uint32_t n_cols,
+                                                     StatPart* out) {
+    reduce_stats_body(parts, n_tiles, n_cols, out, blockIdx.x, blockIdx.y, gridDim.y);
 }
 
 struct NdPart {
@@ -1675,8 +1773,7 @@ struct NdPart {
 // NDJSON id+score: records are split at raw '\n' (ndjson.rs:679-723); a
 // record belongs to the tile holding its '\n' (the data's trailing record to
 // the last tile). Each thread walks whole records; no cross-tile state.
-__global__ __launch_bounds__(WG) void k_ndjson_id_score(const uint8_t* data, uint32_t len, uint32_t n_tiles,
-                                                        NdPart* parts) {
+DEV void ndjson_tile(const uint8_t* data, uint32_t len, uint32_t n_tiles, NdPart* parts, uint32_t tile) {
     __shared__ uint8_t bytes[TILE];
     __shared__ uint16_t nls[TILE];
     __shared__ uint32_t wcnt[NWAVE], wb[NWAVE + 1];
@@ -1685,7 +1782,7 @@ __global__ __launch_bounds__(WG) void k_ndjson_id_score(const uint8_t* data, uin
     __shared__ unsigned long long ush[NWAVE];
     __shared__ long long lsh[NWAVE];
     __shared__ double dsh[NWAVE];
-    const uint32_t tile = blockIdx.x, tid = threadIdx.x, ln = tid & 63u, w = uni(tid >> 6);
+    const uint32_t tid = threadIdx.x, ln = tid & 63u, w = uni(tid >> 6);
     const uint32_t tile_start = tile * (uint32_t)TILE;
     const uint32_t tile_len = min((uint32_t)TILE, len - tile_start);
     for (uint32_t i = tid * 16; i < (uint32_t)TILE; i += WG * 16) {
@@ -1762,12 +1859,17 @@ __global__ __launch_bounds__(WG) void k_ndjson_id_score(const uint8_t* data, uin
     }
 }
 
-__global__ __launch_bounds__(WG) void k_reduce_nd(const NdPart* parts, uint32_t n, NdPart* out) {
+__global__ __launch_bounds__(WG) void k_ndjson_id_score(const uint8_t* data, uint32_t len, uint32_t n_tiles,
+                                                        NdPart* parts) {
+    ndjson_tile(data, len, n_tiles, parts, blockIdx.x);
+}
+
+DEV void reduce_nd_body(const NdPart* parts, uint32_t n, NdPart* out, uint32_t blk, uint32_t nblk) {
     __shared__ unsigned long long ush[NWAVE];
     __shared__ long long lsh[NWAVE];
     __shared__ double dsh[NWAVE];
-    const uint32_t chunk = (n + gridDim.x - 1) / gridDim.x;
-    const uint32_t lo = blockIdx.x * chunk, hi = min(n, lo + chunk);
+    const uint32_t chunk = (n + nblk - 1) / nblk;
+    const uint32_t lo = blk * chunk, hi = min(n, lo + chunk);
     unsigned long long r = 0, iv = 0, fc = 0, id = 0;
     long long fs = 0;
     double ss = 0;
@@ -1783,8 +1885,245 @@ __global__ __launch_bounds__(WG) void k_reduce_nd(const NdPart* parts, uint32_t 
     fs = block_reduce_sum(fs, lsh);
     ss = block_reduce_sum(ss, dsh);
     if (threadIdx.x == 0) {
-        NdPart& o = out[blockIdx.x];
+        NdPart& o = out[blk];
         o.records = r; o.invalid = iv; o.fx_cells = fc; o.id_sum = (long long)id; o.fx_sum = fs; o.score_sum = ss;
+    }
+}
+
+__global__ __launch_bounds__(WG) void k_reduce_nd(const NdPart* parts, uint32_t n, NdPart* out) {
+    reduce_nd_body(parts, n, out, blockIdx.x, gridDim.x);
+}
+
+// --------------------------------------------------------------------------
+// batch stages over the other kernels' bodies
+// --------------------------------------------------------------------------
+// Per-document arguments of the tile-prefix scan.
+struct BPrefix {
+    uint32_t* tsum;
+    uint32_t* gsum;
+    uint32_t* gpre;
+    uint32_t* tpre;
+    unsigned long long* result;
+    uint32_t stride, n_tiles, ns, groups;
+};
+
+// One fixed-order reduction job (the plain entry points' two-level folds):
+// level 1 folds `blocks` contiguous runs of n rows, level 2 folds those.
+enum : uint32_t { RJ_STATS = 0, RJ_U64 = 1, RJ_ND = 2 };
+struct BRed {
+    const void* parts;
+    void* tmp;     // level-1 outputs when blocks > 1
+    void* out;
+    uint32_t kind, n, width, blocks;   // width: columns (stats) or u64 lanes
+};
+
+struct BNd {
+    const uint8_t* data;
+    uint32_t len, n_tiles;
+    NdPart* parts;
+};
+
+struct BFill {
+    unsigned long long* p;
+    uint64_t n;
+    unsigned long long v;
+};
+
+// A segment table by pointers into the batch's uploaded descriptor blob.
+struct SegT {
+    uint32_t n, total;
+    const uint32_t* req;
+    const uint32_t* chunk;
+    const uint32_t* start;
+};
+
+DEV uint32_t seg_find(const SegT& T, uint32_t u) {
+    uint32_t lo = 0, hi = T.n;   // start[lo] <= u < start[hi]
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (T.start[mid] <= u) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+// Stage tables of a batch. Tile stages: summary per dialect kind, emit per
+// (kind, emit mode), NDJSON tiles. Unit stages: prefix groups, documents,
+// string slots, reduction levels, fills (4096 words a unit).
+enum : uint32_t { E_INDEX = 0, E_COLUMNS, E_STATS, E_STRINGS, E_LOGFMT, E_FIELDBYTES, N_EMIT };
+enum : uint32_t {
+    T_SUM = 0, T_GROUPS = 6, T_PREQ = 7, T_SLOT = 8, T_NDJ = 9, T_RED1 = 10, T_RED2 = 11, T_FILL = 12,
+    T_EMIT = 13, N_TAB = T_EMIT + 6 * N_EMIT
+};
+constexpr uint32_t FILL_UNIT = 4096;
+
+struct BatchHdr {
+    const Params* params;
+    const BPrefix* pre;
+    const BRed* red;
+    const BNd* nd;
+    const BFill* fill;
+    unsigned long long* tickets;   // one per table, zero on upload
+    SegT tab[N_TAB];
+};
+
+// A document's Params read through the constant address space: nothing
+// writes them during a batch, and the compiler then loads their fields with
+// scalar loads, as it does for kernel arguments.
+typedef __attribute__((address_space(4))) const Params ConstParams;
+DEV const Params& const_params(const Params* base, uint32_t i) {
+    ConstParams* cp = reinterpret_cast<ConstParams*>(reinterpret_cast<uintptr_t>(base + i));
+    return *(const Params*)cp;
+}
+
+// Persistent tile pass of table `tab` (a summary or emit table).
+template <int MODE>
+constexpr int batch_waves() { return (MODE == M_COLUMNS || MODE == M_STATS) ? 3 : 4; }
+
+template <bool HQ, bool ESC, bool CMT, int MODE>
+__global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(batch_waves<MODE>())))
+void k_tiles_batch(const BatchHdr* H, uint32_t tab) {
+    __shared__ Shared sh;
+    __shared__ uint32_t s_p, s_t0, s_t1;
+    const SegT& T = H->tab[tab];
+    for (;;) {
+        if (threadIdx.x == 0) {
+            const uint32_t u = (uint32_t)atomicAdd(&H->tickets[tab], 1ULL);
+            if (u >= T.total) {
+                s_p = ~0u;
+            } else {
+                const uint32_t i = seg_find(T, u);
+                const uint32_t pi = T.req[i], c = T.chunk[i];
+                const uint32_t t0 = (u - T.start[i]) * c;
+                s_p = pi;
+                s_t0 = t0;
+                s_t1 = min(t0 + c, H->params[pi].n_tiles);
+            }
+        }
+        __syncthreads();
+        const uint32_t pi = uni(s_p), t0 = uni(s_t0), t1 = uni(s_t1);
+        __syncthreads();
+        if (pi == ~0u) return;
+        for (uint32_t t = t0; t < t1; ++t) {
+            // The Params address passes an empty asm each tile, so their
+            // loads stay inside the loop: hoisted, they would stay live
+            // across the whole tile body and spill.
+            const uintptr_t pv = reinterpret_cast<uintptr_t>(H->params);
+            uintptr_t pa = (uintptr_t)uni((uint32_t)pv) | ((uintptr_t)uni((uint32_t)(pv >> 32)) << 32);
+            asm volatile("" : "+s"(pa));
+            scan_tile<HQ, ESC, CMT, MODE>(const_params(reinterpret_cast<const Params*>(pa), pi), t, sh);
+            __syncthreads();
+        }
+    }
+}
+
+// Unit u of table T as (document, local unit).
+struct Unit {
+    uint32_t req, k;
+};
+DEV Unit unit_at(const SegT& T, uint32_t u) {
+    const uint32_t i = seg_find(T, u);
+    return Unit{T.req[i], u - T.start[i]};
+}
+
+__global__ __launch_bounds__(SG) void k_prefix_reduce_batch(const BatchHdr* H) {
+    const SegT& T = H->tab[T_GROUPS];
+    for (uint32_t u = blockIdx.x; u < T.total; u += gridDim.x) {
+        const Unit x = unit_at(T, u);
+        const BPrefix& A = H->pre[x.req];
+        scan_reduce_body(A.tsum, A.stride, A.n_tiles, A.ns, A.gsum, x.k);
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(64) void k_prefix_groups_batch(const BatchHdr* H) {
+    const SegT& T = H->tab[T_PREQ];
+    for (uint32_t u = blockIdx.x; u < T.total; u += gridDim.x) {
+        const BPrefix& A = H->pre[unit_at(T, u).req];
+        scan_groups_body(A.gsum, A.stride, A.groups, A.ns, A.gpre, A.result);
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(SG) void k_prefix_tiles_batch(const BatchHdr* H) {
+    const SegT& T = H->tab[T_GROUPS];
+    for (uint32_t u = blockIdx.x; u < T.total; u += gridDim.x) {
+        const Unit x = unit_at(T, u);
+        const BPrefix& A = H->pre[x.req];
+        scan_tiles_body(A.tsum, A.stride, A.n_tiles, A.ns, A.gpre, A.tpre, x.k);
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(SLOT_WG) void k_slot_scan_batch(const BatchHdr* H) {
+    const SegT& T = H->tab[T_SLOT];
+    for (uint32_t u = blockIdx.x; u < T.total; u += gridDim.x) {
+        const Unit x = unit_at(T, u);
+        const Params& P = H->params[x.req];
+        slot_scan_body(P.str_len, P.n_tiles, P.n_str, const_cast<uint32_t*>(P.str_base), P.result, x.k);
+        __syncthreads();
+    }
+}
+
+// Reduction level 1 (units: stats jobs width x blocks, others blocks) or
+// level 2 (units: stats jobs width, others 1; jobs with blocks > 1 only).
+__global__ __launch_bounds__(WG) void k_reduce_batch(const BatchHdr* H, uint32_t level) {
+    const SegT& T = H->tab[level == 1 ? T_RED1 : T_RED2];
+    for (uint32_t u = blockIdx.x; u < T.total; u += gridDim.x) {
+        const Unit x = unit_at(T, u);
+        const BRed& J = H->red[x.req];
+        const bool l1 = level == 1;
+        const void* src = l1 ? J.parts : J.tmp;
+        void* dst = (l1 && J.blocks > 1) ? J.tmp : J.out;
+        const uint32_t n = l1 ? J.n : J.blocks, nblk = l1 ? J.blocks : 1u;
+        if (J.kind == RJ_STATS) {
+            const uint32_t c = l1 ? x.k / J.blocks : x.k, b = l1 ? x.k % J.blocks : 0u;
+            reduce_stats_body(static_cast<const StatPart*>(src), n, J.width, static_cast<StatPart*>(dst), c, b, nblk);
+        } else if (J.kind == RJ_U64) {
+            reduce_u64_body(static_cast<const unsigned long long*>(src), n, J.width,
+                            static_cast<unsigned long long*>(dst), l1 ? x.k : 0u, nblk);
+        } else {
+            reduce_nd_body(static_cast<const NdPart*>(src), n, static_cast<NdPart*>(dst), l1 ? x.k : 0u, nblk);
+        }
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(WG) void k_ndjson_batch(const BatchHdr* H) {
+    __shared__ uint32_t s_p, s_t0, s_t1;
+    const SegT& T = H->tab[T_NDJ];
+    for (;;) {
+        if (threadIdx.x == 0) {
+            const uint32_t u = (uint32_t)atomicAdd(&H->tickets[T_NDJ], 1ULL);
+            if (u >= T.total) {
+                s_p = ~0u;
+            } else {
+                const uint32_t i = seg_find(T, u);
+                const uint32_t pi = T.req[i], c = T.chunk[i];
+                const uint32_t t0 = (u - T.start[i]) * c;
+                s_p = pi;
+                s_t0 = t0;
+                s_t1 = min(t0 + c, H->nd[pi].n_tiles);
+            }
+        }
+        __syncthreads();
+        const uint32_t pi = uni(s_p), t0 = uni(s_t0), t1 = uni(s_t1);
+        __syncthreads();
+        if (pi == ~0u) return;
+        const BNd& A = H->nd[pi];
+        for (uint32_t t = t0; t < t1; ++t) {
+            ndjson_tile(A.data, A.len, A.n_tiles, A.parts, t);
+            __syncthreads();
+        }
+    }
+}
+
+__global__ __launch_bounds__(WG) void k_fill_batch(const BatchHdr* H) {
+    const SegT& T = H->tab[T_FILL];
+    for (uint32_t u = blockIdx.x; u < T.total; u += gridDim.x) {
+        const Unit x = unit_at(T, u);
+        const BFill& F = H->fill[x.req];
+        const uint64_t lo = (uint64_t)x.k * FILL_UNIT, hi = min(F.n, lo + FILL_UNIT);
+        for (uint64_t i = lo + threadIdx.x; i < hi; i += WG) F.p[i] = F.v;
     }
 }
 

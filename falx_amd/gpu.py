@@ -46,6 +46,15 @@ _SIGS = [
     ("falx_parse_ndjson_lines", C.c_int, [_VP, _VP, _U64, _U64P]),
     ("falx_parse_ndjson_id_score", C.c_int, [_VP, _VP, _U64, C.POINTER(S.CNdjsonIdScore)]),
     ("falx_parse_logfmt_pairs", C.c_int, [_VP, _VP, _U64, C.POINTER(S.CLogfmtStats)]),
+    ("falx_batch_create", C.c_int, [_VP, _U64, C.POINTER(C.c_void_p)]),
+    ("falx_batch_destroy", None, [_VP]),
+    ("falx_request_scratch_bytes", _U64, [C.POINTER(S.CRequest)]),
+    ("falx_batch_get_tuning", C.c_int, [_VP, C.POINTER(S.CBatchTuning)]),
+    ("falx_batch_set_tuning", C.c_int, [_VP, C.POINTER(S.CBatchTuning)]),
+    ("falx_batch_submit", C.c_int, [_VP, C.POINTER(S.CRequest), C.c_uint32]),
+    ("falx_batch_query", C.c_int, [_VP]),
+    ("falx_batch_wait", C.c_int, [_VP, C.POINTER(S.CResult)]),
+    ("falx_batch_last_ms", C.c_double, [_VP]),
 ]
 EXPORTED = [name for name, _, _ in _SIGS]
 
@@ -289,3 +298,107 @@ class Context:
         self._ck("falx_parse_logfmt_pairs", load().falx_parse_logfmt_pairs(self.h, data.ptr, n, C.byref(st)))
         return {"pairs": st.pairs, "key_bytes": st.key_bytes, "value_bytes": st.value_bytes,
                 "checksum": st.checksum}
+
+
+def _result_dict(op: int, spec: Optional[S.Spec], r: S.CResult) -> Dict:
+    out = {"status": r.status}
+    if op == S.OP_INDEX:
+        out["n_index"] = r.n_index
+    elif op in (S.OP_COLUMNS_STATS, S.OP_CSV_STATS):
+        out["stats"] = _stats_dict(spec, r.stats)
+    elif op == S.OP_FIELD_BYTES:
+        out["field_bytes"] = {"fields": r.field_bytes.fields, "bytes": r.field_bytes.bytes}
+    elif op == S.OP_LOGFMT_PAIRS:
+        L = r.logfmt
+        out["logfmt"] = {"pairs": L.pairs, "key_bytes": L.key_bytes, "value_bytes": L.value_bytes,
+                         "checksum": L.checksum}
+    elif op == S.OP_NDJSON_ID_SCORE:
+        N = r.ndjson
+        out["ndjson"] = {"records": N.records, "invalid": N.invalid, "id_sum": N.id_sum,
+                         "score_fixed6_sum": N.score_fixed6_sum, "score_fixed6_cells": N.score_fixed6_cells,
+                         "score_sum": N.score_sum}
+    return out
+
+
+class Request:
+    """One document of a batch: an op of falx_op, its spec (CSV-family ops),
+    its device input and, for COLUMNS ops, device column buffers
+    (:meth:`Context.alloc_columns`) or, for INDEX, an offsets buffer."""
+
+    def __init__(self, op: int, data: DeviceBuffer, length: int, spec: Optional[S.Spec] = None,
+                 columns: Optional[S.CColumns] = None, index: Optional[DeviceBuffer] = None,
+                 index_cap: int = 0, chunk: int = 0):
+        self.op, self.data, self.length, self.spec = op, data, length, spec
+        self.columns, self.index, self.index_cap, self.chunk = columns, index, index_cap, chunk
+        self._cspec = spec.to_c() if spec is not None else None
+
+    def to_c(self) -> S.CRequest:
+        r = S.CRequest()
+        r.op = self.op
+        r.chunk = self.chunk
+        r.spec = C.pointer(self._cspec) if self._cspec is not None else None
+        r.d_data = self.data.ptr if self.data is not None else None
+        r.len = self.length
+        r.columns = C.pointer(self.columns) if self.columns is not None else None
+        r.d_index = self.index.ptr if self.index is not None else None
+        r.index_cap = self.index_cap
+        return r
+
+    def scratch_bytes(self) -> int:
+        r = self.to_c()
+        return load().falx_request_scratch_bytes(C.byref(r))
+
+
+class Batch:
+    """Batched execution over one context (include/falx_gpu.h falx_batch_*):
+    submit() enqueues many documents at once and returns; wait() returns one
+    result dict per document, each equal to the plain entry point's."""
+
+    def __init__(self, ctx: Context, scratch_bytes: int):
+        self.ctx = ctx
+        h = C.c_void_p()
+        ctx._ck("falx_batch_create", load().falx_batch_create(ctx.h, scratch_bytes, C.byref(h)))
+        self.h = h
+        self._pending = None
+
+    def close(self):
+        if self.h:
+            load().falx_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def tuning(self) -> S.CBatchTuning:
+        t = S.CBatchTuning()
+        self.ctx._ck("falx_batch_get_tuning", load().falx_batch_get_tuning(self.h, C.byref(t)))
+        return t
+
+    def set_tuning(self, t: S.CBatchTuning):
+        self.ctx._ck("falx_batch_set_tuning", load().falx_batch_set_tuning(self.h, C.byref(t)))
+
+    def submit(self, reqs):
+        arr = (S.CRequest * max(1, len(reqs)))(*[r.to_c() for r in reqs])
+        self.ctx._ck("falx_batch_submit", load().falx_batch_submit(self.h, arr, len(reqs)))
+        self._pending = (list(reqs), arr)
+
+    def done(self) -> bool:
+        st = load().falx_batch_query(self.h)
+        if st == S.E_BUSY:
+            return False
+        self.ctx._ck("falx_batch_query", st)
+        return True
+
+    def wait(self):
+        reqs, arr = self._pending
+        res = (S.CResult * max(1, len(reqs)))()
+        self.ctx._ck("falx_batch_wait", load().falx_batch_wait(self.h, res))
+        self._pending = None
+        return [_result_dict(r.op, r.spec, res[i]) for i, r in enumerate(reqs)]
+
+    @property
+    def last_ms(self) -> float:
+        return load().falx_batch_last_ms(self.h)

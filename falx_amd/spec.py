@@ -106,6 +106,34 @@ def _byte(x) -> int:
     return b[0]
 
 
+# Batched execution (include/falx_gpu.h, falx_batch_*)
+OP_INDEX, OP_COLUMNS, OP_COLUMNS_STATS, OP_CSV_STATS, OP_FIELD_BYTES, OP_LOGFMT_PAIRS, OP_NDJSON_ID_SCORE = range(1, 8)
+N_SIZE_CLASS = 8
+
+
+class CRequest(C.Structure):
+    _fields_ = [
+        ("op", C.c_uint32), ("chunk", C.c_uint32), ("spec", C.POINTER(CSpec)),
+        ("d_data", C.c_void_p), ("len", C.c_uint64), ("columns", C.POINTER(CColumns)),
+        ("d_index", C.c_void_p), ("index_cap", C.c_uint64),
+    ]
+
+
+class CResult(C.Structure):
+    _fields_ = [
+        ("status", C.c_int32), ("reserved", C.c_uint32), ("n_index", C.c_uint64),
+        ("stats", CCsvStats), ("field_bytes", CFieldByteStats), ("logfmt", CLogfmtStats),
+        ("ndjson", CNdjsonIdScore),
+    ]
+
+
+class CBatchTuning(C.Structure):
+    _fields_ = [
+        ("tile_workgroups", C.c_uint32), ("unit_workgroups", C.c_uint32),
+        ("chunk", C.c_uint32 * N_SIZE_CLASS), ("use_graph", C.c_uint32),
+    ]
+
+
 @dataclass(frozen=True)
 class Column:
     index: int

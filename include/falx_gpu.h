@@ -259,6 +259,80 @@ falx_status falx_parse_ndjson_id_score(falx_ctx* ctx, const uint8_t* d_data,
 falx_status falx_parse_logfmt_pairs(falx_ctx* ctx, const uint8_t* d_data,
                                     uint64_t len, falx_logfmt_stats* out);
 
+/* ---------------------------------------------------------------------
+ * Batches. No counterpart in the reference, which parses one document per
+ * call (the generated module functions above). A batch runs many documents
+ * of uneven sizes and mixed specs through each pipeline stage in one
+ * persistent launch over a device queue of (document, chunk of tiles)
+ * units; the launch sequence is captured in a hipGraph per set of stages
+ * and replayed. Every document's outputs and result are bit-identical to
+ * the plain entry point's (falx_index_structurals, falx_parse_columns[_with_stats],
+ * falx_parse_csv_stats, falx_parse_field_bytes, falx_parse_logfmt_pairs,
+ * falx_parse_ndjson_id_score). One batch in flight per falx_batch; use one
+ * falx_batch (and falx_ctx) per concurrent slot.
+ * ------------------------------------------------------------------- */
+typedef enum falx_op {
+    FALX_OP_INDEX = 1,            /* falx_index_structurals */
+    FALX_OP_COLUMNS = 2,          /* falx_parse_columns */
+    FALX_OP_COLUMNS_STATS = 3,    /* falx_parse_columns_with_stats */
+    FALX_OP_CSV_STATS = 4,        /* falx_parse_csv_stats */
+    FALX_OP_FIELD_BYTES = 5,      /* falx_parse_field_bytes */
+    FALX_OP_LOGFMT_PAIRS = 6,     /* falx_parse_logfmt_pairs (no spec) */
+    FALX_OP_NDJSON_ID_SCORE = 7   /* falx_parse_ndjson_id_score (no spec) */
+} falx_op;
+
+typedef struct falx_request {
+    uint32_t op;                  /* falx_op */
+    uint32_t chunk;               /* tiles per queue unit; 0 = the tuning's choice */
+    const falx_spec* spec;        /* ops 1-5; read at submit */
+    const uint8_t* d_data;        /* device input, 16-byte aligned */
+    uint64_t len;
+    falx_columns* columns;        /* COLUMNS*: device buffers; rows and data_len
+                                     are written by falx_batch_wait */
+    uint32_t* d_index;            /* INDEX: output offsets */
+    uint64_t index_cap;
+} falx_request;
+
+typedef struct falx_result {
+    int32_t status;               /* the plain entry point's status; FALX_E_BUSY
+                                     when the batch's scratch could not hold it */
+    uint32_t reserved;
+    uint64_t n_index;             /* INDEX */
+    falx_csv_stats stats;         /* COLUMNS_STATS, CSV_STATS */
+    falx_field_byte_stats field_bytes;
+    falx_logfmt_stats logfmt;
+    falx_ndjson_id_score_stats ndjson;
+} falx_result;
+
+/* Launch shape. chunk[k]: tiles per queue unit for documents of [4^k,
+ * 4^(k+1)) tiles (8 KiB each); workgroups: persistent grid of the tile
+ * stages and of the other stages. Chosen per spec and size class by
+ * falx_amd/tune.py. */
+typedef struct falx_batch_tuning {
+    uint32_t tile_workgroups;
+    uint32_t unit_workgroups;
+    uint32_t chunk[8];
+    uint32_t use_graph;           /* capture and replay the launch sequence */
+} falx_batch_tuning;
+
+typedef struct falx_batch falx_batch;
+
+/* scratch_bytes: device memory for the per-document scratch of one batch
+ * (see falx_request_scratch_bytes); admission control sizes batches by it. */
+falx_status falx_batch_create(falx_ctx* ctx, uint64_t scratch_bytes, falx_batch** out);
+void falx_batch_destroy(falx_batch* batch);
+uint64_t falx_request_scratch_bytes(const falx_request* req);
+falx_status falx_batch_get_tuning(const falx_batch* batch, falx_batch_tuning* out);
+falx_status falx_batch_set_tuning(falx_batch* batch, const falx_batch_tuning* tuning);
+/* Enqueues the batch on the context's stream and returns. */
+falx_status falx_batch_submit(falx_batch* batch, const falx_request* reqs, uint32_t n);
+/* FALX_OK when the batch in flight (if any) has finished, FALX_E_BUSY if not. */
+falx_status falx_batch_query(falx_batch* batch);
+/* Waits and fills results[0..n) of the last submit. */
+falx_status falx_batch_wait(falx_batch* batch, falx_result* results);
+/* Device time of the last batch: descriptor upload to results download. */
+double falx_batch_last_ms(const falx_batch* batch);
+
 #ifdef __cplusplus
 }
 #endif

@@ -13,7 +13,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 def declared_symbols():
     text = open(os.path.join(ROOT, "include", "falx_gpu.h")).read()
-    return sorted(set(re.findall(r"^\s*(?:falx_status|void\*?|double)\s+(falx_\w+)\s*\(", text, re.M)))
+    return sorted(set(re.findall(r"^\s*(?:falx_status|void\*?|double|uint64_t)\s+(falx_\w+)\s*\(", text, re.M)))
 
 
 def test_header_declares_the_mirror_surface():
@@ -37,6 +37,9 @@ def test_struct_layouts_match_header():
     assert ctypes.sizeof(S.CColumns) == 16 + 40 * 32
     assert ctypes.sizeof(S.CColumnStats) == 48
     assert ctypes.sizeof(S.CNdjsonIdScore) == 48
+    assert ctypes.sizeof(S.CRequest) == 56
+    assert ctypes.sizeof(S.CResult) == 16 + (8 + 48 * 32) + 16 + 32 + 48
+    assert ctypes.sizeof(S.CBatchTuning) == 4 * 11
 
 
 def test_spec_toml_front_end():
