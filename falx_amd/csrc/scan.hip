@@ -1287,43 +1287,52 @@ __device__ __forceinline__ void scan_tile(const Params& P, const uint32_t tile, 
     if (MODE == M_COLUMNS) {
         const bool trailer_row_live = trailer && !trailer_comment;
         vstage_clear();
-        // Optional fused statistics of the first COL_STAT_GROUP columns,
-        // accumulated in registers while their cells are written (further
-        // columns get STATS passes: every accumulator costs registers here,
-        // next to the walk and the parsers).
-        Acc acc[COL_STAT_GROUP];
-#pragma unroll
-        for (int j = 0; j < COL_STAT_GROUP; ++j) acc[j] = Acc{0, 0, 0, 0, 0, 0.0};
+        // Optional fused statistics of the first COL_STAT_GROUP columns
+        // (further columns get STATS passes). They are not accumulated during
+        // the walk, whose registers set this kernel's occupancy: the walk
+        // marks each written valid cell of a stat column (and whether it had
+        // the fixed-6 shape) in LDS, and a sweep over the tile's rows then
+        // reads back the values it wrote. A fixed-6 cell's mantissa is
+        // round(value * 1e6), exact for |m| < 10^9 (the value is m / 1e6
+        // correctly rounded). String columns' statistics come from the
+        // STRINGS pass. Cells past rows_cap are not written, so on a
+        // CAPACITY result the statistics cover the rows that fit.
+        constexpr uint32_t SWORDS = 2 * (TILE / 64 + 2);   // a tile's rows, from row0, in u32 words
+        __shared__ uint32_t sbits[COL_STAT_GROUP][2][SWORDS];
         unsigned long long records = 0;
         const bool with_stats = P.partials != nullptr;
         const bool have_str = P.n_str > 0;
-        // E1: typed cells (+ statistics) and cleaned string lengths. A record
-        // that ends before a column's field leaves that cell null.
+        if (with_stats)
+            for (uint32_t i = tid; i < COL_STAT_GROUP * 2 * SWORDS; i += WG) (&sbits[0][0][0])[i] = 0;
+        auto stat_mark = [&](uint32_t j, uint32_t row, bool fx) {
+            if (!with_stats || j >= (uint32_t)COL_STAT_GROUP) return;
+            const uint32_t r = row - row0;
+            atomicOr(&sbits[j][0][r >> 5], 1u << (r & 31));
+            if (fx) atomicOr(&sbits[j][1][r >> 5], 1u << (r & 31));
+        };
+        // E1: typed cells and cleaned string lengths. A record that ends
+        // before a column's field leaves that cell null.
         if (tid < FALX_MAX_COLUMNS) sh.str_len[tid] = 0;
         __syncthreads();
-        // A typed cell of column j: value, validity, statistics.
+        // A typed cell of column j: value, validity, statistics mark.
         auto num_cell = [&](uint32_t j, const ColDesc c, bool found, const NumVal& nf, const NumVal& ni,
                             uint32_t row) __attribute__((always_inline)) {
-            const bool live = row < rows_cap;
-            CellStat o{false, false, 0, 0, 0, 0.0};
+            if (row >= rows_cap) return;
             if (c.type == FALX_COL_F64) {
-                F64Result r{false, false, false, 0, 0.0};
-                if (found) r = nf.f;
-                if (live) {
-                    reinterpret_cast<double*>(c.values)[row] = r.ok ? r.value : 0.0;
-                    if (r.ok) set_valid(j, c, row);
+                const bool ok = found && nf.f.ok;
+                reinterpret_cast<double*>(c.values)[row] = ok ? nf.f.value : 0.0;
+                if (ok) {
+                    set_valid(j, c, row);
+                    stat_mark(j, row, nf.f.fixed6);
                 }
-                o = CellStat{r.ok, r.fixed6, f64_to_bits(r.value), 0, r.mantissa, r.value};
             } else {
                 const bool ok = found && ni.iok;
-                const int64_t v = ok ? ni.iv : 0;
-                if (live) {
-                    reinterpret_cast<long long*>(c.values)[row] = v;
-                    if (ok) set_valid(j, c, row);
+                reinterpret_cast<long long*>(c.values)[row] = ok ? ni.iv : 0;
+                if (ok) {
+                    set_valid(j, c, row);
+                    stat_mark(j, row, false);
                 }
-                o = CellStat{ok, false, (unsigned long long)v, 0, 0, (double)v};
             }
-            if (with_stats && j < (uint32_t)COL_STAT_GROUP) acc_add(acc, j, o);
         };
         walk([&](const Entry& e, uint32_t, uint32_t) {
             if (!e.valid || e.flag) return;
@@ -1346,35 +1355,57 @@ __device__ __forceinline__ void scan_tile(const Params& P, const uint32_t tile, 
                 const ColDesc& c = P.cols[j];
                 const bool found = c.index == e.ord;
                 if (!found && !(e.is_term && c.index > e.ord)) continue;
-                CellStat o{false, false, 0, 0, 0, 0.0};
                 if (c.type == FALX_COL_STR) {
-                    if (!found) continue;
-                    unsigned long long cs = 0;
-                    const uint32_t n = with_stats ? clean_cell(src, e.fs, to, cd, [&](uint8_t b) { cs += b; })
-                                                  : clean_len(e.fs, to);
-                    atomicAdd(&sh.str_len[c.str_slot], n);
-                    o = CellStat{true, false, cs, n, 0, 0.0};
+                    if (found) atomicAdd(&sh.str_len[c.str_slot], clean_len(e.fs, to));
                 } else if (is_num(c.type)) {
                     num_cell(j, c, found, nf, ni, row);
-                    continue;
-                } else {
+                } else if (live) {
                     const bool ok = found && to != e.fs;
-                    if (live) {
-                        reinterpret_cast<uint2*>(c.values)[row] = ok ? make_uint2(e.fs, to) : make_uint2(0, 0);
-                        if (ok) set_valid(j, c, row);
-                    }
+                    reinterpret_cast<uint2*>(c.values)[row] = ok ? make_uint2(e.fs, to) : make_uint2(0, 0);
                     if (ok) {
-                        o.ok = true;
-                        o.bytes = to - e.fs;
-                        for (uint32_t p = e.fs; p < to; ++p) o.csum += src.at(p);
+                        set_valid(j, c, row);
+                        stat_mark(j, row, false);
                     }
                 }
-                if (with_stats && j < (uint32_t)COL_STAT_GROUP) acc_add(acc, j, o);
             }
         }, trailer_word);
-        if (with_stats) acc_publish(acc, 0, min((uint32_t)COL_STAT_GROUP, n_cols), records);
         if (tid == 0 && last_tile) P.result[R_N_ROWS] = inc.n_rows + (trailer_row_live ? 1 : 0);
         __syncthreads();
+        if (with_stats) {
+            // The statistics sweep over the marked rows, in row order per
+            // thread, then the fixed-order block reduction.
+            Acc acc[COL_STAT_GROUP];
+#pragma unroll
+            for (int j = 0; j < COL_STAT_GROUP; ++j) acc[j] = Acc{0, 0, 0, 0, 0, 0.0};
+            const uint32_t nc = min((uint32_t)COL_STAT_GROUP, n_cols);
+            const uint32_t span = min(32u * SWORDS, inc.n_rows + 2u - row0);
+            for (uint32_t r = tid; r < span; r += WG) {
+#pragma unroll
+                for (uint32_t j = 0; j < (uint32_t)COL_STAT_GROUP; ++j) {
+                    if (j >= nc || !((sbits[j][0][r >> 5] >> (r & 31)) & 1u)) continue;
+                    const ColDesc& c = P.cols[j];
+                    const uint32_t row = row0 + r;
+                    CellStat o{true, false, 0, 0, 0, 0.0};
+                    if (c.type == FALX_COL_F64) {
+                        const double v = reinterpret_cast<const double*>(c.values)[row];
+                        o.fx = (sbits[j][1][r >> 5] >> (r & 31)) & 1u;
+                        o.csum = f64_to_bits(v);
+                        o.fxm = o.fx ? __double2ll_rn(v * 1000000.0) : 0;
+                        o.d = v;
+                    } else if (c.type == FALX_COL_I64) {
+                        const long long v = reinterpret_cast<const long long*>(c.values)[row];
+                        o.csum = (unsigned long long)v;
+                        o.d = (double)v;
+                    } else {
+                        const uint2 sp = reinterpret_cast<const uint2*>(c.values)[row];
+                        o.bytes = sp.y - sp.x;
+                        for (uint32_t q = sp.x; q < sp.y; ++q) o.csum += src.at(q);
+                    }
+                    acc_add(acc, j, o);
+                }
+            }
+            acc_publish(acc, 0, nc, records);
+        }
         flush_valid(false);
         if (!have_str) return;
 
@@ -1407,6 +1438,10 @@ __device__ __forceinline__ void scan_tile(const Params& P, const uint32_t tile, 
                 else if (q < c.data_cap) c.data[q] = b;
             };
             if (tile == 0 && tid == 0) offs[0] = 0;
+            // Fused statistics of a string column among the first
+            // COL_STAT_GROUP: present cells, cleaned bytes, their byte sum.
+            const bool stat_col = P.partials != nullptr && j < (uint32_t)COL_STAT_GROUP;
+            unsigned long long s_cells = 0, s_bytes = 0, s_sum = 0;
             walk_w(
                 Weighted{},
                 [&](const Entry& e) -> uint32_t {
@@ -1422,16 +1457,23 @@ __device__ __forceinline__ void scan_tile(const Params& P, const uint32_t tile, 
                     if (cellhere) {
                         if (raw_cell(e.fs, to)) {
                             n = to - e.fs;
-                            if (live)
-                                for (uint32_t k = 0; k < n; ++k) put((uint64_t)at + k, src.at(e.fs + k));
+                            if (live || stat_col)
+                                for (uint32_t k = 0; k < n; ++k) {
+                                    const uint8_t b = src.at(e.fs + k);
+                                    if (live) put((uint64_t)at + k, b);
+                                    s_sum += b;
+                                }
                         } else {
                             uint64_t q = at;
                             n = clean_cell(src, e.fs, to, cd, [&](uint8_t b) {
                                 if (live) put(q, b);
+                                s_sum += b;
                                 ++q;
                             });
                         }
                         if (live) set_valid(j, c, e.row);
+                        s_cells += 1;
+                        s_bytes += n;
                     }
                     if (e.is_term && live) offs[e.row + 1] = (int32_t)(at + n);
                 },
@@ -1443,6 +1485,22 @@ __device__ __forceinline__ void scan_tile(const Params& P, const uint32_t tile, 
                 const uint64_t end = min((uint64_t)arena + total, c.data_cap);
                 for (uint64_t q = (uint64_t)arena + tid; q < end; q += WG) c.data[q] = sstage[q - arena];
                 __syncthreads();
+            }
+            if (stat_col) {
+                // The column's StatPart of this tile (the COLUMNS pass wrote
+                // zeros there), in the fixed reduction order.
+                const unsigned long long v0 = block_reduce_sum(s_cells, sh.usum_sh);
+                const unsigned long long v1 = block_reduce_sum(s_sum, sh.usum_sh);
+                const unsigned long long v2 = block_reduce_sum(s_bytes, sh.usum_sh);
+                if (tid == 0) {
+                    StatPart& o = reinterpret_cast<StatPart*>(P.partials)[(size_t)tile * P.n_cols + j];
+                    o.values = v0;
+                    o.checksum = v1;
+                    o.bytes = v2;
+                    o.fixed6_sum = 0;
+                    o.fixed6_cells = 0;
+                    o.sum = 0.0;
+                }
             }
         }
         __syncthreads();
@@ -1977,7 +2035,7 @@ DEV const Params& const_params(const Params* base, uint32_t i) {
 
 // Persistent tile pass of table `tab` (a summary or emit table).
 template <int MODE>
-constexpr int batch_waves() { return (MODE == M_COLUMNS || MODE == M_STATS) ? 3 : 4; }
+constexpr int batch_waves() { return MODE == M_STATS ? 3 : 4; }
 
 template <bool HQ, bool ESC, bool CMT, int MODE>
 __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(batch_waves<MODE>())))
