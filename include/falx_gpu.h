@@ -223,7 +223,10 @@ falx_status falx_parse_columns(falx_ctx* ctx, const falx_spec* spec,
  * over the input: the reference's column lane followed by the sums over the
  * materialised columns (examples/bench_columns.rs:151-179,
  * examples/csv_geo_aggregate.rs:42-63), with `stats` as falx_parse_csv_stats
- * would return it.
+ * would return it (counts, checksums and fixed-6 sums equal; FP64 sums are
+ * fixed-order trees that may differ from it in the last bits). On
+ * FALX_E_CAPACITY the statistics of the numeric and bytes columns cover the
+ * rows that fit in `rows_cap` only.
  */
 falx_status falx_parse_columns_with_stats(falx_ctx* ctx, const falx_spec* spec,
                                           const uint8_t* d_data, uint64_t len,
