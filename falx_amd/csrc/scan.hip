@@ -857,15 +857,18 @@ __device__ __forceinline__ void scan_tile(const Params& P, const uint32_t tile, 
     // the weight of everything before it (string arena offsets).
     // `vword` != 0 appends one virtual entry after the list: the terminator
     // at `len` that closes an unterminated last record (last tile only).
+    // Entries per thread in a walk round: the tile's mean fields per record.
+    auto walk_items = [&]() -> uint32_t {
+        const uint32_t t_terms = sh.inc.n_terms - pre.n_terms;
+        return uni(t_terms ? min(ITEMS_MAX, max(1u, (n_list + t_terms / 2) / t_terms)) : ITEMS_MAX);
+    };
     auto walk_w = [&](auto weighted, auto&& weight, auto&& visit, uint32_t vword, auto&& after)
         __attribute__((always_inline)) {
         constexpr bool WEIGHTED = decltype(weighted)::value;
         uint32_t terms_before = 0, rows_before = 0, w_before = 0;
         int32_t lt_before = -1;
         const uint32_t n_walk = n_list + (vword ? 1u : 0u);
-        const uint32_t t_terms = sh.inc.n_terms - pre.n_terms;
-        const uint32_t items =
-            uni(t_terms ? min(ITEMS_MAX, max(1u, (n_list + t_terms / 2) / t_terms)) : ITEMS_MAX);
+        const uint32_t items = walk_items();
         const auto word_at = [&](uint32_t k) -> uint32_t { return k < n_list ? (uint32_t)sh.list[k] : vword; };
         // Entry k without record context; `fs` is the end of entry k-1 plus 1.
         const auto entry_at = [&](uint32_t k, uint32_t fs) -> Entry {
@@ -934,14 +937,14 @@ __device__ __forceinline__ void scan_tile(const Params& P, const uint32_t tile, 
                 visit(e, terms, wsum);
                 if (WEIGHTED && e.valid) wsum += weight(e);
             });
-            after();
+            after(base, min(WG * items, n_walk - base));
             terms_before += tot.a;
             rows_before += tot.b;
             lt_before = tot.c > lt_before ? tot.c : lt_before;
         }
     };
     auto walk = [&](auto&& visit, uint32_t vword = 0u) __attribute__((always_inline)) {
-        walk_w(NoWeight{}, [](const Entry&) { return 0u; }, visit, vword, [] {});
+        walk_w(NoWeight{}, [](const Entry&) { return 0u; }, visit, vword, [](uint32_t, uint32_t) {});
     };
     // Field end without the record's trailing '\r' (csv_geo.rs:1056-1060).
     // A string cell that cleans to itself: no quote convention, or doubled
@@ -1425,6 +1428,13 @@ __device__ __forceinline__ void scan_tile(const Params& P, const uint32_t tile, 
         // contiguous stores instead of per-cell byte stores.
         constexpr uint32_t SSTAGE = 8192;
         __shared__ uint8_t sstage[SSTAGE];
+        // Per list entry of a walk round: end of its output range and
+        // output-to-source offset of its bytes. Rounds of up to four entries
+        // per thread fit; the table's size keeps four workgroups per CU.
+        constexpr uint32_t RT_N = WG * 4;
+        constexpr int32_t RAW_SKIP = -32768;
+        __shared__ uint16_t rt_end[RT_N];
+        __shared__ int16_t rt_delta[RT_N];
         for (uint32_t j = 0; j < n_cols; ++j) {
             const ColDesc& c = P.cols[j];
             if (c.type != FALX_COL_STR) continue;
@@ -1442,27 +1452,45 @@ __device__ __forceinline__ void scan_tile(const Params& P, const uint32_t tile, 
             // COL_STAT_GROUP: present cells, cleaned bytes, their byte sum.
             const bool stat_col = P.partials != nullptr && j < (uint32_t)COL_STAT_GROUP;
             unsigned long long s_cells = 0, s_bytes = 0, s_sum = 0;
+            // Gathered in LDS, a round's cells are placed in two steps. The
+            // visit records, per list entry, the end of its output range and
+            // for an unquoted cell inside the tile the offset from output to
+            // source byte; it cleans the other cells itself. Then all threads
+            // copy the round's raw bytes, each a contiguous run of the output,
+            // finding its first cell by binary search over the range ends:
+            // work per byte, not per cell, so long and short cells share it.
+            uint32_t round_lo = 0;
+            // Worth it for long cells: per-lane copies run as long as the
+            // wave's longest cell, the shared copy costs two barriers and a
+            // search per round. The tile's mean cell length decides.
+            const uint32_t t_rows = sh.inc.n_rows - pre.n_rows + 1u;
+            const bool coop = gather && WG * walk_items() <= RT_N && total > 13u * t_rows;
             walk_w(
                 Weighted{},
                 [&](const Entry& e) -> uint32_t {
                     return (!e.flag && c.index == e.ord) ? clean_len(e.fs, field_end(e)) : 0u;
                 },
                 [&](const Entry& e, uint32_t, uint32_t before) {
-                    if (!e.valid || e.flag) return;
-                    const bool cellhere = c.index == e.ord;
+                    if (!e.valid) return;
+                    const bool cellhere = !e.flag && c.index == e.ord;
                     const bool live = e.row < rows_cap;
                     const uint32_t at = arena + before;
-                    const uint32_t to = field_end(e);
+                    const uint32_t to = cellhere ? field_end(e) : e.fs;
                     uint32_t n = 0;
+                    int32_t delta = RAW_SKIP;
                     if (cellhere) {
+                        const bool in_tile = e.fs >= tile_start && to <= tile_start + tile_len;
                         if (raw_cell(e.fs, to)) {
                             n = to - e.fs;
-                            if (live || stat_col)
+                            if (coop && in_tile && live) {
+                                delta = (int32_t)(e.fs - tile_start) - (int32_t)before;
+                            } else if (live || stat_col) {
                                 for (uint32_t k = 0; k < n; ++k) {
                                     const uint8_t b = src.at(e.fs + k);
                                     if (live) put((uint64_t)at + k, b);
                                     s_sum += b;
                                 }
+                            }
                         } else {
                             uint64_t q = at;
                             n = clean_cell(src, e.fs, to, cd, [&](uint8_t b) {
@@ -1475,9 +1503,39 @@ __device__ __forceinline__ void scan_tile(const Params& P, const uint32_t tile, 
                         s_cells += 1;
                         s_bytes += n;
                     }
-                    if (e.is_term && live) offs[e.row + 1] = (int32_t)(at + n);
+                    if (coop) {
+                        rt_end[e.k & (RT_N - 1)] = (uint16_t)(before + n);
+                        rt_delta[e.k & (RT_N - 1)] = (int16_t)delta;
+                    }
+                    if (e.is_term && !e.flag && live) offs[e.row + 1] = (int32_t)(at + n);
                 },
-                trailer_word, [] {});
+                trailer_word,
+                [&](uint32_t base, uint32_t n_round) {
+                    if (!coop) return;
+                    __syncthreads();
+                    const uint32_t hi = rt_end[(base + n_round - 1) & (RT_N - 1)];
+                    const uint32_t per = (hi - round_lo + WG - 1) / WG;
+                    const uint32_t q0 = round_lo + tid * per, q1 = min(q0 + per, hi);
+                    if (q0 < q1) {
+                        uint32_t lo = 0, up = n_round - 1;   // first entry whose range ends past q0
+                        while (lo < up) {
+                            const uint32_t mid = (lo + up) >> 1;
+                            if (rt_end[(base + mid) & (RT_N - 1)] > q0) up = mid; else lo = mid + 1;
+                        }
+                        uint32_t i = lo;
+                        for (uint32_t q = q0; q < q1; ++q) {
+                            while (rt_end[(base + i) & (RT_N - 1)] <= q) ++i;
+                            const int32_t d = rt_delta[(base + i) & (RT_N - 1)];
+                            if (d != RAW_SKIP) {
+                                const uint8_t b = sh.bytes[(int32_t)q + d];
+                                sstage[q] = b;
+                                s_sum += b;
+                            }
+                        }
+                    }
+                    round_lo = hi;
+                    __syncthreads();
+                });
             if (gather) {
                 // Cells of rows past rows_cap were not staged; the call
                 // reports CAPACITY then, so those bytes need no value.
