@@ -41,7 +41,7 @@ constexpr int BPW = 32;                   // 64-byte blocks per wave
 constexpr int TILE = NWAVE * BPW * 64;    // 8192 bytes per workgroup
 constexpr uint64_t EVEN_BITS = 0x5555555555555555ULL;
 constexpr uint32_t SPIN_LIMIT = 1u << 22;
-constexpr int STAT_GROUP = 4;             // stat columns accumulated per STATS pass
+constexpr int STAT_GROUP = 2;             // stat columns accumulated per STATS pass
 constexpr int COL_STAT_GROUP = 2;         // stat columns fused into a COLUMNS pass
 
 enum Mode : int { M_INDEX = 0, M_TAPE, M_COLUMNS, M_STATS, M_FIELDBYTES, M_LOGFMT, M_SUMMARY, M_STRINGS };
@@ -1566,9 +1566,15 @@ __device__ __forceinline__ void scan_tile(const Params& P, const uint32_t tile, 
     }
 }
 
+// Occupancy the STATS pass is held to (its statistics accumulators sit just
+// over the register budget of 4 waves/SIMD otherwise); 1 = no constraint.
+template <int MODE>
+constexpr int plain_waves() { return MODE == M_STATS ? 4 : 1; }
+
 // One workgroup per tile, tiles in ticket order.
 template <bool HQ, bool ESC, bool CMT, int MODE>
-__global__ __launch_bounds__(WG) void k_scan(Params P) {
+__global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(plain_waves<MODE>())))
+void k_scan(Params P) {
     __shared__ Shared sh;
     if (threadIdx.x == 0) sh.tile = (uint32_t)(atomicAdd(P.ticket, 1ULL) - P.ticket_base);
     __syncthreads();
@@ -2093,7 +2099,7 @@ DEV const Params& const_params(const Params* base, uint32_t i) {
 
 // Persistent tile pass of table `tab` (a summary or emit table).
 template <int MODE>
-constexpr int batch_waves() { return MODE == M_STATS ? 3 : 4; }
+constexpr int batch_waves() { return 4; }
 
 template <bool HQ, bool ESC, bool CMT, int MODE>
 __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(batch_waves<MODE>())))
