@@ -26,6 +26,8 @@
 #include <stdint.h>
 #include <string.h>
 
+#include <type_traits>
+
 #include "../../include/falx_gpu.h"
 #include "cells.h"
 #include "ndjson_walk.h"
@@ -682,19 +684,6 @@ __device__ __forceinline__ void scan_tile(const Params& P, const uint32_t tile, 
         ls0 = uni(v);
     }
     const uint8_t qb = (uint8_t)P.quote, eb = (uint8_t)P.escape, cb = (uint8_t)P.comment;
-    auto classify = [&](uint32_t blk) {
-        uint8_t c = sh.bytes[blk * 64 + ln];
-        BM m;
-        if (P.n_sepb <= 4)
-            m.cand = ballot(c == P.sepb[0] || c == P.sepb[1] || c == P.sepb[2] || c == P.sepb[3]);
-        else
-            m.cand = ballot(sh.cls[c] != 0);
-        m.nl = ballot(c == '\n');
-        m.q = HQ ? ballot(c == qb) : 0ULL;
-        m.esc = ESC ? ballot(c == eb) : 0ULL;
-        m.cm = CMT ? ballot(c == cb) : 0ULL;
-        return m;
-    };
 
     const auto live_mask = [&](uint32_t off) -> uint64_t {
         return off + 64 <= tile_len ? ~0ULL : (off < tile_len ? ((1ULL << (tile_len - off)) - 1) : 0ULL);
@@ -705,16 +694,49 @@ __device__ __forceinline__ void scan_tile(const Params& P, const uint32_t tile, 
     // __ballot, and hands block b's masks to lane b: the mask algebra, the
     // counters and the compaction then run for all blocks at once, one per
     // lane, instead of block by block on the scalar unit.
+    // The candidate set is tested by up to four compares, each its own
+    // ballot ORed as scalar masks (NSEP = their number), or by the 256-entry
+    // table (NSEP = 0); eight blocks' bytes are read before any is used.
     BM mine{0ULL, 0ULL, 0ULL, 0ULL, 0ULL};
-#pragma unroll 4
-    for (int b = 0; b < BPW; ++b) {
-        const BM m = classify(w * BPW + b);
-        const bool me = ln == (uint32_t)b;
-        mine.cand = me ? m.cand : mine.cand;
-        mine.nl = me ? m.nl : mine.nl;
-        if (HQ) mine.q = me ? m.q : mine.q;
-        if (ESC) mine.esc = me ? m.esc : mine.esc;
-        if (CMT) mine.cm = me ? m.cm : mine.cm;
+    auto classify_all = [&](auto nsep) __attribute__((always_inline)) {
+        constexpr int NSEP = decltype(nsep)::value;
+        const uint32_t s0 = P.sepb[0], s1 = P.sepb[1], s2 = P.sepb[2], s3 = P.sepb[3];
+#pragma unroll
+        for (int b0 = 0; b0 < BPW; b0 += 8) {
+            uint32_t cc[8];
+#pragma unroll
+            for (int i = 0; i < 8; ++i) cc[i] = sh.bytes[(w * BPW + b0 + i) * 64 + ln];
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                const uint32_t c = cc[i];
+                uint64_t cand;
+                if (NSEP == 0) {
+                    cand = ballot(sh.cls[c] != 0);
+                } else {
+                    cand = ballot(c == s0);
+                    if (NSEP > 1) cand |= ballot(c == s1);
+                    if (NSEP > 2) cand |= ballot(c == s2);
+                    if (NSEP > 3) cand |= ballot(c == s3);
+                }
+                const uint64_t nl = ballot(c == '\n');
+                const uint64_t q = HQ ? ballot(c == qb) : 0ULL;
+                const uint64_t esc = ESC ? ballot(c == eb) : 0ULL;
+                const uint64_t cm = CMT ? ballot(c == cb) : 0ULL;
+                const bool me = ln == (uint32_t)(b0 + i);
+                mine.cand = me ? cand : mine.cand;
+                mine.nl = me ? nl : mine.nl;
+                if (HQ) mine.q = me ? q : mine.q;
+                if (ESC) mine.esc = me ? esc : mine.esc;
+                if (CMT) mine.cm = me ? cm : mine.cm;
+            }
+        }
+    };
+    switch (P.n_sepb) {
+    case 1: classify_all(std::integral_constant<int, 1>{}); break;
+    case 2: classify_all(std::integral_constant<int, 2>{}); break;
+    case 3: classify_all(std::integral_constant<int, 3>{}); break;
+    case 4: classify_all(std::integral_constant<int, 4>{}); break;
+    default: classify_all(std::integral_constant<int, 0>{}); break;
     }
     const bool own = ln < (uint32_t)BPW;
     const uint32_t off = (w * BPW + (own ? ln : 0u)) * 64;      // this lane's block in the tile
