@@ -955,9 +955,11 @@ __device__ __forceinline__ void scan_tile(const Params& P, const uint32_t tile, 
                 wsum = w_before + block_excl_sum(mine, sh.sum_sh, wt);
                 w_before += wt;
             }
+            // A weighted walk's visit returns the entry's weight (it has just
+            // computed it), so the weight runs once more only for the scan.
             each([&](const Entry& e, uint32_t terms) {
-                visit(e, terms, wsum);
-                if (WEIGHTED && e.valid) wsum += weight(e);
+                if constexpr (WEIGHTED) wsum += visit(e, terms, wsum);
+                else visit(e, terms, wsum);
             });
             after(base, min(WG * items, n_walk - base));
             terms_before += tot.a;
@@ -1507,8 +1509,8 @@ __device__ __forceinline__ void scan_tile(const Params& P, const uint32_t tile, 
                 [&](const Entry& e) -> uint32_t {
                     return (!e.flag && c.index == e.ord) ? clean_len(e.fs, field_end(e)) : 0u;
                 },
-                [&](const Entry& e, uint32_t, uint32_t before) {
-                    if (!e.valid) return;
+                [&](const Entry& e, uint32_t, uint32_t before) -> uint32_t {
+                    if (!e.valid) return 0u;
                     const bool cellhere = !e.flag && c.index == e.ord;
                     const bool live = e.row < rows_cap;
                     const uint32_t at = arena + before;
@@ -1545,6 +1547,7 @@ __device__ __forceinline__ void scan_tile(const Params& P, const uint32_t tile, 
                         rt_delta[e.k & (RT_N - 1)] = (int16_t)delta;
                     }
                     if (e.is_term && !e.flag && live) offs[e.row + 1] = (int32_t)(at + n);
+                    return n;
                 },
                 trailer_word,
                 [&](uint32_t base, uint32_t n_round) {
