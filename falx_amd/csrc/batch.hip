@@ -333,7 +333,6 @@ falx_status falx_batch_submit(falx_batch* b, const falx_request* reqs, uint32_t 
         P.str_len = reinterpret_cast<uint32_t*>(base + L.str_len);
         P.str_base = reinterpret_cast<const uint32_t*>(base + L.str_base);
         P.result = reinterpret_cast<unsigned long long*>(res_rel + RES_SLOTS_OFF);   // patched
-        P.ticket = nullptr;
         char* part_base = base + align_up(L.total, 256);
         const uint32_t nc = D.spec.n_columns ? D.spec.n_columns : 1;
         size_t part_bytes = 0;
@@ -419,8 +418,6 @@ falx_status falx_batch_submit(falx_batch* b, const falx_request* reqs, uint32_t 
     }
     // ---- blob layout -----------------------------------------------------
     size_t o = align_up(sizeof(BatchHdr), 256);
-    const size_t tick_off = o;
-    o = align_up(o + 8 * N_TAB, 256);
     const size_t par_off = o;
     o = align_up(o + sizeof(Params) * B.params.size(), 256);
     const size_t pre_off = o;
@@ -451,7 +448,7 @@ falx_status falx_batch_submit(falx_batch* b, const falx_request* reqs, uint32_t 
     }
     char* h = b->h_blob;
     char* d = b->d_blob;
-    memset(h, 0, o);   // tickets and results records start at zero
+    memset(h, 0, o);   // results records start at zero
     auto dres = [&](const void* rel) { return static_cast<void*>(d + b->res_off + reinterpret_cast<uintptr_t>(rel)); };
     for (Params& P : B.params) P.result = static_cast<unsigned long long*>(dres(P.result));
     for (BPrefix& A : B.pre) A.result = static_cast<unsigned long long*>(dres(A.result));
@@ -463,7 +460,6 @@ falx_status falx_batch_submit(falx_batch* b, const falx_request* reqs, uint32_t 
     H.red = reinterpret_cast<const BRed*>(d + red_off);
     H.nd = reinterpret_cast<const BNd*>(d + nd_off);
     H.fill = reinterpret_cast<const BFill*>(d + fill_off);
-    H.tickets = reinterpret_cast<unsigned long long*>(d + tick_off);
     uint64_t sig = 0;
     for (uint32_t t = 0; t < N_TAB; ++t) {
         const auto& T = B.tab[t];

@@ -14,8 +14,6 @@ struct falx_ctx {
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     float last_ms = 0.f;
-    unsigned long long* d_ticket = nullptr;
-    unsigned long long ticket_next = 0;
     unsigned long long* d_result = nullptr;
     unsigned long long* h_result = nullptr;
     void* d_lb = nullptr;
@@ -108,12 +106,9 @@ int dialect_kind(const falx_spec* s) {
     return esc ? 5 : 4;
 }
 
-// Launches one k_scan pass over all tiles. Tiles take tickets in launch
-// order, so each pass gets its own range of the context's ticket counter.
+// Launches one k_scan pass over all tiles, a workgroup per tile.
 template <int MODE>
 hipError_t launch_scan(falx_ctx* c, Params& P, int kind) {
-    P.ticket_base = c->ticket_next;
-    c->ticket_next += P.n_tiles;
     const hipStream_t s = c->stream;
     dim3 g(P.n_tiles), b(WG);
     switch (kind) {
@@ -175,9 +170,14 @@ void spec_params(const falx_spec* spec, const uint8_t* d, uint64_t len, Params& 
         bool nl = false, sep = false;
         for (uint32_t i = 0; i < spec->n_structural; ++i) (spec->structural[i] == '\n' ? nl : sep) = true;
         if (nl || !sep) P.cand['\n' >> 5] |= 1u << ('\n' & 31);
-        // Up to four candidate bytes are tested by compares instead of the table.
+        // Up to four candidate bytes besides '\n' are tested by compares
+        // instead of the table; '\n' comes from the newline test.
         for (uint32_t b = 0; b < 256; ++b)
             if ((P.cand[b >> 5] >> (b & 31)) & 1u) {
+                if (b == '\n') {
+                    P.nl_cand = 1;
+                    continue;
+                }
                 if (P.n_sepb < 4) P.sepb[P.n_sepb] = b;
                 ++P.n_sepb;
             }
@@ -204,7 +204,6 @@ falx_status prepare(falx_ctx* c, const falx_spec* spec, const uint8_t* d, uint64
     LbLayout L = lb_layout(P.n_tiles, P.n_str, ns);
     P.str_len = reinterpret_cast<uint32_t*>(static_cast<char*>(c->d_lb) + L.str_len);
     P.str_base = reinterpret_cast<const uint32_t*>(static_cast<char*>(c->d_lb) + L.str_base);
-    P.ticket = c->d_ticket;
     P.result = c->d_result;
     return FALX_OK;
 }
@@ -294,10 +293,10 @@ falx_status falx_ctx_create(int device, falx_ctx** out) {
     c->device = device;
     if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
-        hipMalloc(&c->d_ticket, 256) != hipSuccess || hipMalloc(&c->d_result, R_SLOTS * 8) != hipSuccess ||
+        hipMalloc(&c->d_result, R_SLOTS * 8) != hipSuccess ||
         hipMalloc(&c->d_red, RED_BYTES) != hipSuccess ||
         hipHostMalloc(&c->h_result, R_SLOTS * 8, hipHostMallocDefault) != hipSuccess ||
-        hipMemset(c->d_ticket, 0, 256) != hipSuccess || hipDeviceSynchronize() != hipSuccess) {
+        hipDeviceSynchronize() != hipSuccess) {
         falx_ctx_destroy(c);
         return FALX_E_DEVICE;
     }
@@ -313,7 +312,6 @@ void falx_ctx_destroy(falx_ctx* c) {
     if (c->d_lb) (void)hipFree(c->d_lb);
     if (c->d_part) (void)hipFree(c->d_part);
     if (c->d_red) (void)hipFree(c->d_red);
-    if (c->d_ticket) (void)hipFree(c->d_ticket);
     if (c->d_result) (void)hipFree(c->d_result);
     if (c->h_result) (void)hipHostFree(c->h_result);
     if (c->ev0) (void)hipEventDestroy(c->ev0);

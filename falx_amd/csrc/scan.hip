@@ -83,15 +83,14 @@ struct Params {
     uint32_t len;
     uint32_t n_tiles;
     uint32_t cand[8];
-    uint32_t sepb[4];         // the candidate bytes when there are at most 4 (0x100 = unused)
-    uint32_t n_sepb;          // number of candidate bytes; more than 4 uses the cand[] table
+    uint32_t sepb[4];         // the candidate bytes other than '\n' when at most 4 (0x100 = unused)
+    uint32_t n_sepb;          // number of candidate bytes other than '\n'; more than 4 uses the cand[] table
+    uint32_t nl_cand;         // '\n' is a candidate
     int32_t quote, escape, comment;
     uint32_t n_cols, n_str;
     uint32_t col_group;       // STATS: first column of this launch's group
     uint64_t rows_cap;
     ColDesc cols[FALX_MAX_COLUMNS];
-    unsigned long long* ticket;
-    unsigned long long ticket_base;
     uint32_t* tsum;           // M_SUMMARY out / emit in: per tile ts_stride words (tile summary)
     const uint32_t* tpre;     // emit modes: per tile 8 words, entry state + exclusive counters
     uint32_t ts_stride;
@@ -115,6 +114,17 @@ DEV uint32_t mbcnt(uint64_t m) {
 }
 
 DEV uint32_t uni(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// v_writelane_b32 (the LLVM intrinsic; clang has no builtin for it here):
+// `old` with lane `lane` replaced by the wave-uniform `val`.
+extern "C" __device__ int fx_writelane(int val, int lane, int old) __asm("llvm.amdgcn.writelane.i32");
+
+// v with lane `lane` (wave-uniform) replaced by the wave-uniform value s.
+DEV uint64_t put_lane(uint64_t v, uint64_t s, uint32_t lane) {
+    const uint32_t lo = (uint32_t)fx_writelane((int)(uint32_t)s, (int)lane, (int)(uint32_t)v);
+    const uint32_t hi = (uint32_t)fx_writelane((int)(uint32_t)(s >> 32), (int)lane, (int)(uint32_t)(v >> 32));
+    return ((uint64_t)hi << 32) | lo;
+}
 
 DEV uint32_t popc(uint64_t m) { return (uint32_t)__popcll(m); }
 
@@ -250,31 +260,6 @@ DEV void c7_store(uint32_t* p, const Counts& c) {
     p[6] = c.last_term;
 }
 
-// Adds one 64-byte block's structurals S (terminators T, comment-record
-// terminators F) at byte `base` to the running counters.
-DEV void count_block(Counts& wc, uint64_t S, uint64_t T, uint64_t F, uint32_t base) {
-    const uint32_t nS = popc(S);
-    if (!nS) return;
-    const uint64_t seps = S & ~T;
-    wc.n_struct += nS;
-    wc.n_terms += popc(T);
-    wc.n_seps += popc(seps);
-    wc.n_rows += popc(T & ~F);
-    if (T) {
-        const uint32_t hb = hibit(T);
-        wc.has_term = 1;
-        const uint64_t above = hb == 63 ? 0ULL : (seps & ~((2ULL << hb) - 1));
-        wc.ord = popc(above);
-        wc.last_term = base + hb + 1;
-    } else {
-        wc.ord += popc(seps);
-    }
-    const uint32_t h = hibit(S);
-    const uint64_t S2 = S & ~(1ULL << h);
-    wc.prev_struct = S2 ? base + hibit(S2) + 1 : wc.last_struct;
-    wc.last_struct = base + h + 1;
-}
-
 // Tile summary layout (u32 words, stride 8 + 7*NS): [0] tile transfer
 // function, [1..4] the four waves' functions, [8 + 7s ..] counters of the
 // tile entered in state s. Functions pack 4 bits per state.
@@ -391,31 +376,67 @@ struct BM {
 };
 
 
-DEV Counts shfl_down_counts(const Counts& c, int d) {
-    Counts r;
-    r.n_struct = __shfl_down(c.n_struct, d);
-    r.n_seps = __shfl_down(c.n_seps, d);
-    r.n_terms = __shfl_down(c.n_terms, d);
-    r.n_rows = __shfl_down(c.n_rows, d);
-    r.has_term = __shfl_down(c.has_term, d);
-    r.ord = __shfl_down(c.ord, d);
-    r.last_struct = __shfl_down(c.last_struct, d);
-    r.prev_struct = __shfl_down(c.prev_struct, d);
-    r.last_term = __shfl_down(c.last_term, d);
-    r.pad0 = r.pad1 = r.pad2 = 0;
-    return r;
+// v + the value `CTRL` selects by DPP (0 where no lane is selected or the
+// row is outside ROWM). All lanes of the wave must be active.
+template <int CTRL, int ROWM>
+DEV uint32_t dpp_plus(uint32_t v) {
+    return v + (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, ROWM, 0xF, false);
 }
 
-// Counters of lanes 0..n-1 composed in lane order (lane 0 = first bytes);
-// the result is in lane 0. n is a power of two.
-template <int N>
-DEV Counts wave_compose(Counts c) {
+// Inclusive sum over lanes 0..ln of the wave: shifts within rows of 16
+// lanes (row_shr 1, 2, 4, 8), then row 0's total into row 1 and row 2's into
+// row 3 (row_bcast 15), then rows 0-1's total into rows 2-3 (row_bcast 31).
+DEV uint32_t wave_incl_sum(uint32_t v) {
+    v = dpp_plus<0x111, 0xF>(v);
+    v = dpp_plus<0x112, 0xF>(v);
+    v = dpp_plus<0x114, 0xF>(v);
+    v = dpp_plus<0x118, 0xF>(v);
+    v = dpp_plus<0x142, 0xA>(v);
+    v = dpp_plus<0x143, 0xC>(v);
+    return v;
+}
+
+DEV uint32_t lane_value(uint32_t v, uint32_t lane) { return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)lane); }
+
+// The counters of a wave's blocks composed in lane order: lane b holds block
+// b's structurals S, terminators T and comment-record terminators F at byte
+// `base` (lanes without a block hold zero masks); the result is in every
+// lane. Sums go through one scan as two 16-bit halves per word (a wave
+// holds at most 4096 bytes); the positions come from the highest lanes with
+// a structural or a terminator.
+DEV Counts wave_counts(uint64_t S, uint64_t T, uint64_t F, uint32_t base) {
+    Counts c = counts_zero();
+    const uint64_t has_s = ballot(S != 0ULL);
+    if (!has_s) return c;
+    const uint64_t has_t = ballot(T != 0ULL);
     const uint32_t ln = __lane_id();
-#pragma unroll
-    for (int d = 1; d < N; d <<= 1) {
-        const Counts hi = shfl_down_counts(c, d);
-        if ((ln & (2 * d - 1)) == 0) c = compose(c, hi);
+    const uint64_t seps = S & ~T;
+    // Separators after the wave's last terminator: all of a lane's above
+    // the terminator lane, those past its last terminator in that lane.
+    uint32_t ord = popc(seps);
+    if (has_t) {
+        const uint32_t L = hibit(has_t);
+        const uint32_t hb = T ? hibit(T) : 0u;
+        if (ln == L) ord = hb == 63u ? 0u : popc(seps & ~((2ULL << hb) - 1ULL));
+        else if (ln < L) ord = 0u;
+        c.has_term = 1;
+        c.last_term = lane_value(base + hb + 1u, L);
     }
+    const uint32_t a = lane_value(wave_incl_sum(popc(S) | (popc(T) << 16)), 63);
+    const uint32_t b = lane_value(wave_incl_sum(popc(T & ~F) | (ord << 16)), 63);
+    c.n_struct = a & 0xFFFFu;
+    c.n_terms = a >> 16;
+    c.n_seps = c.n_struct - c.n_terms;
+    c.n_rows = b & 0xFFFFu;
+    c.ord = b >> 16;
+    // Last and second-to-last structural of the wave.
+    const uint32_t H = hibit(has_s);
+    const uint32_t h = S ? hibit(S) : 0u;
+    const uint64_t S2 = S & ~(1ULL << h);
+    c.last_struct = lane_value(base + h + 1u, H);
+    const uint32_t p2 = lane_value(S2 ? base + hibit(S2) + 1u : 0u, H);
+    const uint64_t below = has_s & ~(1ULL << H);
+    c.prev_struct = p2 ? p2 : (below ? lane_value(base + h + 1u, hibit(below)) : 0u);
     return c;
 }
 
@@ -691,13 +712,15 @@ __device__ __forceinline__ void scan_tile(const Params& P, const uint32_t tile, 
 
     // ---- classification: ballots per block, then lane b owns block b ------
     // Each wave classifies its BPW blocks with one byte per lane and
-    // __ballot, and hands block b's masks to lane b: the mask algebra, the
-    // counters and the compaction then run for all blocks at once, one per
-    // lane, instead of block by block on the scalar unit.
-    // The candidate set is tested by up to four compares, each its own
-    // ballot ORed as scalar masks (NSEP = their number), or by the 256-entry
-    // table (NSEP = 0); eight blocks' bytes are read before any is used.
+    // __ballot, and writes block b's masks into lane b: the mask algebra,
+    // the counters and the compaction then run for all blocks at once, one
+    // per lane, instead of block by block on the scalar unit.
+    // The candidate bytes other than '\n' are tested by up to four compares,
+    // each its own ballot ORed as scalar masks (NSEP = their number; '\n'
+    // reuses the newline ballot), or by the 256-entry table (NSEP < 0);
+    // eight blocks' bytes are read before any is used.
     BM mine{0ULL, 0ULL, 0ULL, 0ULL, 0ULL};
+    const bool nl_cand = P.nl_cand != 0u;
     auto classify_all = [&](auto nsep) __attribute__((always_inline)) {
         constexpr int NSEP = decltype(nsep)::value;
         const uint32_t s0 = P.sepb[0], s1 = P.sepb[1], s2 = P.sepb[2], s3 = P.sepb[3];
@@ -708,35 +731,34 @@ __device__ __forceinline__ void scan_tile(const Params& P, const uint32_t tile, 
             for (int i = 0; i < 8; ++i) cc[i] = sh.bytes[(w * BPW + b0 + i) * 64 + ln];
 #pragma unroll
             for (int i = 0; i < 8; ++i) {
+                const uint32_t b = (uint32_t)(b0 + i);
                 const uint32_t c = cc[i];
+                const uint64_t nl = ballot(c == '\n');
                 uint64_t cand;
-                if (NSEP == 0) {
+                if (NSEP < 0) {
                     cand = ballot(sh.cls[c] != 0);
                 } else {
-                    cand = ballot(c == s0);
+                    cand = nl_cand ? nl : 0ULL;
+                    if (NSEP > 0) cand |= ballot(c == s0);
                     if (NSEP > 1) cand |= ballot(c == s1);
                     if (NSEP > 2) cand |= ballot(c == s2);
                     if (NSEP > 3) cand |= ballot(c == s3);
                 }
-                const uint64_t nl = ballot(c == '\n');
-                const uint64_t q = HQ ? ballot(c == qb) : 0ULL;
-                const uint64_t esc = ESC ? ballot(c == eb) : 0ULL;
-                const uint64_t cm = CMT ? ballot(c == cb) : 0ULL;
-                const bool me = ln == (uint32_t)(b0 + i);
-                mine.cand = me ? cand : mine.cand;
-                mine.nl = me ? nl : mine.nl;
-                if (HQ) mine.q = me ? q : mine.q;
-                if (ESC) mine.esc = me ? esc : mine.esc;
-                if (CMT) mine.cm = me ? cm : mine.cm;
+                mine.cand = put_lane(mine.cand, cand, b);
+                mine.nl = put_lane(mine.nl, nl, b);
+                if (HQ) mine.q = put_lane(mine.q, ballot(c == qb), b);
+                if (ESC) mine.esc = put_lane(mine.esc, ballot(c == eb), b);
+                if (CMT) mine.cm = put_lane(mine.cm, ballot(c == cb), b);
             }
         }
     };
     switch (P.n_sepb) {
+    case 0: classify_all(std::integral_constant<int, 0>{}); break;
     case 1: classify_all(std::integral_constant<int, 1>{}); break;
     case 2: classify_all(std::integral_constant<int, 2>{}); break;
     case 3: classify_all(std::integral_constant<int, 3>{}); break;
     case 4: classify_all(std::integral_constant<int, 4>{}); break;
-    default: classify_all(std::integral_constant<int, 0>{}); break;
+    default: classify_all(std::integral_constant<int, -1>{}); break;
     }
     const bool own = ln < (uint32_t)BPW;
     const uint32_t off = (w * BPW + (own ? ln : 0u)) * 64;      // this lane's block in the tile
@@ -759,12 +781,14 @@ __device__ __forceinline__ void scan_tile(const Params& P, const uint32_t tile, 
         Counts* wsum = reinterpret_cast<Counts*>(sh.list);   // the list is unused here
 #pragma unroll
         for (int s = 0; s < NS; ++s) {
-            Counts c = counts_zero();
+            uint64_t S = 0, T = 0, F = 0;
             if (own) {
                 const StepOut o = block_step<HQ, ESC, CMT>(mine, fn_apply(f_excl, (uint32_t)s), ls);
-                count_block(c, o.S & live, o.T & live, o.F & live, tile_start + off);
+                S = o.S & live;
+                T = o.T & live;
+                F = o.F & live;
             }
-            c = wave_compose<BPW>(c);
+            const Counts c = wave_counts(S, T, F, tile_start + off);
             if (ln == 0) wsum[w * NS + s] = c;
         }
         if (ln == 63) sh.wfn[w] = f_incl;
@@ -814,17 +838,9 @@ __device__ __forceinline__ void scan_tile(const Params& P, const uint32_t tile, 
             F = o.F & live;
         }
         const uint32_t n = popc(S);
-        uint32_t incl = n;
-#pragma unroll
-        for (int d = 1; d < 64; d <<= 1) {
-            const uint32_t t = __shfl_up(incl, d);
-            if (ln >= (uint32_t)d) incl += t;
-        }
-        const uint32_t cnt = __shfl(incl, 63);
-        Counts wc = counts_zero();
-        count_block(wc, S, T, F, tile_start + off);
-        wc = wave_compose<BPW>(wc);
-        if (ln == 0) { sh.wcnt[w] = wc; sh.wcount[w] = cnt; }
+        const uint32_t incl = wave_incl_sum(n);
+        const Counts wc = wave_counts(S, T, F, tile_start + off);
+        if (ln == 0) { sh.wcnt[w] = wc; sh.wcount[w] = wc.n_struct; }
         __syncthreads();
         // One list for the tile, in byte order: this lane's structurals go
         // after the earlier waves' and the earlier lanes' entries.
@@ -1611,76 +1627,14 @@ __device__ __forceinline__ void scan_tile(const Params& P, const uint32_t tile, 
 template <int MODE>
 constexpr int plain_waves() { return MODE == M_STATS ? 4 : 1; }
 
-// One workgroup per tile, tiles in ticket order.
+// One workgroup per tile. Tiles depend on nothing but the tile prefix
+// (tpre) and their own bytes, so the block index is the tile and its
+// loads start at once.
 template <bool HQ, bool ESC, bool CMT, int MODE>
 __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(plain_waves<MODE>())))
 void k_scan(Params P) {
     __shared__ Shared sh;
-    if (threadIdx.x == 0) sh.tile = (uint32_t)(atomicAdd(P.ticket, 1ULL) - P.ticket_base);
-    __syncthreads();
-    scan_tile<HQ, ESC, CMT, MODE>(P, uni(sh.tile), sh);
-}
-
-// --------------------------------------------------------------------------
-// batches: persistent workgroups over a queue of (document, chunk) units
-// --------------------------------------------------------------------------
-// A stage's work is a table of segments, one per document (or per document
-// and column): segment i covers units [start[i], start[i + 1]) of the
-// stage's unit range and names its Params (req[i]); for tile stages a unit
-// is a chunk of chunk[i] consecutive tiles. Tables live at fixed device
-// addresses, so a captured launch sequence can be replayed for every batch.
-constexpr uint32_t BATCH_MAX_SEG = 1024;
-
-struct SegTable {
-    uint32_t n, total;
-    uint32_t req[BATCH_MAX_SEG];
-    uint32_t chunk[BATCH_MAX_SEG];
-    uint32_t start[BATCH_MAX_SEG + 1];
-};
-
-// Segment of unit u < T.total: the last i with start[i] <= u.
-DEV uint32_t seg_find(const SegTable& T, uint32_t u) {
-    uint32_t lo = 0, hi = T.n;   // start[lo] <= u < start[hi]
-    while (hi - lo > 1) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (T.start[mid] <= u) lo = mid; else hi = mid;
-    }
-    return lo;
-}
-
-// Persistent tile pass: workgroups take chunk tickets until the table is
-// drained; each chunk runs scan_tile over its tiles with its document's
-// Params. Documents of different sizes and specs (of one dialect kind) mix
-// freely in one launch.
-template <bool HQ, bool ESC, bool CMT, int MODE>
-__global__ __launch_bounds__(WG) void k_scan_batch(const Params* params, const SegTable* T,
-                                                   unsigned long long* ticket) {
-    __shared__ Shared sh;
-    __shared__ uint32_t s_p, s_t0, s_t1;
-    for (;;) {
-        if (threadIdx.x == 0) {
-            const uint32_t u = (uint32_t)atomicAdd(ticket, 1ULL);
-            if (u >= T->total) {
-                s_p = ~0u;
-            } else {
-                const uint32_t i = seg_find(*T, u);
-                const uint32_t pi = T->req[i], c = T->chunk[i];
-                const uint32_t t0 = (u - T->start[i]) * c;
-                s_p = pi;
-                s_t0 = t0;
-                s_t1 = min(t0 + c, params[pi].n_tiles);
-            }
-        }
-        __syncthreads();
-        const uint32_t pi = uni(s_p), t0 = uni(s_t0), t1 = uni(s_t1);
-        __syncthreads();
-        if (pi == ~0u) return;
-        const Params& P = params[pi];
-        for (uint32_t t = t0; t < t1; ++t) {
-            scan_tile<HQ, ESC, CMT, MODE>(P, t, sh);
-            __syncthreads();
-        }
-    }
+    scan_tile<HQ, ESC, CMT, MODE>(P, blockIdx.x, sh);
 }
 
 // --------------------------------------------------------------------------
@@ -2099,11 +2053,18 @@ struct SegT {
     const uint32_t* start;
 };
 
+// Segment of unit u < T.total: the last i with start[i] <= u, by a 64-way
+// search (each lane tests one start per round: two rounds up to 4096
+// segments). Called by all lanes of a wave with the same u.
 DEV uint32_t seg_find(const SegT& T, uint32_t u) {
-    uint32_t lo = 0, hi = T.n;   // start[lo] <= u < start[hi]
+    const uint32_t ln = __lane_id();
+    uint32_t lo = 0, hi = T.n;   // start[lo] <= u < start[hi] (start[n] = total)
     while (hi - lo > 1) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (T.start[mid] <= u) lo = mid; else hi = mid;
+        const uint32_t step = (hi - lo + 63u) >> 6;
+        const uint32_t i = lo + ln * step;
+        const uint64_t le = ballot(i < hi && T.start[i] <= u);   // lane 0 (i = lo) always
+        lo += hibit(le) * step;
+        hi = min(hi, lo + step);
     }
     return lo;
 }
@@ -2124,7 +2085,6 @@ struct BatchHdr {
     const BRed* red;
     const BNd* nd;
     const BFill* fill;
-    unsigned long long* tickets;   // one per table, zero on upload
     SegT tab[N_TAB];
 };
 
@@ -2141,19 +2101,19 @@ DEV const Params& const_params(const Params* base, uint32_t i) {
 template <int MODE>
 constexpr int batch_waves() { return 4; }
 
+// Units go to workgroups by stride (unit u to workgroup u mod grid): no
+// shared counter, whose atomics serialise at one address.
 template <bool HQ, bool ESC, bool CMT, int MODE>
 __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(batch_waves<MODE>())))
 void k_tiles_batch(const BatchHdr* H, uint32_t tab) {
     __shared__ Shared sh;
     __shared__ uint32_t s_p, s_t0, s_t1;
     const SegT& T = H->tab[tab];
-    for (;;) {
-        if (threadIdx.x == 0) {
-            const uint32_t u = (uint32_t)atomicAdd(&H->tickets[tab], 1ULL);
-            if (u >= T.total) {
-                s_p = ~0u;
-            } else {
-                const uint32_t i = seg_find(T, u);
+    const uint32_t total = T.total;
+    for (uint32_t u = blockIdx.x; u < total; u += gridDim.x) {
+        if (threadIdx.x < 64) {
+            const uint32_t i = seg_find(T, u);
+            if (threadIdx.x == 0) {
                 const uint32_t pi = T.req[i], c = T.chunk[i];
                 const uint32_t t0 = (u - T.start[i]) * c;
                 s_p = pi;
@@ -2164,7 +2124,6 @@ void k_tiles_batch(const BatchHdr* H, uint32_t tab) {
         __syncthreads();
         const uint32_t pi = uni(s_p), t0 = uni(s_t0), t1 = uni(s_t1);
         __syncthreads();
-        if (pi == ~0u) return;
         for (uint32_t t = t0; t < t1; ++t) {
             // The Params address passes an empty asm each tile, so their
             // loads stay inside the loop: hoisted, they would stay live
@@ -2253,13 +2212,11 @@ __global__ __launch_bounds__(WG) void k_reduce_batch(const BatchHdr* H, uint32_t
 __global__ __launch_bounds__(WG) void k_ndjson_batch(const BatchHdr* H) {
     __shared__ uint32_t s_p, s_t0, s_t1;
     const SegT& T = H->tab[T_NDJ];
-    for (;;) {
-        if (threadIdx.x == 0) {
-            const uint32_t u = (uint32_t)atomicAdd(&H->tickets[T_NDJ], 1ULL);
-            if (u >= T.total) {
-                s_p = ~0u;
-            } else {
-                const uint32_t i = seg_find(T, u);
+    const uint32_t total = T.total;
+    for (uint32_t u = blockIdx.x; u < total; u += gridDim.x) {
+        if (threadIdx.x < 64) {
+            const uint32_t i = seg_find(T, u);
+            if (threadIdx.x == 0) {
                 const uint32_t pi = T.req[i], c = T.chunk[i];
                 const uint32_t t0 = (u - T.start[i]) * c;
                 s_p = pi;
@@ -2270,7 +2227,6 @@ __global__ __launch_bounds__(WG) void k_ndjson_batch(const BatchHdr* H) {
         __syncthreads();
         const uint32_t pi = uni(s_p), t0 = uni(s_t0), t1 = uni(s_t1);
         __syncthreads();
-        if (pi == ~0u) return;
         const BNd& A = H->nd[pi];
         for (uint32_t t = t0; t < t1; ++t) {
             ndjson_tile(A.data, A.len, A.n_tiles, A.parts, t);
