@@ -281,94 +281,6 @@ DEV uint32_t fn_then_rt(uint32_t f, uint32_t g, uint32_t ns) {
 }
 
 // --------------------------------------------------------------------------
-// block-wide scans over 256 threads
-// --------------------------------------------------------------------------
-struct Scan3 {
-    uint32_t a, b, d;
-    int32_t c;  // max
-};
-
-DEV Scan3 wave_incl(Scan3 v) {
-    const uint32_t ln = __lane_id();
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        uint32_t ta = __shfl_up(v.a, d);
-        uint32_t tb = __shfl_up(v.b, d);
-        uint32_t td = __shfl_up(v.d, d);
-        int32_t tc = __shfl_up(v.c, d);
-        if (ln >= (uint32_t)d) { v.a += ta; v.b += tb; v.d += td; v.c = tc > v.c ? tc : v.c; }
-    }
-    return v;
-}
-
-// Exclusive scan (sum a, b, d; max c) across the workgroup; `tot` gets the
-// inclusive total. `sh` is 4-wave scratch.
-DEV Scan3 block_excl(Scan3 v, Scan3* sh, Scan3& tot) {
-    const uint32_t ln = __lane_id(), w = uni(threadIdx.x >> 6);
-    Scan3 inc = wave_incl(v);
-    Scan3 ex;
-    ex.a = __shfl_up(inc.a, 1);
-    ex.b = __shfl_up(inc.b, 1);
-    ex.d = __shfl_up(inc.d, 1);
-    ex.c = __shfl_up(inc.c, 1);
-    if (ln == 0) { ex.a = 0; ex.b = 0; ex.d = 0; ex.c = -1; }
-    if (ln == 63) sh[w] = inc;
-    __syncthreads();
-    Scan3 base{0, 0, 0, -1};
-    for (uint32_t k = 0; k < w; ++k) {
-        base.a += sh[k].a;
-        base.b += sh[k].b;
-        base.d += sh[k].d;
-        base.c = sh[k].c > base.c ? sh[k].c : base.c;
-    }
-    tot = Scan3{0, 0, 0, -1};
-    for (int k = 0; k < NWAVE; ++k) {
-        tot.a += sh[k].a;
-        tot.b += sh[k].b;
-        tot.d += sh[k].d;
-        tot.c = sh[k].c > tot.c ? sh[k].c : tot.c;
-    }
-    __syncthreads();
-    ex.a += base.a;
-    ex.b += base.b;
-    ex.d += base.d;
-    ex.c = base.c > ex.c ? base.c : ex.c;
-    return ex;
-}
-
-DEV uint32_t block_excl_sum(uint32_t v, uint32_t* sh, uint32_t& tot) {
-    const uint32_t ln = __lane_id(), w = uni(threadIdx.x >> 6);
-    uint32_t inc = v;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        uint32_t t = __shfl_up(inc, d);
-        if (ln >= (uint32_t)d) inc += t;
-    }
-    if (ln == 63) sh[w] = inc;
-    __syncthreads();
-    uint32_t base = 0;
-    for (uint32_t k = 0; k < w; ++k) base += sh[k];
-    tot = 0;
-    for (int k = 0; k < NWAVE; ++k) tot += sh[k];
-    __syncthreads();
-    return base + inc - v;
-}
-
-template <class T>
-DEV T block_reduce_sum(T v, T* sh) {
-    // fixed-order tree: wave butterfly, then waves in order
-#pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d);
-    const uint32_t ln = __lane_id(), w = uni(threadIdx.x >> 6);
-    if (ln == 0) sh[w] = v;
-    __syncthreads();
-    T t = sh[0];
-    for (int k = 1; k < NWAVE; ++k) t += sh[k];
-    __syncthreads();
-    return t;
-}
-
-// --------------------------------------------------------------------------
 // dialect step (one 64-byte block, wave-uniform)
 // --------------------------------------------------------------------------
 struct BM {
@@ -394,6 +306,53 @@ DEV uint32_t wave_incl_sum(uint32_t v) {
     v = dpp_plus<0x142, 0xA>(v);
     v = dpp_plus<0x143, 0xC>(v);
     return v;
+}
+
+// The value of the lane DPP control CTRL selects, for controls that select
+// a lane of the row for every lane (quad permutes, row rotations).
+template <int CTRL>
+DEV uint32_t dpp_get(uint32_t v) { return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, 0xF, 0xF, false); }
+template <int CTRL>
+DEV unsigned long long dpp_get(unsigned long long v) {
+    return ((unsigned long long)dpp_get<CTRL>((uint32_t)(v >> 32)) << 32) | dpp_get<CTRL>((uint32_t)v);
+}
+template <int CTRL>
+DEV double dpp_get(double v) {
+    return __longlong_as_double((long long)dpp_get<CTRL>((unsigned long long)__double_as_longlong(v)));
+}
+
+// Sum over the lane's row of 16 (quad swaps 1-0-3-2 and 2-3-0-1, then
+// rotations by 4 and 8); lane 0 of the row has the row's sum in the order
+// ((l0 + l1) + (l2 + l3)) + ... All lanes of the wave must be active.
+template <class T>
+DEV T row_sum16(T v) {
+    v += dpp_get<0xB1>(v);
+    v += dpp_get<0x4E>(v);
+    v += dpp_get<0x124>(v);
+    v += dpp_get<0x128>(v);
+    return v;
+}
+
+template <int CTRL, int ROWM>
+DEV uint32_t dpp_max(uint32_t v) {
+    return max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, ROWM, 0xF, false));
+}
+
+// Inclusive max over lanes 0..ln (values >= 0), the same DPP steps as
+// wave_incl_sum.
+DEV uint32_t wave_incl_max(uint32_t v) {
+    v = dpp_max<0x111, 0xF>(v);
+    v = dpp_max<0x112, 0xF>(v);
+    v = dpp_max<0x114, 0xF>(v);
+    v = dpp_max<0x118, 0xF>(v);
+    v = dpp_max<0x142, 0xA>(v);
+    v = dpp_max<0x143, 0xC>(v);
+    return v;
+}
+
+// Lane ln - 1's value (0 in lane 0): DPP wave_shr 1.
+DEV uint32_t dpp_prev_lane(uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x138, 0xF, 0xF, false);
 }
 
 DEV uint32_t lane_value(uint32_t v, uint32_t lane) { return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)lane); }
@@ -438,6 +397,74 @@ DEV Counts wave_counts(uint64_t S, uint64_t T, uint64_t F, uint32_t base) {
     const uint64_t below = has_s & ~(1ULL << H);
     c.prev_struct = p2 ? p2 : (below ? lane_value(base + h + 1u, hibit(below)) : 0u);
     return c;
+}
+
+// --------------------------------------------------------------------------
+// block-wide scans over 256 threads
+// --------------------------------------------------------------------------
+// A walk round's scan: terminators (a) and rows (b), summed, and the last
+// terminator's list index (c, max; -1 = none).
+struct Scan3 {
+    uint32_t a, b;
+    int32_t c;
+};
+
+// Exclusive scan across the workgroup; `tot` gets the inclusive total. a
+// and b travel as the 16-bit halves of one word (a round holds at most
+// WG * ITEMS_MAX = 2048 entries), c + 1 >= 0 as a max scan; DPP within each
+// wave, then the wave totals through `sh` (NWAVE words of each).
+DEV Scan3 block_excl(Scan3 v, uint32_t* sh, Scan3& tot) {
+    const uint32_t ln = __lane_id(), w = uni(threadIdx.x >> 6);
+    const uint32_t p = v.a | (v.b << 16), m = (uint32_t)(v.c + 1);
+    const uint32_t pi = wave_incl_sum(p), mi = wave_incl_max(m);
+    const uint32_t px = pi - p, mx = dpp_prev_lane(mi);   // exclusive
+    if (ln == 63) { sh[w] = pi; sh[NWAVE + w] = mi; }
+    __syncthreads();
+    uint32_t pb = 0, mb = 0, pt = 0, mt = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < (uint32_t)NWAVE; ++k) {
+        const uint32_t pk = sh[k], mk = sh[NWAVE + k];
+        if (k < w) { pb += pk; mb = max(mb, mk); }
+        pt += pk;
+        mt = max(mt, mk);
+    }
+    __syncthreads();
+    tot = Scan3{pt & 0xFFFFu, pt >> 16, (int32_t)mt - 1};
+    const uint32_t pe = pb + px, me = max(mb, mx);
+    return Scan3{pe & 0xFFFFu, pe >> 16, (int32_t)me - 1};
+}
+
+// Exclusive sum across the workgroup; `tot` gets the total. DPP within
+// each wave, the wave totals through `sh` (NWAVE words).
+DEV uint32_t block_excl_sum(uint32_t v, uint32_t* sh, uint32_t& tot) {
+    const uint32_t ln = __lane_id(), w = uni(threadIdx.x >> 6);
+    const uint32_t inc = wave_incl_sum(v);
+    if (ln == 63) sh[w] = inc;
+    __syncthreads();
+    uint32_t base = 0;
+    tot = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < (uint32_t)NWAVE; ++k) {
+        const uint32_t t = sh[k];
+        if (k < w) base += t;
+        tot += t;
+    }
+    __syncthreads();
+    return base + inc - v;
+}
+
+template <class T>
+DEV T block_reduce_sum(T v, T* sh) {
+    // fixed-order tree: wave butterfly, then waves in order
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d);
+    const uint32_t ln = __lane_id(), w = uni(threadIdx.x >> 6);
+    if (ln == 0) sh[w] = v;
+    __syncthreads();
+    T t = sh[0];
+    for (int k = 1; k < NWAVE; ++k) t += sh[k];
+    __syncthreads();
+    return t;
 }
 
 template <bool HQ, bool ESC, bool CMT>
@@ -589,7 +616,7 @@ struct Shared {
     Counts pre;       // tile exclusive prefix
     Counts inc;       // tile inclusive
     uint32_t tile;
-    Scan3 scan_sh[NWAVE];
+    uint32_t scan_sh[2 * NWAVE];
     uint32_t sum_sh[NWAVE];
     double dsum_sh[NWAVE];
     unsigned long long usum_sh[NWAVE];
@@ -928,7 +955,7 @@ __device__ __forceinline__ void scan_tile(const Params& P, const uint32_t tile, 
             const uint32_t fs0 = k0 == 0 ? pre.last_struct
                                          : (k0 - 1 < n_walk ? tile_start + (word_at(k0 - 1) & 0x3FFFu) + 1 : 0u);
             // This thread's terminators, rows and last terminator.
-            Scan3 agg{0, 0, 0, -1};
+            Scan3 agg{0, 0, -1};
             uint32_t fs = fs0;
             for (uint32_t i = 0; i < items; ++i) {
                 const Entry e = entry_at(k0 + i, fs);
@@ -1128,35 +1155,31 @@ __device__ __forceinline__ void scan_tile(const Params& P, const uint32_t tile, 
             if (jj == j) acc[jj].add(o);
     };
     // Per-tile statistics of up to STAT_GROUP columns plus the record count,
-    // reduced in a fixed order (wave butterfly, then waves in order) with one
-    // exchange through LDS: StatPart field f of column j is slot 6j + f.
-    __shared__ unsigned long long stat_red[NWAVE][STAT_GROUP * 6 + 1];
+    // reduced in a fixed order: sums over each row of 16 lanes by DPP, then
+    // the 16 row sums of the workgroup (waves in order, rows in order) from
+    // one exchange through LDS. StatPart field f of column j is slot 6j + f.
+    constexpr uint32_t NROW = NWAVE * 4;
+    __shared__ unsigned long long stat_red[NROW][STAT_GROUP * 6 + 1];
     auto acc_publish = [&](auto& acc, uint32_t first, uint32_t nc, unsigned long long records) {
         constexpr uint32_t N = sizeof(acc) / sizeof(acc[0]);
-        const auto wsum = [](unsigned long long v) {
-#pragma unroll
-            for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d);
-            return v;
-        };
-        const auto wsum_f = [](double v) {
-#pragma unroll
-            for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d);
-            return v;
-        };
+        const bool row_head = (ln & 15u) == 0u;
+        unsigned long long* red = &stat_red[w * 4 + (ln >> 4)][0];
 #pragma unroll
         for (uint32_t j = 0; j < N; ++j) {
             if (j >= nc) break;
-            const unsigned long long v0 = wsum(acc[j].values), v1 = wsum(acc[j].csum), v2 = wsum(acc[j].bytes),
-                                     v3 = wsum((unsigned long long)acc[j].fx_sum), v4 = wsum(acc[j].fx_cells);
-            const double v5 = wsum_f(acc[j].dsum);
-            if (ln == 0) {
-                unsigned long long* r = &stat_red[w][6 * j];
+            const uint32_t v0 = row_sum16(acc[j].values), v2 = row_sum16(acc[j].bytes),
+                           v4 = row_sum16(acc[j].fx_cells);
+            const unsigned long long v1 = row_sum16(acc[j].csum),
+                                     v3 = row_sum16((unsigned long long)acc[j].fx_sum);
+            const double v5 = row_sum16(acc[j].dsum);
+            if (row_head) {
+                unsigned long long* r = red + 6 * j;
                 r[0] = v0; r[1] = v1; r[2] = v2; r[3] = v3; r[4] = v4;
                 r[5] = (unsigned long long)__double_as_longlong(v5);
             }
         }
-        const unsigned long long rec = wsum(records);
-        if (ln == 0) stat_red[w][STAT_GROUP * 6] = rec;
+        const unsigned long long rec = row_sum16(records);
+        if (row_head) red[STAT_GROUP * 6] = rec;
         __syncthreads();
         StatPart* parts = reinterpret_cast<StatPart*>(P.partials);
         if (tid < nc * 6) {
@@ -1164,17 +1187,17 @@ __device__ __forceinline__ void scan_tile(const Params& P, const uint32_t tile, 
             unsigned long long* dst = reinterpret_cast<unsigned long long*>(&parts[(size_t)tile * P.n_cols + first + j]);
             if (f == 5) {
                 double t = __longlong_as_double((long long)stat_red[0][tid]);
-                for (int k = 1; k < NWAVE; ++k) t += __longlong_as_double((long long)stat_red[k][tid]);
+                for (uint32_t k = 1; k < NROW; ++k) t += __longlong_as_double((long long)stat_red[k][tid]);
                 dst[5] = (unsigned long long)__double_as_longlong(t);
             } else {
                 unsigned long long t = stat_red[0][tid];
-                for (int k = 1; k < NWAVE; ++k) t += stat_red[k][tid];
+                for (uint32_t k = 1; k < NROW; ++k) t += stat_red[k][tid];
                 dst[f] = t;
             }
         }
         if (tid == 0 && first == 0) {
             unsigned long long t = 0;
-            for (int k = 0; k < NWAVE; ++k) t += stat_red[k][STAT_GROUP * 6];
+            for (uint32_t k = 0; k < NROW; ++k) t += stat_red[k][STAT_GROUP * 6];
             reinterpret_cast<unsigned long long*>(parts + (size_t)P.n_tiles * P.n_cols)[tile] = t;
         }
     };
