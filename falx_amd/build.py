@@ -13,7 +13,7 @@ REPO = os.path.dirname(HERE)
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 GPU_SOURCES = ["csrc/falx_gpu.hip"]
-GPU_DEPS = ["csrc/falx_gpu.hip", "csrc/scan.hip", "csrc/batch.hip", "csrc/cells.h", "csrc/ndjson_walk.h", "csrc/num_swar.h",
+GPU_DEPS = ["csrc/falx_gpu.hip", "csrc/scan.hip", "csrc/batch.hip", "csrc/cells.h", "csrc/ndjson_walk.h", "csrc/num_swar.h", "csrc/regions.h",
             "csrc/pow5_table.h"]
 # Packed-FP32 instructions stay off in every device compile.
 HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",

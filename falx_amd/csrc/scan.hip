@@ -32,6 +32,7 @@
 #include "cells.h"
 #include "ndjson_walk.h"
 #include "num_swar.h"
+#include "regions.h"
 
 #define DEV __device__ __forceinline__
 
@@ -129,50 +130,6 @@ DEV uint64_t put_lane(uint64_t v, uint64_t s, uint32_t lane) {
 DEV uint32_t popc(uint64_t m) { return (uint32_t)__popcll(m); }
 
 DEV uint32_t hibit(uint64_t m) { return 63u - (uint32_t)__clzll(m); }
-
-DEV uint64_t prefix_xor(uint64_t x) {
-    x ^= x << 1;
-    x ^= x << 2;
-    x ^= x << 4;
-    x ^= x << 8;
-    x ^= x << 16;
-    x ^= x << 32;
-    return x;
-}
-
-DEV uint64_t range_mask(uint32_t from, uint32_t to) {
-    uint64_t hi = to >= 64 ? ~0ULL : ((1ULL << to) - 1);
-    return hi & ~((1ULL << from) - 1);
-}
-
-// Three-state region resolver (csv_hash.rs:2149-2220; interp.rs:163-203).
-DEV uint64_t resolve_regions(uint64_t q, uint64_t s, uint64_t n, uint32_t& state) {
-    if (state == 0 && (q | s) == 0) return 0;
-    if (state != 2 && s == 0) {
-        uint64_t inert = prefix_xor(q);
-        if (state == 1) inert = ~inert;
-        state = (uint32_t)(inert >> 63);
-        return inert;
-    }
-    uint64_t inert = 0;
-    uint32_t run_start = 0;
-    uint64_t events = q | s | n;
-    while (events) {
-        uint32_t p = (uint32_t)__ffsll((long long)events) - 1;
-        uint64_t bit = 1ULL << p;
-        if (state == 1) {
-            if (q & bit) { inert |= range_mask(run_start, p); state = 0; }
-        } else if (state == 2) {
-            if (n & bit) { inert |= range_mask(run_start, p); state = 0; }
-        } else {
-            if (q & bit) { state = 1; run_start = p; }
-            else if (s & bit) { state = 2; run_start = p; }
-        }
-        events &= events - 1;
-    }
-    if (state != 0) inert |= range_mask(run_start, 64);
-    return inert;
-}
 
 // Transfer functions on <= 8 states, 4 bits per entry.
 DEV uint32_t fn_apply(uint32_t f, uint32_t s) { return (f >> (4 * s)) & 15u; }

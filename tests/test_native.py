@@ -1,6 +1,7 @@
 """CPU sweeps of the device-side code paths that are written once for host
-and device (falx_amd/csrc/cells.h, ndjson_walk.h): their host build is run
-against the oracle on randomized cells and records."""
+and device (falx_amd/csrc/cells.h, ndjson_walk.h, num_swar.h, regions.h):
+their host build is run against the oracle on randomized cells and
+records, and the block region resolver against a byte-at-a-time automaton."""
 import os
 import subprocess
 
@@ -40,4 +41,12 @@ def test_register_number_parser_matches_oracle(ora, seed):
     exe = _build("swar_check", ora)
     r = subprocess.run([exe, str(seed), "300000"], capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
+    assert "agree" in r.stdout
+
+
+@pytest.mark.parametrize("seed", [5, 0xFEED])
+def test_block_regions_match_byte_automaton(ora, seed):
+    exe = _build("regions_check", ora)
+    r = subprocess.run([exe, str(seed), "400000"], capture_output=True, text=True)
+    assert r.returncode == 0, r.stdout + r.stderr
     assert "agree" in r.stdout
