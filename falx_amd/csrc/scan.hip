@@ -478,6 +478,61 @@ DEV StepOut block_step(const BM& m, uint32_t s_in, uint64_t ls) {
     return o;
 }
 
+// a or b, field by field: a conditional over two objects would select
+// between their addresses and keep them in memory.
+DEV StepOut pick(bool first, const StepOut& a, const StepOut& b) {
+    StepOut o;
+    o.S = first ? a.S : b.S;
+    o.T = first ? a.T : b.T;
+    o.F = first ? a.F : b.F;
+    o.s_out = first ? a.s_out : b.s_out;
+    return o;
+}
+
+// One block's step from each of N entry states, held in registers (the
+// members are named, so nothing is indexed at run time).
+template <int N>
+struct BlockSteps;
+template <>
+struct BlockSteps<1> {
+    StepOut s0;
+    template <bool HQ, bool ESC, bool CMT>
+    DEV void fill(const BM& m, uint64_t ls) { s0 = block_step<HQ, ESC, CMT>(m, 0u, ls); }
+    DEV uint32_t fn() const { return s0.s_out; }
+    DEV StepOut at(uint32_t) const { return s0; }
+};
+template <>
+struct BlockSteps<3> {
+    StepOut s0, s1, s2;
+    template <bool HQ, bool ESC, bool CMT>
+    DEV void fill(const BM& m, uint64_t ls) {
+        s0 = block_step<HQ, ESC, CMT>(m, 0u, ls);
+        s1 = block_step<HQ, ESC, CMT>(m, 1u, ls);
+        s2 = block_step<HQ, ESC, CMT>(m, 2u, ls);
+    }
+    DEV uint32_t fn() const { return s0.s_out | (s1.s_out << 4) | (s2.s_out << 8); }
+    DEV StepOut at(uint32_t e) const { return pick(e == 0u, s0, pick(e == 1u, s1, s2)); }
+};
+template <>
+struct BlockSteps<6> {
+    StepOut s0, s1, s2, s3, s4, s5;
+    template <bool HQ, bool ESC, bool CMT>
+    DEV void fill(const BM& m, uint64_t ls) {
+        s0 = block_step<HQ, ESC, CMT>(m, 0u, ls);
+        s1 = block_step<HQ, ESC, CMT>(m, 1u, ls);
+        s2 = block_step<HQ, ESC, CMT>(m, 2u, ls);
+        s3 = block_step<HQ, ESC, CMT>(m, 3u, ls);
+        s4 = block_step<HQ, ESC, CMT>(m, 4u, ls);
+        s5 = block_step<HQ, ESC, CMT>(m, 5u, ls);
+    }
+    DEV uint32_t fn() const {
+        return s0.s_out | (s1.s_out << 4) | (s2.s_out << 8) | (s3.s_out << 12) | (s4.s_out << 16) | (s5.s_out << 20);
+    }
+    DEV StepOut at(uint32_t e) const {
+        return pick(e < 3u, pick(e == 0u, s0, pick(e == 1u, s1, s2)), pick(e == 3u, s3, pick(e == 4u, s4, s5)));
+    }
+};
+
 // Transfer function of one block over all entry states.
 template <bool HQ, bool ESC, bool CMT>
 DEV uint32_t block_fn(const BM& m, uint64_t ls) {
@@ -751,7 +806,22 @@ __device__ __forceinline__ void scan_tile(const Params& P, const uint32_t tile, 
     const uint32_t prev_hi = __shfl_up((uint32_t)(mine.nl >> 32), 1);
     const uint64_t ls = ln == 0 ? ls0 : (uint64_t)(prev_hi >> 31);
     // Exclusive scan of the blocks' transfer functions over the lanes.
-    uint32_t f_incl = own ? block_fn<HQ, ESC, CMT>(mine, ls) : fn_ident<NS>();
+    // Comment dialects resolve regions per entry state: each state's step of
+    // this lane's block is computed once, for its transfer function and for
+    // the passes below, which pick the step of the block's entry state.
+    constexpr bool STEPS = CMT;
+    BlockSteps<STEPS ? NS : 1> steps;
+    uint32_t f_incl;
+    if constexpr (STEPS) {
+        steps.template fill<HQ, ESC, CMT>(mine, ls);
+        f_incl = own ? steps.fn() : fn_ident<NS>();
+    } else {
+        f_incl = own ? block_fn<HQ, ESC, CMT>(mine, ls) : fn_ident<NS>();
+    }
+    const auto step_of = [&](uint32_t entry) __attribute__((always_inline)) -> StepOut {
+        if constexpr (STEPS) return steps.at(entry);
+        else return block_step<HQ, ESC, CMT>(mine, entry, ls);
+    };
 #pragma unroll
     for (int d = 1; d < 64; d <<= 1) {
         const uint32_t g = __shfl_up(f_incl, d);
@@ -767,7 +837,7 @@ __device__ __forceinline__ void scan_tile(const Params& P, const uint32_t tile, 
         for (int s = 0; s < NS; ++s) {
             uint64_t S = 0, T = 0, F = 0;
             if (own) {
-                const StepOut o = block_step<HQ, ESC, CMT>(mine, fn_apply(f_excl, (uint32_t)s), ls);
+                const StepOut o = step_of(fn_apply(f_excl, (uint32_t)s));
                 S = o.S & live;
                 T = o.T & live;
                 F = o.F & live;
@@ -816,7 +886,7 @@ __device__ __forceinline__ void scan_tile(const Params& P, const uint32_t tile, 
         const uint32_t entry = fn_apply(f_excl, uni(sh.wentry[w]));
         uint64_t S = 0, T = 0, F = 0;
         if (own) {
-            const StepOut o = block_step<HQ, ESC, CMT>(mine, entry, ls);
+            const StepOut o = step_of(entry);
             S = o.S & live;
             T = o.T & live;
             F = o.F & live;
