@@ -1962,11 +1962,22 @@ DEV void ndjson_tile(const uint8_t* data, uint32_t len, uint32_t n_tiles, NdPart
         }
         *reinterpret_cast<uint4*>(&bytes[i]) = v;
     }
-    if (tid == 0) {
-        slow.lock = 0;
-        int64_t p = (int64_t)tile_start - 1;
-        while (p >= 0 && data[p] != '\n') --p;
-        prev_nl = p;
+    // The newline before the tile (-1: none), found by wave 0 reading 64
+    // bytes per step backwards, one per lane.
+    if (w == 0) {
+        int64_t found = -1;
+        for (int64_t hi = (int64_t)tile_start; hi > 0; hi -= 64) {
+            const int64_t p = hi - 64 + (int64_t)ln;
+            const uint64_t m = ballot(p >= 0 && data[p] == '\n');
+            if (m) {
+                found = hi - 64 + (int64_t)hibit(m);
+                break;
+            }
+        }
+        if (ln == 0) {
+            slow.lock = 0;
+            prev_nl = found;
+        }
     }
     __syncthreads();
     uint32_t cnt = 0;
