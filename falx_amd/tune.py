@@ -30,7 +30,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 DEFAULT_PATH = os.path.join(HERE, "tuning.json")
 TILE = 8192
 CHUNKS = [1, 2, 4, 8, 16]
-GRID_MULT = [1, 2, 4, 6, 8]
+GRID_MULT = [2, 4, 8, 12, 16]
 CLASS_TILES = [1, 4, 16, 64, 256, 1024, 4096, 16384]   # one size per class
 BATCH_BYTES = 96 << 20                                   # per timed batch (at most)
 MAX_DOCS = 128
