@@ -811,24 +811,37 @@ __device__ __forceinline__ void scan_tile(const Params& P, const uint32_t tile, 
     // the passes below, which pick the step of the block's entry state.
     constexpr bool STEPS = CMT;
     BlockSteps<STEPS ? NS : 1> steps;
-    uint32_t f_incl;
-    if constexpr (STEPS) {
-        steps.template fill<HQ, ESC, CMT>(mine, ls);
-        f_incl = own ? steps.fn() : fn_ident<NS>();
-    } else {
-        f_incl = own ? block_fn<HQ, ESC, CMT>(mine, ls) : fn_ident<NS>();
-    }
     const auto step_of = [&](uint32_t entry) __attribute__((always_inline)) -> StepOut {
         if constexpr (STEPS) return steps.at(entry);
         else return block_step<HQ, ESC, CMT>(mine, entry, ls);
     };
+    // Exclusive scan of the blocks' functions over the lanes (f_excl), and
+    // the wave's whole function (f_incl in lane 63).
+    uint32_t f_incl, f_excl;
+    if constexpr (NS == 1) {
+        f_incl = f_excl = 0u;
+    } else if constexpr (HQ && !ESC && !CMT) {
+        // Outside and inside quotes: a block flips the state when it holds
+        // an odd number of quotes, so a lane's entry function is the parity
+        // of the odd blocks below it.
+        const uint64_t odd = ballot(own && (popc(mine.q) & 1u));
+        f_excl = (mbcnt(odd) & 1u) ? 0x01u : 0x10u;
+        f_incl = (popc(odd) & 1u) ? 0x01u : 0x10u;
+    } else {
+        if constexpr (STEPS) {
+            steps.template fill<HQ, ESC, CMT>(mine, ls);
+            f_incl = own ? steps.fn() : fn_ident<NS>();
+        } else {
+            f_incl = own ? block_fn<HQ, ESC, CMT>(mine, ls) : fn_ident<NS>();
+        }
 #pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const uint32_t g = __shfl_up(f_incl, d);
-        if (ln >= (uint32_t)d) f_incl = fn_then<NS>(g, f_incl);
+        for (int d = 1; d < 64; d <<= 1) {
+            const uint32_t g = __shfl_up(f_incl, d);
+            if (ln >= (uint32_t)d) f_incl = fn_then<NS>(g, f_incl);
+        }
+        f_excl = __shfl_up(f_incl, 1);
+        if (ln == 0) f_excl = fn_ident<NS>();
     }
-    uint32_t f_excl = __shfl_up(f_incl, 1);
-    if (ln == 0) f_excl = fn_ident<NS>();
 
     // ---- summary mode: every entry state's trajectory through the tile ----
     if (MODE == M_SUMMARY) {
