@@ -266,8 +266,9 @@ falx_status falx_parse_logfmt_pairs(falx_ctx* ctx, const uint8_t* d_data,
  * Batches. No counterpart in the reference, which parses one document per
  * call (the generated module functions above). A batch runs many documents
  * of uneven sizes and mixed specs through each pipeline stage in one
- * persistent launch over a device queue of (document, chunk of tiles)
- * units; the launch sequence is captured in a hipGraph per set of stages
+ * persistent launch over a device table of (document, chunk of tiles)
+ * units, which workgroups take by stride; the launch sequence is captured
+ * in a hipGraph per set of stages
  * and replayed. Every document's outputs and result are bit-identical to
  * the plain entry point's (falx_index_structurals, falx_parse_columns[_with_stats],
  * falx_parse_csv_stats, falx_parse_field_bytes, falx_parse_logfmt_pairs,
@@ -286,7 +287,7 @@ typedef enum falx_op {
 
 typedef struct falx_request {
     uint32_t op;                  /* falx_op */
-    uint32_t chunk;               /* tiles per queue unit; 0 = the tuning's choice */
+    uint32_t chunk;               /* tiles per unit; 0 = the tuning's choice */
     const falx_spec* spec;        /* ops 1-5; read at submit */
     const uint8_t* d_data;        /* device input, 16-byte aligned */
     uint64_t len;
@@ -307,7 +308,7 @@ typedef struct falx_result {
     falx_ndjson_id_score_stats ndjson;
 } falx_result;
 
-/* Launch shape. chunk[k]: tiles per queue unit for documents of [4^k,
+/* Launch shape. chunk[k]: tiles per unit for documents of [4^k,
  * 4^(k+1)) tiles (8 KiB each); workgroups: persistent grid of the tile
  * stages and of the other stages. Chosen per spec and size class by
  * falx_amd/tune.py. */
