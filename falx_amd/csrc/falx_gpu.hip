@@ -56,7 +56,7 @@ uint32_t summary_stride(uint32_t ns) { return (uint32_t)TS_COUNTS + (uint32_t)C7
 // summaries, group and tile prefixes, and per tile and string column the
 // cleaned byte count and arena offset.
 struct LbLayout {
-    size_t tsum, gsum, gpre, tpre, str_len, str_base, total;
+    size_t tsum, gsum, gpre, tpre, str_len, str_base, fix, total;
 };
 
 LbLayout lb_layout(size_t n, uint32_t n_str, uint32_t ns) {
@@ -70,6 +70,7 @@ LbLayout lb_layout(size_t n, uint32_t n_str, uint32_t ns) {
     L.tpre = o; o = align_up(o + 32 * n, 256);
     L.str_len = o; o = align_up(o + 4 * (size_t)n_str * n, 256);
     L.str_base = o; o = align_up(o + 4 * (size_t)n_str * n, 256);
+    L.fix = o; o = align_up(o + 4 * n, 256);
     L.total = o;
     return L;
 }
@@ -120,6 +121,19 @@ hipError_t launch_scan(falx_ctx* c, Params& P, int kind) {
     default: hipLaunchKernelGGL((k_scan<true, true, true, MODE>), g, b, 0, s, P); break;
     }
     return hipGetLastError();
+}
+
+// A COLUMNS or STATS pass as its fast form over all tiles, then the full
+// form over the tiles the fast one flagged (cells outside the register
+// parsers' reach: quoted or long numbers, inf/nan, undecided roundings).
+template <int FAST, int FULL>
+hipError_t launch_fast(falx_ctx* c, Params& P, int kind) {
+    const LbLayout L = lb_layout(P.n_tiles, P.n_str, kKindStates[kind]);
+    P.fixup = reinterpret_cast<uint32_t*>(static_cast<char*>(c->d_lb) + L.fix);
+    hipError_t e = launch_scan<FAST>(c, P, kind);
+    if (e == hipSuccess) e = launch_scan<FULL>(c, P, kind);
+    P.fixup = nullptr;
+    return e;
 }
 
 // The tile-prefix pipeline: tile summaries (every entry state), then the
@@ -484,13 +498,13 @@ static falx_status columns_impl(falx_ctx* c, const falx_spec* spec, const uint8_
     }
     HIP_TRY(mark_start(c));
     HIP_TRY(run_prefix(c, P, dialect_kind(spec)));
-    HIP_TRY(launch_scan<M_COLUMNS>(c, P, dialect_kind(spec)));
+    HIP_TRY((launch_fast<M_COLUMNS_FAST, M_COLUMNS>(c, P, dialect_kind(spec))));
     // Statistics of columns past the first group come from extra passes.
     if (stats) {
         Params Q = P;
         for (uint32_t g = COL_STAT_GROUP; g < spec->n_columns; g += STAT_GROUP) {
             Q.col_group = g;
-            HIP_TRY(launch_scan<M_STATS>(c, Q, dialect_kind(spec)));
+            HIP_TRY((launch_fast<M_STATS_FAST, M_STATS>(c, Q, dialect_kind(spec))));
         }
     }
     // String columns: arena bases from the per-tile byte counts, then the
@@ -579,7 +593,7 @@ falx_status falx_parse_csv_stats(falx_ctx* c, const falx_spec* spec, const uint8
     HIP_TRY(run_prefix(c, P, dialect_kind(spec)));
     for (uint32_t g = 0; g < groups; ++g) {
         P.col_group = g * STAT_GROUP;
-        HIP_TRY(launch_scan<M_STATS>(c, P, dialect_kind(spec)));
+        HIP_TRY((launch_fast<M_STATS_FAST, M_STATS>(c, P, dialect_kind(spec))));
     }
     HIP_TRY(mark_end(c));
     return reduce_stats(c, spec, P.n_tiles, out);

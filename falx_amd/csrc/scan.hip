@@ -42,12 +42,25 @@ constexpr int WG = 256;
 constexpr int NWAVE = WG / 64;
 constexpr int BPW = 32;                   // 64-byte blocks per wave
 constexpr int TILE = NWAVE * BPW * 64;    // 8192 bytes per workgroup
+// The LDS copy of a tile also holds PRE bytes before it and HALO bytes
+// after it, so that cells crossing the tile's edges (a tile's first entry
+// usually ends a cell that began in the previous tile) are read from LDS;
+// it has a few more bytes of room after that for word reads.
+constexpr int PRE = 64;
+constexpr int HALO = 64;
+constexpr int TILE_LDS = PRE + TILE + HALO + 16;
 constexpr uint64_t EVEN_BITS = 0x5555555555555555ULL;
 constexpr uint32_t SPIN_LIMIT = 1u << 22;
 constexpr int STAT_GROUP = 2;             // stat columns accumulated per STATS pass
 constexpr int COL_STAT_GROUP = 2;         // stat columns fused into a COLUMNS pass
 
-enum Mode : int { M_INDEX = 0, M_TAPE, M_COLUMNS, M_STATS, M_FIELDBYTES, M_LOGFMT, M_SUMMARY, M_STRINGS };
+enum Mode : int {
+    M_INDEX = 0, M_TAPE, M_COLUMNS, M_STATS, M_FIELDBYTES, M_LOGFMT, M_SUMMARY, M_STRINGS,
+    // COLUMNS and STATS with the register number parsers only: a tile holding
+    // a cell they leave undecided is flagged in Params::fixup and redone by
+    // the full pass, whose launch skips the unflagged tiles.
+    M_COLUMNS_FAST, M_STATS_FAST
+};
 
 // Record/field counters of a byte range; `compose(a, b)` = a then b.
 struct Counts {
@@ -103,6 +116,7 @@ struct Params {
     uint64_t ends_cap;
     void* partials;
     unsigned long long* result;
+    uint32_t* fixup;          // per tile: written by the *_FAST passes; a full pass given it skips zeros
 };
 
 // --------------------------------------------------------------------------
@@ -617,7 +631,7 @@ DEV void resolve_score(SlowF64& s, const Src& src, NdRecord& r) {
 }
 
 struct Shared {
-    uint8_t bytes[TILE];
+    uint8_t lds_bytes[TILE_LDS];   // from tile start - PRE
     uint16_t list[TILE];
     uint8_t cls[256];
     uint32_t wfn[NWAVE];
@@ -635,19 +649,20 @@ struct Shared {
     long long lsum_sh[NWAVE];
     uint32_t str_len[FALX_MAX_COLUMNS];   // tile cleaned bytes per str slot
     SlowF64 slow;                         // exact path for undecided f64 cells
+    uint32_t fix_bad;                     // *_FAST passes: a cell here needs the full parsers
 };
 
 DEV uint32_t list_at(const Shared& sh, uint32_t k) { return sh.list[k]; }
 
-// Fixed-6 cells read straight from the tile in LDS: the last 8 bytes come
-// from three aligned words (fixed6_tail in cells.h decides). Bytes [lf, lt)
-// are local to the tile.
+// Fixed-6 cells read straight from the tile's LDS copy: the last 8 bytes
+// come from three aligned words (fixed6_tail in cells.h decides). [lf, lt)
+// are offsets into the copy.
 DEV bool fixed6_lds(const uint8_t* lds, uint32_t lf, uint32_t lt, F64Result& r) {
     const uint32_t len = lt - lf;
     if (len < 8 || len > 11) return false;
     const uint32_t s8 = lt - 8, a = s8 & ~3u, sh8 = (s8 & 3u) * 8u;
-    // Up to 3 bytes past the tile's last byte are read: still inside the
-    // workgroup's LDS (the list follows the tile) and never used.
+    // Up to 3 bytes past byte lt are read: inside the copy's spare room,
+    // and never used.
     const uint32_t* wp = reinterpret_cast<const uint32_t*>(lds + a);
     const uint32_t w0 = wp[0], w1 = wp[1], w2 = wp[2];
     const uint32_t lo = sh8 ? (w0 >> sh8) | (w1 << (32 - sh8)) : w0;
@@ -655,12 +670,11 @@ DEV bool fixed6_lds(const uint8_t* lds, uint32_t lf, uint32_t lt, F64Result& r) 
     return fixed6_tail(((uint64_t)hi << 32) | lo, lds + lf, len - 8, r);
 }
 
-// The 4 NW bytes of the tile that end at local byte lt, as NW words (the
+// The 4 NW bytes of the LDS copy that end at offset lt, as NW words (the
 // window num_swar reads): NW + 1 aligned LDS words funnel-shifted by lt mod
-// 4. Words below the tile's first byte read as word 0 and only feed window
+// 4. Words below the copy's first byte read as word 0 and only feed window
 // bytes below the cell, which num_swar masks off. The highest word read
-// ends at most 3 bytes past the tile, inside the workgroup's LDS (the list
-// follows the tile).
+// ends at most 3 bytes past lt, inside the copy's spare room.
 template <int NW>
 DEV Win<NW> win_lds(const uint8_t* lds, uint32_t lt) {
     const uint32_t* wp = reinterpret_cast<const uint32_t*>(lds);
@@ -705,12 +719,19 @@ struct Weighted { static constexpr bool value = true; };
 // One tile of one pass (MODE) over the document P describes. Called by every
 // thread of the workgroup; returns with the workgroup converged (every
 // early return is uniform).
-template <bool HQ, bool ESC, bool CMT, int MODE>
+template <bool HQ, bool ESC, bool CMT, int KMODE>
 __device__ __forceinline__ void scan_tile(const Params& P, const uint32_t tile, Shared& sh) {
     constexpr int NS = Dial<HQ, ESC, CMT>::NS;
+    // A *_FAST pass is its pass without the stream parsers (and their
+    // register and code footprint): cells they would take flag the tile.
+    constexpr bool FAST = KMODE == M_COLUMNS_FAST || KMODE == M_STATS_FAST;
+    constexpr int MODE = KMODE == M_COLUMNS_FAST ? M_COLUMNS : (KMODE == M_STATS_FAST ? M_STATS : KMODE);
     const uint32_t tid = threadIdx.x, ln = tid & 63u, w = uni(tid >> 6);  // w is wave-uniform
 
-    if (tid == 0) sh.slow.lock = 0;
+    if (tid == 0) {
+        sh.slow.lock = 0;
+        sh.fix_bad = 0;
+    }
     {
         uint32_t b = tid;
         sh.cls[b] = (uint8_t)((P.cand[b >> 5] >> (b & 31)) & 1u);
@@ -719,18 +740,28 @@ __device__ __forceinline__ void scan_tile(const Params& P, const uint32_t tile, 
     const uint32_t tile_start = tile * (uint32_t)TILE;
     const uint32_t tile_len = min((uint32_t)TILE, P.len - tile_start);
     const bool last_tile = tile + 1 == P.n_tiles;
+    // Bytes [lds_lo, lds_hi) of the document are in LDS; tile-local byte i
+    // is tb[i], document byte x is sh.lds_bytes[x - tile_start + PRE]. The
+    // passes that read cells load the bytes around the tile too.
+    constexpr bool HALOED = MODE == M_COLUMNS || MODE == M_STATS || MODE == M_STRINGS;
+    const uint32_t lds_lo = HALOED ? tile_start - min((uint32_t)PRE, tile_start) : tile_start;
+    const uint32_t lds_hi = tile_start + min((uint32_t)(HALOED ? TILE + HALO : TILE), P.len - tile_start);
+    uint8_t* const tb = sh.lds_bytes + PRE;
 
     // ---- load: 16-byte coalesced reads into LDS, zero padded --------------
-    for (uint32_t i = tid * 16; i < (uint32_t)TILE; i += WG * 16) {
+    // LDS index i holds document byte tile_start - PRE + i for i in [lo_i, hi_i).
+    const uint32_t lo_i = lds_lo + PRE - tile_start, hi_i = lds_hi + PRE - tile_start;
+    for (uint32_t i = (HALOED ? 0u : (uint32_t)PRE) + tid * 16; i < (uint32_t)(HALOED ? PRE + TILE + HALO : PRE + TILE);
+         i += WG * 16) {
         uint4 v = make_uint4(0, 0, 0, 0);
-        if (i + 16 <= tile_len) {
-            v = *reinterpret_cast<const uint4*>(P.data + tile_start + i);
-        } else if (i < tile_len) {
+        if (i >= lo_i && i + 16 <= hi_i) {
+            v = *reinterpret_cast<const uint4*>(P.data + (tile_start - PRE + i));
+        } else if (i >= lo_i && i < hi_i) {
             uint8_t tmp[16];
-            for (int k = 0; k < 16; ++k) tmp[k] = (i + k < tile_len) ? P.data[tile_start + i + k] : 0;
+            for (int k = 0; k < 16; ++k) tmp[k] = (i + k < hi_i) ? P.data[tile_start - PRE + i + k] : 0;
             memcpy(&v, tmp, 16);
         }
-        *reinterpret_cast<uint4*>(&sh.bytes[i]) = v;
+        *reinterpret_cast<uint4*>(&sh.lds_bytes[i]) = v;
     }
     __syncthreads();
 
@@ -739,7 +770,7 @@ __device__ __forceinline__ void scan_tile(const Params& P, const uint32_t tile, 
     {
         uint32_t wb = w * BPW * 64;
         uint32_t v;
-        if (wb > 0) v = sh.bytes[wb - 1] == '\n';
+        if (wb > 0) v = tb[wb - 1] == '\n';
         else v = tile_start == 0 ? 1u : (uint32_t)(P.data[tile_start - 1] == '\n');
         ls0 = uni(v);
     }
@@ -767,7 +798,7 @@ __device__ __forceinline__ void scan_tile(const Params& P, const uint32_t tile, 
         for (int b0 = 0; b0 < BPW; b0 += 8) {
             uint32_t cc[8];
 #pragma unroll
-            for (int i = 0; i < 8; ++i) cc[i] = sh.bytes[(w * BPW + b0 + i) * 64 + ln];
+            for (int i = 0; i < 8; ++i) cc[i] = tb[(w * BPW + b0 + i) * 64 + ln];
 #pragma unroll
             for (int i = 0; i < 8; ++i) {
                 const uint32_t b = (uint32_t)(b0 + i);
@@ -949,7 +980,7 @@ __device__ __forceinline__ void scan_tile(const Params& P, const uint32_t tile, 
         return;
     }
 
-    TileSrc src{P.data, sh.bytes, tile_start, tile_start + tile_len};
+    TileSrc src{P.data, sh.lds_bytes + (lds_lo + PRE - tile_start), lds_lo, lds_hi};
     const CellDialect cd{P.quote, HQ ? P.escape : -1};
 
     // Walk the list with record context: each thread takes `items`
@@ -1250,30 +1281,51 @@ __device__ __forceinline__ void scan_tile(const Params& P, const uint32_t tile, 
         NumVal v;
         bool done = false;
         // The fixed `[-]d{1,3}.d{6}` shape first: three word reads decide it.
-        if (as_f64 && from >= tile_start && to <= tile_start + tile_len &&
-            fixed6_lds(sh.bytes, from - tile_start, to - tile_start, v.f)) {
+        // Offsets into sh.lds_bytes (the cell is inside [lds_lo, lds_hi)).
+        const bool in_lds = from >= lds_lo && to <= lds_hi;
+        const uint32_t lf = from - tile_start + PRE, lt = to - tile_start + PRE;
+        if (as_f64 && in_lds && fixed6_lds(sh.lds_bytes, lf, lt, v.f)) {
             done = true;
-        } else if (swar_ok && from >= tile_start && to <= tile_start + tile_len && to - from <= NUM_WIN) {
-            const uint32_t lt = to - tile_start;
+        } else if (swar_ok && in_lds && to - from <= NUM_WIN) {
             // The narrow window when every lane here can use it.
             if (__all(to - from <= Win12::BYTES)) {
-                auto at = [&](uint32_t p) { return (uint32_t)sh.bytes[lt - Win12::BYTES + p]; };
-                done = num_swar(win_lds<3>(sh.bytes, lt), to - from, as_f64, cd.quote, at, v.f, v.iok, v.iv);
+                auto at = [&](uint32_t p) { return (uint32_t)sh.lds_bytes[lt - Win12::BYTES + p]; };
+                done = num_swar(win_lds<3>(sh.lds_bytes, lt), to - from, as_f64, cd.quote, at, v.f, v.iok, v.iv);
             } else {
-                auto at = [&](uint32_t p) { return (uint32_t)sh.bytes[lt - Win24::BYTES + p]; };
-                done = num_swar(win_lds<6>(sh.bytes, lt), to - from, as_f64, cd.quote, at, v.f, v.iok, v.iv);
+                auto at = [&](uint32_t p) { return (uint32_t)sh.lds_bytes[lt - Win24::BYTES + p]; };
+                done = num_swar(win_lds<6>(sh.lds_bytes, lt), to - from, as_f64, cd.quote, at, v.f, v.iok, v.iv);
             }
         }
-        if (!done) {
-            if (as_f64) {
-                v.f = parse_f64_field(src, from, to, cd);
-            } else {
-                v.iok = parse_i64_field(src, from, to, cd, v.iv);
-                if (!v.iok) v.iv = 0;
+        if constexpr (FAST) {
+            if (!done || (as_f64 && v.f.ok && v.f.deferred)) {
+                sh.fix_bad = 1u;
+                v.f = F64Result{};
+                v.iok = false;
+                v.iv = 0;
             }
+        } else {
+            if (!done) {
+                if (as_f64) {
+                    v.f = parse_f64_field(src, from, to, cd);
+                } else {
+                    v.iok = parse_i64_field(src, from, to, cd, v.iv);
+                    if (!v.iok) v.iv = 0;
+                }
+            }
+            if (as_f64 && v.f.ok && v.f.deferred) v.f.value = resolve_f64(sh.slow, src, from, to, cd);
         }
-        if (as_f64 && v.f.ok && v.f.deferred) v.f.value = resolve_f64(sh.slow, src, from, to, cd);
         return v;
+    };
+    // The i64 reading of a field also read as f64 (two columns of one field).
+    auto num_i64_too = [&](uint32_t from, uint32_t to, NumVal& ni) {
+        if constexpr (FAST) {
+            sh.fix_bad = 1u;
+            ni.iok = false;
+            ni.iv = 0;
+        } else {
+            ni.iok = parse_i64_field(src, from, to, cd, ni.iv);
+            if (!ni.iok) ni.iv = 0;
+        }
     };
     // The f64 and i64 columns that read ordinal `ord`: one parse per entry
     // serves all of them (an entry read both ways also parses as i64).
@@ -1308,10 +1360,7 @@ __device__ __forceinline__ void scan_tile(const Params& P, const uint32_t tile, 
             NumVal nf{}, ni{};
             if (want_f || want_i) nf = num_value(e.fs, to, want_f);
             ni = nf;
-            if (want_f && want_i) {
-                ni.iok = parse_i64_field(src, e.fs, to, cd, ni.iv);
-                if (!ni.iok) ni.iv = 0;
-            }
+            if (want_f && want_i) num_i64_too(e.fs, to, ni);
             for (uint32_t j = 0; j < nc; ++j) {
                 const ColDesc& c = P.cols[c0 + j];
                 if (c.index != e.ord) continue;
@@ -1338,6 +1387,7 @@ __device__ __forceinline__ void scan_tile(const Params& P, const uint32_t tile, 
             }
         }, trailer_word);
         acc_publish(acc, c0, nc, records);
+        if (FAST && tid == 0) P.fixup[tile] = sh.fix_bad;
         return;
     }
 
@@ -1466,10 +1516,7 @@ __device__ __forceinline__ void scan_tile(const Params& P, const uint32_t tile, 
             NumVal nf{}, ni{};
             if (want_f || want_i) nf = num_value(e.fs, to, want_f);
             ni = nf;
-            if (want_f && want_i) {
-                ni.iok = parse_i64_field(src, e.fs, to, cd, ni.iv);
-                if (!ni.iok) ni.iv = 0;
-            }
+            if (want_f && want_i) num_i64_too(e.fs, to, ni);
             for (uint32_t j = 0; j < n_cols; ++j) {
                 const ColDesc& c = P.cols[j];
                 const bool found = c.index == e.ord;
@@ -1490,6 +1537,7 @@ __device__ __forceinline__ void scan_tile(const Params& P, const uint32_t tile, 
         }, trailer_word);
         if (tid == 0 && last_tile) P.result[R_N_ROWS] = inc.n_rows + (trailer_row_live ? 1 : 0);
         __syncthreads();
+        if (FAST && tid == 0) P.fixup[tile] = sh.fix_bad;
         if (with_stats) {
             // The statistics sweep over the marked rows, in row order per
             // thread, then the fixed-order block reduction.
@@ -1646,7 +1694,7 @@ __device__ __forceinline__ void scan_tile(const Params& P, const uint32_t tile, 
                             while (rt_end[(base + i) & (RT_N - 1)] <= q) ++i;
                             const int32_t d = rt_delta[(base + i) & (RT_N - 1)];
                             if (d != RAW_SKIP) {
-                                const uint8_t b = sh.bytes[(int32_t)q + d];
+                                const uint8_t b = tb[(int32_t)q + d];
                                 sstage[q] = b;
                                 s_sum += b;
                             }
@@ -1688,7 +1736,7 @@ __device__ __forceinline__ void scan_tile(const Params& P, const uint32_t tile, 
 // Occupancy the STATS pass is held to (its statistics accumulators sit just
 // over the register budget of 4 waves/SIMD otherwise); 1 = no constraint.
 template <int MODE>
-constexpr int plain_waves() { return MODE == M_STATS ? 4 : 1; }
+constexpr int plain_waves() { return (MODE == M_STATS || MODE == M_STATS_FAST) ? 4 : 1; }
 
 // One workgroup per tile. Tiles depend on nothing but the tile prefix
 // (tpre) and their own bytes, so the block index is the tile and its
@@ -1697,6 +1745,8 @@ template <bool HQ, bool ESC, bool CMT, int MODE>
 __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(plain_waves<MODE>())))
 void k_scan(Params P) {
     __shared__ Shared sh;
+    if constexpr (MODE != M_COLUMNS_FAST && MODE != M_STATS_FAST)
+        if (P.fixup && P.fixup[blockIdx.x] == 0u) return;   // a full pass after a fast one: flagged tiles only
     scan_tile<HQ, ESC, CMT, MODE>(P, blockIdx.x, sh);
 }
 
