@@ -238,6 +238,31 @@ def test_geo_columns_and_stats(ctx, ora, spec):
     assert st["cols"][lat]["checksum"] == truth["lat_checksum"]
 
 
+@pytest.mark.parametrize("spec", [S.GEO_LATLON, S.GEO_ALL], ids=lambda s: s.name)
+def test_columns_fast_pass_redoes_flagged_tiles(ctx, ora, spec):
+    """A multi-tile geo document whose Latitude/Longitude cells are fixed-6
+    except in a few scattered rows (quoted, exponent, 25-digit, inf, empty,
+    and cells across tile edges): the fast columns and stats passes leave
+    those tiles to the full pass, and every cell matches the oracle."""
+    data, _ = datagen.geo_csv(700_000, seed=77)
+    lines = bytes(data).split(b"\n")
+    odd = [b'"12.5"', b"1e5", b"-0000000000000000000000012.75", b"inf", b"", b"  7.25", b"1.0000000000000000000001"]
+    rng = Rng(0xF1A6)
+    for t in range(40):
+        i = 1 + rng.below(len(lines) - 2)
+        f = lines[i].split(b",")
+        if len(f) == 4:
+            f[2 + rng.below(2)] = odd[t % len(odd)]
+            lines[i] = b",".join(f)
+    doc = np.frombuffer(b"\n".join(lines), dtype=np.uint8)
+    buf = ctx.upload(doc)
+    assert_columns_equal(ctx.parse_columns(spec, buf, doc.size), ora.parse_columns(spec, doc), spec, spec.name)
+    assert_stats_equal(ctx.csv_stats(spec, buf, doc.size), ora.csv_stats(spec, doc), spec, spec.name)
+    cols, keep = ctx.alloc_columns(spec, ora.parse_columns(spec, doc)["rows"], doc.size)
+    st = ctx.parse_columns_with_stats_into(spec, buf, doc.size, cols)
+    assert_stats_equal(st, ora.csv_stats(spec, doc), spec, spec.name + " fused")
+
+
 def test_hash_typed_columns_stats(ctx, ora):
     data, truth = datagen.hash_csv(2 * 1024 * 1024 + 333)
     buf = ctx.upload(data)
