@@ -123,6 +123,22 @@ hipError_t launch_scan(falx_ctx* c, Params& P, int kind) {
     return hipGetLastError();
 }
 
+// The full form of a pass over the tiles its fast form flagged.
+template <int MODE>
+hipError_t launch_redo(falx_ctx* c, Params& P, int kind) {
+    const hipStream_t s = c->stream;
+    dim3 g((P.n_tiles + 63) / 64), b(WG);
+    switch (kind) {
+    case 0: hipLaunchKernelGGL((k_redo<false, false, false, MODE>), g, b, 0, s, P); break;
+    case 1: hipLaunchKernelGGL((k_redo<false, false, true, MODE>), g, b, 0, s, P); break;
+    case 2: hipLaunchKernelGGL((k_redo<true, false, false, MODE>), g, b, 0, s, P); break;
+    case 3: hipLaunchKernelGGL((k_redo<true, true, false, MODE>), g, b, 0, s, P); break;
+    case 4: hipLaunchKernelGGL((k_redo<true, false, true, MODE>), g, b, 0, s, P); break;
+    default: hipLaunchKernelGGL((k_redo<true, true, true, MODE>), g, b, 0, s, P); break;
+    }
+    return hipGetLastError();
+}
+
 // A COLUMNS or STATS pass as its fast form over all tiles, then the full
 // form over the tiles the fast one flagged (cells outside the register
 // parsers' reach: quoted or long numbers, inf/nan, undecided roundings).
@@ -131,7 +147,7 @@ hipError_t launch_fast(falx_ctx* c, Params& P, int kind) {
     const LbLayout L = lb_layout(P.n_tiles, P.n_str, kKindStates[kind]);
     P.fixup = reinterpret_cast<uint32_t*>(static_cast<char*>(c->d_lb) + L.fix);
     hipError_t e = launch_scan<FAST>(c, P, kind);
-    if (e == hipSuccess) e = launch_scan<FULL>(c, P, kind);
+    if (e == hipSuccess) e = launch_redo<FULL>(c, P, kind);
     P.fixup = nullptr;
     return e;
 }

@@ -1283,17 +1283,26 @@ __device__ __forceinline__ void scan_tile(const Params& P, const uint32_t tile, 
         // The fixed `[-]d{1,3}.d{6}` shape first: three word reads decide it.
         // Offsets into sh.lds_bytes (the cell is inside [lds_lo, lds_hi)).
         const bool in_lds = from >= lds_lo && to <= lds_hi;
-        const uint32_t lf = from - tile_start + PRE, lt = to - tile_start + PRE;
+        uint32_t lf = from - tile_start + PRE, lt = to - tile_start + PRE;
+        // A cell quoted at both ends cleans to its inside with doubled
+        // quotes collapsed; the register parsers take the inside, and an
+        // inside with a quote byte left in it is not a number either way.
+        if (HQ && !ESC && in_lds && lt - lf >= 2 && sh.lds_bytes[lf] == (uint8_t)cd.quote &&
+            sh.lds_bytes[lt - 1] == (uint8_t)cd.quote) {
+            lf += 1;
+            lt -= 1;
+        }
+        const uint32_t n = lt - lf;
         if (as_f64 && in_lds && fixed6_lds(sh.lds_bytes, lf, lt, v.f)) {
             done = true;
-        } else if (swar_ok && in_lds && to - from <= NUM_WIN) {
+        } else if (swar_ok && in_lds && n <= NUM_WIN) {
             // The narrow window when every lane here can use it.
-            if (__all(to - from <= Win12::BYTES)) {
+            if (__all(n <= Win12::BYTES)) {
                 auto at = [&](uint32_t p) { return (uint32_t)sh.lds_bytes[lt - Win12::BYTES + p]; };
-                done = num_swar(win_lds<3>(sh.lds_bytes, lt), to - from, as_f64, cd.quote, at, v.f, v.iok, v.iv);
+                done = num_swar(win_lds<3>(sh.lds_bytes, lt), n, as_f64, cd.quote, at, v.f, v.iok, v.iv);
             } else {
                 auto at = [&](uint32_t p) { return (uint32_t)sh.lds_bytes[lt - Win24::BYTES + p]; };
-                done = num_swar(win_lds<6>(sh.lds_bytes, lt), to - from, as_f64, cd.quote, at, v.f, v.iok, v.iv);
+                done = num_swar(win_lds<6>(sh.lds_bytes, lt), n, as_f64, cd.quote, at, v.f, v.iok, v.iv);
             }
         }
         if constexpr (FAST) {
@@ -1738,6 +1747,15 @@ __device__ __forceinline__ void scan_tile(const Params& P, const uint32_t tile, 
 template <int MODE>
 constexpr int plain_waves() { return (MODE == M_STATS || MODE == M_STATS_FAST) ? 4 : 1; }
 
+// A document's Params read through the constant address space: nothing
+// writes them during a batch, and the compiler then loads their fields with
+// scalar loads, as it does for kernel arguments.
+typedef __attribute__((address_space(4))) const Params ConstParams;
+DEV const Params& const_params(const Params* base, uint32_t i) {
+    ConstParams* cp = reinterpret_cast<ConstParams*>(reinterpret_cast<uintptr_t>(base + i));
+    return *(const Params*)cp;
+}
+
 // One workgroup per tile. Tiles depend on nothing but the tile prefix
 // (tpre) and their own bytes, so the block index is the tile and its
 // loads start at once.
@@ -1745,9 +1763,39 @@ template <bool HQ, bool ESC, bool CMT, int MODE>
 __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(plain_waves<MODE>())))
 void k_scan(Params P) {
     __shared__ Shared sh;
-    if constexpr (MODE != M_COLUMNS_FAST && MODE != M_STATS_FAST)
-        if (P.fixup && P.fixup[blockIdx.x] == 0u) return;   // a full pass after a fast one: flagged tiles only
     scan_tile<HQ, ESC, CMT, MODE>(P, blockIdx.x, sh);
+}
+
+// The full pass after a fast one (P.fixup holds the fast pass's flags): a
+// workgroup per 64 tiles runs the flagged ones in order. Usually there are
+// none, and the workgroup ends after one read of the flags.
+template <bool HQ, bool ESC, bool CMT, int MODE>
+__global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(4)))
+void k_redo(Params P) {
+    __shared__ Shared sh;
+    __shared__ uint32_t s_mask[2];
+    const uint32_t t0 = blockIdx.x * 64u;
+    if (threadIdx.x < 64) {
+        const uint32_t t = t0 + threadIdx.x;
+        const uint64_t m = ballot(t < P.n_tiles && P.fixup[t] != 0u);
+        if (threadIdx.x == 0) {
+            s_mask[0] = (uint32_t)m;
+            s_mask[1] = (uint32_t)(m >> 32);
+        }
+    }
+    __syncthreads();
+    uint64_t m = ((uint64_t)uni(s_mask[1]) << 32) | uni(s_mask[0]);
+    while (m) {
+        // Params are read from the kernel arguments through an address that
+        // passes an empty asm each tile, so their loads stay in the loop
+        // (hoisted, they would stay live across the tile body and spill).
+        uintptr_t pa = reinterpret_cast<uintptr_t>(__builtin_amdgcn_kernarg_segment_ptr());
+        asm volatile("" : "+s"(pa));
+        scan_tile<HQ, ESC, CMT, MODE>(const_params(reinterpret_cast<const Params*>(pa), 0),
+                                      t0 + (uint32_t)__ffsll((long long)m) - 1u, sh);
+        __syncthreads();
+        m &= m - 1;
+    }
 }
 
 // --------------------------------------------------------------------------
@@ -2212,14 +2260,6 @@ struct BatchHdr {
     SegT tab[N_TAB];
 };
 
-// A document's Params read through the constant address space: nothing
-// writes them during a batch, and the compiler then loads their fields with
-// scalar loads, as it does for kernel arguments.
-typedef __attribute__((address_space(4))) const Params ConstParams;
-DEV const Params& const_params(const Params* base, uint32_t i) {
-    ConstParams* cp = reinterpret_cast<ConstParams*>(reinterpret_cast<uintptr_t>(base + i));
-    return *(const Params*)cp;
-}
 
 // Persistent tile pass of table `tab` (a summary or emit table).
 template <int MODE>
