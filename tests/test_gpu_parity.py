@@ -263,6 +263,24 @@ def test_columns_fast_pass_redoes_flagged_tiles(ctx, ora, spec):
     assert_stats_equal(st, ora.csv_stats(spec, doc), spec, spec.name + " fused")
 
 
+def test_columns_every_tile_flagged(ctx, ora):
+    """Every Latitude quoted and every 7th in exponent form: the register
+    parsers take the quoted cells, the exponent cells flag every tile, and
+    the full pass redoes all of them (many tiles per redo workgroup)."""
+    data, _ = datagen.geo_csv(900_000, seed=5)
+    lines = bytes(data).split(b"\n")
+    for i in range(1, len(lines)):
+        f = lines[i].split(b",")
+        if len(f) == 4:
+            f[2] = b'"' + (f[2].replace(b".", b"e") if i % 7 == 0 else f[2]) + b'"'
+            lines[i] = b",".join(f)
+    doc = np.frombuffer(b"\n".join(lines), dtype=np.uint8)
+    buf = ctx.upload(doc)
+    spec = S.GEO_LATLON
+    assert_columns_equal(ctx.parse_columns(spec, buf, doc.size), ora.parse_columns(spec, doc), spec, "flagged")
+    assert_stats_equal(ctx.csv_stats(spec, buf, doc.size), ora.csv_stats(spec, doc), spec, "flagged stats")
+
+
 def test_hash_typed_columns_stats(ctx, ora):
     data, truth = datagen.hash_csv(2 * 1024 * 1024 + 333)
     buf = ctx.upload(data)
