@@ -1453,40 +1453,19 @@ __device__ __forceinline__ void scan_tile(const Params& P, const uint32_t tile, 
         const bool trailer_row_live = trailer && !trailer_comment;
         vstage_clear();
         // Optional fused statistics of the first COL_STAT_GROUP columns
-        // (further columns get STATS passes). Only the fixed-6 mantissa sums
-        // are accumulated during the walk, whose registers set this kernel's
-        // occupancy; a sweep over the tile's rows afterwards reads back the
-        // values it wrote, taking this tile's valid cells from the staged
-        // validity bits (or, for a tile whose rows do not fit the stage,
-        // from bits marked here for the stat columns alone). String columns'
-        // statistics come from the STRINGS pass. Cells past rows_cap are not
-        // written, so on a CAPACITY result the statistics cover the rows
-        // that fit.
-        constexpr uint32_t SWORDS = 2 * (TILE / 64 + 2);   // a tile's rows, from row0, in u32 words
-        __shared__ uint32_t sbits[COL_STAT_GROUP][SWORDS];
+        // (further columns get STATS passes), accumulated in registers as
+        // the walk writes the cells, then reduced in a fixed order. String
+        // columns' statistics come from the STRINGS pass. Cells past
+        // rows_cap are not written, so on a CAPACITY result the statistics
+        // cover the rows that fit.
         unsigned long long records = 0;
         const bool with_stats = P.partials != nullptr;
         const bool have_str = P.n_str > 0;
-        if (with_stats && !staged)
-            for (uint32_t i = tid; i < COL_STAT_GROUP * SWORDS; i += WG) (&sbits[0][0])[i] = 0;
-        long long fx_sum[COL_STAT_GROUP];
-        uint32_t fx_cells[COL_STAT_GROUP];
+        Acc acc[COL_STAT_GROUP];
 #pragma unroll
-        for (int j = 0; j < COL_STAT_GROUP; ++j) fx_sum[j] = 0, fx_cells[j] = 0;
-        auto stat_mark = [&](uint32_t j, uint32_t row, const F64Result* f) __attribute__((always_inline)) {
-            if (!with_stats || j >= (uint32_t)COL_STAT_GROUP) return;
-            if (!staged) {
-                const uint32_t r = row - row0;
-                atomicOr(&sbits[j][r >> 5], 1u << (r & 31));
-            }
-            if (f && f->fixed6) {
-#pragma unroll
-                for (uint32_t jj = 0; jj < (uint32_t)COL_STAT_GROUP; ++jj)
-                    if (jj == j) {
-                        fx_sum[jj] += f->mantissa;
-                        fx_cells[jj] += 1;
-                    }
-            }
+        for (int j = 0; j < COL_STAT_GROUP; ++j) acc[j] = Acc{0, 0, 0, 0, 0, 0.0};
+        auto stat_add = [&](uint32_t j, const CellStat& o) __attribute__((always_inline)) {
+            if (with_stats && j < (uint32_t)COL_STAT_GROUP) acc_add(acc, j, o);
         };
         // E1: typed cells and cleaned string lengths. A record that ends
         // before a column's field leaves that cell null.
@@ -1501,14 +1480,14 @@ __device__ __forceinline__ void scan_tile(const Params& P, const uint32_t tile, 
                 reinterpret_cast<double*>(c.values)[row] = ok ? nf.f.value : 0.0;
                 if (ok) {
                     set_valid(j, c, row);
-                    stat_mark(j, row, &nf.f);
+                    stat_add(j, CellStat{true, nf.f.fixed6, f64_to_bits(nf.f.value), 0, nf.f.mantissa, nf.f.value});
                 }
             } else {
                 const bool ok = found && ni.iok;
                 reinterpret_cast<long long*>(c.values)[row] = ok ? ni.iv : 0;
                 if (ok) {
                     set_valid(j, c, row);
-                    stat_mark(j, row, nullptr);
+                    stat_add(j, CellStat{true, false, (unsigned long long)ni.iv, 0, 0, (double)ni.iv});
                 }
             }
         };
@@ -1539,7 +1518,11 @@ __device__ __forceinline__ void scan_tile(const Params& P, const uint32_t tile, 
                     reinterpret_cast<uint2*>(c.values)[row] = ok ? make_uint2(e.fs, to) : make_uint2(0, 0);
                     if (ok) {
                         set_valid(j, c, row);
-                        stat_mark(j, row, nullptr);
+                        if (with_stats && j < (uint32_t)COL_STAT_GROUP) {
+                            unsigned long long cs = 0;
+                            for (uint32_t q = e.fs; q < to; ++q) cs += src.at(q);
+                            stat_add(j, CellStat{true, false, cs, to - e.fs, 0, 0.0});
+                        }
                     }
                 }
             }
@@ -1547,43 +1530,7 @@ __device__ __forceinline__ void scan_tile(const Params& P, const uint32_t tile, 
         if (tid == 0 && last_tile) P.result[R_N_ROWS] = inc.n_rows + (trailer_row_live ? 1 : 0);
         __syncthreads();
         if (FAST && tid == 0) P.fixup[tile] = sh.fix_bad;
-        if (with_stats) {
-            // The statistics sweep over the marked rows, in row order per
-            // thread, then the fixed-order block reduction.
-            Acc acc[COL_STAT_GROUP];
-#pragma unroll
-            for (int j = 0; j < COL_STAT_GROUP; ++j) acc[j] = Acc{0, 0, fx_cells[j], 0, fx_sum[j], 0.0};
-            const uint32_t nc = min((uint32_t)COL_STAT_GROUP, n_cols);
-            // Rows this tile writes: row0 .. inc.n_rows (the record in
-            // progress at the tile's end, or the trailer); the staged
-            // validity of column j covers exactly those.
-            const uint32_t span = inc.n_rows + 1u - row0;
-            for (uint32_t r = tid; r < span; r += WG) {
-#pragma unroll
-                for (uint32_t j = 0; j < (uint32_t)COL_STAT_GROUP; ++j) {
-                    const uint32_t vw = staged ? vbits[j * wpc + (r >> 5)] : sbits[j][r >> 5];
-                    if (j >= nc || !((vw >> (r & 31)) & 1u)) continue;
-                    const ColDesc& c = P.cols[j];
-                    const uint32_t row = row0 + r;
-                    CellStat o{true, false, 0, 0, 0, 0.0};
-                    if (c.type == FALX_COL_F64) {
-                        const double v = reinterpret_cast<const double*>(c.values)[row];
-                        o.csum = f64_to_bits(v);
-                        o.d = v;
-                    } else if (c.type == FALX_COL_I64) {
-                        const long long v = reinterpret_cast<const long long*>(c.values)[row];
-                        o.csum = (unsigned long long)v;
-                        o.d = (double)v;
-                    } else {
-                        const uint2 sp = reinterpret_cast<const uint2*>(c.values)[row];
-                        o.bytes = sp.y - sp.x;
-                        for (uint32_t q = sp.x; q < sp.y; ++q) o.csum += src.at(q);
-                    }
-                    acc_add(acc, j, o);
-                }
-            }
-            acc_publish(acc, 0, nc, records);
-        }
+        if (with_stats) acc_publish(acc, 0, min((uint32_t)COL_STAT_GROUP, n_cols), records);
         flush_valid(false);
         if (!have_str) return;
 
