@@ -19,7 +19,9 @@
 //   emit     per tile, with its entry state known: structural/terminator
 //            masks, popcount/mbcnt compaction into an LDS list, then a walk
 //            of the list with record context: index/tape offsets, typed
-//            cells parsed in place, fused statistics;
+//            cells parsed in place, fused statistics (columns and stats run
+//            first in a fast form with only the register number parsers;
+//            tiles holding a cell it leaves undecided are redone in full);
 //   strings  (string columns only) after a scan of the tiles' cleaned byte
 //            counts, the cells are copied into the Arrow arena.
 #include <hip/hip_runtime.h>
