@@ -515,7 +515,7 @@ falx_status falx_batch_submit(falx_batch* b, const falx_request* reqs, uint32_t 
                 if (sig & (1ULL << t)) launch_emit(E_STRINGS, k, tg, ss, dH, t);
             }
         }
-        if (sig & (1ULL << T_NDJ)) hipLaunchKernelGGL(k_ndjson_batch, tg, dim3(WG), 0, ss, dH);
+        if (sig & (1ULL << T_NDJ)) hipLaunchKernelGGL(k_ndjson_batch, tg, dim3(ND_WG), 0, ss, dH);
         if (sig & (1ULL << T_RED1)) hipLaunchKernelGGL(k_reduce_batch, ug, dim3(WG), 0, ss, dH, 1u);
         if (sig & (1ULL << T_RED2)) hipLaunchKernelGGL(k_reduce_batch, ug, dim3(WG), 0, ss, dH, 2u);
         return hipGetLastError();

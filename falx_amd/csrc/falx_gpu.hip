@@ -686,7 +686,7 @@ falx_status falx_parse_ndjson_id_score(falx_ctx* c, const uint8_t* d, uint64_t l
     uint32_t n_tiles = (uint32_t)((len + TILE - 1) / TILE);
     if ((st = ensure_part(c, (size_t)n_tiles * sizeof(NdPart))) != FALX_OK) return st;
     HIP_TRY(mark_start(c));
-    hipLaunchKernelGGL(k_ndjson_id_score, dim3(n_tiles), dim3(WG), 0, c->stream, d, (uint32_t)len, n_tiles,
+    hipLaunchKernelGGL(k_ndjson_id_score, dim3(n_tiles), dim3(ND_WG), 0, c->stream, d, (uint32_t)len, n_tiles,
                        static_cast<NdPart*>(c->d_part));
     HIP_TRY(mark_end(c));
     {

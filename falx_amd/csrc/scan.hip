@@ -426,7 +426,7 @@ DEV uint32_t block_excl_sum(uint32_t v, uint32_t* sh, uint32_t& tot) {
     return base + inc - v;
 }
 
-template <class T>
+template <class T, int NW = NWAVE>
 DEV T block_reduce_sum(T v, T* sh) {
     // fixed-order tree: wave butterfly, then waves in order
 #pragma unroll
@@ -435,7 +435,7 @@ DEV T block_reduce_sum(T v, T* sh) {
     if (ln == 0) sh[w] = v;
     __syncthreads();
     T t = sh[0];
-    for (int k = 1; k < NWAVE; ++k) t += sh[k];
+    for (int k = 1; k < NW; ++k) t += sh[k];
     __syncthreads();
     return t;
 }
@@ -1999,19 +1999,26 @@ struct NdPart {
 // NDJSON id+score: records are split at raw '\n' (ndjson.rs:679-723); a
 // record belongs to the tile holding its '\n' (the data's trailing record to
 // the last tile). Each thread walks whole records; no cross-tile state.
+// NDJSON tiles run on workgroups of two waves: a tile holds some 60-100
+// records, one per lane, so two waves hold them all, and smaller workgroups
+// let more tiles share a CU to hide the walker's latency.
+constexpr int ND_WAVES = 2;
+constexpr int ND_WG = 64 * ND_WAVES;
+constexpr int ND_BPW = TILE / 64 / ND_WAVES;   // 64-byte blocks per wave
+
 DEV void ndjson_tile(const uint8_t* data, uint32_t len, uint32_t n_tiles, NdPart* parts, uint32_t tile) {
     __shared__ uint8_t bytes[TILE];
     __shared__ uint16_t nls[TILE];
-    __shared__ uint32_t wcnt[NWAVE], wb[NWAVE + 1];
+    __shared__ uint32_t wcnt[ND_WAVES], wb[ND_WAVES + 1];
     __shared__ int64_t prev_nl;
     __shared__ SlowF64 slow;
-    __shared__ unsigned long long ush[NWAVE];
-    __shared__ long long lsh[NWAVE];
-    __shared__ double dsh[NWAVE];
+    __shared__ unsigned long long ush[ND_WAVES];
+    __shared__ long long lsh[ND_WAVES];
+    __shared__ double dsh[ND_WAVES];
     const uint32_t tid = threadIdx.x, ln = tid & 63u, w = uni(tid >> 6);
     const uint32_t tile_start = tile * (uint32_t)TILE;
     const uint32_t tile_len = min((uint32_t)TILE, len - tile_start);
-    for (uint32_t i = tid * 16; i < (uint32_t)TILE; i += WG * 16) {
+    for (uint32_t i = tid * 16; i < (uint32_t)TILE; i += ND_WG * 16) {
         uint4 v = make_uint4(0, 0, 0, 0);
         if (i + 16 <= tile_len) {
             v = *reinterpret_cast<const uint4*>(data + tile_start + i);
@@ -2041,9 +2048,9 @@ DEV void ndjson_tile(const uint8_t* data, uint32_t len, uint32_t n_tiles, NdPart
     }
     __syncthreads();
     uint32_t cnt = 0;
-    const uint32_t seg = w * BPW * 64;
-    for (int b = 0; b < BPW; ++b) {
-        uint32_t off = (w * BPW + b) * 64;
+    const uint32_t seg = w * ND_BPW * 64;
+    for (int b = 0; b < ND_BPW; ++b) {
+        uint32_t off = (w * ND_BPW + b) * 64;
         bool isnl = off + ln < tile_len && bytes[off + ln] == '\n';
         uint64_t m = ballot(isnl);
         if (isnl) nls[seg + cnt + mbcnt(m)] = (uint16_t)(off + ln);
@@ -2053,22 +2060,22 @@ DEV void ndjson_tile(const uint8_t* data, uint32_t len, uint32_t n_tiles, NdPart
     __syncthreads();
     if (tid == 0) {
         wb[0] = 0;
-        for (int k = 0; k < NWAVE; ++k) wb[k + 1] = wb[k] + wcnt[k];
+        for (int k = 0; k < ND_WAVES; ++k) wb[k + 1] = wb[k] + wcnt[k];
     }
     __syncthreads();
-    const uint32_t n = uni(wb[NWAVE]);
+    const uint32_t n = uni(wb[ND_WAVES]);
     const bool tail = tile + 1 == n_tiles && len > 0 && data[len - 1] != '\n';
     TileSrc src{data, bytes, tile_start, tile_start + tile_len};
     auto nl_at = [&](uint32_t k) -> uint32_t {
         uint32_t ww = 0;
-        for (int i = 1; i < NWAVE; ++i) ww += k >= wb[i];
-        return tile_start + nls[ww * BPW * 64 + (k - wb[ww])];
+        for (int i = 1; i < ND_WAVES; ++i) ww += k >= wb[i];
+        return tile_start + nls[ww * ND_BPW * 64 + (k - wb[ww])];
     };
     unsigned long long recs = 0, inval = 0, fxc = 0, ids = 0;
     long long fxs = 0;
     double ss = 0;
     const uint32_t total = n + (tail ? 1u : 0u);
-    for (uint32_t k = tid; k < total; k += WG) {
+    for (uint32_t k = tid; k < total; k += ND_WG) {
         uint32_t end = k < n ? nl_at(k) : len;
         int64_t pn = k == 0 ? prev_nl : (int64_t)nl_at(k - 1);
         uint32_t from = (uint32_t)(pn + 1);
@@ -2084,19 +2091,19 @@ DEV void ndjson_tile(const uint8_t* data, uint32_t len, uint32_t n_tiles, NdPart
             inval += 1;
         }
     }
-    recs = block_reduce_sum(recs, ush);
-    inval = block_reduce_sum(inval, ush);
-    fxc = block_reduce_sum(fxc, ush);
-    ids = block_reduce_sum(ids, ush);
-    fxs = block_reduce_sum(fxs, lsh);
-    ss = block_reduce_sum(ss, dsh);
+    recs = block_reduce_sum<unsigned long long, ND_WAVES>(recs, ush);
+    inval = block_reduce_sum<unsigned long long, ND_WAVES>(inval, ush);
+    fxc = block_reduce_sum<unsigned long long, ND_WAVES>(fxc, ush);
+    ids = block_reduce_sum<unsigned long long, ND_WAVES>(ids, ush);
+    fxs = block_reduce_sum<long long, ND_WAVES>(fxs, lsh);
+    ss = block_reduce_sum<double, ND_WAVES>(ss, dsh);
     if (tid == 0) {
         NdPart& p = parts[tile];
         p.records = recs; p.invalid = inval; p.id_sum = (long long)ids; p.fx_sum = fxs; p.fx_cells = fxc; p.score_sum = ss;
     }
 }
 
-__global__ __launch_bounds__(WG) void k_ndjson_id_score(const uint8_t* data, uint32_t len, uint32_t n_tiles,
+__global__ __launch_bounds__(ND_WG) void k_ndjson_id_score(const uint8_t* data, uint32_t len, uint32_t n_tiles,
                                                         NdPart* parts) {
     ndjson_tile(data, len, n_tiles, parts, blockIdx.x);
 }
@@ -2322,7 +2329,7 @@ __global__ __launch_bounds__(WG) void k_reduce_batch(const BatchHdr* H, uint32_t
     }
 }
 
-__global__ __launch_bounds__(WG) void k_ndjson_batch(const BatchHdr* H) {
+__global__ __launch_bounds__(ND_WG) void k_ndjson_batch(const BatchHdr* H) {
     __shared__ uint32_t s_p, s_t0, s_t1;
     const SegT& T = H->tab[T_NDJ];
     const uint32_t total = T.total;
