@@ -514,7 +514,8 @@ static falx_status columns_impl(falx_ctx* c, const falx_spec* spec, const uint8_
     }
     HIP_TRY(mark_start(c));
     HIP_TRY(run_prefix(c, P, dialect_kind(spec)));
-    HIP_TRY((launch_fast<M_COLUMNS_FAST, M_COLUMNS>(c, P, dialect_kind(spec))));
+    if (stats) HIP_TRY((launch_fast<M_COLUMNS_FAST_STATS, M_COLUMNS>(c, P, dialect_kind(spec))));
+    else HIP_TRY((launch_fast<M_COLUMNS_FAST, M_COLUMNS>(c, P, dialect_kind(spec))));
     // Statistics of columns past the first group come from extra passes.
     if (stats) {
         Params Q = P;

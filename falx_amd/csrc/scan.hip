@@ -61,7 +61,8 @@ enum Mode : int {
     // COLUMNS and STATS with the register number parsers only: a tile holding
     // a cell they leave undecided is flagged in Params::fixup and redone by
     // the full pass, whose launch skips the unflagged tiles.
-    M_COLUMNS_FAST, M_STATS_FAST
+    M_COLUMNS_FAST, M_STATS_FAST,
+    M_COLUMNS_FAST_STATS   // M_COLUMNS_FAST with the fused statistics (the plain fast form has none)
 };
 
 // Record/field counters of a byte range; `compose(a, b)` = a then b.
@@ -726,8 +727,10 @@ __device__ __forceinline__ void scan_tile(const Params& P, const uint32_t tile, 
     constexpr int NS = Dial<HQ, ESC, CMT>::NS;
     // A *_FAST pass is its pass without the stream parsers (and their
     // register and code footprint): cells they would take flag the tile.
-    constexpr bool FAST = KMODE == M_COLUMNS_FAST || KMODE == M_STATS_FAST;
-    constexpr int MODE = KMODE == M_COLUMNS_FAST ? M_COLUMNS : (KMODE == M_STATS_FAST ? M_STATS : KMODE);
+    constexpr bool FAST = KMODE == M_COLUMNS_FAST || KMODE == M_COLUMNS_FAST_STATS || KMODE == M_STATS_FAST;
+    constexpr int MODE = (KMODE == M_COLUMNS_FAST || KMODE == M_COLUMNS_FAST_STATS)
+                             ? M_COLUMNS
+                             : (KMODE == M_STATS_FAST ? M_STATS : KMODE);
     const uint32_t tid = threadIdx.x, ln = tid & 63u, w = uni(tid >> 6);  // w is wave-uniform
 
     if (tid == 0) {
@@ -1461,7 +1464,11 @@ __device__ __forceinline__ void scan_tile(const Params& P, const uint32_t tile, 
         // rows_cap are not written, so on a CAPACITY result the statistics
         // cover the rows that fit.
         unsigned long long records = 0;
-        const bool with_stats = P.partials != nullptr;
+        // Known at compile time in the fast forms, which then carry no
+        // statistics registers when there are none.
+        const bool with_stats = KMODE == M_COLUMNS_FAST         ? false
+                                : KMODE == M_COLUMNS_FAST_STATS ? true
+                                                                : P.partials != nullptr;
         const bool have_str = P.n_str > 0;
         Acc acc[COL_STAT_GROUP];
 #pragma unroll
