@@ -1701,7 +1701,9 @@ __device__ __forceinline__ void scan_tile(const Params& P, const uint32_t tile, 
 // Occupancy the STATS pass is held to (its statistics accumulators sit just
 // over the register budget of 4 waves/SIMD otherwise); 1 = no constraint.
 template <int MODE>
-constexpr int plain_waves() { return (MODE == M_STATS || MODE == M_STATS_FAST) ? 4 : 1; }
+constexpr int plain_waves() {
+    return (MODE == M_STATS || MODE == M_STATS_FAST) ? 4 : (MODE == M_COLUMNS_FAST_STATS ? 5 : 1);
+}
 
 // A document's Params read through the constant address space: nothing
 // writes them during a batch, and the compiler then loads their fields with
