@@ -1702,7 +1702,7 @@ __device__ __forceinline__ void scan_tile(const Params& P, const uint32_t tile, 
 // over the register budget of 4 waves/SIMD otherwise); 1 = no constraint.
 template <int MODE>
 constexpr int plain_waves() {
-    return (MODE == M_STATS || MODE == M_STATS_FAST) ? 4 : (MODE == M_COLUMNS_FAST_STATS ? 5 : 1);
+    return MODE == M_STATS ? 4 : ((MODE == M_STATS_FAST || MODE == M_COLUMNS_FAST_STATS) ? 5 : 1);
 }
 
 // A document's Params read through the constant address space: nothing
