@@ -32,25 +32,18 @@ uint32_t size_class(uint32_t n_tiles) {
 bool op_uses_spec(uint32_t op) { return op >= FALX_OP_INDEX && op <= FALX_OP_FIELD_BYTES; }
 
 template <int MODE>
-void launch_tiles_batch(int kind, dim3 g, hipStream_t s, const BatchHdr* H, uint32_t tab) {
-    switch (kind) {
-    case 0: hipLaunchKernelGGL((k_tiles_batch<false, false, false, MODE>), g, dim3(WG), 0, s, H, tab); break;
-    case 1: hipLaunchKernelGGL((k_tiles_batch<false, false, true, MODE>), g, dim3(WG), 0, s, H, tab); break;
-    case 2: hipLaunchKernelGGL((k_tiles_batch<true, false, false, MODE>), g, dim3(WG), 0, s, H, tab); break;
-    case 3: hipLaunchKernelGGL((k_tiles_batch<true, true, false, MODE>), g, dim3(WG), 0, s, H, tab); break;
-    case 4: hipLaunchKernelGGL((k_tiles_batch<true, false, true, MODE>), g, dim3(WG), 0, s, H, tab); break;
-    default: hipLaunchKernelGGL((k_tiles_batch<true, true, true, MODE>), g, dim3(WG), 0, s, H, tab); break;
-    }
+hipError_t launch_tiles_batch(int kind, dim3 g, hipStream_t s, const BatchHdr* H, uint32_t tab) {
+    return launch_tile_kernel(kind, F_BATCH, MODE, g, s, nullptr, H, tab);
 }
 
-void launch_emit(uint32_t e, int kind, dim3 g, hipStream_t s, const BatchHdr* H, uint32_t tab) {
+hipError_t launch_emit(uint32_t e, int kind, dim3 g, hipStream_t s, const BatchHdr* H, uint32_t tab) {
     switch (e) {
-    case E_INDEX: launch_tiles_batch<M_INDEX>(kind, g, s, H, tab); break;
-    case E_COLUMNS: launch_tiles_batch<M_COLUMNS>(kind, g, s, H, tab); break;
-    case E_STATS: launch_tiles_batch<M_STATS>(kind, g, s, H, tab); break;
-    case E_STRINGS: launch_tiles_batch<M_STRINGS>(kind, g, s, H, tab); break;
-    case E_LOGFMT: launch_tiles_batch<M_LOGFMT>(kind, g, s, H, tab); break;
-    default: launch_tiles_batch<M_FIELDBYTES>(kind, g, s, H, tab); break;
+    case E_INDEX: return launch_tiles_batch<M_INDEX>(kind, g, s, H, tab);
+    case E_COLUMNS: return launch_tiles_batch<M_COLUMNS>(kind, g, s, H, tab);
+    case E_STATS: return launch_tiles_batch<M_STATS>(kind, g, s, H, tab);
+    case E_STRINGS: return launch_tiles_batch<M_STRINGS>(kind, g, s, H, tab);
+    case E_LOGFMT: return launch_tiles_batch<M_LOGFMT>(kind, g, s, H, tab);
+    default: return launch_tiles_batch<M_FIELDBYTES>(kind, g, s, H, tab);
     }
 }
 
@@ -494,9 +487,15 @@ falx_status falx_batch_submit(falx_batch* b, const falx_request* reqs, uint32_t 
     const BatchHdr* dH = reinterpret_cast<const BatchHdr*>(d);
     const dim3 tg(b->tune.tile_workgroups), ug(b->tune.unit_workgroups);
     auto launch_all = [&](hipStream_t ss) -> hipError_t {
+        // The tile launches check their own errors; the first one found is
+        // returned (the others are checked at the end).
+        hipError_t first = hipSuccess;
+        auto keep = [&](hipError_t e) {
+            if (first == hipSuccess) first = e;
+        };
         if (sig & (1ULL << T_FILL)) hipLaunchKernelGGL(k_fill_batch, ug, dim3(WG), 0, ss, dH);
         for (int k = 0; k < 6; ++k)
-            if (sig & (1ULL << (T_SUM + k))) launch_tiles_batch<M_SUMMARY>(k, tg, ss, dH, T_SUM + k);
+            if (sig & (1ULL << (T_SUM + k))) keep(launch_tiles_batch<M_SUMMARY>(k, tg, ss, dH, T_SUM + k));
         if (sig & (1ULL << T_GROUPS)) {
             hipLaunchKernelGGL(k_prefix_reduce_batch, ug, dim3(SG), 0, ss, dH);
             hipLaunchKernelGGL(k_prefix_groups_batch, ug, dim3(64), 0, ss, dH);
@@ -506,19 +505,20 @@ falx_status falx_batch_submit(falx_batch* b, const falx_request* reqs, uint32_t 
         for (uint32_t e : order)
             for (int k = 0; k < 6; ++k) {
                 const uint32_t t = T_EMIT + k * N_EMIT + e;
-                if (sig & (1ULL << t)) launch_emit(e, k, tg, ss, dH, t);
+                if (sig & (1ULL << t)) keep(launch_emit(e, k, tg, ss, dH, t));
             }
         if (sig & (1ULL << T_SLOT)) {
             hipLaunchKernelGGL(k_slot_scan_batch, ug, dim3(SLOT_WG), 0, ss, dH);
             for (int k = 0; k < 6; ++k) {
                 const uint32_t t = T_EMIT + k * N_EMIT + E_STRINGS;
-                if (sig & (1ULL << t)) launch_emit(E_STRINGS, k, tg, ss, dH, t);
+                if (sig & (1ULL << t)) keep(launch_emit(E_STRINGS, k, tg, ss, dH, t));
             }
         }
         if (sig & (1ULL << T_NDJ)) hipLaunchKernelGGL(k_ndjson_batch, tg, dim3(ND_WG), 0, ss, dH);
         if (sig & (1ULL << T_RED1)) hipLaunchKernelGGL(k_reduce_batch, ug, dim3(WG), 0, ss, dH, 1u);
         if (sig & (1ULL << T_RED2)) hipLaunchKernelGGL(k_reduce_batch, ug, dim3(WG), 0, ss, dH, 2u);
-        return hipGetLastError();
+        keep(hipGetLastError());
+        return first;
     };
     static_assert(N_TAB <= 64, "signature bits");
     if (sig) {

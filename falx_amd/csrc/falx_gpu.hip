@@ -110,33 +110,13 @@ int dialect_kind(const falx_spec* s) {
 // Launches one k_scan pass over all tiles, a workgroup per tile.
 template <int MODE>
 hipError_t launch_scan(falx_ctx* c, Params& P, int kind) {
-    const hipStream_t s = c->stream;
-    dim3 g(P.n_tiles), b(WG);
-    switch (kind) {
-    case 0: hipLaunchKernelGGL((k_scan<false, false, false, MODE>), g, b, 0, s, P); break;
-    case 1: hipLaunchKernelGGL((k_scan<false, false, true, MODE>), g, b, 0, s, P); break;
-    case 2: hipLaunchKernelGGL((k_scan<true, false, false, MODE>), g, b, 0, s, P); break;
-    case 3: hipLaunchKernelGGL((k_scan<true, true, false, MODE>), g, b, 0, s, P); break;
-    case 4: hipLaunchKernelGGL((k_scan<true, false, true, MODE>), g, b, 0, s, P); break;
-    default: hipLaunchKernelGGL((k_scan<true, true, true, MODE>), g, b, 0, s, P); break;
-    }
-    return hipGetLastError();
+    return launch_tile_kernel(kind, F_PLAIN, MODE, dim3(P.n_tiles), c->stream, &P, nullptr, 0);
 }
 
 // The full form of a pass over the tiles its fast form flagged.
 template <int MODE>
 hipError_t launch_redo(falx_ctx* c, Params& P, int kind) {
-    const hipStream_t s = c->stream;
-    dim3 g((P.n_tiles + 63) / 64), b(WG);
-    switch (kind) {
-    case 0: hipLaunchKernelGGL((k_redo<false, false, false, MODE>), g, b, 0, s, P); break;
-    case 1: hipLaunchKernelGGL((k_redo<false, false, true, MODE>), g, b, 0, s, P); break;
-    case 2: hipLaunchKernelGGL((k_redo<true, false, false, MODE>), g, b, 0, s, P); break;
-    case 3: hipLaunchKernelGGL((k_redo<true, true, false, MODE>), g, b, 0, s, P); break;
-    case 4: hipLaunchKernelGGL((k_redo<true, false, true, MODE>), g, b, 0, s, P); break;
-    default: hipLaunchKernelGGL((k_redo<true, true, true, MODE>), g, b, 0, s, P); break;
-    }
-    return hipGetLastError();
+    return launch_tile_kernel(kind, F_REDO, MODE, dim3((P.n_tiles + 63) / 64), c->stream, &P, nullptr, 0);
 }
 
 // A COLUMNS or STATS pass as its fast form over all tiles, then the full

@@ -1759,9 +1759,11 @@ void k_redo(Params P) {
 // --------------------------------------------------------------------------
 // reductions, fills, NDJSON id+score
 // --------------------------------------------------------------------------
+#ifndef FX_KIND   // defined once, in the host translation unit
 __global__ __launch_bounds__(WG) void k_fill_u64(unsigned long long* p, uint64_t n, unsigned long long v) {
     for (uint64_t i = (uint64_t)blockIdx.x * WG + threadIdx.x; i < n; i += (uint64_t)gridDim.x * WG) p[i] = v;
 }
+#endif
 
 // --------------------------------------------------------------------------
 // tile-prefix scan: tile summaries -> each tile's entry state and exclusive
@@ -1806,10 +1808,12 @@ DEV void scan_reduce_body(const uint32_t* ts, uint32_t stride, uint32_t n_tiles,
     }
 }
 
+#ifndef FX_KIND   // defined once, in the host translation unit
 __global__ __launch_bounds__(SG) void k_scan_reduce(const uint32_t* ts, uint32_t stride, uint32_t n_tiles,
                                                      uint32_t ns, uint32_t* gs) {
     scan_reduce_body(ts, stride, n_tiles, ns, gs, blockIdx.x);
 }
+#endif
 
 // Level 2: one wave over the group summaries. Each lane folds a run of
 // groups, lane 0 chains the lanes from state 0, and each lane then walks its
@@ -1868,10 +1872,12 @@ DEV void scan_groups_body(const uint32_t* gs, uint32_t stride, uint32_t n_groups
     }
 }
 
+#ifndef FX_KIND   // defined once, in the host translation unit
 __global__ __launch_bounds__(64) void k_scan_groups(const uint32_t* gs, uint32_t stride, uint32_t n_groups,
                                                      uint32_t ns, uint32_t* gpre, unsigned long long* result) {
     scan_groups_body(gs, stride, n_groups, ns, gpre, result);
 }
+#endif
 
 // String arena bases: exclusive prefix over tiles of one string slot's
 // cleaned byte counts (blockIdx.x = slot). Each thread sums a run of tiles,
@@ -1904,10 +1910,12 @@ DEV void slot_scan_body(const uint32_t* len, uint32_t n_tiles, uint32_t ns, uint
     if (t == SLOT_WG - 1) result[R_STR_LEN + slot] = run;
 }
 
+#ifndef FX_KIND   // defined once, in the host translation unit
 __global__ __launch_bounds__(SLOT_WG) void k_slot_scan(const uint32_t* len, uint32_t n_tiles, uint32_t ns,
                                                        uint32_t* base, unsigned long long* result) {
     slot_scan_body(len, n_tiles, ns, base, result, blockIdx.x);
 }
+#endif
 
 // Level 3: per group, thread 0 walks its SG tiles from the group's entry.
 DEV void scan_tiles_body(const uint32_t* ts, uint32_t stride, uint32_t n_tiles, uint32_t ns, const uint32_t* gpre,
@@ -1935,10 +1943,12 @@ DEV void scan_tiles_body(const uint32_t* ts, uint32_t stride, uint32_t n_tiles, 
         for (int i = 0; i < 8; ++i) tpre[(size_t)tile * 8 + i] = out[t][i];
 }
 
+#ifndef FX_KIND   // defined once, in the host translation unit
 __global__ __launch_bounds__(SG) void k_scan_tiles(const uint32_t* ts, uint32_t stride, uint32_t n_tiles,
                                                     uint32_t ns, const uint32_t* gpre, uint32_t* tpre) {
     scan_tiles_body(ts, stride, n_tiles, ns, gpre, tpre, blockIdx.x);
 }
+#endif
 
 // Deterministic two-level reductions over per-tile partials: level 1 folds
 // RED_BLOCKS contiguous tile ranges in a fixed order, level 2 folds those.
@@ -1959,10 +1969,12 @@ DEV void reduce_u64_body(const unsigned long long* parts, uint32_t n, uint32_t w
     }
 }
 
+#ifndef FX_KIND   // defined once, in the host translation unit
 __global__ __launch_bounds__(WG) void k_reduce_u64(const unsigned long long* parts, uint32_t n, uint32_t width,
                                                    unsigned long long* out) {
     reduce_u64_body(parts, n, width, out, blockIdx.x, gridDim.x);
 }
+#endif
 
 // Per-column stat reduction. grid = (n_cols, chunks); fixed-order FP64 folds.
 DEV void reduce_stats_body(const StatPart* parts, uint32_t n_tiles, uint32_t n_cols, StatPart* out, uint32_t c,
@@ -1992,10 +2004,12 @@ DEV void reduce_stats_body(const StatPart* parts, uint32_t n_tiles, uint32_t n_c
     }
 }
 
+#ifndef FX_KIND   // defined once, in the host translation unit
 __global__ __launch_bounds__(WG) void k_reduce_stats(const StatPart* parts, uint32_t n_tiles, uint32_t n_cols,
                                                      StatPart* out) {
     reduce_stats_body(parts, n_tiles, n_cols, out, blockIdx.x, blockIdx.y, gridDim.y);
 }
+#endif
 
 struct NdPart {
     unsigned long long records, invalid;
@@ -2112,10 +2126,12 @@ DEV void ndjson_tile(const uint8_t* data, uint32_t len, uint32_t n_tiles, NdPart
     }
 }
 
+#ifndef FX_KIND   // defined once, in the host translation unit
 __global__ __launch_bounds__(ND_WG) void k_ndjson_id_score(const uint8_t* data, uint32_t len, uint32_t n_tiles,
                                                         NdPart* parts) {
     ndjson_tile(data, len, n_tiles, parts, blockIdx.x);
 }
+#endif
 
 DEV void reduce_nd_body(const NdPart* parts, uint32_t n, NdPart* out, uint32_t blk, uint32_t nblk) {
     __shared__ unsigned long long ush[NWAVE];
@@ -2143,9 +2159,11 @@ DEV void reduce_nd_body(const NdPart* parts, uint32_t n, NdPart* out, uint32_t b
     }
 }
 
+#ifndef FX_KIND   // defined once, in the host translation unit
 __global__ __launch_bounds__(WG) void k_reduce_nd(const NdPart* parts, uint32_t n, NdPart* out) {
     reduce_nd_body(parts, n, out, blockIdx.x, gridDim.x);
 }
+#endif
 
 // --------------------------------------------------------------------------
 // batch stages over the other kernels' bodies
@@ -2275,6 +2293,7 @@ DEV Unit unit_at(const SegT& T, uint32_t u) {
     return Unit{T.req[i], u - T.start[i]};
 }
 
+#ifndef FX_KIND   // defined once, in the host translation unit
 __global__ __launch_bounds__(SG) void k_prefix_reduce_batch(const BatchHdr* H) {
     const SegT& T = H->tab[T_GROUPS];
     for (uint32_t u = blockIdx.x; u < T.total; u += gridDim.x) {
@@ -2284,7 +2303,9 @@ __global__ __launch_bounds__(SG) void k_prefix_reduce_batch(const BatchHdr* H) {
         __syncthreads();
     }
 }
+#endif
 
+#ifndef FX_KIND   // defined once, in the host translation unit
 __global__ __launch_bounds__(64) void k_prefix_groups_batch(const BatchHdr* H) {
     const SegT& T = H->tab[T_PREQ];
     for (uint32_t u = blockIdx.x; u < T.total; u += gridDim.x) {
@@ -2293,7 +2314,9 @@ __global__ __launch_bounds__(64) void k_prefix_groups_batch(const BatchHdr* H) {
         __syncthreads();
     }
 }
+#endif
 
+#ifndef FX_KIND   // defined once, in the host translation unit
 __global__ __launch_bounds__(SG) void k_prefix_tiles_batch(const BatchHdr* H) {
     const SegT& T = H->tab[T_GROUPS];
     for (uint32_t u = blockIdx.x; u < T.total; u += gridDim.x) {
@@ -2303,7 +2326,9 @@ __global__ __launch_bounds__(SG) void k_prefix_tiles_batch(const BatchHdr* H) {
         __syncthreads();
     }
 }
+#endif
 
+#ifndef FX_KIND   // defined once, in the host translation unit
 __global__ __launch_bounds__(SLOT_WG) void k_slot_scan_batch(const BatchHdr* H) {
     const SegT& T = H->tab[T_SLOT];
     for (uint32_t u = blockIdx.x; u < T.total; u += gridDim.x) {
@@ -2313,9 +2338,11 @@ __global__ __launch_bounds__(SLOT_WG) void k_slot_scan_batch(const BatchHdr* H) 
         __syncthreads();
     }
 }
+#endif
 
 // Reduction level 1 (units: stats jobs width x blocks, others blocks) or
 // level 2 (units: stats jobs width, others 1; jobs with blocks > 1 only).
+#ifndef FX_KIND   // defined once, in the host translation unit
 __global__ __launch_bounds__(WG) void k_reduce_batch(const BatchHdr* H, uint32_t level) {
     const SegT& T = H->tab[level == 1 ? T_RED1 : T_RED2];
     for (uint32_t u = blockIdx.x; u < T.total; u += gridDim.x) {
@@ -2337,7 +2364,9 @@ __global__ __launch_bounds__(WG) void k_reduce_batch(const BatchHdr* H, uint32_t
         __syncthreads();
     }
 }
+#endif
 
+#ifndef FX_KIND   // defined once, in the host translation unit
 __global__ __launch_bounds__(ND_WG) void k_ndjson_batch(const BatchHdr* H) {
     __shared__ uint32_t s_p, s_t0, s_t1;
     const SegT& T = H->tab[T_NDJ];
@@ -2363,7 +2392,9 @@ __global__ __launch_bounds__(ND_WG) void k_ndjson_batch(const BatchHdr* H) {
         }
     }
 }
+#endif
 
+#ifndef FX_KIND   // defined once, in the host translation unit
 __global__ __launch_bounds__(WG) void k_fill_batch(const BatchHdr* H) {
     const SegT& T = H->tab[T_FILL];
     for (uint32_t u = blockIdx.x; u < T.total; u += gridDim.x) {
@@ -2371,6 +2402,48 @@ __global__ __launch_bounds__(WG) void k_fill_batch(const BatchHdr* H) {
         const BFill& F = H->fill[x.req];
         const uint64_t lo = (uint64_t)x.k * FILL_UNIT, hi = min(F.n, lo + FILL_UNIT);
         for (uint64_t i = lo + threadIdx.x; i < hi; i += WG) F.p[i] = F.v;
+    }
+}
+#endif
+
+// --------------------------------------------------------------------------
+// tile-kernel launches, one translation unit per dialect kind (kinds.hip)
+// --------------------------------------------------------------------------
+// Dialect kinds: 0 no quote, 1 comment only, 2 quote, 3 quote + backslash
+// escape, 4 quote + comment, 5 quote + backslash escape + comment.
+enum Form : int {
+    F_PLAIN = 0,   // k_scan, a workgroup per tile
+    F_REDO = 1,    // k_redo over the tiles a fast pass flagged
+    F_BATCH = 2    // k_tiles_batch over table `tab` of the batch H
+};
+
+// Launches the tile kernel of dialect kind K for (form, mode); defined in
+// kinds.hip, which is compiled once per kind. hipErrorInvalidValue for a
+// (form, mode) that is never launched for this kind.
+template <int K>
+hipError_t launch_kind(int form, int mode, dim3 g, hipStream_t s, const Params* P, const BatchHdr* H, uint32_t tab);
+template <>
+hipError_t launch_kind<0>(int, int, dim3, hipStream_t, const Params*, const BatchHdr*, uint32_t);
+template <>
+hipError_t launch_kind<1>(int, int, dim3, hipStream_t, const Params*, const BatchHdr*, uint32_t);
+template <>
+hipError_t launch_kind<2>(int, int, dim3, hipStream_t, const Params*, const BatchHdr*, uint32_t);
+template <>
+hipError_t launch_kind<3>(int, int, dim3, hipStream_t, const Params*, const BatchHdr*, uint32_t);
+template <>
+hipError_t launch_kind<4>(int, int, dim3, hipStream_t, const Params*, const BatchHdr*, uint32_t);
+template <>
+hipError_t launch_kind<5>(int, int, dim3, hipStream_t, const Params*, const BatchHdr*, uint32_t);
+
+inline hipError_t launch_tile_kernel(int kind, int form, int mode, dim3 g, hipStream_t s, const Params* P,
+                                     const BatchHdr* H, uint32_t tab) {
+    switch (kind) {
+    case 0: return launch_kind<0>(form, mode, g, s, P, H, tab);
+    case 1: return launch_kind<1>(form, mode, g, s, P, H, tab);
+    case 2: return launch_kind<2>(form, mode, g, s, P, H, tab);
+    case 3: return launch_kind<3>(form, mode, g, s, P, H, tab);
+    case 4: return launch_kind<4>(form, mode, g, s, P, H, tab);
+    default: return launch_kind<5>(form, mode, g, s, P, H, tab);
     }
 }
 
