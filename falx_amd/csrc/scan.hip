@@ -2052,23 +2052,6 @@ DEV void ndjson_tile(const uint8_t* data, uint32_t len, uint32_t n_tiles, NdPart
         }
         *reinterpret_cast<uint4*>(&bytes[i]) = v;
     }
-    // The newline before the tile (-1: none), found by wave 0 reading 64
-    // bytes per step backwards, one per lane.
-    if (w == 0) {
-        int64_t found = -1;
-        for (int64_t hi = (int64_t)tile_start; hi > 0; hi -= 64) {
-            const int64_t p = hi - 64 + (int64_t)ln;
-            const uint64_t m = ballot(p >= 0 && data[p] == '\n');
-            if (m) {
-                found = hi - 64 + (int64_t)hibit(m);
-                break;
-            }
-        }
-        if (ln == 0) {
-            slow.lock = 0;
-            prev_nl = found;
-        }
-    }
     __syncthreads();
     uint32_t cnt = 0;
     const uint32_t seg = w * ND_BPW * 64;
@@ -2086,8 +2069,38 @@ DEV void ndjson_tile(const uint8_t* data, uint32_t len, uint32_t n_tiles, NdPart
         for (int k = 0; k < ND_WAVES; ++k) wb[k + 1] = wb[k] + wcnt[k];
     }
     __syncthreads();
-    const uint32_t n = uni(wb[ND_WAVES]);
     const bool tail = tile + 1 == n_tiles && len > 0 && data[len - 1] != '\n';
+    // The newline before the tile (-1: none), needed only by a tile that owns
+    // a record (it holds a '\n', or it is the last tile and the data does not
+    // end in one): so a stretch of the input without newlines is searched
+    // once, by the tile that ends it, not by every tile inside it. Wave 0
+    // reads 512 bytes per step backwards, 8 per lane (tile starts are
+    // multiples of 512, so every word read is aligned and below the tile).
+    if (w == 0) {
+        int64_t found = -1;
+        if (wb[ND_WAVES] > 0 || tail) {
+            for (int64_t hi = (int64_t)tile_start; hi > 0; hi -= 512) {
+                const int64_t p = hi - 512 + 8 * (int64_t)ln;
+                uint64_t x = 0;
+                if (p >= 0) x = *reinterpret_cast<const uint64_t*>(data + p) ^ 0x0A0A0A0A0A0A0A0AULL;
+                // Bytes of x that are zero (exact, no borrow across bytes).
+                const uint64_t z = ~(((x & 0x7F7F7F7F7F7F7F7FULL) + 0x7F7F7F7F7F7F7F7FULL) | x | 0x7F7F7F7F7F7F7F7FULL);
+                const uint64_t m = ballot(p >= 0 && z != 0ULL);
+                if (m) {
+                    const uint32_t L = hibit(m);
+                    const uint32_t byte = hibit(z) >> 3;
+                    found = hi - 512 + 8 * (int64_t)L + (int64_t)lane_value(byte, L);
+                    break;
+                }
+            }
+        }
+        if (ln == 0) {
+            slow.lock = 0;
+            prev_nl = found;
+        }
+    }
+    __syncthreads();
+    const uint32_t n = uni(wb[ND_WAVES]);
     TileSrc src{data, bytes, tile_start, tile_start + tile_len};
     auto nl_at = [&](uint32_t k) -> uint32_t {
         uint32_t ww = 0;

@@ -18,7 +18,7 @@ SEED_CSV_HASH = 0xC0FFEE15600DF00D
 SEED_TSV = 0xA0761D6478BD642F
 SEED_LOGFMT = 0xE7037ED1A0B428DB
 SEED_NDJSON = 0x853C49E6748FEA9B
-GENERATOR_VERSION = 1
+GENERATOR_VERSION = 2
 
 
 class GeoTruth(C.Structure):
@@ -44,7 +44,8 @@ class TsvTruth(C.Structure):
 
 
 class LogfmtTruth(C.Structure):
-    _fields_ = [("lines", C.c_uint64)]
+    _fields_ = [("lines", C.c_uint64), ("pairs", C.c_uint64), ("key_bytes", C.c_uint64),
+                ("value_bytes", C.c_uint64), ("checksum", C.c_uint64)]
 
 
 _lib = None

@@ -321,9 +321,15 @@ uint64_t datagen_tsv(uint8_t* buf, uint64_t target, uint64_t cap, uint64_t seed,
 
 struct logfmt_truth {
     uint64_t lines;
+    uint64_t pairs;        // key=value pairs written
+    uint64_t key_bytes;    // key lengths
+    uint64_t value_bytes;  // value lengths after quote stripping and unescaping
+    uint64_t checksum;     // wrapping byte sum of the keys and the unescaped values
 };
 
 // logfmt `k=v` lines with quoted values holding escaped quotes/backslashes.
+// The truths count what each pair decodes to: a quoted value without its
+// quotes, and each `\x` of it as the one byte x.
 uint64_t datagen_logfmt(uint8_t* buf, uint64_t target, uint64_t cap, uint64_t seed,
                         logfmt_truth* t) {
     Rng r{seed};
@@ -331,26 +337,53 @@ uint64_t datagen_logfmt(uint8_t* buf, uint64_t target, uint64_t cap, uint64_t se
     std::memset(t, 0, sizeof *t);
     static const char* levels[] = {"debug", "info", "warn", "error"};
     static const char alnum[] = "abcdefghijklmnopqrstuvwxyz0123456789";
+    // Byte sum and length of what was written since `from`.
+    auto since = [&](uint64_t from, uint64_t& bytes, uint64_t& sum) {
+        for (uint64_t i = from; i < o.n; ++i) sum += o.p[i];
+        bytes += o.n - from;
+    };
+    // A pair whose key is `k` and whose (unquoted) value is written by `value`.
+    auto pair = [&](const char* k, auto&& value) {
+        const uint64_t k0 = o.n;
+        o.put(k);
+        since(k0, t->key_bytes, t->checksum);
+        o.put('=');
+        const uint64_t v0 = o.n;
+        value();
+        since(v0, t->value_bytes, t->checksum);
+        t->pairs += 1;
+    };
     while (o.n < target && o.ok) {
-        o.put("ts=");
-        o.put_u64(1700000000 + r.below(10000000));
-        o.put(" level=");
-        o.put(levels[r.below(4)]);
+        pair("ts", [&] { o.put_u64(1700000000 + r.below(10000000)); });
+        o.put(' ');
+        pair("level", [&] { o.put(levels[r.below(4)]); });
         o.put(" msg=\"");
+        t->key_bytes += 3;
+        t->checksum += 'm' + 's' + 'g';
         int len = 4 + (int)r.below(40);
         for (int i = 0; i < len; ++i) {
             uint64_t c = r.below(40);
-            if (c == 0) { o.put('\\'); o.put('"'); }
-            else if (c == 1) { o.put('\\'); o.put('\\'); }
-            else if (c < 6) o.put(' ');
-            else if (c == 6) o.put('=');
-            else o.put((uint8_t)alnum[r.below(36)]);
+            uint8_t b;
+            if (c == 0) { o.put('\\'); b = '"'; }
+            else if (c == 1) { o.put('\\'); b = '\\'; }
+            else if (c < 6) b = ' ';
+            else if (c == 6) b = '=';
+            else b = (uint8_t)alnum[r.below(36)];
+            o.put(b);
+            t->value_bytes += 1;
+            t->checksum += b;
         }
         o.put('"');
-        o.put(" dur=");
-        o.put_fixed((int64_t)r.below(100000), 2);
-        o.put("ms");
-        if (r.below(4) == 0) { o.put(" user="); for (int i = 0; i < 6; ++i) o.put((uint8_t)alnum[r.below(36)]); }
+        t->pairs += 1;
+        o.put(' ');
+        pair("dur", [&] {
+            o.put_fixed((int64_t)r.below(100000), 2);
+            o.put("ms");
+        });
+        if (r.below(4) == 0) {
+            o.put(' ');
+            pair("user", [&] { for (int i = 0; i < 6; ++i) o.put((uint8_t)alnum[r.below(36)]); });
+        }
         o.put('\n');
         t->lines += 1;
     }

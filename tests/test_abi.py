@@ -58,3 +58,30 @@ def test_spec_toml_front_end():
                 'name = "x"\nstructural = [","]\nescape = "weird"\n']:
         with pytest.raises(ValueError):
             S.parse_spec_toml(bad)
+
+
+def test_spec_validate_rejections_through_the_c_abi():
+    """falx_spec_validate (no device needed) applies the spec.rs / formats.rs
+    rules: every reference dialect passes; each broken rule is rejected."""
+    path = build.gpu_lib_path()
+    if not os.path.exists(path):
+        pytest.skip("libfalx_gpu.so not built")
+    L = gpu.load()
+    for spec in list(S.DIALECTS.values()) + [S.CSV_HASH, S.CSV_TYPED, S.GEO_ALL, S.HASH_TYPED, S.TSV_TYPED]:
+        assert L.falx_spec_validate(ctypes.byref(spec.to_c())) == S.OK, spec.name
+
+    def broken(**kw):
+        c = S.CSV_HASH.to_c()
+        for k, v in kw.items():
+            setattr(c, k, v)
+        return c
+
+    too_many_cols = S.CSV_HASH.to_c()
+    too_many_cols.n_columns = 33
+    bad_type = S.CSV_HASH.to_c()
+    bad_type.columns[1].type = 4
+    for c in [broken(n_structural=0), broken(n_structural=17), broken(quote=256), broken(quote=-2),
+              broken(escape=ord('"')), broken(comment=ord("\n")), broken(comment=ord("\r")),
+              broken(comment=ord('"')), broken(escape=300), too_many_cols, bad_type]:
+        assert L.falx_spec_validate(ctypes.byref(c)) == S.E_INVALID
+    assert L.falx_spec_validate(None) == S.E_INVALID

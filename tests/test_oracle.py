@@ -14,6 +14,7 @@ import pytest
 from falx_amd import datagen
 from falx_amd import spec as S
 from falx_amd.records import Parsed
+from falx_amd.records import clean as clean_field
 from helpers import Rng, assert_columns_equal, assert_stats_equal, dialect_alphabet, random_doc
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -206,3 +207,191 @@ def test_generator_closed_forms(ora):
     h, ht = datagen.hash_csv(1_000_000)
     hs = ora.csv_stats(S.HASH_TYPED, h)
     assert hs["records"] == ht["records"] and hs["cols"][0]["checksum"] == ht["i64_sum"] % 2**64
+
+
+# ---- field bytes: pinned by the independent records() walk ----------------
+FIELD_BYTE_HAND_CASES = [
+    b"a,1\nb,2\nc,3\n", b"# header\na,1\nb,2\n", b"a,1\n# tail\n", b"# just a comment\n",
+    b"\"a\nb\",1\nc,2\n", b"\"a#b\",1\n\"#\",2\n", b"# a \" b\na,1\n", b"# \"\na,1\nb,2\n",
+    b"\"a\"\"b\",1\n# c\n", b",\n,\n", b"a,1\nb,2", b"a,1\r\n# c\r\nb,2\r\n",
+]
+
+
+def records_field_bytes(ora, spec, data):
+    """(fields, bytes) from the tape's record iterator: every field of every
+    non-comment record, cleaned (csv_hash_fused_par.rs:153-167)."""
+    p = Parsed(data, *ora.parse(spec, data), spec)
+    fields = nbytes = 0
+    for rec in p.records():
+        for f in rec.fields():
+            fields += 1
+            nbytes += len(f)
+    return fields, nbytes
+
+
+def field_byte_random_docs(seed=0x0FB72C419E55D003, n=120):
+    """The random documents of csv_hash_fused_par.rs:203-229 (same seed and
+    generator, so the same documents)."""
+    rng = Rng(seed)
+    docs = []
+    for _ in range(n):
+        rows = 1 + rng.next() % 50
+        d = bytearray()
+        for _ in range(rows):
+            c = rng.next() % 7
+            if c == 0:
+                d += b"# comment with \" quote\n"
+            elif c == 1:
+                d += b"\"quoted\nnewline\",42\n"
+            elif c == 2:
+                d += b"\"has#hash\",7\n"
+            elif c == 3:
+                d += b"\"a\"\"b\",13\n"
+            elif c == 4:
+                d += b",\n"
+            elif c == 5:
+                d += b"\"a,b\",5\n"
+            else:
+                k = rng.next() % 1000
+                d += b"key%d,%d\n" % (k, k)
+        docs.append(bytes(d))
+    return docs
+
+
+def test_field_bytes_match_records_reference(ora):
+    """csv_hash_fused_par.rs:186-233: the fused field-byte sink equals the
+    tape's records() walk on the 12 hazard cases and the 120 random
+    documents, serial and at 1, 2, 3, 5, 8 and 17 threads."""
+    spec = S.CSV_HASH_DIALECT
+    for doc in FIELD_BYTE_HAND_CASES + field_byte_random_docs():
+        want = records_field_bytes(ora, spec, doc)
+        for t in (1, 2, 3, 5, 8, 17):
+            got = ora.field_bytes(spec, doc, t)
+            assert (got["fields"], got["bytes"]) == want, (doc, t)
+
+
+def test_field_bytes_known_answers(ora):
+    """Hand-counted answers: cleaned lengths with doubled quotes collapsed,
+    comment records skipped, the record's trailing CR trimmed."""
+    spec = S.CSV_HASH_DIALECT
+    cases = {
+        b"a,1\nb,2\nc,3\n": (6, 6),
+        b"\"a\"\"b\",1\n# c\n": (2, 4),        # a"b + 1
+        b",\n,\n": (4, 0),
+        b"a,1\r\n# c\r\nb,2\r\n": (4, 4),       # CR trimmed from the last field
+        b"# just a comment\n": (0, 0),
+        b"\"a\nb\",1\nc,2\n": (4, 6),           # a\nb is 3 bytes
+        b"a,1\nb,2": (4, 4),                    # trailing unterminated record
+        b"\n": (1, 0),                          # an empty line is one empty field
+    }
+    for doc, want in cases.items():
+        got = ora.field_bytes(spec, doc)
+        assert (got["fields"], got["bytes"]) == want, doc
+
+
+@pytest.mark.parametrize("spec", [S.CSV, S.TSV, S.CSV_HASH_DIALECT], ids=lambda s: s.name)
+def test_field_bytes_random_hazard_docs(ora, spec):
+    """Random documents over each dialect's hazard alphabet (quotes, comment
+    bytes, CR, blank lines): the sink equals the records() walk."""
+    rng = Rng(0x5EED_F1E1_DB17_E500)
+    alpha = dialect_alphabet(spec) + b"\r\n"
+    for _ in range(400):
+        doc = random_doc(rng, alpha, rng.below(200))
+        want = records_field_bytes(ora, spec, doc)
+        for t in (1, 3):
+            got = ora.field_bytes(spec, doc, t)
+            assert (got["fields"], got["bytes"]) == want, (doc, t)
+
+
+# ---- logfmt pairs: records() walk, hand answers, generator truths ---------
+def records_logfmt(ora, data):
+    """logfmt.rs:686-858 restated over the tape: consecutive fields of a
+    record pair up (key, value); a value that starts with a quote loses its
+    quotes and escapes as the dialect's cleaning does, any other value is
+    counted raw."""
+    spec = S.LOGFMT
+    p = Parsed(data, *ora.parse(spec, data), spec)
+    pairs = kb = vb = cs = 0
+    for rec in p.records():
+        raw = rec.fields_raw()
+        for i in range(0, len(raw) - 1, 2):
+            k, v = raw[i], raw[i + 1]
+            if v[:1] == b'"':
+                v = clean_field(spec, v)
+            pairs += 1
+            kb += len(k)
+            vb += len(v)
+            cs += sum(k) + sum(v)
+    return {"pairs": pairs, "key_bytes": kb, "value_bytes": vb, "checksum": cs % 2**64}
+
+
+def test_logfmt_known_answers(ora):
+    """Hand-counted logfmt pairs (keys and values counted after unquoting and
+    unescaping quoted values, logfmt.rs:826-858)."""
+    s = lambda b: sum(b)  # noqa: E731
+    cases = {
+        b'a="x \\" y" b=plain\n': (2, 2, 10, s(b"a") + s(b'x " y') + s(b"b") + s(b"plain")),
+        b"k=v\n\nk2=\n": (2, 3, 1, s(b"kvk2")),
+        b"lonely\n": (0, 0, 0, 0),
+        b"a=1 b\r\n": (1, 1, 1, s(b"a1")),
+        b'x="q\\\\z" y="" z=w\n': (3, 3, 4, s(b"xq\\zyzw")),
+        b'k="abc': (1, 1, 4, s(b'k"abc')),         # unterminated quote: counted raw
+        b"a=1 b=2": (2, 2, 2, s(b"a1b2")),
+        b"a=1 ": (1, 1, 1, s(b"a1")),
+        b"a=1 b=2 c\n": (2, 2, 2, s(b"a1b2")),
+        b"p=a\\b\n": (1, 1, 3, s(b"pa\\b")),       # unquoted value: escapes kept
+        b'q="\\\\"\n': (1, 1, 1, s(b"q\\")),
+        b'e="a=b c" f=1\n': (2, 2, 6, s(b"ea=b cf1")),
+    }
+    for doc, (pairs, kb, vb, cs) in cases.items():
+        got = ora.logfmt_pairs(doc)
+        assert (got["pairs"], got["key_bytes"], got["value_bytes"], got["checksum"]) == (pairs, kb, vb, cs), doc
+        assert records_logfmt(ora, doc) == got, doc
+
+
+def test_logfmt_random_hazard_docs(ora):
+    """Random logfmt documents (quotes, escapes, '=', spaces, CR, blank lines,
+    odd field counts, no final newline) against the records() walk."""
+    rng = Rng(0x10F3_A7C0_FFEE_0001)
+    alpha = b'ab= =  ""\\\\\r\n\n'
+    for _ in range(600):
+        doc = random_doc(rng, alpha, rng.below(160))
+        assert ora.logfmt_pairs(doc) == records_logfmt(ora, doc), doc
+
+
+def test_logfmt_generator_closed_forms(ora):
+    """The generator's own count of what it wrote: pairs, key and unescaped
+    value bytes, byte checksum; serial and newline-chunked alike."""
+    data, t = datagen.logfmt(3_000_000)
+    want = {"pairs": t["pairs"], "key_bytes": t["key_bytes"], "value_bytes": t["value_bytes"],
+            "checksum": t["checksum"] % 2**64}
+    for threads in (1, 4, 16):
+        assert ora.logfmt_pairs(data, threads) == want, threads
+    assert records_logfmt(ora, bytes(data[:200_000]).rsplit(b"\n", 1)[0] + b"\n")["pairs"] > 0
+
+
+# ---- string cleaning, backslash dialects ------------------------------------
+def test_backslash_clean_known_answers(ora):
+    """codegen.rs:8078-8110 (append_clean, backslash escape): outer quotes
+    stripped, each escape byte followed by a byte becomes that byte, a final
+    lone escape byte is kept; the same rule unquoted."""
+    cases = {
+        b'"a\\"b"': b'a"b', b"plain": b"plain", b'"x"': b"x", b"a\\\\b": b"a\\b",
+        b'"ab\\"': b"ab\\", b"\\": b"\\", b'"': b'"', b'""': b"", b'"\\\\"': b"\\",
+        b"x\\y\\": b"xy\\", b'"unterminated': b'"unterminated', b"\\n\\t": b"nt",
+        b'"a,b"': b"a,b", b"": b"",
+    }
+    for spec in (S.LOGFMT, S.NDJSON):
+        for cell, want in cases.items():
+            assert ora.clean(spec, cell) == want, (spec.name, cell)
+            assert clean_field(spec, cell) == want, (spec.name, cell)
+
+
+def test_doubled_quote_clean_known_answers(ora):
+    """csv_hash.rs:1749-1779: quoted at both ends -> inside with each doubled
+    quote collapsed (a lone quote copied); otherwise raw."""
+    cases = {b'"a""b"': b'a"b', b'"a"b"': b'a"b', b'""': b"", b'"': b'"', b'x"y': b'x"y',
+             b'""""': b'"', b'"a,b"': b"a,b", b"plain": b"plain", b'"""': b'"'}
+    for cell, want in cases.items():
+        assert ora.clean(S.CSV, cell) == want, cell
+        assert clean_field(S.CSV, cell) == want, cell
