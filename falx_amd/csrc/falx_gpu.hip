@@ -44,13 +44,13 @@ thread_local int32_t tl_hip_error = 0;
         }                                              \
     } while (0)
 
-inline size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+constexpr size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 
 // Entry states of each dialect kind (see dialect_kind): quote x2, backslash
 // escape x2, comment regions x3.
 constexpr uint32_t kKindStates[6] = {1, 3, 2, 4, 3, 6};
 
-uint32_t summary_stride(uint32_t ns) { return (uint32_t)TS_COUNTS + (uint32_t)C7 * ns; }
+constexpr uint32_t summary_stride(uint32_t ns) { return (uint32_t)TS_COUNTS + (uint32_t)C7 * ns; }
 
 // Per-context scratch for a launch over n tiles: tile summaries, group
 // summaries, group and tile prefixes, and per tile and string column the
@@ -59,10 +59,10 @@ struct LbLayout {
     size_t tsum, gsum, gpre, tpre, str_len, str_base, fix, total;
 };
 
-LbLayout lb_layout(size_t n, uint32_t n_str, uint32_t ns) {
+constexpr LbLayout lb_layout(size_t n, uint32_t n_str, uint32_t ns) {
     const size_t stride = 4 * (size_t)summary_stride(ns);
     const size_t groups = (n + SG - 1) / SG;
-    LbLayout L;
+    LbLayout L{};
     size_t o = 0;
     L.tsum = o; o = align_up(o + stride * n, 256);
     L.gsum = o; o = align_up(o + stride * groups, 256);
@@ -135,12 +135,13 @@ hipError_t launch_fast(falx_ctx* c, Params& P, int kind) {
 // The tile-prefix pipeline: tile summaries (every entry state), then the
 // three-level scan to each tile's entry state and exclusive counters, and
 // the grand totals in the result slots. Emit passes read P.tpre.
-hipError_t run_prefix(falx_ctx* c, Params& P, int kind) {
+// `base` is the scratch laid out by lb_layout (the context's, by default).
+hipError_t run_prefix(falx_ctx* c, Params& P, int kind, char* base = nullptr) {
     const uint32_t ns = kKindStates[kind];
     const uint32_t stride = summary_stride(ns);
     const uint32_t groups = (P.n_tiles + SG - 1) / SG;
     LbLayout L = lb_layout(P.n_tiles, P.n_str, ns);
-    char* base = static_cast<char*>(c->d_lb);
+    if (!base) base = static_cast<char*>(c->d_lb);
     uint32_t* tsum = reinterpret_cast<uint32_t*>(base + L.tsum);
     uint32_t* gsum = reinterpret_cast<uint32_t*>(base + L.gsum);
     uint32_t* gpre = reinterpret_cast<uint32_t*>(base + L.gpre);
@@ -718,6 +719,80 @@ falx_status falx_parse_logfmt_pairs(falx_ctx* c, const uint8_t* d, uint64_t len,
     out->value_bytes = h[2];
     out->checksum = h[3];
     return FALX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Index without allocation: the scratch of falx_index_structurals (tile
+// summaries, group summaries, prefixes, result slots) in one file-scope
+// device array, so the call makes no allocation at all.
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr uint64_t kStaticMaxTiles = FALX_STATIC_MAX_LEN / TILE;
+constexpr LbLayout kStaticLayout = lb_layout(kStaticMaxTiles, 0, NS_MAX);
+constexpr size_t kStaticResult = align_up(kStaticLayout.total, 256);
+constexpr size_t kStaticBytes = kStaticResult + 8 * R_SLOTS;
+static_assert(FALX_STATIC_MAX_LEN % TILE == 0, "whole tiles");
+
+// Each device holds its own copy until the process ends (about 2 MiB).
+__device__ __align__(256) uint8_t scratch[kStaticBytes];
+
+// Device address of `scratch` per device (the name is a host stand-in), set
+// once per device by hipGetSymbolAddress with that device current.
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+falx_status static_scratch(int device, char** out) {
+    if (device < 0 || device >= kMaxDevices) return FALX_E_INVALID;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void* p = nullptr;
+        HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return FALX_E_DEVICE;
+        g_scratch_ptr[device] = p;
+    }
+    *out = static_cast<char*>(g_scratch_ptr[device]);
+    return FALX_OK;
+}
+
+}  // namespace
+
+falx_status falx_index_structurals_static(falx_ctx* c, const falx_spec* spec, const uint8_t* d, uint64_t len,
+                                          uint32_t* d_out, uint64_t out_cap, uint64_t* n_out) {
+    falx_status st = check_input(c, d, len);
+    if (st != FALX_OK) return st;
+    if ((st = falx_spec_validate(spec)) != FALX_OK) return st;
+    if (!n_out) return FALX_E_INVALID;
+    *n_out = 0;
+    if (len > FALX_STATIC_MAX_LEN) return FALX_E_TOO_LARGE;
+    if (len == 0) return FALX_OK;
+    tl_hip_error = 0;
+    c->err_flags = c->err_tile = 0;
+    (void)hipGetLastError();
+    HIP_TRY(hipSetDevice(c->device));
+    char* base = nullptr;
+    if ((st = static_scratch(c->device, &base)) != FALX_OK) return st;
+    unsigned long long* result = reinterpret_cast<unsigned long long*>(base + kStaticResult);
+    Params P;
+    spec_params(spec, d, len, P);
+    P.n_cols = P.n_str = 0;   // the index reads no columns
+    P.result = result;
+    P.out = d_out;
+    P.out_cap = d_out ? out_cap : 0;
+    // Result slots start at zero: cleared by a fill kernel, not a memset.
+    HIP_TRY(fill_u64(c->stream, result, R_SLOTS, 0ULL));
+    HIP_TRY(mark_start(c));
+    HIP_TRY(run_prefix(c, P, dialect_kind(spec), base));
+    HIP_TRY(launch_scan<M_INDEX>(c, P, dialect_kind(spec)));
+    HIP_TRY(mark_end(c));
+    HIP_TRY(hipMemcpyAsync(c->h_result, result, R_SLOTS * 8, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    float ms = 0;
+    HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    c->last_ms = ms;
+    *n_out = c->h_result[R_N_STRUCT];
+    return *n_out > P.out_cap ? FALX_E_CAPACITY : FALX_OK;
 }
 
 }  // extern "C"

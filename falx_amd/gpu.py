@@ -37,6 +37,7 @@ _SIGS = [
     ("falx_download", C.c_int, [_VP, _VP, _VP, _U64]),
     ("falx_spec_validate", C.c_int, [C.POINTER(S.CSpec)]),
     ("falx_index_structurals", C.c_int, [_VP, C.POINTER(S.CSpec), _VP, _U64, _VP, _U64, _U64P]),
+    ("falx_index_structurals_static", C.c_int, [_VP, C.POINTER(S.CSpec), _VP, _U64, _VP, _U64, _U64P]),
     ("falx_parse", C.c_int, [_VP, C.POINTER(S.CSpec), _VP, _U64, _VP, _U64, _VP, _U64, _U64P, _U64P]),
     ("falx_parse_columns", C.c_int, [_VP, C.POINTER(S.CSpec), _VP, _U64, C.POINTER(S.CColumns)]),
     ("falx_parse_columns_with_stats", C.c_int,

@@ -193,6 +193,25 @@ falx_status falx_index_structurals(falx_ctx* ctx, const falx_spec* spec,
                                    uint32_t* d_out, uint64_t out_cap,
                                    uint64_t* n_out);
 
+/*
+ * falx_index_structurals without any allocation. No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). Same arguments, outputs and statuses as
+ * falx_index_structurals, but the call's scratch (tile summaries, prefixes,
+ * result slots) lives in a file-scope device array of the library, about
+ * 2 MiB per device, held until the process ends; it makes no hipMalloc,
+ * hipFree or other allocating call (the context is created beforehand).
+ * Inputs longer than FALX_STATIC_MAX_LEN return FALX_E_TOO_LARGE. All calls
+ * on one device share that array: calls that overlap on one GPU (from
+ * several contexts or threads) race and must be serialised by the caller.
+ */
+#define FALX_STATIC_MAX_LEN (64ull << 20)
+falx_status falx_index_structurals_static(falx_ctx* ctx, const falx_spec* spec,
+                                          const uint8_t* d_data, uint64_t len,
+                                          uint32_t* d_out, uint64_t out_cap,
+                                          uint64_t* n_out);
+
 /* ---- tape ----------------------------------------------------------------- */
 
 /*
