@@ -1330,78 +1330,214 @@ __device__ __forceinline__ void scan_tile(const Params& P, const uint32_t tile, 
         }
         return v;
     };
-    // The i64 reading of a field also read as f64 (two columns of one field).
-    auto num_i64_too = [&](uint32_t from, uint32_t to, NumVal& ni) {
-        if constexpr (FAST) {
-            sh.fix_bad = 1u;
-            ni.iok = false;
-            ni.iv = 0;
-        } else {
-            ni.iok = parse_i64_field(src, from, to, cd, ni.iv);
-            if (!ni.iok) ni.iv = 0;
-        }
-    };
-    // The f64 and i64 columns that read ordinal `ord`: one parse per entry
-    // serves all of them (an entry read both ways also parses as i64).
-    auto num_wants = [&](uint32_t c0, uint32_t nc, uint32_t ord, bool& want_f, bool& want_i)
-        __attribute__((always_inline)) {
-        want_f = want_i = false;
-        for (uint32_t j = c0; j < c0 + nc; ++j) {
-            const ColDesc& c = P.cols[j];
-            if (c.index != ord) continue;
-            want_f |= c.type == FALX_COL_F64;
-            want_i |= c.type == FALX_COL_I64;
-        }
-    };
-
-    auto is_num = [](uint32_t type) { return type == FALX_COL_F64 || type == FALX_COL_I64; };
-
-    if (MODE == M_STATS) {
-        const uint32_t c0 = P.col_group;
-        const uint32_t nc = min((uint32_t)STAT_GROUP, P.n_cols - c0);
-        Acc acc[STAT_GROUP];
+    // ---- COLUMNS and STATS: one record per thread --------------------------
+    // The tile's records end at its terminator entries (and at the virtual
+    // terminator of an unterminated last record); a record still open after
+    // the tile's last terminator is finished by the next tile. A thread takes
+    // one record: its fields that end in this tile are the list entries
+    // e0 .. e0 + n - 1, so column c's cell is entry e0 + (c.index - ord0),
+    // found by index, and the lanes of a wave run the same steps for each
+    // column. Terminators are listed per segment of up to TL_MAX records (one
+    // block scan over the list entries per segment). Validity words are built
+    // per wave (rows of consecutive records are consecutive) and ORed into
+    // the column with one atomic per word.
+    if constexpr (MODE == M_COLUMNS || MODE == M_STATS) {
+        constexpr uint32_t TL_MAX = 512;
+        constexpr uint32_t NACC = 2;
+        static_assert(NACC == STAT_GROUP && NACC == COL_STAT_GROUP, "accumulators per pass");
+        __shared__ uint32_t tl[TL_MAX];   // list index | (row rank in the segment << 16)
+        constexpr bool IS_COLS = MODE == M_COLUMNS;
+        const uint32_t c0 = IS_COLS ? 0u : P.col_group;
+        const uint32_t nc = IS_COLS ? P.n_cols : min((uint32_t)STAT_GROUP, P.n_cols - c0);
+        // Known at compile time in the fast forms, which then carry no
+        // statistics registers when there are none.
+        const bool with_stats = !IS_COLS                         ? true
+                                : KMODE == M_COLUMNS_FAST       ? false
+                                : KMODE == M_COLUMNS_FAST_STATS ? true
+                                                                : P.partials != nullptr;
+        const bool trailer_row_live = trailer && !trailer_comment;
+        const uint32_t n_ent = n_list + (trailer ? 1u : 0u);   // the virtual terminator is entry n_list
+        const auto word_at = [&](uint32_t k) -> uint32_t { return k < n_list ? (uint32_t)sh.list[k] : trailer_word; };
+        const auto pos_of = [&](uint32_t wd) -> uint32_t { return tile_start + (wd & 0x3FFFu); };
+        const uint64_t rows_cap = P.rows_cap;
+        Acc acc[NACC];
 #pragma unroll
-        for (int j = 0; j < STAT_GROUP; ++j) acc[j] = Acc{0, 0, 0, 0, 0, 0.0};
+        for (uint32_t j = 0; j < NACC; ++j) acc[j] = Acc{0, 0, 0, 0, 0, 0.0};
         unsigned long long records = 0;
-        walk([&](const Entry& e, uint32_t, uint32_t) {
-            if (!e.valid || e.flag) return;
-            const uint32_t to = field_end(e);
-            if (e.is_term) records += 1;
-            bool want_f, want_i;
-            num_wants(c0, nc, e.ord, want_f, want_i);
-            // One parse for both kinds, so lanes at f64 and i64 columns run
-            // the same instructions; nf serves the f64 columns, ni the i64.
-            NumVal nf{}, ni{};
-            if (want_f || want_i) nf = num_value(e.fs, to, want_f);
-            ni = nf;
-            if (want_f && want_i) num_i64_too(e.fs, to, ni);
-            for (uint32_t j = 0; j < nc; ++j) {
-                const ColDesc& c = P.cols[c0 + j];
-                if (c.index != e.ord) continue;
-                CellStat o{false, false, 0, 0, 0, 0.0};
-                if (c.type == FALX_COL_F64) {
-                    const F64Result& r = nf.f;
-                    o = CellStat{r.ok, r.fixed6, f64_to_bits(r.value), 0, r.mantissa, r.value};
-                } else if (c.type == FALX_COL_I64) {
-                    const int64_t v = ni.iv;
-                    o.ok = ni.iok;
-                    o.csum = (unsigned long long)v;
-                    o.d = (double)v;
-                } else if (c.type == FALX_COL_STR) {
-                    unsigned long long cs = 0;
-                    o.bytes = clean_cell(src, e.fs, to, cd, [&](uint8_t b) { cs += b; });
-                    o.ok = true;
-                    o.csum = cs;
-                } else if (to > e.fs) {
-                    o.ok = true;
-                    o.bytes = to - e.fs;
-                    for (uint32_t p = e.fs; p < to; ++p) o.csum += src.at(p);
-                }
-                acc_add(acc, j, o);
+        if (IS_COLS && tid < FALX_MAX_COLUMNS) sh.str_len[tid] = 0;   // read after the segment scans' barriers
+
+        // Validity of one column for the wave's rows: lanes with a row (RL)
+        // hold consecutive rows in lane order, so bit (row - first row) of
+        // the wave's word is the lane's cell.
+        auto wave_valid = [&](const ColDesc& col, bool vb, uint32_t row, uint64_t RL) __attribute__((always_inline)) {
+            if (!RL) return;
+            const uint32_t f = (uint32_t)__ffsll((long long)RL) - 1u;
+            const uint32_t rfirst = lane_value(row, f);
+            const uint64_t run = RL >> f;
+            uint64_t W;
+            if ((run & (run + 1ULL)) == 0ULL) {
+                W = ballot(vb) >> f;
+            } else {
+                unsigned long long v = vb ? (1ULL << ((row - rfirst) & 63u)) : 0ULL;
+                v |= dpp_get<0xB1>(v);
+                v |= dpp_get<0x4E>(v);
+                v |= dpp_get<0x124>(v);
+                v |= dpp_get<0x128>(v);
+                const uint64_t r0 = ((uint64_t)lane_value((uint32_t)(v >> 32), 0) << 32) | lane_value((uint32_t)v, 0);
+                const uint64_t r1 = ((uint64_t)lane_value((uint32_t)(v >> 32), 16) << 32) | lane_value((uint32_t)v, 16);
+                const uint64_t r2 = ((uint64_t)lane_value((uint32_t)(v >> 32), 32) << 32) | lane_value((uint32_t)v, 32);
+                const uint64_t r3 = ((uint64_t)lane_value((uint32_t)(v >> 32), 48) << 32) | lane_value((uint32_t)v, 48);
+                W = r0 | r1 | r2 | r3;
             }
-        }, trailer_word);
-        acc_publish(acc, c0, nc, records);
+            if (W && ln == 0) {
+                const uint32_t wi = rfirst >> 6, sb = rfirst & 63u;
+                atomicOr(&col.valid[wi], (unsigned long long)(W << sb));
+                if (sb && (W >> (64u - sb))) atomicOr(&col.valid[wi + 1], (unsigned long long)(W >> (64u - sb)));
+            }
+        };
+
+        uint32_t s0 = 0, row_base = pre.n_rows;
+        while (true) {
+            // List the segment's terminators: entries [s0, n_ent) in runs of
+            // `per` per thread, counted, scanned, then written by rank.
+            const uint32_t rem = n_ent - s0;
+            const uint32_t per = (rem + WG - 1) / WG;
+            const uint32_t k0 = min(n_ent, s0 + tid * per), k1 = min(n_ent, k0 + per);
+            Scan3 cnt{0, 0, -1};
+            for (uint32_t k = k0; k < k1; ++k) {
+                const uint32_t wd = word_at(k);
+                if (wd & 0x4000u) {
+                    cnt.a += 1;
+                    cnt.b += (wd & 0x8000u) ? 0u : 1u;
+                }
+            }
+            Scan3 tot;
+            const Scan3 ex = block_excl(cnt, sh.scan_sh, tot);
+            {
+                uint32_t ra = ex.a, rb = ex.b;
+                for (uint32_t k = k0; k < k1; ++k) {
+                    const uint32_t wd = word_at(k);
+                    if (wd & 0x4000u) {
+                        if (ra < TL_MAX) tl[ra] = k | (rb << 16);
+                        ra += 1;
+                        rb += (wd & 0x8000u) ? 0u : 1u;
+                    }
+                }
+            }
+            __syncthreads();
+            const bool last_seg = tot.a <= TL_MAX;
+            const uint32_t n_seg = uni(min(tot.a, TL_MAX));
+            const uint32_t t_last = n_seg ? uni(tl[n_seg - 1]) : 0u;
+            const uint32_t last_k = t_last & 0xFFFFu;
+            // An unfinished record after the tile's last terminator (never
+            // with a trailer: the virtual terminator is the last entry).
+            const uint32_t after = n_seg ? last_k + 1u : s0;
+            const bool partial = last_seg && after < n_ent;
+            const uint32_t n_rec = n_seg + (partial ? 1u : 0u);
+            const uint32_t seg_rows = n_seg ? (t_last >> 16) + ((word_at(last_k) & 0x8000u) ? 0u : 1u) : 0u;
+            for (uint32_t r0 = 0; r0 < n_rec; r0 += WG) {
+                const uint32_t r = r0 + tid;
+                const bool has = r < n_rec;
+                bool term = false, comment = false;
+                uint32_t e0 = 0, n = 0, fs0 = 0, rs = 0, ord0 = 0, row = 0;
+                if (has) {
+                    e0 = r == 0 ? s0 : (tl[r - 1] & 0xFFFFu) + 1u;
+                    if (r < n_seg) {
+                        const uint32_t t = tl[r];
+                        const uint32_t e1 = t & 0xFFFFu;
+                        term = true;
+                        comment = (word_at(e1) & 0x8000u) != 0u;
+                        n = e1 - e0 + 1u;
+                        row = row_base + (t >> 16);
+                    } else {
+                        n = n_ent - e0;
+                        comment = (word_at(e0) & 0x8000u) != 0u;
+                        row = row_base + seg_rows;
+                    }
+                    if (e0 == 0) {
+                        fs0 = pre.last_struct;
+                        rs = pre.last_term;
+                        ord0 = pre.ord;
+                    } else {
+                        fs0 = pos_of(word_at(e0 - 1)) + 1u;
+                        rs = fs0;
+                    }
+                }
+                const bool live_rec = has && !comment;
+                if (live_rec && term) records += 1;
+                const bool live = row < rows_cap;
+                const uint64_t RL = IS_COLS ? ballot(live_rec) : 0ULL;
+                for (uint32_t jj = 0; jj < nc; ++jj) {
+                    const uint32_t j = c0 + jj;
+                    const ColDesc& c = P.cols[j];
+                    const int32_t i = (int32_t)c.index - (int32_t)ord0;
+                    const bool here = live_rec && i >= 0 && (uint32_t)i < n;
+                    const bool missing = live_rec && term && i >= (int32_t)n;
+                    uint32_t fs = 0, fe = 0;
+                    if (here) {
+                        const uint32_t k = e0 + (uint32_t)i;
+                        fe = pos_of(word_at(k));
+                        fs = i == 0 ? fs0 : pos_of(word_at(k - 1)) + 1u;
+                        // The record's last field loses a trailing '\r' (csv_geo.rs:1056-1060).
+                        if (term && (uint32_t)i + 1u == n && fe > rs && src.at(fe - 1) == '\r') fe -= 1;
+                    }
+                    // Statistics of COLUMNS cover the rows that fit rows_cap.
+                    const bool stat_ok = with_stats && jj < NACC && (IS_COLS ? live : true);
+                    bool vb = false;
+                    if (c.type == FALX_COL_F64 || c.type == FALX_COL_I64) {
+                        const bool as_f64 = c.type == FALX_COL_F64;
+                        NumVal nv{};
+                        if (here) nv = num_value(fs, fe, as_f64);
+                        CellStat o{false, false, 0, 0, 0, 0.0};
+                        if (as_f64) {
+                            vb = here && nv.f.ok;
+                            if (IS_COLS && (here || missing) && live)
+                                reinterpret_cast<double*>(c.values)[row] = vb ? nv.f.value : 0.0;
+                            if (vb) o = CellStat{true, nv.f.fixed6, f64_to_bits(nv.f.value), 0, nv.f.mantissa, nv.f.value};
+                        } else {
+                            vb = here && nv.iok;
+                            if (IS_COLS && (here || missing) && live)
+                                reinterpret_cast<long long*>(c.values)[row] = vb ? nv.iv : 0;
+                            if (vb) o = CellStat{true, false, (unsigned long long)nv.iv, 0, 0, (double)nv.iv};
+                        }
+                        if (stat_ok) acc_add(acc, jj, o);
+                    } else if (c.type == FALX_COL_STR) {
+                        if (IS_COLS) {
+                            // Cleaned byte counts only; the STRINGS pass places
+                            // the bytes and writes the validity.
+                            uint32_t len = here ? clean_len(fs, fe) : 0u;
+                            len = row_sum16(len);
+                            const uint32_t wsum = lane_value(len, 0) + lane_value(len, 16) + lane_value(len, 32) +
+                                                  lane_value(len, 48);
+                            if (ln == 0 && wsum) atomicAdd(&sh.str_len[c.str_slot], wsum);
+                        } else if (here) {
+                            unsigned long long cs = 0;
+                            const uint32_t nb = clean_cell(src, fs, fe, cd, [&](uint8_t b) { cs += b; });
+                            if (stat_ok) acc_add(acc, jj, CellStat{true, false, cs, nb, 0, 0.0});
+                        }
+                    } else {
+                        vb = here && fe != fs;
+                        if (IS_COLS && (here || missing) && live)
+                            reinterpret_cast<uint2*>(c.values)[row] = vb ? make_uint2(fs, fe) : make_uint2(0, 0);
+                        if (vb && stat_ok) {
+                            unsigned long long cs = 0;
+                            for (uint32_t q = fs; q < fe; ++q) cs += src.at(q);
+                            acc_add(acc, jj, CellStat{true, false, cs, fe - fs, 0, 0.0});
+                        }
+                    }
+                    if (IS_COLS && c.type != FALX_COL_STR) wave_valid(c, vb && live, row, RL);
+                }
+            }
+            row_base += seg_rows;
+            if (last_seg) break;
+            s0 = last_k + 1u;
+            __syncthreads();   // every thread has read tl before the next segment rewrites it
+        }
+        if (IS_COLS && tid == 0 && last_tile) P.result[R_N_ROWS] = inc.n_rows + (trailer_row_live ? 1 : 0);
+        __syncthreads();
         if (FAST && tid == 0) P.fixup[tile] = sh.fix_bad;
+        if (with_stats) acc_publish(acc, c0, min(NACC, nc), records);
+        if (IS_COLS && P.n_str > 0 && tid < P.n_str) P.str_len[(size_t)tile * P.n_str + tid] = sh.str_len[tid];
         return;
     }
 
@@ -1452,101 +1588,6 @@ __device__ __forceinline__ void scan_tile(const Params& P, const uint32_t tile, 
             }
         }
     };
-
-    // ---- M_COLUMNS --------------------------------------------------------
-    if (MODE == M_COLUMNS) {
-        const bool trailer_row_live = trailer && !trailer_comment;
-        vstage_clear();
-        // Optional fused statistics of the first COL_STAT_GROUP columns
-        // (further columns get STATS passes), accumulated in registers as
-        // the walk writes the cells, then reduced in a fixed order. String
-        // columns' statistics come from the STRINGS pass. Cells past
-        // rows_cap are not written, so on a CAPACITY result the statistics
-        // cover the rows that fit.
-        unsigned long long records = 0;
-        // Known at compile time in the fast forms, which then carry no
-        // statistics registers when there are none.
-        const bool with_stats = KMODE == M_COLUMNS_FAST         ? false
-                                : KMODE == M_COLUMNS_FAST_STATS ? true
-                                                                : P.partials != nullptr;
-        const bool have_str = P.n_str > 0;
-        Acc acc[COL_STAT_GROUP];
-#pragma unroll
-        for (int j = 0; j < COL_STAT_GROUP; ++j) acc[j] = Acc{0, 0, 0, 0, 0, 0.0};
-        auto stat_add = [&](uint32_t j, const CellStat& o) __attribute__((always_inline)) {
-            if (with_stats && j < (uint32_t)COL_STAT_GROUP) acc_add(acc, j, o);
-        };
-        // E1: typed cells and cleaned string lengths. A record that ends
-        // before a column's field leaves that cell null.
-        if (tid < FALX_MAX_COLUMNS) sh.str_len[tid] = 0;
-        __syncthreads();
-        // A typed cell of column j: value, validity, statistics mark.
-        auto num_cell = [&](uint32_t j, const ColDesc c, bool found, const NumVal& nf, const NumVal& ni,
-                            uint32_t row) __attribute__((always_inline)) {
-            if (row >= rows_cap) return;
-            if (c.type == FALX_COL_F64) {
-                const bool ok = found && nf.f.ok;
-                reinterpret_cast<double*>(c.values)[row] = ok ? nf.f.value : 0.0;
-                if (ok) {
-                    set_valid(j, c, row);
-                    stat_add(j, CellStat{true, nf.f.fixed6, f64_to_bits(nf.f.value), 0, nf.f.mantissa, nf.f.value});
-                }
-            } else {
-                const bool ok = found && ni.iok;
-                reinterpret_cast<long long*>(c.values)[row] = ok ? ni.iv : 0;
-                if (ok) {
-                    set_valid(j, c, row);
-                    stat_add(j, CellStat{true, false, (unsigned long long)ni.iv, 0, 0, (double)ni.iv});
-                }
-            }
-        };
-        walk([&](const Entry& e, uint32_t, uint32_t) {
-            if (!e.valid || e.flag) return;
-            const uint32_t to = field_end(e);
-            if (e.is_term) records += 1;
-            const uint32_t row = e.row;
-            const bool live = row < rows_cap;
-            bool want_f, want_i;
-            num_wants(0, n_cols, e.ord, want_f, want_i);
-            // One parse for both kinds, so lanes at f64 and i64 columns run
-            // the same instructions; nf serves the f64 columns, ni the i64.
-            NumVal nf{}, ni{};
-            if (want_f || want_i) nf = num_value(e.fs, to, want_f);
-            ni = nf;
-            if (want_f && want_i) num_i64_too(e.fs, to, ni);
-            for (uint32_t j = 0; j < n_cols; ++j) {
-                const ColDesc& c = P.cols[j];
-                const bool found = c.index == e.ord;
-                if (!found && !(e.is_term && c.index > e.ord)) continue;
-                if (c.type == FALX_COL_STR) {
-                    if (found) atomicAdd(&sh.str_len[c.str_slot], clean_len(e.fs, to));
-                } else if (is_num(c.type)) {
-                    num_cell(j, c, found, nf, ni, row);
-                } else if (live) {
-                    const bool ok = found && to != e.fs;
-                    reinterpret_cast<uint2*>(c.values)[row] = ok ? make_uint2(e.fs, to) : make_uint2(0, 0);
-                    if (ok) {
-                        set_valid(j, c, row);
-                        if (with_stats && j < (uint32_t)COL_STAT_GROUP) {
-                            unsigned long long cs = 0;
-                            for (uint32_t q = e.fs; q < to; ++q) cs += src.at(q);
-                            stat_add(j, CellStat{true, false, cs, to - e.fs, 0, 0.0});
-                        }
-                    }
-                }
-            }
-        }, trailer_word);
-        if (tid == 0 && last_tile) P.result[R_N_ROWS] = inc.n_rows + (trailer_row_live ? 1 : 0);
-        __syncthreads();
-        if (FAST && tid == 0) P.fixup[tile] = sh.fix_bad;
-        if (with_stats) acc_publish(acc, 0, min((uint32_t)COL_STAT_GROUP, n_cols), records);
-        flush_valid(false);
-        if (!have_str) return;
-
-        // String columns: the tile's cleaned byte counts go out for the
-        // arena scan; the bytes themselves are placed by the STRINGS pass.
-        if (tid < P.n_str) P.str_len[(size_t)tile * P.n_str + tid] = sh.str_len[tid];
-    }
 
     // ---- M_STRINGS: arena placement of the string columns ------------------
     if (MODE == M_STRINGS) {
