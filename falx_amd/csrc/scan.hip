@@ -641,7 +641,6 @@ struct Shared {
     uint32_t wentry[NWAVE];
     uint32_t wcount[NWAVE];
     uint32_t wbase[NWAVE + 1];
-    Counts wcnt[NWAVE];
     Counts pre;       // tile exclusive prefix
     Counts inc;       // tile inclusive
     uint32_t tile;
@@ -921,12 +920,16 @@ __device__ __forceinline__ void scan_tile(const Params& P, const uint32_t tile, 
     if (tid == 0) {
         const uint32_t* tp = P.tpre + (size_t)tile * 8;
         const uint32_t* ts = P.tsum + (size_t)tile * P.ts_stride;
-        uint32_t s = tp[0];
+        const uint32_t s0 = tp[0];
+        uint32_t s = s0;
         for (int k = 0; k < NWAVE; ++k) {
             sh.wentry[k] = s;
             s = fn_apply(ts[1 + k], s);
         }
+        // The tile's inclusive counters: the prefix, then the tile's own
+        // counters from its summary for its entry state.
         sh.pre = c7_load(tp + 1);
+        sh.inc = compose(sh.pre, c7_load(ts + TS_COUNTS + C7 * s0));
     }
     __syncthreads();
 
@@ -942,8 +945,7 @@ __device__ __forceinline__ void scan_tile(const Params& P, const uint32_t tile, 
         }
         const uint32_t n = popc(S);
         const uint32_t incl = wave_incl_sum(n);
-        const Counts wc = wave_counts(S, T, F, tile_start + off);
-        if (ln == 0) { sh.wcnt[w] = wc; sh.wcount[w] = wc.n_struct; }
+        if (ln == 63) sh.wcount[w] = incl;
         __syncthreads();
         // One list for the tile, in byte order: this lane's structurals go
         // after the earlier waves' and the earlier lanes' entries.
@@ -958,12 +960,9 @@ __device__ __forceinline__ void scan_tile(const Params& P, const uint32_t tile, 
     }
     __syncthreads();
 
-    // ---- tile counters: exclusive prefix (from the scan) and inclusive ------
+    // ---- tile counters: list offsets of the waves; totals of the last tile -
     if (tid == 0) {
-        Counts agg = sh.wcnt[0];
-        for (int k = 1; k < NWAVE; ++k) agg = compose(agg, sh.wcnt[k]);
-        const Counts inc = compose(sh.pre, agg);
-        sh.inc = inc;
+        const Counts inc = sh.inc;
         sh.wbase[0] = 0;
         for (int k = 0; k < NWAVE; ++k) sh.wbase[k + 1] = sh.wbase[k] + sh.wcount[k];
         if (last_tile) {
@@ -1243,14 +1242,23 @@ __device__ __forceinline__ void scan_tile(const Params& P, const uint32_t tile, 
 #pragma unroll
         for (uint32_t j = 0; j < N; ++j) {
             if (j >= nc) break;
-            const uint32_t v0 = row_sum16(acc[j].values), v2 = row_sum16(acc[j].bytes),
-                           v4 = row_sum16(acc[j].fx_cells);
-            const unsigned long long v1 = row_sum16(acc[j].csum),
-                                     v3 = row_sum16((unsigned long long)acc[j].fx_sum);
-            const double v5 = row_sum16(acc[j].dsum);
+            // Only the fields the column's type can make non-zero are
+            // reduced (the others are zero in every lane): counts of values
+            // and of fixed-6 cells travel as the 16-bit halves of one word
+            // (a thread holds at most a few dozen cells of a tile).
+            const uint32_t type = P.cols[first + j].type;
+            const bool is_f64 = type == FALX_COL_F64, is_num = is_f64 || type == FALX_COL_I64;
+            const uint32_t vc = row_sum16(acc[j].values | (acc[j].fx_cells << 16));
+            const unsigned long long v1 = row_sum16(acc[j].csum);
+            uint32_t v2 = 0;
+            unsigned long long v3 = 0;
+            double v5 = 0.0;
+            if (!is_num) v2 = row_sum16(acc[j].bytes);
+            if (is_f64) v3 = row_sum16((unsigned long long)acc[j].fx_sum);
+            if (is_num) v5 = row_sum16(acc[j].dsum);
             if (row_head) {
                 unsigned long long* r = red + 6 * j;
-                r[0] = v0; r[1] = v1; r[2] = v2; r[3] = v3; r[4] = v4;
+                r[0] = vc & 0xFFFFu; r[1] = v1; r[2] = v2; r[3] = v3; r[4] = vc >> 16;
                 r[5] = (unsigned long long)__double_as_longlong(v5);
             }
         }
