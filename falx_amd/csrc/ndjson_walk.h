@@ -29,16 +29,56 @@ struct JCur {
     }
 };
 
+// Bytes of x equal to zero, as the top bit of each byte (exact: no borrow
+// runs from one byte into the next).
+FX_HD uint64_t zero_bytes64(uint64_t x) {
+    return ~(((x & 0x7F7F7F7F7F7F7F7FULL) + 0x7F7F7F7F7F7F7F7FULL) | x | 0x7F7F7F7F7F7F7F7FULL);
+}
+
+// A source whose bytes can also be read as aligned 8-byte words (word8(p)
+// for p a multiple of 8, bytes past the record readable) lets strings be
+// skipped a word at a time.
+template <class Src, class = void>
+struct has_word8 { static constexpr bool value = false; };
+template <class Src>
+struct has_word8<Src, decltype((void)Src::kWord8)> { static constexpr bool value = Src::kWord8; };
+
 template <class Src>
 FX_HD bool j_skip_string(JCur<Src>& c) {  // at the opening quote
     ++c.i;
-    while (c.i < c.n) {
-        uint8_t b = c.s->at(c.i);
-        if (b == '\\') { c.i += 2; continue; }
-        ++c.i;
-        if (b == '"') return true;
+    if constexpr (has_word8<Src>::value) {
+        // The byte loop below, eight bytes per step: plain bytes are passed
+        // over, a backslash skips the byte after it, a quote ends the string.
+        uint32_t i = c.i;
+        while (i < c.n) {
+            const uint32_t a = i & ~7u;
+            const uint64_t x = c.s->word8(a);
+            const uint64_t zq = zero_bytes64(x ^ 0x2222222222222222ULL);
+            const uint64_t hit = (zq | zero_bytes64(x ^ 0x5C5C5C5C5C5C5C5CULL)) & (~0ULL << (8u * (i - a)));
+            if (!hit) {
+                i = a + 8u;
+                continue;
+            }
+            const uint32_t sb = (uint32_t)__builtin_ctzll(hit);
+            const uint32_t p = a + (sb >> 3);
+            if (p >= c.n) break;
+            if ((zq >> sb) & 1ULL) {
+                c.i = p + 1u;
+                return true;
+            }
+            i = p + 2u;
+        }
+        c.i = i;
+        return false;
+    } else {
+        while (c.i < c.n) {
+            uint8_t b = c.s->at(c.i);
+            if (b == '\\') { c.i += 2; continue; }
+            ++c.i;
+            if (b == '"') return true;
+        }
+        return false;
     }
-    return false;
 }
 
 template <class Src>
@@ -132,13 +172,31 @@ FX_HD bool j_number(JCur<Src>& c, double& v, bool& fx6, int64_t& mant, bool& def
         if (c.i == es) return false;
     }
     if (!c.delim()) return false;
+    from = s0;
+    to = c.i;
+    fx6 = false;
+    if (!has_exp && int_digits + frac_digits <= 15) {
+        // At most 15 digits and no exponent: the digits as an integer m
+        // (< 2^53) over 10^frac_digits (<= 10^15, exact) is one correctly
+        // rounded IEEE division (Clinger 1990), the value the general
+        // parser gives.
+        int64_t m = 0;
+        for (uint32_t k = ip; k < ip + int_digits; ++k) m = m * 10 + (c.s->at(k) - '0');
+        for (uint32_t k = 0; k < frac_digits; ++k) m = m * 10 + (c.s->at(fs + k) - '0');
+        const double mag = (double)m / pow10_exact((int)frac_digits);
+        v = neg ? -mag : mag;
+        deferred = false;
+        if (frac_digits <= 6 && int_digits <= 12) {
+            for (uint32_t k = frac_digits; k < 6; ++k) m *= 10;
+            mant = neg ? -m : m;
+            fx6 = true;
+        }
+        return true;
+    }
     F64Result r = parse_f64_stream(RawStream<Src>{c.s, s0, c.i});
     if (!r.ok) return false;
     v = r.value;
     deferred = r.deferred;
-    from = s0;
-    to = c.i;
-    fx6 = false;
     if (!has_exp && frac_digits <= 6 && int_digits <= 12) {
         int64_t m = 0;
         for (uint32_t k = ip; k < ip + int_digits; ++k) m = m * 10 + (c.s->at(k) - '0');

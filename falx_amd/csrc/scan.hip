@@ -2068,123 +2068,209 @@ struct NdPart {
     unsigned long long pad[2];
 };
 
-// NDJSON id+score: records are split at raw '\n' (ndjson.rs:679-723); a
-// record belongs to the tile holding its '\n' (the data's trailing record to
-// the last tile). Each thread walks whole records; no cross-tile state.
-// NDJSON tiles run on workgroups of two waves: a tile holds some 60-100
-// records, one per lane, so two waves hold them all, and smaller workgroups
-// let more tiles share a CU to hide the walker's latency.
-constexpr int ND_WAVES = 2;
-constexpr int ND_WG = 64 * ND_WAVES;
-constexpr int ND_BPW = TILE / 64 / ND_WAVES;   // 64-byte blocks per wave
+// NDJSON id+score (ndjson.rs:679-723): records are the bytes between raw
+// '\n's. A tile owns the records that start in it and holds in LDS its 8 KiB
+// plus ND_HALO bytes after them, so a record is read from LDS alone unless it
+// runs past the halo (then one lane walks it with HBM reads). One wave per
+// tile: the newline positions are listed in order 1 KiB per step (16 bytes
+// per lane, SWAR byte tests, ranks by a wave scan), and lanes take records
+// from a per-wave queue as they finish their previous one, so no lane waits
+// for a long neighbour while records remain.
+constexpr int ND_WG = 64;
+constexpr uint32_t ND_HALO = 512;
+constexpr uint32_t ND_WIN = TILE + ND_HALO;
+constexpr uint32_t ND_MAXNL = 512;      // newlines listed per round
+constexpr uint32_t ND_NONE = 0xFFFFFFFFu;
+
+// The tile's LDS copy as a walker source (ndjson_walk.h): bytes, and
+// aligned 8-byte words for skipping strings (tile starts are multiples of 8).
+struct NdLds {
+    static constexpr bool kWord8 = true;
+    const uint8_t* b;   // document bytes [lo, lo + ND_WIN), zero padded
+    uint32_t lo;
+    DEV uint8_t at(uint32_t p) const { return b[p - lo]; }
+    DEV uint64_t word8(uint32_t p) const { return *reinterpret_cast<const uint64_t*>(b + (p - lo)); }
+};
+
+struct NdAcc {
+    unsigned long long recs, inval, fxc, ids;
+    long long fxs;
+    double ss;
+};
+
+template <class Src>
+DEV void nd_record(const Src& src, uint32_t from, uint32_t to, NdAcc& a, SlowF64& slow) {
+    if (blank_span(src, from, to)) return;
+    NdRecord r;
+    if (ndjson_record(src, from, to, r)) {
+        if (r.score_deferred) resolve_score(slow, src, r);
+        a.recs += 1;
+        a.ids += (unsigned long long)r.id;
+        a.ss += r.score;
+        if (r.fx6) {
+            a.fxs += r.mant;
+            a.fxc += 1;
+        }
+    } else {
+        a.inval += 1;
+    }
+}
+
+// Newline bytes of four little-endian words as a 16-bit mask.
+DEV uint32_t nl_mask16(uint4 v) {
+    uint32_t m = 0;
+    const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const uint32_t x = w[k] ^ 0x0A0A0A0Au;
+        const uint32_t z = ~(((x & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | x | 0x7F7F7F7Fu);
+        m |= (((z >> 7) & 1u) | ((z >> 14) & 2u) | ((z >> 21) & 4u) | ((z >> 28) & 8u)) << (4 * k);
+    }
+    return m;
+}
 
 DEV void ndjson_tile(const uint8_t* data, uint32_t len, uint32_t n_tiles, NdPart* parts, uint32_t tile) {
-    __shared__ uint8_t bytes[TILE];
-    __shared__ uint16_t nls[TILE];
-    __shared__ uint32_t wcnt[ND_WAVES], wb[ND_WAVES + 1];
-    __shared__ int64_t prev_nl;
+    __shared__ __align__(16) uint8_t bytes[ND_WIN + 16];
+    __shared__ uint16_t nls[ND_MAXNL];   // newline positions - tile_start, in order
+    __shared__ uint32_t qnext;
     __shared__ SlowF64 slow;
-    __shared__ unsigned long long ush[ND_WAVES];
-    __shared__ long long lsh[ND_WAVES];
-    __shared__ double dsh[ND_WAVES];
-    const uint32_t tid = threadIdx.x, ln = tid & 63u, w = uni(tid >> 6);
+    __shared__ unsigned long long ush[1];
+    __shared__ long long lsh[1];
+    __shared__ double dsh[1];
+    (void)n_tiles;
+    const uint32_t ln = threadIdx.x;
     const uint32_t tile_start = tile * (uint32_t)TILE;
-    const uint32_t tile_len = min((uint32_t)TILE, len - tile_start);
-    for (uint32_t i = tid * 16; i < (uint32_t)TILE; i += ND_WG * 16) {
+    const uint32_t tile_end = min(tile_start + (uint32_t)TILE, len);
+    const uint32_t win_hi = min(tile_start + ND_WIN, len);
+    const uint32_t win_len = win_hi - tile_start;
+    for (uint32_t i = ln * 16; i < ND_WIN + 16; i += ND_WG * 16) {
         uint4 v = make_uint4(0, 0, 0, 0);
-        if (i + 16 <= tile_len) {
+        if (i + 16 <= win_len) {
             v = *reinterpret_cast<const uint4*>(data + tile_start + i);
-        } else if (i < tile_len) {
+        } else if (i < win_len) {
             uint8_t tmp[16];
-            for (int k = 0; k < 16; ++k) tmp[k] = (i + k < tile_len) ? data[tile_start + i + k] : 0;
+            for (int k = 0; k < 16; ++k) tmp[k] = (i + k < win_len) ? data[tile_start + i + k] : 0;
             memcpy(&v, tmp, 16);
         }
         *reinterpret_cast<uint4*>(&bytes[i]) = v;
     }
+    if (ln == 0) slow.lock = 0;
     __syncthreads();
-    uint32_t cnt = 0;
-    const uint32_t seg = w * ND_BPW * 64;
-    for (int b = 0; b < ND_BPW; ++b) {
-        uint32_t off = (w * ND_BPW + b) * 64;
-        bool isnl = off + ln < tile_len && bytes[off + ln] == '\n';
-        uint64_t m = ballot(isnl);
-        if (isnl) nls[seg + cnt + mbcnt(m)] = (uint16_t)(off + ln);
-        cnt += popc(m);
-    }
-    if (ln == 0) wcnt[w] = cnt;
-    __syncthreads();
-    if (tid == 0) {
-        wb[0] = 0;
-        for (int k = 0; k < ND_WAVES; ++k) wb[k + 1] = wb[k] + wcnt[k];
-    }
-    __syncthreads();
-    const bool tail = tile + 1 == n_tiles && len > 0 && data[len - 1] != '\n';
-    // The newline before the tile (-1: none), needed only by a tile that owns
-    // a record (it holds a '\n', or it is the last tile and the data does not
-    // end in one): so a stretch of the input without newlines is searched
-    // once, by the tile that ends it, not by every tile inside it. Wave 0
-    // reads 512 bytes per step backwards, 8 per lane (tile starts are
-    // multiples of 512, so every word read is aligned and below the tile).
-    if (w == 0) {
-        int64_t found = -1;
-        if (wb[ND_WAVES] > 0 || tail) {
-            for (int64_t hi = (int64_t)tile_start; hi > 0; hi -= 512) {
-                const int64_t p = hi - 512 + 8 * (int64_t)ln;
-                uint64_t x = 0;
-                if (p >= 0) x = *reinterpret_cast<const uint64_t*>(data + p) ^ 0x0A0A0A0A0A0A0A0AULL;
-                // Bytes of x that are zero (exact, no borrow across bytes).
-                const uint64_t z = ~(((x & 0x7F7F7F7F7F7F7F7FULL) + 0x7F7F7F7F7F7F7F7FULL) | x | 0x7F7F7F7F7F7F7F7FULL);
-                const uint64_t m = ballot(p >= 0 && z != 0ULL);
-                if (m) {
-                    const uint32_t L = hibit(m);
-                    const uint32_t byte = hibit(z) >> 3;
-                    found = hi - 512 + 8 * (int64_t)L + (int64_t)lane_value(byte, L);
-                    break;
-                }
+    const NdLds src{bytes, tile_start};
+    NdAcc acc{0, 0, 0, 0, 0, 0.0};
+    // Start of the record being listed: the tile's first byte when a record
+    // starts there, else it is the byte after the tile's first newline.
+    uint32_t cur = (tile_start == 0 || data[tile_start - 1] == '\n') ? tile_start : ND_NONE;
+    uint32_t scan = tile_start;     // first byte not yet searched for newlines
+    bool done = tile_start >= len;
+    while (!done) {
+        // List newlines from `scan` on, stopping after the first one at or
+        // past tile_end - 1 (the record after it is the next tile's), or
+        // when the list is full.
+        uint32_t n = 0;
+        bool stop = false, full = false;
+        while (!stop && !full && scan < win_hi) {
+            const uint32_t base = scan & ~15u;
+            const uint32_t off = base - tile_start + 16 * ln;   // LDS offset of this lane's 16 bytes
+            const uint32_t p0 = base + 16 * ln;
+            uint32_t m = 0;
+            if (off + 16 <= ND_WIN + 16) m = nl_mask16(*reinterpret_cast<const uint4*>(&bytes[off]));
+            // Positions >= scan and < win_hi only.
+            if (p0 < scan) m &= scan - p0 >= 16 ? 0u : (0xFFFFu << (scan - p0)) & 0xFFFFu;
+            if (p0 + 16 > win_hi) m &= win_hi <= p0 ? 0u : (1u << (win_hi - p0)) - 1u;
+            // The stopping newline: the first one at or past tile_end - 1.
+            const uint32_t lim = tile_end - 1;
+            uint32_t cand = m;
+            if (p0 + 16 <= lim) cand = 0;
+            else if (p0 < lim) cand &= (0xFFFFu << (lim - p0)) & 0xFFFFu;
+            const uint64_t cl = ballot(cand != 0);
+            uint32_t stop_pos = ND_NONE;
+            if (cl) {
+                const uint32_t L = (uint32_t)__ffsll((long long)cl) - 1u;
+                stop_pos = lane_value(p0 + (uint32_t)__ffs((int)cand) - 1u, L);
+                if (p0 > stop_pos) m = 0;
+                else if (p0 + 16 > stop_pos) m &= (2u << (stop_pos - p0)) - 1u;
+                stop = true;
+            }
+            const uint32_t cnt = popc((uint64_t)m);
+            const uint32_t incl = wave_incl_sum(cnt);
+            const uint32_t total = lane_value(incl, 63);
+            uint32_t rank = n + incl - cnt;
+            for (uint32_t mm = m; mm; mm &= mm - 1) {
+                if (rank < ND_MAXNL) nls[rank] = (uint16_t)(p0 + (uint32_t)__ffs((int)mm) - 1u - tile_start);
+                ++rank;
+            }
+            if (n + total >= ND_MAXNL && (n + total > ND_MAXNL || !stop)) {
+                // Full: the next round resumes after the last listed newline.
+                n = ND_MAXNL;
+                full = true;
+                stop = false;
+            } else {
+                n += total;
+                scan = base + 16 * ND_WG;
             }
         }
-        if (ln == 0) {
-            slow.lock = 0;
-            prev_nl = found;
+        __syncthreads();
+        if (full) scan = tile_start + nls[ND_MAXNL - 1] + 1u;
+        // Record k ends at newline k; its start is `cur` for k = 0, else the
+        // byte after newline k - 1. Lanes take records from the queue.
+        if (ln == 0) qnext = ND_WG;
+        __syncthreads();
+        for (uint32_t k = ln; k < n;) {
+            const uint32_t from = k == 0 ? cur : tile_start + nls[k - 1] + 1u;
+            if (from != ND_NONE) nd_record(src, from, tile_start + nls[k], acc, slow);
+            k = atomicAdd(&qnext, 1u);
+        }
+        __syncthreads();
+        if (n) cur = tile_start + nls[n - 1] + 1u;
+        if (stop) break;
+        if (!full && scan >= win_hi) {
+            // No newline ends the record at `cur` inside the window.
+            if (cur != ND_NONE && cur < tile_end) {
+                if (win_hi == len) {
+                    // The data's last record, without a final newline.
+                    if (ln == 0 && cur < len) nd_record(src, cur, len, acc, slow);
+                } else {
+                    // A record running past the halo: its end is searched in
+                    // HBM 512 bytes per step, and one lane walks it there.
+                    uint32_t end = len;
+                    for (uint32_t lo = win_hi & ~7u; lo < len; lo += 512) {
+                        const uint32_t p = lo + 8 * ln;
+                        uint64_t z = 0;
+                        if (p + 8 <= len) {
+                            z = zero_bytes64(*reinterpret_cast<const uint64_t*>(data + p) ^ 0x0A0A0A0A0A0A0A0AULL);
+                        } else if (p < len) {
+                            for (uint32_t q = p; q < len; ++q)
+                                if (data[q] == '\n') z |= 0x80ULL << (8 * (q - p));
+                        }
+                        // Bytes below win_hi were searched in the window.
+                        if (p < win_hi) z = win_hi - p >= 8 ? 0ULL : z & (~0ULL << (8 * (win_hi - p)));
+                        const uint64_t hit = ballot(z != 0ULL);
+                        if (hit) {
+                            const uint32_t L = (uint32_t)__ffsll((long long)hit) - 1u;
+                            end = lane_value(p + ((uint32_t)__builtin_ctzll(z) >> 3), L);
+                            break;
+                        }
+                    }
+                    if (ln == 0) {
+                        const TileSrc gsrc{data, bytes, tile_start, win_hi};
+                        nd_record(gsrc, cur, end, acc, slow);
+                    }
+                }
+            }
+            break;
         }
     }
-    __syncthreads();
-    const uint32_t n = uni(wb[ND_WAVES]);
-    TileSrc src{data, bytes, tile_start, tile_start + tile_len};
-    auto nl_at = [&](uint32_t k) -> uint32_t {
-        uint32_t ww = 0;
-        for (int i = 1; i < ND_WAVES; ++i) ww += k >= wb[i];
-        return tile_start + nls[ww * ND_BPW * 64 + (k - wb[ww])];
-    };
-    unsigned long long recs = 0, inval = 0, fxc = 0, ids = 0;
-    long long fxs = 0;
-    double ss = 0;
-    const uint32_t total = n + (tail ? 1u : 0u);
-    for (uint32_t k = tid; k < total; k += ND_WG) {
-        uint32_t end = k < n ? nl_at(k) : len;
-        int64_t pn = k == 0 ? prev_nl : (int64_t)nl_at(k - 1);
-        uint32_t from = (uint32_t)(pn + 1);
-        if (blank_span(src, from, end)) continue;
-        NdRecord r;
-        if (ndjson_record(src, from, end, r)) {
-            if (r.score_deferred) resolve_score(slow, src, r);
-            recs += 1;
-            ids += (unsigned long long)r.id;
-            ss += r.score;
-            if (r.fx6) { fxs += r.mant; fxc += 1; }
-        } else {
-            inval += 1;
-        }
-    }
-    recs = block_reduce_sum<unsigned long long, ND_WAVES>(recs, ush);
-    inval = block_reduce_sum<unsigned long long, ND_WAVES>(inval, ush);
-    fxc = block_reduce_sum<unsigned long long, ND_WAVES>(fxc, ush);
-    ids = block_reduce_sum<unsigned long long, ND_WAVES>(ids, ush);
-    fxs = block_reduce_sum<long long, ND_WAVES>(fxs, lsh);
-    ss = block_reduce_sum<double, ND_WAVES>(ss, dsh);
-    if (tid == 0) {
+    acc.recs = block_reduce_sum<unsigned long long, 1>(acc.recs, ush);
+    acc.inval = block_reduce_sum<unsigned long long, 1>(acc.inval, ush);
+    acc.fxc = block_reduce_sum<unsigned long long, 1>(acc.fxc, ush);
+    acc.ids = block_reduce_sum<unsigned long long, 1>(acc.ids, ush);
+    acc.fxs = block_reduce_sum<long long, 1>(acc.fxs, lsh);
+    acc.ss = block_reduce_sum<double, 1>(acc.ss, dsh);
+    if (ln == 0) {
         NdPart& p = parts[tile];
-        p.records = recs; p.invalid = inval; p.id_sum = (long long)ids; p.fx_sum = fxs; p.fx_cells = fxc; p.score_sum = ss;
+        p.records = acc.recs; p.invalid = acc.inval; p.id_sum = (long long)acc.ids;
+        p.fx_sum = acc.fxs; p.fx_cells = acc.fxc; p.score_sum = acc.ss;
     }
 }
 

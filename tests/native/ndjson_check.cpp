@@ -30,6 +30,25 @@ struct BufSrc {
     uint8_t at(uint32_t i) const { return p[i]; }
 };
 
+// The kernel's LDS source: bytes also read as aligned 8-byte words (p is
+// 8-aligned, bytes past the record are readable and hold decoys).
+struct WordSrc {
+    static constexpr bool kWord8 = true;
+    const uint8_t* p;
+    uint8_t at(uint32_t i) const { return p[i]; }
+    uint64_t word8(uint32_t a) const { uint64_t w; std::memcpy(&w, p + a, 8); return w; }
+};
+
+// One record through the walker: 1 valid, 0 invalid, -1 blank.
+template <class Src>
+int walk(const Src& src, uint32_t from, uint32_t to, fx::NdRecord& rec) {
+    if (fx::blank_span(src, from, to)) return -1;
+    int ok = fx::ndjson_record(src, from, to, rec);
+    static fx::BigDec big;
+    if (ok == 1) fx::ndjson_resolve(src, rec, big);
+    return ok;
+}
+
 std::string ws(Rng& r) {
     const char* w[] = {"", "", "", " ", "  ", "\t", " \r "};
     return w[r.below(7)];
@@ -115,11 +134,25 @@ int main(int argc, char** argv) {
         falx_ndjson_id_score_stats want;
         ora_ndjson_id_score((const uint8_t*)withnl.data(), withnl.size(), 1, &want);
         BufSrc src{(const uint8_t*)line.data()};
-        bool blank = fx::blank_span(src, 0, (uint32_t)line.size());
         fx::NdRecord rec;
-        int ok = blank ? -1 : fx::ndjson_record(src, 0, (uint32_t)line.size(), rec);
-        static fx::BigDec big;
-        if (ok == 1) fx::ndjson_resolve(src, rec, big);
+        int ok = walk(src, 0, (uint32_t)line.size(), rec);
+        // The same record at every alignment in an 8-aligned buffer, with
+        // quotes and backslashes after it: the word walker must agree.
+        {
+            alignas(8) static uint8_t wbuf[1 << 16];
+            const uint32_t off = (uint32_t)r.below(8);
+            if (off + line.size() + 16 <= sizeof wbuf) {
+                std::memcpy(wbuf + off, line.data(), line.size());
+                for (int k = 0; k < 16; ++k) wbuf[off + line.size() + k] = "\"\\x"[r.below(3)];
+                fx::NdRecord wrec;
+                int wok = walk(WordSrc{wbuf}, off, off + (uint32_t)line.size(), wrec);
+                if (wok != ok || (ok == 1 && (wrec.id != rec.id || wrec.fx6 != rec.fx6 || wrec.mant != rec.mant ||
+                                             fx::f64_to_bits(wrec.score) != fx::f64_to_bits(rec.score)))) {
+                    std::fprintf(stderr, "WORD WALKER MISMATCH on record [%s]\n", line.c_str());
+                    return 1;
+                }
+            }
+        }
         uint64_t g_rec = ok == 1, g_inv = ok == 0;
         int64_t g_id = ok == 1 ? rec.id : 0;
         int64_t g_fx = ok == 1 && rec.fx6 ? rec.mant : 0;
