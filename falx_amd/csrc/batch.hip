@@ -112,7 +112,8 @@ uint64_t scratch_of(const falx_request* r, uint32_t* n_tiles_out = nullptr) {
     const uint32_t n_tiles = (uint32_t)((r->len + TILE - 1) / TILE);
     if (n_tiles_out) *n_tiles_out = n_tiles;
     if (r->op == FALX_OP_NDJSON_ID_SCORE)
-        return align_up((size_t)n_tiles * sizeof(NdPart), 256) + align_up(RED_BLOCKS * sizeof(NdPart), 256);
+        return align_up((size_t)2 * n_tiles * sizeof(NdPart), 256) + align_up((size_t)n_tiles * sizeof(uint2), 256) +
+               align_up(RED_BLOCKS * sizeof(NdPart), 256);
     falx_spec fixed;
     const falx_spec* spec = r->spec;
     if (r->op == FALX_OP_LOGFMT_PAIRS) {
@@ -306,11 +307,14 @@ falx_status falx_batch_submit(falx_batch* b, const falx_request* reqs, uint32_t 
         // addresses once the layout is known.
         const size_t res_rel = (size_t)i * RES_STRIDE;
         if (r.op == FALX_OP_NDJSON_ID_SCORE) {
+            // Per tile its partial, then (second half) its far record's.
             NdPart* parts = reinterpret_cast<NdPart*>(base);
-            NdPart* tmp = reinterpret_cast<NdPart*>(base + align_up((size_t)n_tiles * sizeof(NdPart), 256));
-            B.nd.push_back(BNd{r.d_data, (uint32_t)r.len, n_tiles, parts});
+            const size_t far_off = align_up((size_t)2 * n_tiles * sizeof(NdPart), 256);
+            uint2* far = reinterpret_cast<uint2*>(base + far_off);
+            NdPart* tmp = reinterpret_cast<NdPart*>(base + far_off + align_up((size_t)n_tiles * sizeof(uint2), 256));
+            B.nd.push_back(BNd{r.d_data, (uint32_t)r.len, n_tiles, parts, far});
             B.tab[T_NDJ].add((uint32_t)B.nd.size() - 1, (n_tiles + chunk - 1) / chunk, chunk);
-            add_red(B, RJ_ND, parts, n_tiles, 1, tmp, reinterpret_cast<void*>(res_rel + RES_ND_OFF));
+            add_red(B, RJ_ND, parts, 2 * n_tiles, 1, tmp, reinterpret_cast<void*>(res_rel + RES_ND_OFF));
             continue;
         }
         // CSV-family and logfmt documents: the tile-prefix pipeline.
@@ -514,7 +518,10 @@ falx_status falx_batch_submit(falx_batch* b, const falx_request* reqs, uint32_t 
                 if (sig & (1ULL << t)) keep(launch_emit(E_STRINGS, k, tg, ss, dH, t));
             }
         }
-        if (sig & (1ULL << T_NDJ)) hipLaunchKernelGGL(k_ndjson_batch, tg, dim3(ND_WG), 0, ss, dH);
+        if (sig & (1ULL << T_NDJ)) {
+            hipLaunchKernelGGL(k_ndjson_batch, tg, dim3(ND_WG), 0, ss, dH);
+            hipLaunchKernelGGL(k_ndjson_far_batch, ug, dim3(64), 0, ss, dH);
+        }
         if (sig & (1ULL << T_RED1)) hipLaunchKernelGGL(k_reduce_batch, ug, dim3(WG), 0, ss, dH, 1u);
         if (sig & (1ULL << T_RED2)) hipLaunchKernelGGL(k_reduce_batch, ug, dim3(WG), 0, ss, dH, 2u);
         keep(hipGetLastError());

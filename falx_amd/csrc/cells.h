@@ -30,8 +30,12 @@
 
 #if defined(__HIPCC__)
 #define FX_HD __host__ __device__ __forceinline__
+// Out of line on the device: for paths taken rarely, whose inlined code
+// would only bloat the kernels that reach them.
+#define FX_HD_COLD __host__ __device__ __attribute__((noinline))
 #else
 #define FX_HD inline
+#define FX_HD_COLD inline
 #endif
 
 namespace fx {

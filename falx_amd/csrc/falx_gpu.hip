@@ -666,17 +666,24 @@ falx_status falx_parse_ndjson_id_score(falx_ctx* c, const uint8_t* d, uint64_t l
     if (len == 0) return FALX_OK;
     if ((st = begin(c)) != FALX_OK) return st;
     uint32_t n_tiles = (uint32_t)((len + TILE - 1) / TILE);
-    if ((st = ensure_part(c, (size_t)n_tiles * sizeof(NdPart))) != FALX_OK) return st;
+    // Per tile: the tile's partial, then (second half) its far record's.
+    if ((st = ensure_part(c, (size_t)2 * n_tiles * sizeof(NdPart) + (size_t)n_tiles * sizeof(uint2))) != FALX_OK)
+        return st;
+    NdPart* parts = static_cast<NdPart*>(c->d_part);
+    uint2* far = reinterpret_cast<uint2*>(parts + 2 * (size_t)n_tiles);
     HIP_TRY(mark_start(c));
     hipLaunchKernelGGL(k_ndjson_id_score, dim3(n_tiles), dim3(ND_WG), 0, c->stream, d, (uint32_t)len, n_tiles,
-                       static_cast<NdPart*>(c->d_part));
+                       parts, far);
+    hipLaunchKernelGGL(k_ndjson_far, dim3((n_tiles + 63) / 64), dim3(64), 0, c->stream, d, n_tiles,
+                       static_cast<const uint2*>(far), parts + n_tiles);
     HIP_TRY(mark_end(c));
     {
         NdPart* tmp = static_cast<NdPart*>(c->d_red);
         NdPart* fin = reinterpret_cast<NdPart*>(static_cast<char*>(c->d_red) + RED_OUT);
-        uint32_t blocks = n_tiles < RED_BLOCKS * 4 ? 1u : RED_BLOCKS;
-        hipLaunchKernelGGL(k_reduce_nd, dim3(blocks), dim3(WG), 0, c->stream, static_cast<const NdPart*>(c->d_part),
-                           n_tiles, blocks == 1 ? fin : tmp);
+        const uint32_t n_parts = 2 * n_tiles;
+        uint32_t blocks = n_parts < RED_BLOCKS * 4 ? 1u : RED_BLOCKS;
+        hipLaunchKernelGGL(k_reduce_nd, dim3(blocks), dim3(WG), 0, c->stream, static_cast<const NdPart*>(parts),
+                           n_parts, blocks == 1 ? fin : tmp);
         if (blocks > 1) hipLaunchKernelGGL(k_reduce_nd, dim3(1), dim3(WG), 0, c->stream, tmp, blocks, fin);
         HIP_TRY(hipGetLastError());
     }

@@ -193,10 +193,10 @@ FX_HD bool j_number(JCur<Src>& c, double& v, bool& fx6, int64_t& mant, bool& def
         }
         return true;
     }
-    F64Result r = parse_f64_stream(RawStream<Src>{c.s, s0, c.i});
-    if (!r.ok) return false;
-    v = r.value;
-    deferred = r.deferred;
+    // Any other JSON number (it is a valid number of the general grammar):
+    // its value is computed after the walk, from [from, to).
+    v = 0.0;
+    deferred = true;
     if (!has_exp && frac_digits <= 6 && int_digits <= 12) {
         int64_t m = 0;
         for (uint32_t k = ip; k < ip + int_digits; ++k) m = m * 10 + (c.s->at(k) - '0');
@@ -212,7 +212,7 @@ struct NdRecord {
     int64_t id;
     double score;
     bool fx6;
-    bool score_deferred;          // score needs the exact path over [sfrom, sto)
+    bool score_deferred;          // score is computed after the walk from [sfrom, sto)
     uint32_t sfrom, sto;
     int64_t mant;
 };
@@ -281,11 +281,28 @@ FX_HD int ndjson_record(const Src& s, uint32_t from, uint32_t to, NdRecord& out)
     return (have_id && have_score) ? 1 : 0;
 }
 
-// Resolves a deferred score with the exact path (storage in `b`).
+// A deferred score by the general parser; true when that leaves it to the
+// exact path (ndjson_resolve_exact).
+template <class Src>
+FX_HD bool ndjson_score_general(const Src& s, NdRecord& r) {
+    const F64Result f = parse_f64_stream(RawStream<Src>{&s, r.sfrom, r.sto});
+    r.score = f.value;
+    return f.deferred;
+}
+
+// The exact path for a score (storage in `b`).
+template <class Src>
+FX_HD void ndjson_resolve_exact(const Src& s, NdRecord& r, BigDec& b) {
+    r.score = bits_to_f64(exact_f64_bits(RawStream<Src>{&s, r.sfrom, r.sto}, b));
+    r.score_deferred = false;
+}
+
+// Resolves a deferred score: the general parser, then the exact path if it
+// needs it.
 template <class Src>
 FX_HD void ndjson_resolve(const Src& s, NdRecord& r, BigDec& b) {
     if (!r.score_deferred) return;
-    r.score = bits_to_f64(exact_f64_bits(RawStream<Src>{&s, r.sfrom, r.sto}, b));
+    if (ndjson_score_general(s, r)) ndjson_resolve_exact(s, r, b);
     r.score_deferred = false;
 }
 

@@ -626,7 +626,7 @@ DEV void resolve_score(SlowF64& s, const Src& src, NdRecord& r) {
         const uint32_t l = (uint32_t)__ffsll((long long)m) - 1u;
         if (__lane_id() == l) {
             slow_lock(s);
-            ndjson_resolve(src, r, s.big);
+            ndjson_resolve_exact(src, r, s.big);
             slow_unlock(s);
         }
         m &= m - 1;
@@ -2098,23 +2098,51 @@ struct NdAcc {
     double ss;
 };
 
+// One record's outcome: 0 blank, 1 valid, 2 invalid (returned by value and
+// folded with selects, so the accumulators stay in registers).
+struct NdOne {
+    uint32_t kind;
+    bool fx6;
+    int64_t id, mant;
+    double score;
+};
+
 template <class Src>
-DEV void nd_record(const Src& src, uint32_t from, uint32_t to, NdAcc& a, SlowF64& slow) {
-    if (blank_span(src, from, to)) return;
+DEV NdOne nd_record(const Src& src, uint32_t from, uint32_t to, SlowF64& slow) {
+    NdOne o{0, false, 0, 0, 0.0};
+    if (blank_span(src, from, to)) return o;
     NdRecord r;
     if (ndjson_record(src, from, to, r)) {
-        if (r.score_deferred) resolve_score(slow, src, r);
-        a.recs += 1;
-        a.ids += (unsigned long long)r.id;
-        a.ss += r.score;
-        if (r.fx6) {
-            a.fxs += r.mant;
-            a.fxc += 1;
+        if (r.score_deferred) {
+            if (ndjson_score_general(src, r)) resolve_score(slow, src, r);
+            r.score_deferred = false;
         }
+        o.kind = 1;
+        o.fx6 = r.fx6;
+        o.id = r.id;
+        o.mant = r.mant;
+        o.score = r.score;
     } else {
-        a.inval += 1;
+        o.kind = 2;
     }
+    return o;
 }
+
+DEV void nd_add(NdAcc& a, const NdOne& o) {
+    const bool v = o.kind == 1;
+    a.recs += v ? 1u : 0u;
+    a.inval += o.kind == 2 ? 1u : 0u;
+    a.ids += v ? (unsigned long long)o.id : 0ULL;
+    a.ss += v ? o.score : 0.0;
+    a.fxs += (v && o.fx6) ? o.mant : 0;
+    a.fxc += (v && o.fx6) ? 1u : 0u;
+}
+
+// Document bytes read from HBM (records running past a tile's LDS window).
+struct GlobalSrc {
+    const uint8_t* g;
+    DEV uint8_t at(uint32_t p) const { return g[p]; }
+};
 
 // Newline bytes of four little-endian words as a 16-bit mask.
 DEV uint32_t nl_mask16(uint4 v) {
@@ -2129,7 +2157,9 @@ DEV uint32_t nl_mask16(uint4 v) {
     return m;
 }
 
-DEV void ndjson_tile(const uint8_t* data, uint32_t len, uint32_t n_tiles, NdPart* parts, uint32_t tile) {
+// far[tile] receives the one record of the tile that runs past its LDS
+// window, (start, end), or (ND_NONE, 0); k_ndjson_far walks those records.
+DEV void ndjson_tile(const uint8_t* data, uint32_t len, uint32_t n_tiles, NdPart* parts, uint2* far, uint32_t tile) {
     __shared__ __align__(16) uint8_t bytes[ND_WIN + 16];
     __shared__ uint16_t nls[ND_MAXNL];   // newline positions - tile_start, in order
     __shared__ uint32_t qnext;
@@ -2158,6 +2188,7 @@ DEV void ndjson_tile(const uint8_t* data, uint32_t len, uint32_t n_tiles, NdPart
     __syncthreads();
     const NdLds src{bytes, tile_start};
     NdAcc acc{0, 0, 0, 0, 0, 0.0};
+    uint2 far_rec = make_uint2(ND_NONE, 0u);
     // Start of the record being listed: the tile's first byte when a record
     // starts there, else it is the byte after the tile's first newline.
     uint32_t cur = (tile_start == 0 || data[tile_start - 1] == '\n') ? tile_start : ND_NONE;
@@ -2212,25 +2243,29 @@ DEV void ndjson_tile(const uint8_t* data, uint32_t len, uint32_t n_tiles, NdPart
         }
         __syncthreads();
         if (full) scan = tile_start + nls[ND_MAXNL - 1] + 1u;
-        // Record k ends at newline k; its start is `cur` for k = 0, else the
-        // byte after newline k - 1. Lanes take records from the queue.
+        const uint32_t next_cur = n ? tile_start + nls[n - 1] + 1u : cur;
+        const bool open_end = !stop && !full && scan >= win_hi;   // no newline ends the record at next_cur
+        // The data's last record, without a final newline, is one more item.
+        const bool last_rec = open_end && win_hi == len && next_cur != ND_NONE && next_cur < tile_end &&
+                              next_cur < len;
+        const uint32_t items = n + (last_rec ? 1u : 0u);
+        // Item k ends at newline k (at len for the last record); its start is
+        // `cur` for k = 0, else the byte after newline k - 1. Lanes take
+        // items from the queue.
         if (ln == 0) qnext = ND_WG;
         __syncthreads();
-        for (uint32_t k = ln; k < n;) {
+        for (uint32_t k = ln; k < items;) {
             const uint32_t from = k == 0 ? cur : tile_start + nls[k - 1] + 1u;
-            if (from != ND_NONE) nd_record(src, from, tile_start + nls[k], acc, slow);
+            const uint32_t to = k < n ? tile_start + nls[k] : len;
+            if (from != ND_NONE) nd_add(acc, nd_record(src, from, to, slow));
             k = atomicAdd(&qnext, 1u);
         }
         __syncthreads();
-        if (n) cur = tile_start + nls[n - 1] + 1u;
+        cur = next_cur;
         if (stop) break;
-        if (!full && scan >= win_hi) {
-            // No newline ends the record at `cur` inside the window.
+        if (open_end) {
             if (cur != ND_NONE && cur < tile_end) {
-                if (win_hi == len) {
-                    // The data's last record, without a final newline.
-                    if (ln == 0 && cur < len) nd_record(src, cur, len, acc, slow);
-                } else {
+                if (win_hi < len) {
                     // A record running past the halo: its end is searched in
                     // HBM 512 bytes per step, and one lane walks it there.
                     uint32_t end = len;
@@ -2252,10 +2287,7 @@ DEV void ndjson_tile(const uint8_t* data, uint32_t len, uint32_t n_tiles, NdPart
                             break;
                         }
                     }
-                    if (ln == 0) {
-                        const TileSrc gsrc{data, bytes, tile_start, win_hi};
-                        nd_record(gsrc, cur, end, acc, slow);
-                    }
+                    far_rec = make_uint2(cur, end);
                 }
             }
             break;
@@ -2271,13 +2303,37 @@ DEV void ndjson_tile(const uint8_t* data, uint32_t len, uint32_t n_tiles, NdPart
         NdPart& p = parts[tile];
         p.records = acc.recs; p.invalid = acc.inval; p.id_sum = (long long)acc.ids;
         p.fx_sum = acc.fxs; p.fx_cells = acc.fxc; p.score_sum = acc.ss;
+        far[tile] = far_rec;
+    }
+}
+
+// The records that ran past their tile's LDS window, one lane per tile,
+// read from HBM; results go to fparts[tile] (zeros for tiles without one).
+DEV void ndjson_far_tile(const uint8_t* data, const uint2* far, NdPart* fparts, uint32_t tile, bool have,
+                         SlowF64& slow) {
+    NdAcc acc{0, 0, 0, 0, 0, 0.0};
+    const uint2 f = have ? far[tile] : make_uint2(ND_NONE, 0u);
+    if (f.x != ND_NONE) nd_add(acc, nd_record(GlobalSrc{data}, f.x, f.y, slow));
+    if (have) {
+        NdPart& p = fparts[tile];
+        p.records = acc.recs; p.invalid = acc.inval; p.id_sum = (long long)acc.ids;
+        p.fx_sum = acc.fxs; p.fx_cells = acc.fxc; p.score_sum = acc.ss;
     }
 }
 
 #ifndef FX_KIND   // defined once, in the host translation unit
 __global__ __launch_bounds__(ND_WG) void k_ndjson_id_score(const uint8_t* data, uint32_t len, uint32_t n_tiles,
-                                                        NdPart* parts) {
-    ndjson_tile(data, len, n_tiles, parts, blockIdx.x);
+                                                        NdPart* parts, uint2* far) {
+    ndjson_tile(data, len, n_tiles, parts, far, blockIdx.x);
+}
+
+__global__ __launch_bounds__(64) void k_ndjson_far(const uint8_t* data, uint32_t n_tiles, const uint2* far,
+                                                   NdPart* fparts) {
+    __shared__ SlowF64 slow;
+    if (threadIdx.x == 0) slow.lock = 0;
+    __syncthreads();
+    const uint32_t t = blockIdx.x * 64u + threadIdx.x;
+    ndjson_far_tile(data, far, fparts, t, t < n_tiles, slow);
 }
 #endif
 
@@ -2339,7 +2395,8 @@ struct BRed {
 struct BNd {
     const uint8_t* data;
     uint32_t len, n_tiles;
-    NdPart* parts;
+    NdPart* parts;    // 2 n_tiles: the tiles', then the far records'
+    uint2* far;
 };
 
 struct BFill {
@@ -2535,8 +2592,27 @@ __global__ __launch_bounds__(ND_WG) void k_ndjson_batch(const BatchHdr* H) {
         __syncthreads();
         const BNd& A = H->nd[pi];
         for (uint32_t t = t0; t < t1; ++t) {
-            ndjson_tile(A.data, A.len, A.n_tiles, A.parts, t);
+            ndjson_tile(A.data, A.len, A.n_tiles, A.parts, A.far, t);
             __syncthreads();
+        }
+    }
+}
+
+// The far records of the NDJSON documents, over the same units (a lane per
+// tile of the unit).
+__global__ __launch_bounds__(64) void k_ndjson_far_batch(const BatchHdr* H) {
+    __shared__ SlowF64 slow;
+    const SegT& T = H->tab[T_NDJ];
+    if (threadIdx.x == 0) slow.lock = 0;
+    __syncthreads();
+    for (uint32_t u = blockIdx.x; u < T.total; u += gridDim.x) {
+        const uint32_t i = seg_find(T, u);
+        const uint32_t pi = T.req[i], c = T.chunk[i];
+        const BNd& A = H->nd[pi];
+        const uint32_t t0 = (u - T.start[i]) * c, t1 = min(t0 + c, A.n_tiles);
+        for (uint32_t t = t0; t < t1; t += 64) {
+            const uint32_t tt = t + threadIdx.x;
+            ndjson_far_tile(A.data, A.far, A.parts + A.n_tiles, tt, tt < t1, slow);
         }
     }
 }
