@@ -127,13 +127,71 @@ FX_HD bool key_eq(const JCur<Src>& c, uint32_t ks, uint32_t ke, const char* lit,
     return true;
 }
 
+// Bytes of x that are not ASCII digits, as the top bit of each byte (exact).
+FX_HD uint64_t nondigit_bytes64(uint64_t x) {
+    const uint64_t t = x ^ 0x3030303030303030ULL;
+    return (((t & 0x7F7F7F7F7F7F7F7FULL) + 0x7676767676767676ULL) | t) & 0x8080808080808080ULL;
+}
+
+// The value of the first k (1..8) bytes of x, all digits, the first byte the
+// most significant: Lemire's 8-digit fold (pairs, then quads, then the
+// halves) after moving the k digits to the top (the bytes below read as 0).
+FX_HD uint64_t fold_digits8(uint64_t x, uint32_t k) {
+    uint64_t v = (x & 0x0F0F0F0F0F0F0F0FULL) << (8u * (8u - k));
+    v = (v * 2561ULL) >> 8;
+    v = ((v & 0x00FF00FF00FF00FFULL) * 6553601ULL) >> 16;
+    return ((v & 0x0000FFFF0000FFFFULL) * 42949672960001ULL) >> 32;
+}
+
+FX_HD uint64_t pow10_u64(uint32_t k) {
+    uint64_t p = 1;
+    for (uint32_t j = 0; j < k; ++j) p *= 10;
+    return p;
+}
+
+// The run of decimal digits at c.i (before c.n): its length and, when it is
+// at most 16 digits long, its value. A word source tests 16 bytes at once;
+// len 17 stands for "more than 16" (no value).
+struct DigitRun {
+    uint32_t len;
+    uint64_t val;
+};
+
+template <class Src>
+FX_HD DigitRun j_digits(const JCur<Src>& c) {
+    DigitRun d{0, 0};
+    const uint32_t i = c.i;
+    if (i >= c.n) return d;
+    const uint32_t a = i & ~7u, sh = 8u * (i - a);
+    const uint64_t w0 = c.s->word8(a), w1 = c.s->word8(a + 8), w2 = c.s->word8(a + 16);
+    const uint64_t x = sh ? (w0 >> sh) | (w1 << (64u - sh)) : w0;
+    const uint64_t y = sh ? (w1 >> sh) | (w2 << (64u - sh)) : w1;
+    const uint64_t ndx = nondigit_bytes64(x), ndy = nondigit_bytes64(y);
+    uint32_t len = ndx ? ((uint32_t)__builtin_ctzll(ndx) >> 3) : (ndy ? 8u + ((uint32_t)__builtin_ctzll(ndy) >> 3) : 17u);
+    if (len > c.n - i) len = c.n - i;
+    d.len = len;
+    if (len == 0 || len > 16) return d;
+    d.val = len <= 8 ? fold_digits8(x, len) : fold_digits8(x, 8) * pow10_u64(len - 8) + fold_digits8(y, len - 8);
+    return d;
+}
+
 template <class Src>
 FX_HD bool j_int(JCur<Src>& c, int64_t& v) {
     bool neg = false;
     if (c.peek() == '-') { neg = true; ++c.i; }
     if (!is_digit(c.peek())) return false;
     uint64_t acc = 0;
-    while (is_digit(c.peek())) { acc = acc * 10 + (uint64_t)(c.s->at(c.i) - '0'); ++c.i; }
+    bool done = false;
+    if constexpr (has_word8<Src>::value) {
+        const DigitRun d = j_digits(c);
+        if (d.len <= 16) {
+            acc = d.val;
+            c.i += d.len;
+            done = true;
+        }
+    }
+    if (!done)
+        while (is_digit(c.peek())) { acc = acc * 10 + (uint64_t)(c.s->at(c.i) - '0'); ++c.i; }
     if (!c.delim()) return false;
     v = neg ? (int64_t)(0 - acc) : (int64_t)acc;
     return true;
@@ -150,14 +208,33 @@ FX_HD bool j_number(JCur<Src>& c, double& v, bool& fx6, int64_t& mant, bool& def
     if (c.peek() == '-') { neg = true; ++c.i; }
     if (!is_digit(c.peek())) return false;
     uint32_t ip = c.i;
-    if (c.peek() == '0') ++c.i;
-    else while (is_digit(c.peek())) ++c.i;
+    // Digit runs by word where the source allows it; the values of runs of
+    // at most 16 digits come with them (ival, fval; 0 otherwise).
+    uint64_t ival = 0, fval = 0;
+    bool vals = true;
+    if (c.peek() == '0') {
+        ++c.i;
+    } else if constexpr (has_word8<Src>::value) {
+        const DigitRun d = j_digits(c);
+        if (d.len <= 16) { ival = d.val; c.i += d.len; }
+        else { vals = false; while (is_digit(c.peek())) ++c.i; }
+    } else {
+        vals = false;
+        while (is_digit(c.peek())) ++c.i;
+    }
     uint32_t int_digits = c.i - ip;
     uint32_t frac_digits = 0, fs = 0;
     if (c.peek() == '.') {
         ++c.i;
         fs = c.i;
-        while (is_digit(c.peek())) ++c.i;
+        if constexpr (has_word8<Src>::value) {
+            const DigitRun d = j_digits(c);
+            if (d.len <= 16) { fval = d.val; c.i += d.len; }
+            else { vals = false; while (is_digit(c.peek())) ++c.i; }
+        } else {
+            vals = false;
+            while (is_digit(c.peek())) ++c.i;
+        }
         frac_digits = c.i - fs;
         if (frac_digits == 0) return false;
     }
@@ -181,8 +258,12 @@ FX_HD bool j_number(JCur<Src>& c, double& v, bool& fx6, int64_t& mant, bool& def
         // rounded IEEE division (Clinger 1990), the value the general
         // parser gives.
         int64_t m = 0;
-        for (uint32_t k = ip; k < ip + int_digits; ++k) m = m * 10 + (c.s->at(k) - '0');
-        for (uint32_t k = 0; k < frac_digits; ++k) m = m * 10 + (c.s->at(fs + k) - '0');
+        if (vals) {
+            m = (int64_t)(ival * pow10_u64(frac_digits) + fval);
+        } else {
+            for (uint32_t k = ip; k < ip + int_digits; ++k) m = m * 10 + (c.s->at(k) - '0');
+            for (uint32_t k = 0; k < frac_digits; ++k) m = m * 10 + (c.s->at(fs + k) - '0');
+        }
         const double mag = (double)m / pow10_exact((int)frac_digits);
         v = neg ? -mag : mag;
         deferred = false;

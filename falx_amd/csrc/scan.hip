@@ -2080,13 +2080,16 @@ constexpr int ND_WG = 64;
 constexpr uint32_t ND_HALO = 512;
 constexpr uint32_t ND_WIN = TILE + ND_HALO;
 constexpr uint32_t ND_MAXNL = 512;      // newlines listed per round
+// Zero bytes after the window: the walker's word reads (three aligned words
+// from a byte of the record) end at most 24 bytes past its last byte.
+constexpr uint32_t ND_PAD = 32;
 constexpr uint32_t ND_NONE = 0xFFFFFFFFu;
 
 // The tile's LDS copy as a walker source (ndjson_walk.h): bytes, and
 // aligned 8-byte words for skipping strings (tile starts are multiples of 8).
 struct NdLds {
     static constexpr bool kWord8 = true;
-    const uint8_t* b;   // document bytes [lo, lo + ND_WIN), zero padded
+    const uint8_t* b;   // document bytes [lo, lo + ND_WIN), then ND_PAD zero bytes
     uint32_t lo;
     DEV uint8_t at(uint32_t p) const { return b[p - lo]; }
     DEV uint64_t word8(uint32_t p) const { return *reinterpret_cast<const uint64_t*>(b + (p - lo)); }
@@ -2160,7 +2163,7 @@ DEV uint32_t nl_mask16(uint4 v) {
 // far[tile] receives the one record of the tile that runs past its LDS
 // window, (start, end), or (ND_NONE, 0); k_ndjson_far walks those records.
 DEV void ndjson_tile(const uint8_t* data, uint32_t len, uint32_t n_tiles, NdPart* parts, uint2* far, uint32_t tile) {
-    __shared__ __align__(16) uint8_t bytes[ND_WIN + 16];
+    __shared__ __align__(16) uint8_t bytes[ND_WIN + ND_PAD];
     __shared__ uint16_t nls[ND_MAXNL];   // newline positions - tile_start, in order
     __shared__ uint32_t qnext;
     __shared__ SlowF64 slow;
@@ -2173,7 +2176,7 @@ DEV void ndjson_tile(const uint8_t* data, uint32_t len, uint32_t n_tiles, NdPart
     const uint32_t tile_end = min(tile_start + (uint32_t)TILE, len);
     const uint32_t win_hi = min(tile_start + ND_WIN, len);
     const uint32_t win_len = win_hi - tile_start;
-    for (uint32_t i = ln * 16; i < ND_WIN + 16; i += ND_WG * 16) {
+    for (uint32_t i = ln * 16; i < ND_WIN + ND_PAD; i += ND_WG * 16) {
         uint4 v = make_uint4(0, 0, 0, 0);
         if (i + 16 <= win_len) {
             v = *reinterpret_cast<const uint4*>(data + tile_start + i);

@@ -31,7 +31,8 @@ struct BufSrc {
 };
 
 // The kernel's LDS source: bytes also read as aligned 8-byte words (p is
-// 8-aligned, bytes past the record are readable and hold decoys).
+// 8-aligned, 32 bytes past the record are readable and hold decoys, digits
+// among them).
 struct WordSrc {
     static constexpr bool kWord8 = true;
     const uint8_t* p;
@@ -141,9 +142,9 @@ int main(int argc, char** argv) {
         {
             alignas(8) static uint8_t wbuf[1 << 16];
             const uint32_t off = (uint32_t)r.below(8);
-            if (off + line.size() + 16 <= sizeof wbuf) {
+            if (off + line.size() + 32 <= sizeof wbuf) {
                 std::memcpy(wbuf + off, line.data(), line.size());
-                for (int k = 0; k < 16; ++k) wbuf[off + line.size() + k] = "\"\\x"[r.below(3)];
+                for (int k = 0; k < 32; ++k) wbuf[off + line.size() + k] = "\"\\x7"[r.below(4)];
                 fx::NdRecord wrec;
                 int wok = walk(WordSrc{wbuf}, off, off + (uint32_t)line.size(), wrec);
                 if (wok != ok || (ok == 1 && (wrec.id != rec.id || wrec.fx6 != rec.fx6 || wrec.mant != rec.mant ||
