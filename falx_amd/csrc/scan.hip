@@ -1349,34 +1349,122 @@ __device__ __forceinline__ void scan_tile(const Params& P, const uint32_t tile, 
     // block scan over the list entries per segment). Validity words are built
     // per wave (rows of consecutive records are consecutive) and ORed into
     // the column with one atomic per word.
-    if constexpr (MODE == M_COLUMNS || MODE == M_STATS) {
+    if constexpr (MODE == M_COLUMNS || MODE == M_STATS || MODE == M_STRINGS) {
         constexpr uint32_t TL_MAX = 512;
-        constexpr uint32_t NACC = 2;
-        static_assert(NACC == STAT_GROUP && NACC == COL_STAT_GROUP, "accumulators per pass");
         __shared__ uint32_t tl[TL_MAX];   // list index | (row rank in the segment << 16)
-        constexpr bool IS_COLS = MODE == M_COLUMNS;
-        const uint32_t c0 = IS_COLS ? 0u : P.col_group;
-        const uint32_t nc = IS_COLS ? P.n_cols : min((uint32_t)STAT_GROUP, P.n_cols - c0);
-        // Known at compile time in the fast forms, which then carry no
-        // statistics registers when there are none.
-        const bool with_stats = !IS_COLS                         ? true
-                                : KMODE == M_COLUMNS_FAST       ? false
-                                : KMODE == M_COLUMNS_FAST_STATS ? true
-                                                                : P.partials != nullptr;
-        const bool trailer_row_live = trailer && !trailer_comment;
         const uint32_t n_ent = n_list + (trailer ? 1u : 0u);   // the virtual terminator is entry n_list
         const auto word_at = [&](uint32_t k) -> uint32_t { return k < n_list ? (uint32_t)sh.list[k] : trailer_word; };
         const auto pos_of = [&](uint32_t wd) -> uint32_t { return tile_start + (wd & 0x3FFFu); };
         const uint64_t rows_cap = P.rows_cap;
-        Acc acc[NACC];
-#pragma unroll
-        for (uint32_t j = 0; j < NACC; ++j) acc[j] = Acc{0, 0, 0, 0, 0, 0.0};
-        unsigned long long records = 0;
-        if (IS_COLS && tid < FALX_MAX_COLUMNS) sh.str_len[tid] = 0;   // read after the segment scans' barriers
 
+        // A record of the tile: its fields that end here are list entries
+        // e0 .. e0 + n - 1; `term` when its terminator is here (then it has a
+        // row unless it is a comment record); ord0 is the ordinal of field
+        // e0 (non-zero only for the record open at the tile's start).
+        struct Rec {
+            bool has, term, comment;
+            uint32_t e0, n, fs0, rs, ord0, row;
+        };
+        // Calls round(rec) once per round of WG records, with every thread
+        // (a thread without a record gets has = false). Terminators are
+        // listed per segment of up to TL_MAX records (one block scan over
+        // the list entries per segment).
+        auto for_records = [&](auto&& round) __attribute__((always_inline)) {
+            uint32_t s0 = 0, row_base = pre.n_rows;
+            while (true) {
+                // The segment's terminators: entries [s0, n_ent) in runs of
+                // `per` per thread, counted, scanned, then written by rank.
+                const uint32_t rem = n_ent - s0;
+                const uint32_t per = (rem + WG - 1) / WG;
+                const uint32_t k0 = min(n_ent, s0 + tid * per), k1 = min(n_ent, k0 + per);
+                Scan3 cnt{0, 0, -1};
+                for (uint32_t k = k0; k < k1; ++k) {
+                    const uint32_t wd = word_at(k);
+                    if (wd & 0x4000u) {
+                        cnt.a += 1;
+                        cnt.b += (wd & 0x8000u) ? 0u : 1u;
+                    }
+                }
+                Scan3 tot;
+                const Scan3 ex = block_excl(cnt, sh.scan_sh, tot);
+                {
+                    uint32_t ra = ex.a, rb = ex.b;
+                    for (uint32_t k = k0; k < k1; ++k) {
+                        const uint32_t wd = word_at(k);
+                        if (wd & 0x4000u) {
+                            if (ra < TL_MAX) tl[ra] = k | (rb << 16);
+                            ra += 1;
+                            rb += (wd & 0x8000u) ? 0u : 1u;
+                        }
+                    }
+                }
+                __syncthreads();
+                const bool last_seg = tot.a <= TL_MAX;
+                const uint32_t n_seg = uni(min(tot.a, TL_MAX));
+                const uint32_t t_last = n_seg ? uni(tl[n_seg - 1]) : 0u;
+                const uint32_t last_k = t_last & 0xFFFFu;
+                // An unfinished record after the tile's last terminator (never
+                // with a trailer: the virtual terminator is the last entry).
+                const uint32_t after = n_seg ? last_k + 1u : s0;
+                const bool partial = last_seg && after < n_ent;
+                const uint32_t n_rec = n_seg + (partial ? 1u : 0u);
+                const uint32_t seg_rows = n_seg ? (t_last >> 16) + ((word_at(last_k) & 0x8000u) ? 0u : 1u) : 0u;
+                for (uint32_t r0 = 0; r0 < n_rec; r0 += WG) {
+                    const uint32_t r = r0 + tid;
+                    Rec R{r < n_rec, false, false, 0, 0, 0, 0, 0, 0};
+                    if (R.has) {
+                        R.e0 = r == 0 ? s0 : (tl[r - 1] & 0xFFFFu) + 1u;
+                        if (r < n_seg) {
+                            const uint32_t t = tl[r];
+                            const uint32_t e1 = t & 0xFFFFu;
+                            R.term = true;
+                            R.comment = (word_at(e1) & 0x8000u) != 0u;
+                            R.n = e1 - R.e0 + 1u;
+                            R.row = row_base + (t >> 16);
+                        } else {
+                            R.n = n_ent - R.e0;
+                            R.comment = (word_at(R.e0) & 0x8000u) != 0u;
+                            R.row = row_base + seg_rows;
+                        }
+                        if (R.e0 == 0) {
+                            R.fs0 = pre.last_struct;
+                            R.rs = pre.last_term;
+                            R.ord0 = pre.ord;
+                        } else {
+                            R.fs0 = pos_of(word_at(R.e0 - 1)) + 1u;
+                            R.rs = R.fs0;
+                        }
+                    }
+                    round(R);
+                }
+                row_base += seg_rows;
+                if (last_seg) break;
+                s0 = last_k + 1u;
+                __syncthreads();   // every thread has read tl before the next segment rewrites it
+            }
+        };
+        // The cell of field ordinal `index` of record R: `here` when the field
+        // ends in this tile (its bytes [fs, fe), the record's last field
+        // without a trailing '\r', csv_geo.rs:1056-1060), `missing` when the
+        // record ends before it.
+        auto cell_of = [&](const Rec& R, uint32_t index, bool& here, bool& missing, uint32_t& fs, uint32_t& fe)
+            __attribute__((always_inline)) {
+            const bool live_rec = R.has && !R.comment;
+            const int32_t i = (int32_t)index - (int32_t)R.ord0;
+            here = live_rec && i >= 0 && (uint32_t)i < R.n;
+            missing = live_rec && R.term && i >= (int32_t)R.n;
+            fs = fe = 0;
+            if (here) {
+                const uint32_t k = R.e0 + (uint32_t)i;
+                fe = pos_of(word_at(k));
+                fs = i == 0 ? R.fs0 : pos_of(word_at(k - 1)) + 1u;
+                if (R.term && (uint32_t)i + 1u == R.n && fe > R.rs && src.at(fe - 1) == '\r') fe -= 1;
+            }
+        };
         // Validity of one column for the wave's rows: lanes with a row (RL)
         // hold consecutive rows in lane order, so bit (row - first row) of
-        // the wave's word is the lane's cell.
+        // the wave's word is the lane's cell; words are ORed in by atomics
+        // (the entry point cleared them).
         auto wave_valid = [&](const ColDesc& col, bool vb, uint32_t row, uint64_t RL) __attribute__((always_inline)) {
             if (!RL) return;
             const uint32_t f = (uint32_t)__ffsll((long long)RL) - 1u;
@@ -1404,346 +1492,177 @@ __device__ __forceinline__ void scan_tile(const Params& P, const uint32_t tile, 
             }
         };
 
-        uint32_t s0 = 0, row_base = pre.n_rows;
-        while (true) {
-            // List the segment's terminators: entries [s0, n_ent) in runs of
-            // `per` per thread, counted, scanned, then written by rank.
-            const uint32_t rem = n_ent - s0;
-            const uint32_t per = (rem + WG - 1) / WG;
-            const uint32_t k0 = min(n_ent, s0 + tid * per), k1 = min(n_ent, k0 + per);
-            Scan3 cnt{0, 0, -1};
-            for (uint32_t k = k0; k < k1; ++k) {
-                const uint32_t wd = word_at(k);
-                if (wd & 0x4000u) {
-                    cnt.a += 1;
-                    cnt.b += (wd & 0x8000u) ? 0u : 1u;
-                }
-            }
-            Scan3 tot;
-            const Scan3 ex = block_excl(cnt, sh.scan_sh, tot);
-            {
-                uint32_t ra = ex.a, rb = ex.b;
-                for (uint32_t k = k0; k < k1; ++k) {
-                    const uint32_t wd = word_at(k);
-                    if (wd & 0x4000u) {
-                        if (ra < TL_MAX) tl[ra] = k | (rb << 16);
-                        ra += 1;
-                        rb += (wd & 0x8000u) ? 0u : 1u;
-                    }
-                }
-            }
-            __syncthreads();
-            const bool last_seg = tot.a <= TL_MAX;
-            const uint32_t n_seg = uni(min(tot.a, TL_MAX));
-            const uint32_t t_last = n_seg ? uni(tl[n_seg - 1]) : 0u;
-            const uint32_t last_k = t_last & 0xFFFFu;
-            // An unfinished record after the tile's last terminator (never
-            // with a trailer: the virtual terminator is the last entry).
-            const uint32_t after = n_seg ? last_k + 1u : s0;
-            const bool partial = last_seg && after < n_ent;
-            const uint32_t n_rec = n_seg + (partial ? 1u : 0u);
-            const uint32_t seg_rows = n_seg ? (t_last >> 16) + ((word_at(last_k) & 0x8000u) ? 0u : 1u) : 0u;
-            for (uint32_t r0 = 0; r0 < n_rec; r0 += WG) {
-                const uint32_t r = r0 + tid;
-                const bool has = r < n_rec;
-                bool term = false, comment = false;
-                uint32_t e0 = 0, n = 0, fs0 = 0, rs = 0, ord0 = 0, row = 0;
-                if (has) {
-                    e0 = r == 0 ? s0 : (tl[r - 1] & 0xFFFFu) + 1u;
-                    if (r < n_seg) {
-                        const uint32_t t = tl[r];
-                        const uint32_t e1 = t & 0xFFFFu;
-                        term = true;
-                        comment = (word_at(e1) & 0x8000u) != 0u;
-                        n = e1 - e0 + 1u;
-                        row = row_base + (t >> 16);
-                    } else {
-                        n = n_ent - e0;
-                        comment = (word_at(e0) & 0x8000u) != 0u;
-                        row = row_base + seg_rows;
-                    }
-                    if (e0 == 0) {
-                        fs0 = pre.last_struct;
-                        rs = pre.last_term;
-                        ord0 = pre.ord;
-                    } else {
-                        fs0 = pos_of(word_at(e0 - 1)) + 1u;
-                        rs = fs0;
-                    }
-                }
-                const bool live_rec = has && !comment;
-                if (live_rec && term) records += 1;
-                const bool live = row < rows_cap;
-                const uint64_t RL = IS_COLS ? ballot(live_rec) : 0ULL;
-                for (uint32_t jj = 0; jj < nc; ++jj) {
-                    const uint32_t j = c0 + jj;
-                    const ColDesc& c = P.cols[j];
-                    const int32_t i = (int32_t)c.index - (int32_t)ord0;
-                    const bool here = live_rec && i >= 0 && (uint32_t)i < n;
-                    const bool missing = live_rec && term && i >= (int32_t)n;
-                    uint32_t fs = 0, fe = 0;
+        // ---- STRINGS: arena placement of the string columns ---------------
+        // Per string column, a block scan over each round's cleaned lengths
+        // gives every cell its arena offset (the tile's cells of a column fill
+        // one contiguous range, whose base the slot scan computed). Cells are
+        // written into an LDS stage when the tile's range fits, then stored
+        // out contiguously; offsets and validity go straight to the column.
+        if constexpr (MODE == M_STRINGS) {
+            constexpr uint32_t SSTAGE = 8192;
+            __shared__ uint8_t sstage[SSTAGE];
+            const uint32_t n_cols = P.n_cols;
+            for (uint32_t j = 0; j < n_cols; ++j) {
+                const ColDesc& c = P.cols[j];
+                if (c.type != FALX_COL_STR) continue;
+                int32_t* offs = reinterpret_cast<int32_t*>(c.values);
+                const uint32_t arena = P.str_base[(size_t)tile * P.n_str + c.str_slot];
+                const uint32_t total = P.str_len[(size_t)tile * P.n_str + c.str_slot];
+                const bool gather = total <= SSTAGE;
+                if (tile == 0 && tid == 0) offs[0] = 0;
+                // Fused statistics of a string column among the first
+                // COL_STAT_GROUP: present cells, cleaned bytes, their byte sum.
+                const bool stat_col = P.partials != nullptr && j < (uint32_t)COL_STAT_GROUP;
+                unsigned long long s_cells = 0, s_bytes = 0, s_sum = 0;
+                uint32_t run = 0;   // bytes of this tile's range placed by earlier rounds
+                for_records([&](const Rec& R) {
+                    bool here, missing;
+                    uint32_t fs, fe;
+                    cell_of(R, c.index, here, missing, fs, fe);
+                    const bool raw = here && raw_cell(fs, fe);
+                    const uint32_t L = here ? (raw ? fe - fs : clean_len(fs, fe)) : 0u;
+                    uint32_t rtot;
+                    const uint32_t at = run + block_excl_sum(L, sh.sum_sh, rtot);
+                    const bool live = R.row < rows_cap;
                     if (here) {
-                        const uint32_t k = e0 + (uint32_t)i;
-                        fe = pos_of(word_at(k));
-                        fs = i == 0 ? fs0 : pos_of(word_at(k - 1)) + 1u;
-                        // The record's last field loses a trailing '\r' (csv_geo.rs:1056-1060).
-                        if (term && (uint32_t)i + 1u == n && fe > rs && src.at(fe - 1) == '\r') fe -= 1;
-                    }
-                    // Statistics of COLUMNS cover the rows that fit rows_cap.
-                    const bool stat_ok = with_stats && jj < NACC && (IS_COLS ? live : true);
-                    bool vb = false;
-                    if (c.type == FALX_COL_F64 || c.type == FALX_COL_I64) {
-                        const bool as_f64 = c.type == FALX_COL_F64;
-                        NumVal nv{};
-                        if (here) nv = num_value(fs, fe, as_f64);
-                        CellStat o{false, false, 0, 0, 0, 0.0};
-                        if (as_f64) {
-                            vb = here && nv.f.ok;
-                            if (IS_COLS && (here || missing) && live)
-                                reinterpret_cast<double*>(c.values)[row] = vb ? nv.f.value : 0.0;
-                            if (vb) o = CellStat{true, nv.f.fixed6, f64_to_bits(nv.f.value), 0, nv.f.mantissa, nv.f.value};
+                        // Bytes of rows past rows_cap are not placed: the call
+                        // reports CAPACITY then.
+                        auto put = [&](uint32_t q, uint8_t b) {
+                            if (!live) return;
+                            if (gather) sstage[q] = b;
+                            else if ((uint64_t)arena + q < c.data_cap) c.data[arena + q] = b;
+                        };
+                        if (raw) {
+                            for (uint32_t k = 0; k < L; ++k) {
+                                const uint8_t b = src.at(fs + k);
+                                put(at + k, b);
+                                s_sum += b;
+                            }
                         } else {
-                            vb = here && nv.iok;
-                            if (IS_COLS && (here || missing) && live)
-                                reinterpret_cast<long long*>(c.values)[row] = vb ? nv.iv : 0;
-                            if (vb) o = CellStat{true, false, (unsigned long long)nv.iv, 0, 0, (double)nv.iv};
+                            uint32_t q = at;
+                            clean_cell(src, fs, fe, cd, [&](uint8_t b) {
+                                put(q++, b);
+                                s_sum += b;
+                            });
                         }
-                        if (stat_ok) acc_add(acc, jj, o);
-                    } else if (c.type == FALX_COL_STR) {
-                        if (IS_COLS) {
-                            // Cleaned byte counts only; the STRINGS pass places
-                            // the bytes and writes the validity.
-                            uint32_t len = here ? clean_len(fs, fe) : 0u;
-                            len = row_sum16(len);
-                            const uint32_t wsum = lane_value(len, 0) + lane_value(len, 16) + lane_value(len, 32) +
-                                                  lane_value(len, 48);
-                            if (ln == 0 && wsum) atomicAdd(&sh.str_len[c.str_slot], wsum);
-                        } else if (here) {
-                            unsigned long long cs = 0;
-                            const uint32_t nb = clean_cell(src, fs, fe, cd, [&](uint8_t b) { cs += b; });
-                            if (stat_ok) acc_add(acc, jj, CellStat{true, false, cs, nb, 0, 0.0});
-                        }
-                    } else {
-                        vb = here && fe != fs;
-                        if (IS_COLS && (here || missing) && live)
-                            reinterpret_cast<uint2*>(c.values)[row] = vb ? make_uint2(fs, fe) : make_uint2(0, 0);
-                        if (vb && stat_ok) {
-                            unsigned long long cs = 0;
-                            for (uint32_t q = fs; q < fe; ++q) cs += src.at(q);
-                            acc_add(acc, jj, CellStat{true, false, cs, fe - fs, 0, 0.0});
-                        }
+                        s_cells += 1;
+                        s_bytes += L;
                     }
-                    if (IS_COLS && c.type != FALX_COL_STR) wave_valid(c, vb && live, row, RL);
+                    if (R.term && !R.comment && R.has && live) offs[R.row + 1] = (int32_t)(arena + at + L);
+                    wave_valid(c, here && live, R.row, ballot(R.has && !R.comment));
+                    run += rtot;
+                });
+                if (gather) {
+                    __syncthreads();
+                    const uint64_t end = min((uint64_t)arena + total, c.data_cap);
+                    for (uint64_t q = (uint64_t)arena + tid; q < end; q += WG) c.data[q] = sstage[q - arena];
+                    __syncthreads();
+                }
+                if (stat_col) {
+                    // The column's StatPart of this tile (the COLUMNS pass
+                    // wrote zeros there), in the fixed reduction order.
+                    const unsigned long long v0 = block_reduce_sum(s_cells, sh.usum_sh);
+                    const unsigned long long v1 = block_reduce_sum(s_sum, sh.usum_sh);
+                    const unsigned long long v2 = block_reduce_sum(s_bytes, sh.usum_sh);
+                    if (tid == 0) {
+                        StatPart& o = reinterpret_cast<StatPart*>(P.partials)[(size_t)tile * P.n_cols + j];
+                        o.values = v0;
+                        o.checksum = v1;
+                        o.bytes = v2;
+                        o.fixed6_sum = 0;
+                        o.fixed6_cells = 0;
+                        o.sum = 0.0;
+                    }
                 }
             }
-            row_base += seg_rows;
-            if (last_seg) break;
-            s0 = last_k + 1u;
-            __syncthreads();   // every thread has read tl before the next segment rewrites it
+            return;
         }
+
+        // ---- COLUMNS and STATS: one record per thread ----------------------
+        // A thread takes one record; column c's cell is its entry
+        // e0 + (c.index - ord0), found by index, so the lanes of a wave run
+        // the same steps for each column.
+        constexpr uint32_t NACC = 2;
+        static_assert(NACC == STAT_GROUP && NACC == COL_STAT_GROUP, "accumulators per pass");
+        constexpr bool IS_COLS = MODE == M_COLUMNS;
+        const uint32_t c0 = IS_COLS ? 0u : P.col_group;
+        const uint32_t nc = IS_COLS ? P.n_cols : min((uint32_t)STAT_GROUP, P.n_cols - c0);
+        // Known at compile time in the fast forms, which then carry no
+        // statistics registers when there are none.
+        const bool with_stats = !IS_COLS                         ? true
+                                : KMODE == M_COLUMNS_FAST       ? false
+                                : KMODE == M_COLUMNS_FAST_STATS ? true
+                                                                : P.partials != nullptr;
+        const bool trailer_row_live = trailer && !trailer_comment;
+        Acc acc[NACC];
+#pragma unroll
+        for (uint32_t j = 0; j < NACC; ++j) acc[j] = Acc{0, 0, 0, 0, 0, 0.0};
+        unsigned long long records = 0;
+        if (IS_COLS && tid < FALX_MAX_COLUMNS) sh.str_len[tid] = 0;   // read after the segment scans' barriers
+        for_records([&](const Rec& R) {
+            const bool live_rec = R.has && !R.comment;
+            if (live_rec && R.term) records += 1;
+            const uint32_t row = R.row;
+            const bool live = row < rows_cap;
+            const uint64_t RL = IS_COLS ? ballot(live_rec) : 0ULL;
+            for (uint32_t jj = 0; jj < nc; ++jj) {
+                const uint32_t j = c0 + jj;
+                const ColDesc& c = P.cols[j];
+                bool here, missing;
+                uint32_t fs, fe;
+                cell_of(R, c.index, here, missing, fs, fe);
+                // Statistics of COLUMNS cover the rows that fit rows_cap.
+                const bool stat_ok = with_stats && jj < NACC && (IS_COLS ? live : true);
+                bool vb = false;
+                if (c.type == FALX_COL_F64 || c.type == FALX_COL_I64) {
+                    const bool as_f64 = c.type == FALX_COL_F64;
+                    NumVal nv{};
+                    if (here) nv = num_value(fs, fe, as_f64);
+                    CellStat o{false, false, 0, 0, 0, 0.0};
+                    if (as_f64) {
+                        vb = here && nv.f.ok;
+                        if (IS_COLS && (here || missing) && live)
+                            reinterpret_cast<double*>(c.values)[row] = vb ? nv.f.value : 0.0;
+                        if (vb) o = CellStat{true, nv.f.fixed6, f64_to_bits(nv.f.value), 0, nv.f.mantissa, nv.f.value};
+                    } else {
+                        vb = here && nv.iok;
+                        if (IS_COLS && (here || missing) && live)
+                            reinterpret_cast<long long*>(c.values)[row] = vb ? nv.iv : 0;
+                        if (vb) o = CellStat{true, false, (unsigned long long)nv.iv, 0, 0, (double)nv.iv};
+                    }
+                    if (stat_ok) acc_add(acc, jj, o);
+                } else if (c.type == FALX_COL_STR) {
+                    if (IS_COLS) {
+                        // Cleaned byte counts only; the STRINGS pass places
+                        // the bytes and writes the validity.
+                        uint32_t len = here ? clean_len(fs, fe) : 0u;
+                        len = row_sum16(len);
+                        const uint32_t wsum = lane_value(len, 0) + lane_value(len, 16) + lane_value(len, 32) +
+                                              lane_value(len, 48);
+                        if (ln == 0 && wsum) atomicAdd(&sh.str_len[c.str_slot], wsum);
+                    } else if (here) {
+                        unsigned long long cs = 0;
+                        const uint32_t nb = clean_cell(src, fs, fe, cd, [&](uint8_t b) { cs += b; });
+                        if (stat_ok) acc_add(acc, jj, CellStat{true, false, cs, nb, 0, 0.0});
+                    }
+                } else {
+                    vb = here && fe != fs;
+                    if (IS_COLS && (here || missing) && live)
+                        reinterpret_cast<uint2*>(c.values)[row] = vb ? make_uint2(fs, fe) : make_uint2(0, 0);
+                    if (vb && stat_ok) {
+                        unsigned long long cs = 0;
+                        for (uint32_t q = fs; q < fe; ++q) cs += src.at(q);
+                        acc_add(acc, jj, CellStat{true, false, cs, fe - fs, 0, 0.0});
+                    }
+                }
+                if (IS_COLS && c.type != FALX_COL_STR) wave_valid(c, vb && live, row, RL);
+            }
+        });
         if (IS_COLS && tid == 0 && last_tile) P.result[R_N_ROWS] = inc.n_rows + (trailer_row_live ? 1 : 0);
         __syncthreads();
         if (FAST && tid == 0) P.fixup[tile] = sh.fix_bad;
         if (with_stats) acc_publish(acc, c0, min(NACC, nc), records);
         if (IS_COLS && P.n_str > 0 && tid < P.n_str) P.str_len[(size_t)tile * P.n_str + tid] = sh.str_len[tid];
-        return;
-    }
-
-    // ---- validity bits for the column passes --------------------------------
-    // Bits of the rows this tile's cells can touch, [pre.n_rows, inc.n_rows],
-    // are staged in LDS (bit i of column j's region is row row0 + i) and
-    // written out once: words wholly inside the tile with plain stores, the
-    // two boundary words (shared with the neighbouring tiles' records) with
-    // atomics. Tiles whose span does not fit the stage use global atomics
-    // per cell.
-    constexpr uint32_t VSTAGE = 512;
-    __shared__ uint32_t vbits[VSTAGE];
-    const uint64_t rows_cap = P.rows_cap;
-    const uint32_t n_cols = P.n_cols;
-    const uint32_t row0 = pre.n_rows & ~63u;
-    const uint32_t words64 = (inc.n_rows - row0) / 64 + 1;
-    const uint32_t wpc = 2 * words64;
-    const bool staged = n_cols * wpc <= VSTAGE;
-    const uint64_t vwords = (rows_cap + 63) / 64;
-    auto vstage_clear = [&]() {
-        for (uint32_t i = tid; i < VSTAGE; i += WG) vbits[i] = 0;
-    };
-    auto set_valid = [&](uint32_t j, const ColDesc& c, uint32_t row) {
-        if (staged) {
-            const uint32_t r = row - row0;
-            atomicOr(&vbits[j * wpc + (r >> 5)], 1u << (r & 31));
-        } else {
-            atomicOr(&c.valid[row >> 6], 1ULL << (row & 63));
-        }
-    };
-    // After a barrier: writes the staged words of the string columns
-    // (strings) or of the others.
-    auto flush_valid = [&](bool strings) {
-        if (!staged) return;
-        for (uint32_t j = 0; j < n_cols; ++j) {
-            const ColDesc& c = P.cols[j];
-            if ((c.type == FALX_COL_STR) != strings) continue;
-            for (uint32_t wi = tid; wi < words64; wi += WG) {
-                const uint64_t gw = (uint64_t)(row0 >> 6) + wi;
-                if (gw >= vwords) continue;
-                const unsigned long long v =
-                    vbits[j * wpc + 2 * wi] | ((unsigned long long)vbits[j * wpc + 2 * wi + 1] << 32);
-                if (wi == 0 || wi == words64 - 1) {
-                    if (v) atomicOr(&c.valid[gw], v);
-                } else {
-                    c.valid[gw] = v;
-                }
-            }
-        }
-    };
-
-    // ---- M_STRINGS: arena placement of the string columns ------------------
-    if (MODE == M_STRINGS) {
-        vstage_clear();
-        __syncthreads();
-        // E2: ordered arena placement per string column: the walk's weight
-        // is the cell's cleaned length, so each cell learns its arena offset.
-        // A tile's cells of one column fill one contiguous arena range; when
-        // it fits, the bytes are gathered in LDS and written out with
-        // contiguous stores instead of per-cell byte stores.
-        constexpr uint32_t SSTAGE = 8192;
-        __shared__ uint8_t sstage[SSTAGE];
-        // Per list entry of a walk round: end of its output range and
-        // output-to-source offset of its bytes. Rounds of up to four entries
-        // per thread fit; the table's size keeps four workgroups per CU.
-        constexpr uint32_t RT_N = WG * 4;
-        constexpr int32_t RAW_SKIP = -32768;
-        __shared__ uint16_t rt_end[RT_N];
-        __shared__ int16_t rt_delta[RT_N];
-        for (uint32_t j = 0; j < n_cols; ++j) {
-            const ColDesc& c = P.cols[j];
-            if (c.type != FALX_COL_STR) continue;
-            int32_t* offs = reinterpret_cast<int32_t*>(c.values);
-            const uint32_t arena = P.str_base[(size_t)tile * P.n_str + c.str_slot];
-            const uint32_t total = P.str_len[(size_t)tile * P.n_str + c.str_slot];
-            const bool gather = total <= SSTAGE;
-            // Byte `b` of the tile's range: to the LDS stage or to the arena.
-            auto put = [&](uint64_t q, uint8_t b) {
-                if (gather) sstage[q - arena] = b;
-                else if (q < c.data_cap) c.data[q] = b;
-            };
-            if (tile == 0 && tid == 0) offs[0] = 0;
-            // Fused statistics of a string column among the first
-            // COL_STAT_GROUP: present cells, cleaned bytes, their byte sum.
-            const bool stat_col = P.partials != nullptr && j < (uint32_t)COL_STAT_GROUP;
-            unsigned long long s_cells = 0, s_bytes = 0, s_sum = 0;
-            // Gathered in LDS, a round's cells are placed in two steps. The
-            // visit records, per list entry, the end of its output range and
-            // for an unquoted cell inside the tile the offset from output to
-            // source byte; it cleans the other cells itself. Then all threads
-            // copy the round's raw bytes, each a contiguous run of the output,
-            // finding its first cell by binary search over the range ends:
-            // work per byte, not per cell, so long and short cells share it.
-            uint32_t round_lo = 0;
-            // Worth it for long cells: per-lane copies run as long as the
-            // wave's longest cell, the shared copy costs two barriers and a
-            // search per round. The tile's mean cell length decides.
-            const uint32_t t_rows = sh.inc.n_rows - pre.n_rows + 1u;
-            const bool coop = gather && WG * walk_items() <= RT_N && total > 13u * t_rows;
-            walk_w(
-                Weighted{},
-                [&](const Entry& e) -> uint32_t {
-                    return (!e.flag && c.index == e.ord) ? clean_len(e.fs, field_end(e)) : 0u;
-                },
-                [&](const Entry& e, uint32_t, uint32_t before) -> uint32_t {
-                    if (!e.valid) return 0u;
-                    const bool cellhere = !e.flag && c.index == e.ord;
-                    const bool live = e.row < rows_cap;
-                    const uint32_t at = arena + before;
-                    const uint32_t to = cellhere ? field_end(e) : e.fs;
-                    uint32_t n = 0;
-                    int32_t delta = RAW_SKIP;
-                    if (cellhere) {
-                        const bool in_tile = e.fs >= tile_start && to <= tile_start + tile_len;
-                        if (raw_cell(e.fs, to)) {
-                            n = to - e.fs;
-                            if (coop && in_tile && live) {
-                                delta = (int32_t)(e.fs - tile_start) - (int32_t)before;
-                            } else if (live || stat_col) {
-                                for (uint32_t k = 0; k < n; ++k) {
-                                    const uint8_t b = src.at(e.fs + k);
-                                    if (live) put((uint64_t)at + k, b);
-                                    s_sum += b;
-                                }
-                            }
-                        } else {
-                            uint64_t q = at;
-                            n = clean_cell(src, e.fs, to, cd, [&](uint8_t b) {
-                                if (live) put(q, b);
-                                s_sum += b;
-                                ++q;
-                            });
-                        }
-                        if (live) set_valid(j, c, e.row);
-                        s_cells += 1;
-                        s_bytes += n;
-                    }
-                    if (coop) {
-                        rt_end[e.k & (RT_N - 1)] = (uint16_t)(before + n);
-                        rt_delta[e.k & (RT_N - 1)] = (int16_t)delta;
-                    }
-                    if (e.is_term && !e.flag && live) offs[e.row + 1] = (int32_t)(at + n);
-                    return n;
-                },
-                trailer_word,
-                [&](uint32_t base, uint32_t n_round) {
-                    if (!coop) return;
-                    __syncthreads();
-                    const uint32_t hi = rt_end[(base + n_round - 1) & (RT_N - 1)];
-                    const uint32_t per = (hi - round_lo + WG - 1) / WG;
-                    const uint32_t q0 = round_lo + tid * per, q1 = min(q0 + per, hi);
-                    if (q0 < q1) {
-                        uint32_t lo = 0, up = n_round - 1;   // first entry whose range ends past q0
-                        while (lo < up) {
-                            const uint32_t mid = (lo + up) >> 1;
-                            if (rt_end[(base + mid) & (RT_N - 1)] > q0) up = mid; else lo = mid + 1;
-                        }
-                        uint32_t i = lo;
-                        for (uint32_t q = q0; q < q1; ++q) {
-                            while (rt_end[(base + i) & (RT_N - 1)] <= q) ++i;
-                            const int32_t d = rt_delta[(base + i) & (RT_N - 1)];
-                            if (d != RAW_SKIP) {
-                                const uint8_t b = tb[(int32_t)q + d];
-                                sstage[q] = b;
-                                s_sum += b;
-                            }
-                        }
-                    }
-                    round_lo = hi;
-                    __syncthreads();
-                });
-            if (gather) {
-                // Cells of rows past rows_cap were not staged; the call
-                // reports CAPACITY then, so those bytes need no value.
-                __syncthreads();
-                const uint64_t end = min((uint64_t)arena + total, c.data_cap);
-                for (uint64_t q = (uint64_t)arena + tid; q < end; q += WG) c.data[q] = sstage[q - arena];
-                __syncthreads();
-            }
-            if (stat_col) {
-                // The column's StatPart of this tile (the COLUMNS pass wrote
-                // zeros there), in the fixed reduction order.
-                const unsigned long long v0 = block_reduce_sum(s_cells, sh.usum_sh);
-                const unsigned long long v1 = block_reduce_sum(s_sum, sh.usum_sh);
-                const unsigned long long v2 = block_reduce_sum(s_bytes, sh.usum_sh);
-                if (tid == 0) {
-                    StatPart& o = reinterpret_cast<StatPart*>(P.partials)[(size_t)tile * P.n_cols + j];
-                    o.values = v0;
-                    o.checksum = v1;
-                    o.bytes = v2;
-                    o.fixed6_sum = 0;
-                    o.fixed6_cells = 0;
-                    o.sum = 0.0;
-                }
-            }
-        }
-        __syncthreads();
-        flush_valid(true);
     }
 }
 
