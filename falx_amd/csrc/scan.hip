@@ -1501,6 +1501,12 @@ __device__ __forceinline__ void scan_tile(const Params& P, const uint32_t tile, 
         if constexpr (MODE == M_STRINGS) {
             constexpr uint32_t SSTAGE = 8192;
             __shared__ uint8_t sstage[SSTAGE];
+            // Per record of a round: end of its cell in the tile's range, and
+            // the offset from output position to LDS source byte of a raw cell
+            // copied by all threads together (RAW_SKIP: its owner wrote it).
+            constexpr int32_t RAW_SKIP = -32768;
+            __shared__ uint16_t rt_end[WG];
+            __shared__ int16_t rt_delta[WG];
             const uint32_t n_cols = P.n_cols;
             for (uint32_t j = 0; j < n_cols; ++j) {
                 const ColDesc& c = P.cols[j];
@@ -1509,6 +1515,9 @@ __device__ __forceinline__ void scan_tile(const Params& P, const uint32_t tile, 
                 const uint32_t arena = P.str_base[(size_t)tile * P.n_str + c.str_slot];
                 const uint32_t total = P.str_len[(size_t)tile * P.n_str + c.str_slot];
                 const bool gather = total <= SSTAGE;
+                // The shared copy pays for its two barriers per round when the
+                // tile's cells are long: its mean cleaned length decides.
+                const bool coop_tile = gather && total > 13u * (inc.n_rows - pre.n_rows + 1u);
                 if (tile == 0 && tid == 0) offs[0] = 0;
                 // Fused statistics of a string column among the first
                 // COL_STAT_GROUP: present cells, cleaned bytes, their byte sum.
@@ -1524,7 +1533,14 @@ __device__ __forceinline__ void scan_tile(const Params& P, const uint32_t tile, 
                     uint32_t rtot;
                     const uint32_t at = run + block_excl_sum(L, sh.sum_sh, rtot);
                     const bool live = R.row < rows_cap;
-                    if (here) {
+                    // A live raw cell inside the tile's LDS copy joins the
+                    // shared copy below: work per byte, not per longest cell.
+                    const bool coop = coop_tile && raw && live && fs >= lds_lo && fe <= lds_hi;
+                    if (coop_tile) {
+                        rt_end[tid] = (uint16_t)(at + L);
+                        rt_delta[tid] = coop ? (int16_t)((int32_t)(fs - tile_start + PRE) - (int32_t)at) : (int16_t)RAW_SKIP;
+                    }
+                    if (here && !coop) {
                         // Bytes of rows past rows_cap are not placed: the call
                         // reports CAPACITY then.
                         auto put = [&](uint32_t q, uint8_t b) {
@@ -1545,11 +1561,39 @@ __device__ __forceinline__ void scan_tile(const Params& P, const uint32_t tile, 
                                 s_sum += b;
                             });
                         }
+                    }
+                    if (here) {
                         s_cells += 1;
                         s_bytes += L;
                     }
                     if (R.term && !R.comment && R.has && live) offs[R.row + 1] = (int32_t)(arena + at + L);
                     wave_valid(c, here && live, R.row, ballot(R.has && !R.comment));
+                    if (coop_tile) {
+                        // The round's raw cells: each thread copies a run of
+                        // the round's output range, finding its first cell by
+                        // binary search over the cells' ends.
+                        __syncthreads();
+                        const uint32_t per = (rtot + WG - 1) / WG;
+                        const uint32_t q0 = run + tid * per, q1 = min(q0 + per, run + rtot);
+                        if (q0 < q1) {
+                            uint32_t lo = 0, up = WG - 1;   // first record whose cell ends past q0
+                            while (lo < up) {
+                                const uint32_t mid = (lo + up) >> 1;
+                                if (rt_end[mid] > q0) up = mid; else lo = mid + 1;
+                            }
+                            uint32_t i = lo;
+                            for (uint32_t q = q0; q < q1; ++q) {
+                                while (rt_end[i] <= q) ++i;
+                                const int32_t d = rt_delta[i];
+                                if (d != RAW_SKIP) {
+                                    const uint8_t b = sh.lds_bytes[(int32_t)q + d];
+                                    sstage[q] = b;
+                                    s_sum += b;
+                                }
+                            }
+                        }
+                        __syncthreads();
+                    }
                     run += rtot;
                 });
                 if (gather) {
