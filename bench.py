@@ -63,9 +63,11 @@ def time_steps(fn, steps: int, warmup: int, ctx):
     return (t1 - t0) * 1000.0 / steps, sum(kms) / len(kms)
 
 
-def cpu_time(fn, runs: int):
-    """Median seconds of `runs` calls of fn (one untimed warm-up)."""
-    fn()
+def cpu_time(fn, runs: int, warmups: int = 2):
+    """Median seconds of `runs` calls of fn after `warmups` untimed calls
+    (the reference's bench_columns protocol: 2 warm-ups, 9 runs, median)."""
+    for _ in range(warmups):
+        fn()
     times = []
     for _ in range(runs):
         t0 = time.perf_counter()
@@ -73,6 +75,23 @@ def cpu_time(fn, runs: int):
         times.append(time.perf_counter() - t0)
     times.sort()
     return times[len(times) // 2]
+
+
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_threads(requested: int) -> int:
+    """Host threads for the CPU legs: the box's CPU share for one GPU (16),
+    or fewer when the host has fewer."""
+    return requested or min(os.cpu_count() or 1, 16)
 
 
 def cpu_columns(spec, data, rows: int, data_cap: int, threads: int):
@@ -90,10 +109,11 @@ def cpu_columns(spec, data, rows: int, data_cap: int, threads: int):
     return run
 
 
-def cpu_baseline(data, rows: int, runs: int, threads: int):
+def cpu_baseline(data, rows: int, runs: int, threads: int, warmups: int = 2):
     """The oracle's restatement of parse_columns_par (csv_geo.rs:1167-1284)
     followed by the sequential sums of the two columns (bench_columns.rs:
-    151-179), at `threads` host threads, on the same input."""
+    151-179), at `threads` host threads, on the same input: `warmups`
+    untimed runs, then `runs` timed ones (median and best)."""
     import numpy as np
 
     from falx_amd import spec as S
@@ -103,18 +123,29 @@ def cpu_baseline(data, rows: int, runs: int, threads: int):
     lib = O.lib()
     times = []
     lat = lon = None
-    for _ in range(runs):
+    for it in range(warmups + runs):
         cols, keep = O.alloc_columns(spec, rows, 16)
         cs = spec.to_c()
         t0 = time.perf_counter()
         st = lib.ora_parse_columns(C.byref(cs), C.c_void_p(data.ctypes.data), data.size, threads, C.byref(cols))
         lat, lon = keep[0][0][:rows], keep[1][0][:rows]
         s_lat, s_lon = float(np.cumsum(lat)[-1]), float(np.cumsum(lon)[-1])
-        times.append(time.perf_counter() - t0)
+        if it >= warmups:
+            times.append(time.perf_counter() - t0)
         assert st == 0 and cols.rows == rows
     times.sort()
     med = times[len(times) // 2]
     return data.size / GIB / med, med, times, (s_lat, s_lon)
+
+
+def sum_errors(name: str, gpu_tree: float, ref_fold: float, exact) -> dict:
+    """FP64 sum of a column: the GPU's blocked tree and the reference's
+    sequential fold, each against the exact rational sum of the decimal
+    values written (fixed-6 mantissas / 10^6)."""
+    return {f"{name}_sum_gpu_tree": gpu_tree, f"{name}_sum_ref_fold": ref_fold, f"{name}_exact": float(exact),
+            f"{name}_err_gpu_vs_exact": float(abs(Fraction(gpu_tree) - exact)),
+            f"{name}_err_fold_vs_exact": float(abs(Fraction(ref_fold) - exact)),
+            f"{name}_err_gpu_vs_fold": abs(gpu_tree - ref_fold)}
 
 
 def served_stream(ctx) -> dict:
@@ -158,7 +189,7 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--size", type=int, default=GIB)
-    ap.add_argument("--cpu-runs", type=int, default=3)
+    ap.add_argument("--cpu-runs", type=int, default=9)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-lanes", action="store_true")
@@ -234,6 +265,9 @@ def main() -> int:
                 ab = n_in + 4 * n_struct[0]
             lanes[name] = {"gib_s": round(n_in / GIB / (lms / 1000.0), 2), "ms_per_step": round(lms, 3),
                            "kernel_ms": round(lkms, 3), "hbm_frac": round(ab / (lkms / 1000.0) / 1e9 / HBM_PEAK_GBS, 3)}
+            if name == "index_structurals":
+                # Closed form: three separators and a newline per record.
+                lanes[name]["verified"] = n_struct[0] == 4 * rows
         # end-to-end from pinned host memory: H2D + the headline step
         hp = C.c_void_p()
         gpu._check("falx_host_alloc_pinned", gpu.load().falx_host_alloc_pinned(ctx.h, n_in, C.byref(hp)))
@@ -248,7 +282,7 @@ def main() -> int:
         gpu.load().falx_host_free_pinned(ctx.h, hp)
         if not args.no_cpu:
             from oracle import oracle as O
-            thr = args.cpu_threads or min(os.cpu_count() or 1, 16)
+            thr = cpu_threads(args.cpu_threads)
             cpu_lane = {
                 "latlon_columns": cpu_columns(spec, data, rows, 16, thr),
                 "fused_count_sum": lambda: O.csv_stats(spec, data, thr),
@@ -284,6 +318,11 @@ def main() -> int:
             lanes["ndjson_id_score_sum"]["cpu_gib_s"] = round(
                 nd_n / GIB / cpu_time(lambda: O.ndjson_id_score(nd, thr), args.cpu_runs), 3)
             lanes["ndjson_id_score_sum"]["cpu_threads"] = thr
+            # Score sum: the GPU tree and the reference's sequential fold
+            # (the oracle at one thread) against the exact decimal sum.
+            fold = O.ndjson_id_score(nd, 1)["score_sum"]
+            lanes["ndjson_id_score_sum"]["sums"] = sum_errors(
+                "score", r["score_sum"], fold, Fraction(nd_truth["score_fixed6_sum"], 10**6))
         nd_buf.free()
         del nd
 
@@ -293,8 +332,8 @@ def main() -> int:
         hs_n = int(hs.size)
         hs_buf = ctx.upload(hs)
         hspec = S.HASH_TYPED
-        hs_rows = int(hs_truth["records"]) + 1
-        hcols, _hk = ctx.alloc_columns(hspec, hs_rows, hs_n)
+        hs_rows = int(hs_truth["records"])
+        hcols, hkeep = ctx.alloc_columns(hspec, hs_rows, hs_n)
         hidx = ctx.alloc(4 * (hs_n // 4 + 64))
         hs_struct = [0]
 
@@ -314,7 +353,10 @@ def main() -> int:
             if name == "csv_hash_index_structurals":
                 ab = hs_n + 4 * hs_struct[0]
             elif ab is None:
-                ab = hs_n + 16 * hcols.rows + 4 * (hcols.rows + 1) + 3 * 8 * ((hcols.rows + 63) // 64)
+                # input + i64 and f64 values + string offsets and arena +
+                # three validity bitmaps
+                hr = int(hcols.rows)
+                ab = (hs_n + 16 * hr + 4 * (hr + 1) + int(hcols.cols[2].data_len) + 3 * 8 * ((hr + 63) // 64))
             lanes[name] = {"gib_s": round(hs_n / GIB / (lms / 1000.0), 2), "ms_per_step": round(lms, 3),
                            "kernel_ms": round(lkms, 3),
                            "hbm_frac": round(ab / (lkms / 1000.0) / 1e9 / HBM_PEAK_GBS, 3), "bytes": hs_n}
@@ -322,8 +364,18 @@ def main() -> int:
                 log(f"timing CPU oracle lane {name}")
                 lanes[name]["cpu_gib_s"] = round(hs_n / GIB / cpu_time(cfn, args.cpu_runs), 3)
                 lanes[name]["cpu_threads"] = thr
+        # Closed forms of the generator: the row count and the wrapping sum
+        # of the i64 column; the index and field bytes equal the oracle's.
+        i64_col = hkeep[0][0].download(np.int64, int(hcols.rows))
         lanes["csv_hash_typed_columns"]["rows"] = int(hcols.rows)
-        lanes["csv_hash_typed_columns"]["verified"] = int(hcols.rows) == hs_rows - 1 or int(hcols.rows) == hs_rows
+        lanes["csv_hash_typed_columns"]["verified"] = (
+            int(hcols.rows) == hs_rows and int(i64_col.view(np.uint64).sum(dtype=np.uint64)) ==
+            hs_truth["i64_sum"] % 2**64)
+        del i64_col
+        if not args.no_cpu:
+            lanes["csv_hash_index_structurals"]["verified"] = hs_struct[0] == O.index_structurals(hspec, hs, thr).size
+            lanes["csv_hash_field_bytes"]["verified"] = (ctx.field_bytes(hspec, hs_buf, hs_n) ==
+                                                         O.field_bytes(hspec, hs, thr))
 
     served = None
     if not args.no_served:
@@ -332,27 +384,27 @@ def main() -> int:
     cpu = None
     sums = {}
     if not args.no_cpu:
-        threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
+        threads = cpu_threads(args.cpu_threads)
         log(f"timing CPU oracle baseline at {threads} threads")
         gibs, med, times, (s_lat, s_lon) = cpu_baseline(data, rows, args.cpu_runs, threads)
-        # one thread on a 128 MiB prefix of the same input
-        pre = data[: min(n_in, 128 << 20)]
+        # one thread on a 256 MiB prefix of the same input (1 warm-up, 3 runs)
+        pre = data[: min(n_in, 256 << 20)]
         from oracle import oracle as O1
-        one = pre.size / GIB / cpu_time(cpu_columns(spec, pre, rows, 16, 1), 1)
+        one = pre.size / GIB / cpu_time(cpu_columns(spec, pre, rows, 16, 1), 3, warmups=1)
         cpu = {"value": round(gibs, 3), "unit": "GiB/s", "cores": threads, "kind": "port",
-               "sample": f"the same {n_in} B input, {args.cpu_runs} runs of parse_columns_par "
-                         f"(lat/lon) + sequential column sums, median {med * 1000:.0f} ms",
+               "sample": f"the same {n_in} B input, 2 warm-ups then {args.cpu_runs} runs of parse_columns_par "
+                         f"(lat/lon) + sequential column sums: median {med * 1000:.0f} ms, "
+                         f"best {times[0] * 1000:.0f} ms",
+               "best_gib_s": round(n_in / GIB / times[0], 3),
                "one_thread_gib_s": round(one, 3),
-               "one_thread_sample": f"first {pre.size} B of the same input, parse_columns at 1 thread",
+               "one_thread_sample": f"first {pre.size} B of the same input, parse_columns at 1 thread, "
+                                    f"1 warm-up then 3 runs, median",
+               "cpu_model": cpu_model(), "host_cpus": os.cpu_count(),
+               "threads_note": "the box's CPU share for one GPU is 16 threads",
                "isa": O1.isa()}
-        exact_lat = Fraction(truth["lat_fixed6_sum"], 10**6)
-        sums = {
-            "lat_sum_gpu_tree": st["cols"][0]["sum"], "lat_sum_ref_fold": s_lat,
-            "lat_exact": float(exact_lat),
-            "lat_err_gpu_vs_exact": float(abs(Fraction(st["cols"][0]["sum"]) - exact_lat)),
-            "lat_err_fold_vs_exact": float(abs(Fraction(s_lat) - exact_lat)),
-            "lat_err_gpu_vs_fold": abs(st["cols"][0]["sum"] - s_lat),
-        }
+        sums = {}
+        sums.update(sum_errors("lat", st["cols"][0]["sum"], s_lat, Fraction(truth["lat_fixed6_sum"], 10**6)))
+        sums.update(sum_errors("lon", st["cols"][1]["sum"], s_lon, Fraction(truth["lon_fixed6_sum"], 10**6)))
 
     out = {
         "metric": "input GiB/s (1 GiB CSV cols + fused sum), % HBM peak; batched p99 latency",

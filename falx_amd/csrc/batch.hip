@@ -595,7 +595,6 @@ falx_status falx_batch_wait(falx_batch* b, falx_result* results) {
         const char* rec = b->h_res + (size_t)D.res * RES_STRIDE;
         const unsigned long long* slots = reinterpret_cast<const unsigned long long*>(rec + RES_SLOTS_OFF);
         falx_status st = FALX_OK;
-        if (r.op != FALX_OP_NDJSON_ID_SCORE && slots[R_ERROR]) st = FALX_E_DEVICE;
         switch (r.op) {
         case FALX_OP_INDEX:
             R.n_index = slots[R_N_STRUCT];

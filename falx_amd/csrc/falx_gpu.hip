@@ -1,6 +1,6 @@
 // falx_gpu.hip — C ABI of include/falx_gpu.h over the gfx950 kernels in
 // scan.hip. One context = one device + one HIP stream + reusable scratch
-// (look-back state, per-tile partials, result slots).
+// (tile summaries and prefixes, per-tile partials, result slots).
 #include "scan.hip"
 
 #include <algorithm>
@@ -22,7 +22,6 @@ struct falx_ctx {
     size_t part_cap = 0;
     void* d_red = nullptr;       // reduction outputs (StatPart[32] + extras)
     bool timed = false;
-    uint64_t err_flags = 0, err_tile = 0;  // kernel-side detail of the last E_DEVICE
 };
 
 namespace {
@@ -229,7 +228,6 @@ hipError_t fill_u64(hipStream_t s, unsigned long long* p, uint64_t n, unsigned l
 
 falx_status begin(falx_ctx* c) {
     tl_hip_error = 0;
-    c->err_flags = c->err_tile = 0;
     (void)hipGetLastError();  // launches below report their own errors only
     HIP_TRY(hipSetDevice(c->device));
     HIP_TRY(fill_u64(c->stream, c->d_result, R_SLOTS, 0ULL));
@@ -257,11 +255,6 @@ falx_status finish(falx_ctx* c) {
     float ms = 0;
     HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
     c->last_ms = ms;
-    if (c->h_result[R_ERROR]) {
-        c->err_flags = c->h_result[R_ERROR];
-        c->err_tile = c->h_result[R_ERR_TILE];
-        return FALX_E_DEVICE;
-    }
     return FALX_OK;
 }
 
@@ -335,11 +328,9 @@ void* falx_ctx_stream(falx_ctx* c) { return c ? (void*)c->stream : nullptr; }
 
 double falx_ctx_last_kernel_ms(falx_ctx* c) { return c ? (double)c->last_ms : 0.0; }
 
-falx_status falx_ctx_last_error(const falx_ctx* c, int32_t* hip_error, uint64_t* device_flags, uint64_t* tile) {
+falx_status falx_ctx_last_error(const falx_ctx* c, int32_t* hip_error) {
     if (!c) return FALX_E_INVALID;
     if (hip_error) *hip_error = tl_hip_error;
-    if (device_flags) *device_flags = c->err_flags;
-    if (tile) *tile = c->err_tile;
     return FALX_OK;
 }
 
@@ -775,7 +766,6 @@ falx_status falx_index_structurals_static(falx_ctx* c, const falx_spec* spec, co
     if (len > FALX_STATIC_MAX_LEN) return FALX_E_TOO_LARGE;
     if (len == 0) return FALX_OK;
     tl_hip_error = 0;
-    c->err_flags = c->err_tile = 0;
     (void)hipGetLastError();
     HIP_TRY(hipSetDevice(c->device));
     char* base = nullptr;

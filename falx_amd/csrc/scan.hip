@@ -52,7 +52,6 @@ constexpr int PRE = 64;
 constexpr int HALO = 64;
 constexpr int TILE_LDS = PRE + TILE + HALO + 16;
 constexpr uint64_t EVEN_BITS = 0x5555555555555555ULL;
-constexpr uint32_t SPIN_LIMIT = 1u << 22;
 constexpr int STAT_GROUP = 2;             // stat columns accumulated per STATS pass
 constexpr int COL_STAT_GROUP = 2;         // stat columns fused into a COLUMNS pass
 
@@ -89,8 +88,7 @@ struct StatPart {
 
 // result[] slots written by the last tile / reduce kernels
 enum : int {
-    R_N_STRUCT = 0, R_N_SEPS, R_N_TERMS, R_N_ROWS, R_ERROR, R_RECORDS, R_F0, R_F1, R_F2, R_F3,
-    R_ERR_TILE,       // first tile whose look-back gave up (+1)
+    R_N_STRUCT = 0, R_N_SEPS, R_N_TERMS, R_N_ROWS,
     R_STR_LEN = 16,   // + str_slot (32)
     R_SLOTS = 64
 };

@@ -146,8 +146,9 @@ struct hash_truth {
 
 // csv_hash dialect: header `amount,value,label`; 3% of lines are `#`
 // comments holding stray quotes and commas; i64 U(+-2^40); f64 with 1-9
-// decimals and an occasional exponent; string length log-uniform 1-64, with
-// 4% of string fields quoted and holding `#`, `,`, `""` and an embedded LF.
+// decimals and an occasional exponent; string length log-uniform 1-64; 14%
+// of string fields are quoted, and 4% of the quoted ones hold `#`, `,`, `""`
+// and an embedded LF.
 uint64_t datagen_hash_csv(uint8_t* buf, uint64_t target, uint64_t cap, uint64_t seed,
                           hash_truth* t) {
     Rng r{seed};
@@ -192,8 +193,8 @@ uint64_t datagen_hash_csv(uint8_t* buf, uint64_t target, uint64_t cap, uint64_t 
         int len = (int)std::floor(std::exp(r.uniform() * std::log(64.0)));
         if (len < 1) len = 1;
         if (len > 64) len = 64;
-        uint64_t kind = r.below(100);
-        if (kind < 4) {
+        const bool quoted = r.below(100) < 14;
+        if (quoted && r.below(100) < 4) {
             t->hazard_fields += 1;
             o.put('"');
             for (int i = 0; i < len; ++i) {
@@ -206,7 +207,7 @@ uint64_t datagen_hash_csv(uint8_t* buf, uint64_t target, uint64_t cap, uint64_t 
             }
             if (len < 4) { o.put('#'); o.put(','); o.put('\n'); o.put('"'); o.put('"'); }
             o.put('"');
-        } else if (kind < 14) {
+        } else if (quoted) {
             o.put('"');
             for (int i = 0; i < len; ++i) o.put((uint8_t)alnum[r.below(36)]);
             o.put('"');

@@ -28,7 +28,7 @@ _SIGS = [
     ("falx_ctx_destroy", None, [_VP]),
     ("falx_ctx_stream", _VP, [_VP]),
     ("falx_ctx_last_kernel_ms", C.c_double, [_VP]),
-    ("falx_ctx_last_error", C.c_int, [_VP, C.POINTER(C.c_int32), _U64P, _U64P]),
+    ("falx_ctx_last_error", C.c_int, [_VP, C.POINTER(C.c_int32)]),
     ("falx_device_alloc", C.c_int, [_VP, _U64, C.POINTER(C.c_void_p)]),
     ("falx_device_free", C.c_int, [_VP, _VP]),
     ("falx_host_alloc_pinned", C.c_int, [_VP, _U64, C.POINTER(C.c_void_p)]),
@@ -90,9 +90,9 @@ def _check(fn: str, st: int, allow=(S.OK,), ctx=None):
     if st not in allow:
         detail = ""
         if st == S.E_DEVICE and ctx is not None:
-            hip, flags, tile = C.c_int32(), C.c_uint64(), C.c_uint64()
-            if load().falx_ctx_last_error(ctx, C.byref(hip), C.byref(flags), C.byref(tile)) == S.OK:
-                detail = f" (hip error {hip.value}, look-back flags {flags.value:#x}, tile {tile.value})"
+            hip = C.c_int32()
+            if load().falx_ctx_last_error(ctx, C.byref(hip)) == S.OK:
+                detail = f" (hip error {hip.value})"
         raise FalxError(fn, st, detail)
     return st
 

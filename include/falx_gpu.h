@@ -163,13 +163,10 @@ void* falx_ctx_stream(falx_ctx* ctx);
  * on the context stream around the scan launches (fills and final
  * reductions excluded). */
 double falx_ctx_last_kernel_ms(falx_ctx* ctx);
-/* Detail of the last FALX_E_DEVICE from an entry point on this context (no
- * counterpart in the reference, whose CPU path has no device to fail): the
- * HIP error code of the failing runtime call on the calling thread (0 =
- * none), and the kernel-side look-back flags (1 = state, 2 = counters,
- * 4 = string arena) with the first tile that gave up, plus one (0 = none). */
-falx_status falx_ctx_last_error(const falx_ctx* ctx, int32_t* hip_error, uint64_t* device_flags,
-                                uint64_t* tile);
+/* Detail of the last FALX_E_DEVICE from an entry point (no counterpart in the
+ * reference, whose CPU path has no device to fail): the HIP error code of the
+ * failing runtime call on the calling thread (0 = none). */
+falx_status falx_ctx_last_error(const falx_ctx* ctx, int32_t* hip_error);
 
 falx_status falx_device_alloc(falx_ctx* ctx, uint64_t bytes, void** d_out);
 falx_status falx_device_free(falx_ctx* ctx, void* d_ptr);
