@@ -193,6 +193,8 @@ def main() -> int:
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-lanes", action="store_true")
+    ap.add_argument("--lanes", default="geo,ndjson,csv_hash",
+                    help="comma-separated lane groups to run: geo, ndjson, csv_hash")
     ap.add_argument("--no-served", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0,
                     help="only run a plain device copy of this many bytes (rocprof counter calibration)")
@@ -244,138 +246,143 @@ def main() -> int:
     value = n_in / GIB / (ms / 1000.0)
 
     lanes = {}
+    want = set(args.lanes.split(","))
     if not args.no_lanes:
-        cols3, _k3 = ctx.alloc_columns(S.GEO_CITY_LATLON, rows, n_in)
-        lane_specs = [
-            ("latlon_columns", lambda: ctx.parse_columns_into(spec, dbuf, n_in, cols), algo),
-            ("fused_count_sum", lambda: ctx.csv_stats(spec, dbuf, n_in), n_in),
-            ("city_latlon_columns", lambda: ctx.parse_columns_into(S.GEO_CITY_LATLON, dbuf, n_in, cols3),
-             algo + 4 * (rows + 1) + truth["city_bytes"] + 8 * ((rows + 63) // 64)),
-        ]
-        idx_out = ctx.alloc(4 * (n_in // 8 + 64))
-        n_struct = [0]
-
-        def idx():
-            _, n_struct[0] = ctx.index_structurals(spec, dbuf, n_in, out=idx_out, download=False)
-        lane_specs.append(("index_structurals", idx, None))
-        for name, fn, ab in lane_specs:
-            log(f"timing lane {name}")
-            lms, lkms = time_steps(fn, max(3, args.steps // 2), 1, ctx)
-            if name == "index_structurals":
-                ab = n_in + 4 * n_struct[0]
-            lanes[name] = {"gib_s": round(n_in / GIB / (lms / 1000.0), 2), "ms_per_step": round(lms, 3),
-                           "kernel_ms": round(lkms, 3), "hbm_frac": round(ab / (lkms / 1000.0) / 1e9 / HBM_PEAK_GBS, 3)}
-            if name == "index_structurals":
-                # Closed form: three separators and a newline per record.
-                lanes[name]["verified"] = n_struct[0] == 4 * rows
-        # end-to-end from pinned host memory: H2D + the headline step
-        hp = C.c_void_p()
-        gpu._check("falx_host_alloc_pinned", gpu.load().falx_host_alloc_pinned(ctx.h, n_in, C.byref(hp)))
-        C.memmove(hp, data.ctypes.data, n_in)
-
-        def e2e():
-            gpu._check("falx_upload", gpu.load().falx_upload(ctx.h, dbuf.ptr, hp, n_in))
-            step()
-        log("timing lane e2e_pinned_h2d")
-        ems, _ = time_steps(e2e, max(3, args.steps // 2), 1, ctx)
-        lanes["e2e_pinned_h2d"] = {"gib_s": round(n_in / GIB / (ems / 1000.0), 2), "ms_per_step": round(ems, 3)}
-        gpu.load().falx_host_free_pinned(ctx.h, hp)
+        thr = cpu_threads(args.cpu_threads)
         if not args.no_cpu:
             from oracle import oracle as O
-            thr = cpu_threads(args.cpu_threads)
-            cpu_lane = {
-                "latlon_columns": cpu_columns(spec, data, rows, 16, thr),
-                "fused_count_sum": lambda: O.csv_stats(spec, data, thr),
-                "city_latlon_columns": cpu_columns(S.GEO_CITY_LATLON, data, rows, n_in, thr),
-                "index_structurals": lambda: O.index_structurals(spec, data, thr),
-            }
-            for name, fn in cpu_lane.items():
-                log(f"timing CPU oracle lane {name}")
-                lanes[name]["cpu_gib_s"] = round(n_in / GIB / cpu_time(fn, args.cpu_runs), 3)
-                lanes[name]["cpu_threads"] = thr
-        del cols3, _k3, idx_out
+        if "geo" in want:
+            cols3, _k3 = ctx.alloc_columns(S.GEO_CITY_LATLON, rows, n_in)
+            lane_specs = [
+                ("latlon_columns", lambda: ctx.parse_columns_into(spec, dbuf, n_in, cols), algo),
+                ("fused_count_sum", lambda: ctx.csv_stats(spec, dbuf, n_in), n_in),
+                ("city_latlon_columns", lambda: ctx.parse_columns_into(S.GEO_CITY_LATLON, dbuf, n_in, cols3),
+                 algo + 4 * (rows + 1) + truth["city_bytes"] + 8 * ((rows + 63) // 64)),
+            ]
+            idx_out = ctx.alloc(4 * (n_in // 8 + 64))
+            n_struct = [0]
 
-        # NDJSON sum(id + nested.score), 1 GiB
-        log(f"generating {args.size} B NDJSON")
-        nd, nd_truth = datagen.ndjson(args.size)
-        nd_n = int(nd.size)
-        nd_buf = ctx.upload(nd)
-        nd_res = [None]
+            def idx():
+                _, n_struct[0] = ctx.index_structurals(spec, dbuf, n_in, out=idx_out, download=False)
+            lane_specs.append(("index_structurals", idx, None))
+            for name, fn, ab in lane_specs:
+                log(f"timing lane {name}")
+                lms, lkms = time_steps(fn, max(3, args.steps // 2), 1, ctx)
+                if name == "index_structurals":
+                    ab = n_in + 4 * n_struct[0]
+                lanes[name] = {"gib_s": round(n_in / GIB / (lms / 1000.0), 2), "ms_per_step": round(lms, 3),
+                               "kernel_ms": round(lkms, 3), "hbm_frac": round(ab / (lkms / 1000.0) / 1e9 / HBM_PEAK_GBS, 3)}
+                if name == "index_structurals":
+                    # Closed form: three separators and a newline per record.
+                    lanes[name]["verified"] = n_struct[0] == 4 * rows
+            # end-to-end from pinned host memory: H2D + the headline step
+            hp = C.c_void_p()
+            gpu._check("falx_host_alloc_pinned", gpu.load().falx_host_alloc_pinned(ctx.h, n_in, C.byref(hp)))
+            C.memmove(hp, data.ctypes.data, n_in)
 
-        def nd_step():
-            nd_res[0] = ctx.ndjson_id_score(nd_buf, nd_n)
-        log("timing lane ndjson_id_score_sum")
-        lms, lkms = time_steps(nd_step, max(3, args.steps // 2), 1, ctx)
-        r = nd_res[0]
-        lanes["ndjson_id_score_sum"] = {
-            "gib_s": round(nd_n / GIB / (lms / 1000.0), 2), "ms_per_step": round(lms, 3),
-            "kernel_ms": round(lkms, 3), "hbm_frac": round(nd_n / (lkms / 1000.0) / 1e9 / HBM_PEAK_GBS, 3),
-            "bytes": nd_n, "records": r["records"],
-            "verified": (r["records"] == nd_truth["records"] and r["id_sum"] == nd_truth["id_sum"]
-                         and r["score_fixed6_sum"] == nd_truth["score_fixed6_sum"] and r["invalid"] == 0)}
-        if not args.no_cpu:
-            log("timing CPU oracle lane ndjson_id_score_sum")
-            lanes["ndjson_id_score_sum"]["cpu_gib_s"] = round(
-                nd_n / GIB / cpu_time(lambda: O.ndjson_id_score(nd, thr), args.cpu_runs), 3)
-            lanes["ndjson_id_score_sum"]["cpu_threads"] = thr
-            # Score sum: the GPU tree and the reference's sequential fold
-            # (the oracle at one thread) against the exact decimal sum.
-            fold = O.ndjson_id_score(nd, 1)["score_sum"]
-            lanes["ndjson_id_score_sum"]["sums"] = sum_errors(
-                "score", r["score_sum"], fold, Fraction(nd_truth["score_fixed6_sum"], 10**6))
-        nd_buf.free()
-        del nd
-
-        # csv_hash 1 GiB: index, typed columns, field bytes
-        log(f"generating {args.size} B csv_hash")
-        hs, hs_truth = datagen.hash_csv(args.size)
-        hs_n = int(hs.size)
-        hs_buf = ctx.upload(hs)
-        hspec = S.HASH_TYPED
-        hs_rows = int(hs_truth["records"])
-        hcols, hkeep = ctx.alloc_columns(hspec, hs_rows, hs_n)
-        hidx = ctx.alloc(4 * (hs_n // 4 + 64))
-        hs_struct = [0]
-
-        def hs_index():
-            _, hs_struct[0] = ctx.index_structurals(hspec, hs_buf, hs_n, out=hidx, download=False)
-        hash_lanes = [
-            ("csv_hash_index_structurals", hs_index, None,
-             lambda: O.index_structurals(hspec, hs, thr)),
-            ("csv_hash_typed_columns", lambda: ctx.parse_columns_into(hspec, hs_buf, hs_n, hcols), None,
-             (cpu_columns(hspec, hs, hs_rows, hs_n, thr) if not args.no_cpu else None)),
-            ("csv_hash_field_bytes", lambda: ctx.field_bytes(hspec, hs_buf, hs_n), hs_n,
-             lambda: O.field_bytes(hspec, hs, thr)),
-        ]
-        for name, fn, ab, cfn in hash_lanes:
-            log(f"timing lane {name}")
-            lms, lkms = time_steps(fn, max(3, args.steps // 2), 1, ctx)
-            if name == "csv_hash_index_structurals":
-                ab = hs_n + 4 * hs_struct[0]
-            elif ab is None:
-                # input + i64 and f64 values + string offsets and arena +
-                # three validity bitmaps
-                hr = int(hcols.rows)
-                ab = (hs_n + 16 * hr + 4 * (hr + 1) + int(hcols.cols[2].data_len) + 3 * 8 * ((hr + 63) // 64))
-            lanes[name] = {"gib_s": round(hs_n / GIB / (lms / 1000.0), 2), "ms_per_step": round(lms, 3),
-                           "kernel_ms": round(lkms, 3),
-                           "hbm_frac": round(ab / (lkms / 1000.0) / 1e9 / HBM_PEAK_GBS, 3), "bytes": hs_n}
+            def e2e():
+                gpu._check("falx_upload", gpu.load().falx_upload(ctx.h, dbuf.ptr, hp, n_in))
+                step()
+            log("timing lane e2e_pinned_h2d")
+            ems, _ = time_steps(e2e, max(3, args.steps // 2), 1, ctx)
+            lanes["e2e_pinned_h2d"] = {"gib_s": round(n_in / GIB / (ems / 1000.0), 2), "ms_per_step": round(ems, 3)}
+            gpu.load().falx_host_free_pinned(ctx.h, hp)
             if not args.no_cpu:
-                log(f"timing CPU oracle lane {name}")
-                lanes[name]["cpu_gib_s"] = round(hs_n / GIB / cpu_time(cfn, args.cpu_runs), 3)
-                lanes[name]["cpu_threads"] = thr
-        # Closed forms of the generator: the row count and the wrapping sum
-        # of the i64 column; the index and field bytes equal the oracle's.
-        i64_col = hkeep[0][0].download(np.int64, int(hcols.rows))
-        lanes["csv_hash_typed_columns"]["rows"] = int(hcols.rows)
-        lanes["csv_hash_typed_columns"]["verified"] = (
-            int(hcols.rows) == hs_rows and int(i64_col.view(np.uint64).sum(dtype=np.uint64)) ==
-            hs_truth["i64_sum"] % 2**64)
-        del i64_col
-        if not args.no_cpu:
-            lanes["csv_hash_index_structurals"]["verified"] = hs_struct[0] == O.index_structurals(hspec, hs, thr).size
-            lanes["csv_hash_field_bytes"]["verified"] = (ctx.field_bytes(hspec, hs_buf, hs_n) ==
-                                                         O.field_bytes(hspec, hs, thr))
+                cpu_lane = {
+                    "latlon_columns": cpu_columns(spec, data, rows, 16, thr),
+                    "fused_count_sum": lambda: O.csv_stats(spec, data, thr),
+                    "city_latlon_columns": cpu_columns(S.GEO_CITY_LATLON, data, rows, n_in, thr),
+                    "index_structurals": lambda: O.index_structurals(spec, data, thr),
+                }
+                for name, fn in cpu_lane.items():
+                    log(f"timing CPU oracle lane {name}")
+                    lanes[name]["cpu_gib_s"] = round(n_in / GIB / cpu_time(fn, args.cpu_runs), 3)
+                    lanes[name]["cpu_threads"] = thr
+            del cols3, _k3, idx_out
+
+        if "ndjson" in want:
+            # NDJSON sum(id + nested.score), 1 GiB
+            log(f"generating {args.size} B NDJSON")
+            nd, nd_truth = datagen.ndjson(args.size)
+            nd_n = int(nd.size)
+            nd_buf = ctx.upload(nd)
+            nd_res = [None]
+
+            def nd_step():
+                nd_res[0] = ctx.ndjson_id_score(nd_buf, nd_n)
+            log("timing lane ndjson_id_score_sum")
+            lms, lkms = time_steps(nd_step, max(3, args.steps // 2), 1, ctx)
+            r = nd_res[0]
+            lanes["ndjson_id_score_sum"] = {
+                "gib_s": round(nd_n / GIB / (lms / 1000.0), 2), "ms_per_step": round(lms, 3),
+                "kernel_ms": round(lkms, 3), "hbm_frac": round(nd_n / (lkms / 1000.0) / 1e9 / HBM_PEAK_GBS, 3),
+                "bytes": nd_n, "records": r["records"],
+                "verified": (r["records"] == nd_truth["records"] and r["id_sum"] == nd_truth["id_sum"]
+                             and r["score_fixed6_sum"] == nd_truth["score_fixed6_sum"] and r["invalid"] == 0)}
+            if not args.no_cpu:
+                log("timing CPU oracle lane ndjson_id_score_sum")
+                lanes["ndjson_id_score_sum"]["cpu_gib_s"] = round(
+                    nd_n / GIB / cpu_time(lambda: O.ndjson_id_score(nd, thr), args.cpu_runs), 3)
+                lanes["ndjson_id_score_sum"]["cpu_threads"] = thr
+                # Score sum: the GPU tree and the reference's sequential fold
+                # (the oracle at one thread) against the exact decimal sum.
+                fold = O.ndjson_id_score(nd, 1)["score_sum"]
+                lanes["ndjson_id_score_sum"]["sums"] = sum_errors(
+                    "score", r["score_sum"], fold, Fraction(nd_truth["score_fixed6_sum"], 10**6))
+            nd_buf.free()
+            del nd
+
+        if "csv_hash" in want:
+            # csv_hash 1 GiB: index, typed columns, field bytes
+            log(f"generating {args.size} B csv_hash")
+            hs, hs_truth = datagen.hash_csv(args.size)
+            hs_n = int(hs.size)
+            hs_buf = ctx.upload(hs)
+            hspec = S.HASH_TYPED
+            hs_rows = int(hs_truth["records"])
+            hcols, hkeep = ctx.alloc_columns(hspec, hs_rows, hs_n)
+            hidx = ctx.alloc(4 * (hs_n // 4 + 64))
+            hs_struct = [0]
+
+            def hs_index():
+                _, hs_struct[0] = ctx.index_structurals(hspec, hs_buf, hs_n, out=hidx, download=False)
+            hash_lanes = [
+                ("csv_hash_index_structurals", hs_index, None,
+                 lambda: O.index_structurals(hspec, hs, thr)),
+                ("csv_hash_typed_columns", lambda: ctx.parse_columns_into(hspec, hs_buf, hs_n, hcols), None,
+                 (cpu_columns(hspec, hs, hs_rows, hs_n, thr) if not args.no_cpu else None)),
+                ("csv_hash_field_bytes", lambda: ctx.field_bytes(hspec, hs_buf, hs_n), hs_n,
+                 lambda: O.field_bytes(hspec, hs, thr)),
+            ]
+            for name, fn, ab, cfn in hash_lanes:
+                log(f"timing lane {name}")
+                lms, lkms = time_steps(fn, max(3, args.steps // 2), 1, ctx)
+                if name == "csv_hash_index_structurals":
+                    ab = hs_n + 4 * hs_struct[0]
+                elif ab is None:
+                    # input + i64 and f64 values + string offsets and arena +
+                    # three validity bitmaps
+                    hr = int(hcols.rows)
+                    ab = (hs_n + 16 * hr + 4 * (hr + 1) + int(hcols.cols[2].data_len) + 3 * 8 * ((hr + 63) // 64))
+                lanes[name] = {"gib_s": round(hs_n / GIB / (lms / 1000.0), 2), "ms_per_step": round(lms, 3),
+                               "kernel_ms": round(lkms, 3),
+                               "hbm_frac": round(ab / (lkms / 1000.0) / 1e9 / HBM_PEAK_GBS, 3), "bytes": hs_n}
+                if not args.no_cpu:
+                    log(f"timing CPU oracle lane {name}")
+                    lanes[name]["cpu_gib_s"] = round(hs_n / GIB / cpu_time(cfn, args.cpu_runs), 3)
+                    lanes[name]["cpu_threads"] = thr
+            # Closed forms of the generator: the row count and the wrapping sum
+            # of the i64 column; the index and field bytes equal the oracle's.
+            i64_col = hkeep[0][0].download(np.int64, int(hcols.rows))
+            lanes["csv_hash_typed_columns"]["rows"] = int(hcols.rows)
+            lanes["csv_hash_typed_columns"]["verified"] = (
+                int(hcols.rows) == hs_rows and int(i64_col.view(np.uint64).sum(dtype=np.uint64)) ==
+                hs_truth["i64_sum"] % 2**64)
+            del i64_col
+            if not args.no_cpu:
+                lanes["csv_hash_index_structurals"]["verified"] = hs_struct[0] == O.index_structurals(hspec, hs, thr).size
+                lanes["csv_hash_field_bytes"]["verified"] = (ctx.field_bytes(hspec, hs_buf, hs_n) ==
+                                                             O.field_bytes(hspec, hs, thr))
 
     served = None
     if not args.no_served:
