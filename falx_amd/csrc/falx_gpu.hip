@@ -4,6 +4,7 @@
 #include "scan.hip"
 
 #include <algorithm>
+#include <cstdio>
 #include <mutex>
 #include <new>
 
@@ -790,6 +791,120 @@ falx_status falx_index_structurals_static(falx_ctx* c, const falx_spec* spec, co
     c->last_ms = ms;
     *n_out = c->h_result[R_N_STRUCT];
     return *n_out > P.out_cap ? FALX_E_CAPACITY : FALX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Arrow export (Arrow C Device Data Interface, zero-copy over the column).
+// ---------------------------------------------------------------------------
+namespace {
+
+// What the export allocates on the host for one array/schema pair; the
+// release callbacks free it (the column buffers are the caller's).
+struct ArrowPriv {
+    const void* buffers[3];
+    const void* child_buffers[2];
+    ArrowArray child;
+    ArrowArray* child_ptr;
+    char name[24];
+};
+
+struct ArrowSchemaPriv {
+    ArrowSchema child;
+    ArrowSchema* child_ptr;
+    char name[24];
+};
+
+void arrow_release_array(ArrowArray* a) {
+    if (!a || !a->release) return;
+    if (a->private_data) delete static_cast<ArrowPriv*>(a->private_data);
+    a->release = nullptr;
+    a->private_data = nullptr;
+}
+
+// Children are released with their parent (they share its allocation).
+void arrow_release_child_array(ArrowArray* a) {
+    if (a) a->release = nullptr;
+}
+
+void arrow_release_schema(ArrowSchema* s) {
+    if (!s || !s->release) return;
+    if (s->private_data) delete static_cast<ArrowSchemaPriv*>(s->private_data);
+    s->release = nullptr;
+    s->private_data = nullptr;
+}
+
+void arrow_release_child_schema(ArrowSchema* s) {
+    if (s) s->release = nullptr;
+}
+
+}  // namespace
+
+falx_status falx_export_arrow(const falx_spec* spec, const falx_columns* cols, uint32_t k, int32_t device_type,
+                              int64_t device_id, ArrowDeviceArray* out, ArrowSchema* schema) {
+    falx_status st = falx_spec_validate(spec);
+    if (st != FALX_OK) return st;
+    if (!cols || !out || !schema || k >= spec->n_columns) return FALX_E_INVALID;
+    if (cols->rows > cols->rows_cap || cols->rows > (uint64_t)INT32_MAX) return FALX_E_INVALID;
+    const falx_column_buf& b = cols->cols[k];
+    const uint32_t type = spec->columns[k].type;
+    ArrowPriv* ap = new (std::nothrow) ArrowPriv();
+    ArrowSchemaPriv* sp = new (std::nothrow) ArrowSchemaPriv();
+    if (!ap || !sp) {
+        delete ap;
+        delete sp;
+        return FALX_E_DEVICE;
+    }
+    const int64_t rows = (int64_t)cols->rows;
+    snprintf(sp->name, sizeof sp->name, "c%u", spec->columns[k].index);
+    memset(out, 0, sizeof *out);
+    memset(schema, 0, sizeof *schema);
+    ArrowArray& a = out->array;
+    a.length = rows;
+    a.null_count = -1;
+    a.offset = 0;
+    a.buffers = ap->buffers;
+    a.release = arrow_release_array;
+    a.private_data = ap;
+    schema->name = sp->name;
+    schema->flags = ARROW_FLAG_NULLABLE;
+    schema->release = arrow_release_schema;
+    schema->private_data = sp;
+    ap->buffers[0] = b.valid;
+    if (type == FALX_COL_I64 || type == FALX_COL_F64) {
+        schema->format = type == FALX_COL_I64 ? "l" : "g";
+        a.n_buffers = 2;
+        ap->buffers[1] = b.values;
+    } else if (type == FALX_COL_STR) {
+        schema->format = "z";
+        a.n_buffers = 3;
+        ap->buffers[1] = b.values;
+        ap->buffers[2] = b.data;
+    } else {
+        // (start, end) pairs: a fixed-size list of two uint32 per cell.
+        schema->format = "+w:2";
+        a.n_buffers = 1;
+        a.n_children = 1;
+        ap->child_buffers[0] = nullptr;
+        ap->child_buffers[1] = b.values;
+        ap->child.length = 2 * rows;
+        ap->child.null_count = 0;
+        ap->child.n_buffers = 2;
+        ap->child.buffers = ap->child_buffers;
+        ap->child.release = arrow_release_child_array;
+        ap->child_ptr = &ap->child;
+        a.children = &ap->child_ptr;
+        schema->n_children = 1;
+        sp->child.format = "I";
+        snprintf(sp->name + 12, 12, "span");
+        sp->child.name = sp->name + 12;
+        sp->child.release = arrow_release_child_schema;
+        sp->child_ptr = &sp->child;
+        schema->children = &sp->child_ptr;
+    }
+    out->device_id = device_id;
+    out->device_type = device_type;
+    out->sync_event = nullptr;
+    return FALX_OK;
 }
 
 }  // extern "C"

@@ -47,6 +47,8 @@ _SIGS = [
     ("falx_parse_ndjson_lines", C.c_int, [_VP, _VP, _U64, _U64P]),
     ("falx_parse_ndjson_id_score", C.c_int, [_VP, _VP, _U64, C.POINTER(S.CNdjsonIdScore)]),
     ("falx_parse_logfmt_pairs", C.c_int, [_VP, _VP, _U64, C.POINTER(S.CLogfmtStats)]),
+    ("falx_export_arrow", C.c_int, [C.POINTER(S.CSpec), C.POINTER(S.CColumns), C.c_uint32, C.c_int32, C.c_int64,
+                                    _VP, _VP]),
     ("falx_batch_create", C.c_int, [_VP, _U64, C.POINTER(C.c_void_p)]),
     ("falx_batch_destroy", None, [_VP]),
     ("falx_request_scratch_bytes", _U64, [C.POINTER(S.CRequest)]),

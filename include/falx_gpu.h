@@ -279,6 +279,76 @@ falx_status falx_parse_logfmt_pairs(falx_ctx* ctx, const uint8_t* d_data,
                                     uint64_t len, falx_logfmt_stats* out);
 
 /* ---------------------------------------------------------------------
+ * Arrow export. The reference hands its columns to Arrow by wrapping the
+ * buffers, since they are Arrow's own layouts (examples/arrow_interop.rs:
+ * 1-115). Here the columns live in HBM, so a column is exported through the
+ * Arrow C Device Data Interface (the public ABI below, as the Arrow project
+ * specifies it) with device_type ARROW_DEVICE_ROCM: the array's buffers are
+ * the column's device buffers themselves, nothing is copied.
+ * ------------------------------------------------------------------- */
+#ifndef ARROW_C_DATA_INTERFACE
+#define ARROW_C_DATA_INTERFACE
+#define ARROW_FLAG_DICTIONARY_ORDERED 1
+#define ARROW_FLAG_NULLABLE 2
+#define ARROW_FLAG_MAP_KEYS_SORTED 4
+struct ArrowSchema {
+    const char* format;
+    const char* name;
+    const char* metadata;
+    int64_t flags;
+    int64_t n_children;
+    struct ArrowSchema** children;
+    struct ArrowSchema* dictionary;
+    void (*release)(struct ArrowSchema*);
+    void* private_data;
+};
+struct ArrowArray {
+    int64_t length;
+    int64_t null_count;
+    int64_t offset;
+    int64_t n_buffers;
+    int64_t n_children;
+    const void** buffers;
+    struct ArrowArray** children;
+    struct ArrowArray* dictionary;
+    void (*release)(struct ArrowArray*);
+    void* private_data;
+};
+#endif
+#ifndef ARROW_C_DEVICE_DATA_INTERFACE
+#define ARROW_C_DEVICE_DATA_INTERFACE
+typedef int32_t ArrowDeviceType;
+#define ARROW_DEVICE_CPU 1
+#define ARROW_DEVICE_ROCM 10
+struct ArrowDeviceArray {
+    struct ArrowArray array;
+    int64_t device_id;
+    ArrowDeviceType device_type;
+    void* sync_event;
+    int64_t reserved[3];
+};
+#endif
+
+/*
+ * Column k of a columns result (falx_parse_columns[_with_stats] or a batch)
+ * as an Arrow array of out->rows rows, zero-copy over its buffers:
+ *   I64 -> int64 ("l"), F64 -> float64 ("g"): validity, values;
+ *   STR -> binary ("z"): validity, int32 offsets, data (the reference's
+ *          BinaryArray, arrow_interop.rs:67-74);
+ *   BYTES -> fixed-size list of 2 uint32 ("+w:2" of "I"): the (start, end)
+ *          span of each cell into the input.
+ * null_count is -1 (not counted: the bitmap stays on the device). The
+ * arrays do not own the buffers: their release callbacks free only what
+ * the export allocated on the host, so the column buffers must outlive
+ * them. device_type is `device_type` (ARROW_DEVICE_ROCM for HBM buffers,
+ * ARROW_DEVICE_CPU for columns in host memory), sync_event is NULL (the
+ * entry points return after their stream has finished).
+ */
+falx_status falx_export_arrow(const falx_spec* spec, const falx_columns* cols, uint32_t k,
+                              int32_t device_type, int64_t device_id,
+                              struct ArrowDeviceArray* out, struct ArrowSchema* schema);
+
+/* ---------------------------------------------------------------------
  * Batches. No counterpart in the reference, which parses one document per
  * call (the generated module functions above). A batch runs many documents
  * of uneven sizes and mixed specs through each pipeline stage in one
