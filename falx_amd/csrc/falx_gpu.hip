@@ -2,6 +2,7 @@
 // scan.hip. One context = one device + one HIP stream + reusable scratch
 // (tile summaries and prefixes, per-tile partials, result slots).
 #include "scan.hip"
+#include "nested.h"
 
 #include <algorithm>
 #include <cstdio>
@@ -717,6 +718,132 @@ falx_status falx_parse_logfmt_pairs(falx_ctx* c, const uint8_t* d, uint64_t len,
     out->key_bytes = h[1];
     out->value_bytes = h[2];
     out->checksum = h[3];
+    return FALX_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// Scratch of the bracket matcher over n_tiles tiles of NT index entries.
+struct NestLayout {
+    size_t tsum, tmin, tmis, rcount, res, bstat, base, lv[NT_LEVELS], total;
+    uint32_t lv_n[NT_LEVELS];
+};
+
+NestLayout nest_layout(uint32_t n_tiles) {
+    NestLayout L{};
+    size_t o = 0;
+    auto take = [&](size_t bytes) {
+        const size_t at = o;
+        o = align_up(o + bytes, 256);
+        return at;
+    };
+    L.tsum = take(4 * (size_t)n_tiles);
+    L.tmin = take(4 * (size_t)n_tiles);
+    L.tmis = take(4 * (size_t)n_tiles);
+    L.rcount = take(4 * (size_t)n_tiles);
+    L.res = take(4 * (size_t)NT * n_tiles);
+    L.bstat = take(4 * (size_t)NT_BLOCKS * n_tiles);
+    L.base = take(8 * (size_t)n_tiles);
+    uint32_t m = n_tiles;
+    for (int k = 0; k < NT_LEVELS; ++k) {
+        L.lv_n[k] = m;
+        L.lv[k] = take(8 * (size_t)m);
+        m = (m + 63) / 64;
+    }
+    L.total = o;
+    return L;
+}
+
+}  // namespace
+
+extern "C" {
+
+falx_status falx_parse_nested(falx_ctx* c, const uint8_t* d, uint64_t len, uint64_t* d_tape, uint64_t tape_cap,
+                              uint64_t* n_tape, int32_t* error, uint64_t* error_pos) {
+    falx_status st = check_input(c, d, len);
+    if (st != FALX_OK) return st;
+    if (!n_tape || !error || !error_pos) return FALX_E_INVALID;
+    *n_tape = 0;
+    *error = FALX_NEST_OK;
+    *error_pos = 0;
+    if (len == 0) return FALX_OK;
+    // The JSON dialect of json.rs: `"` strings with backslash escapes,
+    // structural bytes , : [ ] { } (formats.rs:164-173).
+    falx_spec spec = fixed_spec(",:[]{}", '"', '\\');
+    const int kind = dialect_kind(&spec);
+    Params P;
+    if ((st = begin(c)) != FALX_OK) return st;
+    if ((st = prepare(c, &spec, d, len, P)) != FALX_OK) return st;
+    HIP_TRY(mark_start(c));
+    HIP_TRY(run_prefix(c, P, kind));
+    // The entry count sizes the matcher's grid and scratch: one round trip.
+    HIP_TRY(hipMemcpyAsync(c->h_result, c->d_result, 8, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    const uint64_t n = c->h_result[R_N_STRUCT];
+    if (n == 0 || n > tape_cap || !d_tape) {
+        HIP_TRY(mark_end(c));
+        if ((st = finish(c)) != FALX_OK) return st;
+        *n_tape = n;
+        return n == 0 ? FALX_OK : FALX_E_CAPACITY;
+    }
+    const uint32_t n_tiles = (uint32_t)((n + NT - 1) / NT);
+    const NestLayout L = nest_layout(n_tiles);
+    if ((st = ensure_part(c, L.total)) != FALX_OK) return st;
+    char* base = static_cast<char*>(c->d_part);
+    NestScratch S;
+    S.tsum = reinterpret_cast<int32_t*>(base + L.tsum);
+    S.tmin = reinterpret_cast<int32_t*>(base + L.tmin);
+    S.tmis = reinterpret_cast<uint32_t*>(base + L.tmis);
+    S.rcount = reinterpret_cast<uint32_t*>(base + L.rcount);
+    S.res = reinterpret_cast<uint32_t*>(base + L.res);
+    S.bstat = reinterpret_cast<uint32_t*>(base + L.bstat);
+    S.base = reinterpret_cast<long long*>(base + L.base);
+    for (int k = 0; k < NT_LEVELS; ++k) {
+        S.lv[k] = reinterpret_cast<long long*>(base + L.lv[k]);
+        S.lv_n[k] = L.lv_n[k];
+    }
+    S.result = c->d_result;
+    unsigned long long* tape = reinterpret_cast<unsigned long long*>(d_tape);
+    P.ends = tape;
+    P.ends_cap = tape_cap;
+    HIP_TRY(launch_scan<M_INDEX>(c, P, kind));
+    hipLaunchKernelGGL(k_nest_local, dim3(n_tiles), dim3(WG), 0, c->stream, tape, (uint32_t)n, S);
+    hipLaunchKernelGGL(k_nest_prefix, dim3(1), dim3(NT_PREFIX_WG), 0, c->stream, n_tiles, S);
+    for (int k = 1; k < NT_LEVELS; ++k)
+        hipLaunchKernelGGL(k_nest_tree, dim3((L.lv_n[k] + WG - 1) / WG), dim3(WG), 0, c->stream,
+                           static_cast<const long long*>(S.lv[k - 1]), L.lv_n[k - 1], S.lv[k], L.lv_n[k]);
+    hipLaunchKernelGGL(k_nest_far, dim3(n_tiles), dim3(NT_FAR_WG), 0, c->stream, d, tape, (uint32_t)n, S);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(mark_end(c));
+    if ((st = finish(c)) != FALX_OK) return st;
+    const uint64_t cut = std::min(c->h_result[R_NT_NEG], c->h_result[R_NT_MIS]);
+    const uint64_t open = c->h_result[R_NT_OPEN];
+    const long long depth = (long long)c->h_result[R_NT_DEPTH];
+    *n_tape = n;
+    uint64_t at = UINT64_MAX;
+    if (cut < n) {
+        // json.rs:733-752: the tape stops before the first unmatched close.
+        *n_tape = cut;
+        *error = FALX_NEST_UNMATCHED_CLOSE;
+        at = cut;
+        if (cut > 0) {
+            const uint32_t blocks = (uint32_t)std::min<uint64_t>((cut + WG - 1) / WG, 65536);
+            hipLaunchKernelGGL(k_nest_cut, dim3(blocks), dim3(WG), 0, c->stream, tape, (uint32_t)cut);
+            HIP_TRY(hipGetLastError());
+        }
+    } else if (depth > 0 && open > 0) {
+        // json.rs:651-653: the innermost open still open at the end.
+        *error = FALX_NEST_UNCLOSED_OPEN;
+        at = open - 1;
+    }
+    if (at != UINT64_MAX) {
+        unsigned long long v = 0;
+        HIP_TRY(hipMemcpyAsync(&v, tape + at, 8, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        *error_pos = (uint32_t)v;
+    }
     return FALX_OK;
 }
 

@@ -90,6 +90,9 @@ struct StatPart {
 enum : int {
     R_N_STRUCT = 0, R_N_SEPS, R_N_TERMS, R_N_ROWS,
     R_STR_LEN = 16,   // + str_slot (32)
+    // nested tape: first entry below depth 0, first mismatched close, innermost
+    // open never closed + 1, final depth
+    R_NT_NEG = 56, R_NT_MIS, R_NT_OPEN, R_NT_DEPTH,
     R_SLOTS = 64
 };
 
@@ -975,6 +978,19 @@ __device__ __forceinline__ void scan_tile(const Params& P, const uint32_t tile, 
     const Counts pre = sh.pre;
 
     if (MODE == M_INDEX) {
+        if (P.ends) {
+            // Nested-tape form: 64-bit entries, the position in the low
+            // word and, for a bracket, its byte in the high word (the
+            // bracket matcher replaces it by the partner's tape index).
+            for (uint32_t k = tid; k < n_list; k += WG) {
+                const uint64_t idx = (uint64_t)pre.n_struct + k;
+                const uint32_t at = list_at(sh, k) & 0x3FFFu;
+                const uint32_t b = tb[at];
+                const uint32_t hi = (b == '{' || b == '[' || b == '}' || b == ']') ? b : 0u;
+                if (idx < P.ends_cap) P.ends[idx] = ((unsigned long long)hi << 32) | (tile_start + at);
+            }
+            return;
+        }
         for (uint32_t k = tid; k < n_list; k += WG) {
             uint64_t idx = (uint64_t)pre.n_struct + k;
             if (idx < P.out_cap) P.out[idx] = tile_start + (list_at(sh, k) & 0x3FFFu);

@@ -47,6 +47,7 @@ _SIGS = [
     ("falx_parse_ndjson_lines", C.c_int, [_VP, _VP, _U64, _U64P]),
     ("falx_parse_ndjson_id_score", C.c_int, [_VP, _VP, _U64, C.POINTER(S.CNdjsonIdScore)]),
     ("falx_parse_logfmt_pairs", C.c_int, [_VP, _VP, _U64, C.POINTER(S.CLogfmtStats)]),
+    ("falx_parse_nested", C.c_int, [_VP, _VP, _U64, _VP, _U64, _U64P, C.POINTER(C.c_int32), _U64P]),
     ("falx_export_arrow", C.c_int, [C.POINTER(S.CSpec), C.POINTER(S.CColumns), C.c_uint32, C.c_int32, C.c_int64,
                                     _VP, _VP]),
     ("falx_batch_create", C.c_int, [_VP, _U64, C.POINTER(C.c_void_p)]),
@@ -301,6 +302,34 @@ class Context:
         self._ck("falx_parse_logfmt_pairs", load().falx_parse_logfmt_pairs(self.h, data.ptr, n, C.byref(st)))
         return {"pairs": st.pairs, "key_bytes": st.key_bytes, "value_bytes": st.value_bytes,
                 "checksum": st.checksum}
+
+    # -- JSON nested tape -----------------------------------------------------
+    def parse_nested_into(self, data: DeviceBuffer, length: int, tape: DeviceBuffer):
+        """falx_parse_nested into `tape` (8-byte entries): (status, n_tape,
+        error, error_pos); status is OK or CAPACITY (n_tape = entries needed)."""
+        n, err, pos = C.c_uint64(), C.c_int32(), C.c_uint64()
+        st = load().falx_parse_nested(self.h, data.ptr, length, tape.ptr, tape.nbytes // 8, C.byref(n),
+                                      C.byref(err), C.byref(pos))
+        self._ck("falx_parse_nested", st, allow=(S.OK, S.E_CAPACITY))
+        return st, n.value, err.value, pos.value
+
+    def parse_nested(self, data: DeviceBuffer, length: Optional[int] = None, download: bool = True):
+        """json::parse_nested on the device: (tape, n_tape, error, error_pos),
+        the tape downloaded as uint64 (or left as a DeviceBuffer). The tape
+        is sized for one entry per 4 input bytes and grown once if needed."""
+        n = data.nbytes if length is None else length
+        tape = self.alloc(8 * max(1, n // 4 + 64))
+        st, cnt, err, pos = self.parse_nested_into(data, n, tape)
+        if st == S.E_CAPACITY:
+            tape.free()
+            tape = self.alloc(8 * max(1, cnt))
+            st, cnt, err, pos = self.parse_nested_into(data, n, tape)
+            self._ck("falx_parse_nested", st)
+        if download:
+            out = tape.download(np.uint64, cnt)
+            tape.free()
+            return out, cnt, err, pos
+        return tape, cnt, err, pos
 
 
 def _result_dict(op: int, spec: Optional[S.Spec], r: S.CResult) -> Dict:

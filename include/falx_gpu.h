@@ -278,6 +278,28 @@ falx_status falx_parse_ndjson_id_score(falx_ctx* ctx, const uint8_t* d_data,
 falx_status falx_parse_logfmt_pairs(falx_ctx* ctx, const uint8_t* d_data,
                                     uint64_t len, falx_logfmt_stats* out);
 
+/* ---- JSON nested tape ------------------------------------------------------
+ * `json::parse_nested[_par]` (kernels/json.rs:636-656, 996-1108) and the
+ * `Nested` tape it returns (json.rs:617-630): one entry per
+ * structural byte outside strings (`{ } [ ] , :`, formats.rs:164-173), in
+ * input order. A bracket's entry is (partner's tape index << 32) | byte
+ * position, the partner being 0xFFFFFFFF for an open never closed; a `,` or
+ * `:` entry is its position. `error` is NestError (json.rs:608-615): on a
+ * close with no open or of the wrong kind the tape stops before it
+ * (*n_tape = its tape index) and error = FALX_NEST_UNMATCHED_CLOSE at its
+ * position; input ending with brackets open gives FALX_NEST_UNCLOSED_OPEN at
+ * the innermost open's position. FALX_E_CAPACITY when the tape needs more
+ * than tape_cap entries (*n_tape = the count needed). */
+typedef enum falx_nest_error {
+    FALX_NEST_OK = 0,
+    FALX_NEST_UNMATCHED_CLOSE = 1,
+    FALX_NEST_UNCLOSED_OPEN = 2,
+} falx_nest_error;
+
+falx_status falx_parse_nested(falx_ctx* ctx, const uint8_t* d_data, uint64_t len,
+                              uint64_t* d_tape, uint64_t tape_cap, uint64_t* n_tape,
+                              int32_t* error, uint64_t* error_pos);
+
 /* ---------------------------------------------------------------------
  * Arrow export. The reference hands its columns to Arrow by wrapping the
  * buffers, since they are Arrow's own layouts (examples/arrow_interop.rs:

@@ -1129,6 +1129,134 @@ uint64_t ndjson_terms(const uint8_t* data, uint64_t len) {
     return count;
 }
 
+// ---- JSON nested tape (kernels/json.rs:636-760, 996-1108) -----------------
+// json_dialect (formats.rs:164-173).
+falx_spec json_spec() {
+    falx_spec s;
+    std::memset(&s, 0, sizeof s);
+    const char* b = "{}[],:";
+    for (int i = 0; i < 6; ++i) s.structural[i] = (uint8_t)b[i];
+    s.n_structural = 6;
+    s.quote = '"'; s.escape = '\\'; s.comment = -1;
+    return s;
+}
+
+constexpr uint64_t kNoPartner = 0xFFFFFFFFull;
+
+uint8_t close_of(uint8_t b) { return b == '{' ? '}' : b == '[' ? ']' : 0; }
+
+struct Nest {
+    std::vector<uint64_t> tape;
+    int32_t error = 0;       // falx_nest_error
+    uint64_t error_pos = 0;
+};
+
+// push_nested (json.rs:692-760) over the structural positions, serially:
+// separators append their position; an open pushes (tape index << 8) | the
+// close byte it expects and appends (no partner << 32) | pos; a close pops
+// and patches both entries, or stops the tape at an empty stack or a wrong
+// kind. Then the innermost open left on the stack (json.rs:651-653).
+void nest_serial(const uint8_t* data, const std::vector<uint32_t>& idx, Nest& out) {
+    std::vector<uint64_t>& tape = out.tape;
+    tape.clear();
+    tape.reserve(idx.size());
+    std::vector<uint64_t> stack;
+    for (uint32_t pos : idx) {
+        const uint8_t b = data[pos];
+        if (b == '{' || b == '[') {
+            stack.push_back(((uint64_t)tape.size() << 8) | close_of(b));
+            tape.push_back((kNoPartner << 32) | pos);
+        } else if (b == '}' || b == ']') {
+            if (stack.empty() || (uint8_t)stack.back() != b) {
+                out.error = FALX_NEST_UNMATCHED_CLOSE;
+                out.error_pos = pos;
+                return;
+            }
+            const uint64_t open = stack.back() >> 8;
+            stack.pop_back();
+            tape[open] = ((uint64_t)tape.size() << 32) | (uint32_t)tape[open];
+            tape.push_back((open << 32) | pos);
+        } else {
+            tape.push_back(pos);
+        }
+    }
+    if (!stack.empty()) {
+        out.error = FALX_NEST_UNCLOSED_OPEN;
+        out.error_pos = (uint32_t)tape[stack.back() >> 8];
+    }
+}
+
+// parse_nested_par (json.rs:1010-1108): chunks of the tape matched
+// concurrently into their own slot ranges (here chunks of the structural
+// index, whose entry counts are the slot counts); a close with no local open
+// is left pending, opens left on a chunk's stack are carried; a pending
+// close is matched in order against the opens of earlier chunks (the
+// parenthesis reduction). Any definite mismatch falls back to the serial
+// pass, as the reference does.
+void nest_par(const uint8_t* data, const std::vector<uint32_t>& idx, int threads, Nest& out) {
+    const uint64_t n = idx.size();
+    const int T = std::max(1, std::min<int>(threads, (int)(n / 64 + 1)));
+    if (T == 1) return nest_serial(data, idx, out);
+    std::vector<uint64_t>& tape = out.tape;
+    tape.assign(n, 0);
+    struct Part {
+        bool bad = false;
+        std::vector<uint64_t> opens;     // (tape index << 8) | close byte, bottom to top
+        std::vector<uint64_t> pending;   // (tape index << 32) | pos, in order
+    };
+    std::vector<Part> parts(T);
+    std::vector<std::thread> th;
+    for (int t = 0; t < T; ++t)
+        th.emplace_back([&, t] {
+            const uint64_t lo = n * t / T, hi = n * (t + 1) / T;
+            Part& P = parts[t];
+            for (uint64_t i = lo; i < hi; ++i) {
+                const uint32_t pos = idx[i];
+                const uint8_t b = data[pos];
+                if (b == '{' || b == '[') {
+                    P.opens.push_back((i << 8) | close_of(b));
+                    tape[i] = (kNoPartner << 32) | pos;
+                } else if (b == '}' || b == ']') {
+                    if (P.opens.empty()) {
+                        P.pending.push_back((i << 32) | pos);
+                        tape[i] = pos;
+                        continue;
+                    }
+                    if ((uint8_t)P.opens.back() != b) {
+                        P.bad = true;
+                        return;
+                    }
+                    const uint64_t open = P.opens.back() >> 8;
+                    P.opens.pop_back();
+                    tape[open] = (i << 32) | (uint32_t)tape[open];
+                    tape[i] = (open << 32) | pos;
+                } else {
+                    tape[i] = pos;
+                }
+            }
+        });
+    for (auto& x : th) x.join();
+    for (const Part& P : parts)
+        if (P.bad) return nest_serial(data, idx, out);
+    std::vector<uint64_t> g;
+    for (const Part& P : parts) {
+        for (uint64_t pend : P.pending) {
+            const uint64_t ci = pend >> 32;
+            const uint32_t cpos = (uint32_t)pend;
+            if (g.empty() || (uint8_t)g.back() != data[cpos]) return nest_serial(data, idx, out);
+            const uint64_t oi = g.back() >> 8;
+            g.pop_back();
+            tape[oi] = (ci << 32) | (uint32_t)tape[oi];
+            tape[ci] = (oi << 32) | cpos;
+        }
+        g.insert(g.end(), P.opens.begin(), P.opens.end());
+    }
+    if (!g.empty()) {
+        out.error = FALX_NEST_UNCLOSED_OPEN;
+        out.error_pos = (uint32_t)tape[g.back() >> 8];
+    }
+}
+
 }  // namespace
 
 // ===========================================================================
@@ -1446,6 +1574,25 @@ uint64_t ora_clean(const falx_spec* spec, const uint8_t* s, uint64_t n, uint8_t*
     uint64_t k = 0;
     clean_cell(d, s, n, [&](uint8_t b) { out[k++] = b; });
     return k;
+}
+
+}  // extern "C"
+
+extern "C" {
+
+uint64_t ora_parse_nested(const uint8_t* data, uint64_t len, int threads, uint64_t* tape, uint64_t cap,
+                          int32_t* error, uint64_t* error_pos) {
+    falx_spec sp = json_spec();
+    std::vector<uint32_t> idx(len + 1);
+    idx.resize(ora_index_structurals(&sp, data, len, threads, idx.data()));
+    Nest out;
+    if (threads <= 1) nest_serial(data, idx, out);
+    else nest_par(data, idx, threads, out);
+    const uint64_t n = out.tape.size();
+    std::memcpy(tape, out.tape.data(), std::min(n, cap) * 8);
+    *error = out.error;
+    *error_pos = out.error_pos;
+    return n;
 }
 
 }  // extern "C"

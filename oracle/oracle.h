@@ -57,6 +57,13 @@ int ora_parse_i64(const uint8_t* s, uint64_t n, int64_t* out);
 /* Dialect cleaning (codegen.rs:8008-8185). Returns the cleaned length. */
 uint64_t ora_clean(const falx_spec* spec, const uint8_t* s, uint64_t n, uint8_t* out);
 
+/* json::parse_nested / parse_nested_par (kernels/json.rs:636-760,
+ * 996-1108): the nested tape of the JSON structural index into tape[0, cap)
+ * (cap >= len always suffices); returns its length. error / error_pos as
+ * falx_parse_nested. */
+uint64_t ora_parse_nested(const uint8_t* data, uint64_t len, int threads, uint64_t* tape,
+                          uint64_t cap, int32_t* error, uint64_t* error_pos);
+
 /* Which block-classifier tier the CPU dispatches ("avx512bw" or "scalar"). */
 const char* ora_isa(void);
 

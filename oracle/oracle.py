@@ -58,6 +58,9 @@ def lib():
         L.ora_parse_i64.argtypes = [u8p, C.c_uint64, P(C.c_int64)]
         L.ora_clean.restype = C.c_uint64
         L.ora_clean.argtypes = [specp, u8p, C.c_uint64, u8p]
+        L.ora_parse_nested.restype = C.c_uint64
+        L.ora_parse_nested.argtypes = [u8p, C.c_uint64, C.c_int, u64p, C.c_uint64, P(C.c_int32),
+                                       P(C.c_uint64)]
         L.ora_isa.restype = C.c_char_p
         _lib = L
     return _lib
@@ -231,3 +234,12 @@ def clean(spec: S.Spec, cell: bytes) -> bytes:
     cs = spec.to_c()
     n = lib().ora_clean(C.byref(cs), _ptr(d), d.size, _ptr(out))
     return out[:n].tobytes()
+
+
+def parse_nested(data, threads: int = 1):
+    """(tape uint64 array, error, error_pos): json::parse_nested[_par]."""
+    d = _buf(data)
+    tape = np.zeros(max(1, d.size), dtype=np.uint64)
+    err, pos = C.c_int32(), C.c_uint64()
+    n = lib().ora_parse_nested(_ptr(d), d.size, threads, _ptr(tape), tape.size, C.byref(err), C.byref(pos))
+    return tape[:n].copy(), err.value, pos.value
