@@ -148,6 +148,69 @@ def sum_errors(name: str, gpu_tree: float, ref_fold: float, exact) -> dict:
             f"{name}_err_gpu_vs_fold": abs(gpu_tree - ref_fold)}
 
 
+def nested_lane(ctx, nd, nd_buf, args, lanes, thr) -> None:
+    """The JSON nested tape (json::parse_nested) of the 1 GiB NDJSON document:
+    GPU throughput, algorithmic bytes = input + 8 B per tape entry, checked
+    at full size against the JSON structural index (positions), the pairing
+    invariants (partners mutual, opens before closes, matching kinds) and,
+    with the CPU leg, the oracle's tape word for word."""
+    import numpy as np
+
+    from falx_amd import spec as S
+    from oracle import oracle as O
+
+    nd_n = int(nd.size)
+    probe = ctx.alloc(8)
+    _, need, _, _ = ctx.parse_nested_into(nd_buf, nd_n, probe)
+    probe.free()
+    tape = ctx.alloc(8 * max(1, need))
+    res = [None]
+
+    def step():
+        res[0] = ctx.parse_nested_into(nd_buf, nd_n, tape)
+    log("timing lane json_nested_tape")
+    lms, lkms = time_steps(step, max(3, args.steps // 2), 1, ctx)
+    st, cnt, err, _ = res[0]
+    ab = nd_n + 8 * cnt
+    lane = {"gib_s": round(nd_n / GIB / (lms / 1000.0), 2), "ms_per_step": round(lms, 3),
+            "kernel_ms": round(lkms, 3), "hbm_frac": round(ab / (lkms / 1000.0) / 1e9 / HBM_PEAK_GBS, 3),
+            "bytes": nd_n, "entries": int(cnt), "algorithmic_bytes": int(ab), "error": int(err)}
+    got = tape.download(np.uint64, cnt)
+    tape.free()
+    jspec = S.Spec("json", b"{}[],:", ord('"'), ord("\\"), None, ())
+    idx = ctx.index_structurals(jspec, nd_buf, nd_n)
+    pos = (got & np.uint64(0xFFFFFFFF)).astype(np.uint32)
+    part = (got >> np.uint64(32)).astype(np.int64)
+    del got
+    kinds = nd[pos]
+    opens = (kinds == ord("{")) | (kinds == ord("["))
+    closes = (kinds == ord("}")) | (kinds == ord("]"))
+    i = np.arange(cnt, dtype=np.int64)
+    po = part[opens]
+    ok = (st == S.OK and err == 0 and np.array_equal(pos, idx) and bool(np.all(part[~(opens | closes)] == 0))
+          and bool(np.all((po > i[opens]) & (po < cnt))) and bool(np.all(part[po] == i[opens]))
+          and bool(np.all(kinds[po] == kinds[opens] + 2)) and bool(np.all(part[closes] < i[closes])))
+    lane["verified"] = bool(ok)
+    del idx, kinds, opens, closes, po, i
+    if not args.no_cpu:
+        log("timing CPU oracle lane json_nested_tape")
+        lib = O.lib()
+        ct = np.empty(max(1, nd_n), dtype=np.uint64)
+        e, p = C.c_int32(), C.c_uint64()
+
+        def cpu():
+            return lib.ora_parse_nested(C.c_void_p(nd.ctypes.data), nd_n, thr, C.c_void_p(ct.ctypes.data), ct.size,
+                                        C.byref(e), C.byref(p))
+        lane["cpu_gib_s"] = round(nd_n / GIB / cpu_time(cpu, args.cpu_runs), 3)
+        lane["cpu_threads"] = thr
+        n_c = cpu()
+        lane["oracle_equal"] = bool(n_c == cnt and e.value == err and np.array_equal(
+            (ct[:n_c] >> np.uint64(32)).astype(np.int64), part) and np.array_equal(
+            (ct[:n_c] & np.uint64(0xFFFFFFFF)).astype(np.uint32), pos))
+        del ct
+    lanes["json_nested_tape"] = lane
+
+
 def served_stream(ctx) -> dict:
     """The mixed request stream (geo CSV, csv_hash, TSV, logfmt, NDJSON;
     sizes log-uniform 4 KiB-256 MiB; Poisson arrivals) through continuous
@@ -329,6 +392,7 @@ def main() -> int:
                 fold = O.ndjson_id_score(nd, 1)["score_sum"]
                 lanes["ndjson_id_score_sum"]["sums"] = sum_errors(
                     "score", r["score_sum"], fold, Fraction(nd_truth["score_fixed6_sum"], 10**6))
+            nested_lane(ctx, nd, nd_buf, args, lanes, thr)
             nd_buf.free()
             del nd
 

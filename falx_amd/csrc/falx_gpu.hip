@@ -15,6 +15,7 @@ struct falx_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipEvent_t ev_count = nullptr;   // a count copied home mid-call
     float last_ms = 0.f;
     unsigned long long* d_result = nullptr;
     unsigned long long* h_result = nullptr;
@@ -299,6 +300,7 @@ falx_status falx_ctx_create(int device, falx_ctx** out) {
     c->device = device;
     if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
+        hipEventCreateWithFlags(&c->ev_count, hipEventDisableTiming) != hipSuccess ||
         hipMalloc(&c->d_result, R_SLOTS * 8) != hipSuccess ||
         hipMalloc(&c->d_red, RED_BYTES) != hipSuccess ||
         hipHostMalloc(&c->h_result, R_SLOTS * 8, hipHostMallocDefault) != hipSuccess ||
@@ -322,6 +324,7 @@ void falx_ctx_destroy(falx_ctx* c) {
     if (c->h_result) (void)hipHostFree(c->h_result);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->ev_count) (void)hipEventDestroy(c->ev_count);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -778,11 +781,18 @@ falx_status falx_parse_nested(falx_ctx* c, const uint8_t* d, uint64_t len, uint6
     if ((st = prepare(c, &spec, d, len, P)) != FALX_OK) return st;
     HIP_TRY(mark_start(c));
     HIP_TRY(run_prefix(c, P, kind));
-    // The entry count sizes the matcher's grid and scratch: one round trip.
+    // The entry count sizes the matcher's grid and scratch: it is copied
+    // home behind the prefix while the index pass (which needs only the
+    // tape's capacity) runs.
     HIP_TRY(hipMemcpyAsync(c->h_result, c->d_result, 8, hipMemcpyDeviceToHost, c->stream));
-    HIP_TRY(hipStreamSynchronize(c->stream));
+    HIP_TRY(hipEventRecord(c->ev_count, c->stream));
+    unsigned long long* tape = reinterpret_cast<unsigned long long*>(d_tape);
+    P.ends = tape;
+    P.ends_cap = d_tape ? tape_cap : 0;
+    if (d_tape) HIP_TRY(launch_scan<M_INDEX>(c, P, kind));
+    HIP_TRY(hipEventSynchronize(c->ev_count));
     const uint64_t n = c->h_result[R_N_STRUCT];
-    if (n == 0 || n > tape_cap || !d_tape) {
+    if (n == 0 || n > P.ends_cap) {
         HIP_TRY(mark_end(c));
         if ((st = finish(c)) != FALX_OK) return st;
         *n_tape = n;
@@ -805,10 +815,6 @@ falx_status falx_parse_nested(falx_ctx* c, const uint8_t* d, uint64_t len, uint6
         S.lv_n[k] = L.lv_n[k];
     }
     S.result = c->d_result;
-    unsigned long long* tape = reinterpret_cast<unsigned long long*>(d_tape);
-    P.ends = tape;
-    P.ends_cap = tape_cap;
-    HIP_TRY(launch_scan<M_INDEX>(c, P, kind));
     hipLaunchKernelGGL(k_nest_local, dim3(n_tiles), dim3(WG), 0, c->stream, tape, (uint32_t)n, S);
     hipLaunchKernelGGL(k_nest_prefix, dim3(1), dim3(NT_PREFIX_WG), 0, c->stream, n_tiles, S);
     for (int k = 1; k < NT_LEVELS; ++k)

@@ -71,9 +71,9 @@ struct NestScratch {
 DEV uint32_t nt_blocks_of(uint32_t tile, uint32_t n) { return (min(NT, n - tile * NT) + 63u) / 64u; }
 
 __global__ __launch_bounds__(WG) void k_nest_local(unsigned long long* tape, uint32_t n, NestScratch S) {
-    __shared__ uint32_t spos[NT];
-    __shared__ uint8_t skind[NT];
-    __shared__ int16_t sdep[NT];      // tile-relative depth after each entry
+    __shared__ __align__(16) uint32_t shi[NT];     // high words: the bracket byte as indexed, then the partner
+    __shared__ __align__(16) uint8_t skind[NT];
+    __shared__ __align__(16) int16_t sdep[NT];     // tile-relative depth after each entry
     __shared__ int32_t sbmin[NT_BLOCKS];
     __shared__ uint32_t scan_sh[NWAVE];
     __shared__ uint32_t nres, smis;
@@ -85,32 +85,44 @@ __global__ __launch_bounds__(WG) void k_nest_local(unsigned long long* tape, uin
         nres = 0;
         smis = NT_NONE;
     }
-    for (uint32_t k = tid; k < NT; k += WG) {
-        uint32_t pos = 0, kind = 0;
-        if (k < cnt) {
-            const unsigned long long v = tape[e0 + k];
-            pos = (uint32_t)v;
-            kind = nt_kind((uint32_t)(v >> 32));
-        }
-        spos[k] = pos;
-        skind[k] = (uint8_t)kind;
+    // The thread's entries k = tid + WG j stay in registers for the final
+    // write; LDS holds their high words and kinds.
+    constexpr uint32_t PER = NT / WG;
+    unsigned long long word[PER];
+#pragma unroll
+    for (uint32_t j = 0; j < PER; ++j) {
+        const uint32_t k = tid + WG * j;
+        word[j] = k < cnt ? tape[e0 + k] : 0ull;
+    }
+#pragma unroll
+    for (uint32_t j = 0; j < PER; ++j) {
+        const uint32_t k = tid + WG * j;
+        const uint32_t hi = (uint32_t)(word[j] >> 32);
+        shi[k] = hi;
+        skind[k] = (uint8_t)nt_kind(hi);
     }
     __syncthreads();
 
     // Depths: thread t owns entries [16t, 16t + 16), one block scan.
+    const uint4 kw = *reinterpret_cast<const uint4*>(&skind[tid * 16]);
+    const uint32_t kwords[4] = {kw.x, kw.y, kw.z, kw.w};
     int32_t sum = 0;
 #pragma unroll
-    for (uint32_t q = 0; q < 16; ++q) sum += nt_delta(skind[tid * 16 + q]);
+    for (uint32_t q = 0; q < 16; ++q) sum += nt_delta((kwords[q >> 2] >> (8 * (q & 3))) & 0xFFu);
     uint32_t tot;
     const int32_t before = (int32_t)block_excl_sum((uint32_t)sum, scan_sh, tot);
     int32_t run = before, mn = 0x7FFF;
+    uint32_t dw[8];
 #pragma unroll
     for (uint32_t q = 0; q < 16; ++q) {
-        const uint32_t k = tid * 16 + q;
-        run += nt_delta(skind[k]);
-        sdep[k] = (int16_t)run;
-        if (k < cnt) mn = min(mn, run);
+        run += nt_delta((kwords[q >> 2] >> (8 * (q & 3))) & 0xFFu);
+        if (tid * 16 + q < cnt) mn = min(mn, run);
+        const uint32_t h = (uint32_t)(uint16_t)(int16_t)run;
+        if (q & 1) dw[q >> 1] |= h << 16;
+        else dw[q >> 1] = h;
     }
+    *reinterpret_cast<uint4*>(&sdep[tid * 16]) = make_uint4(dw[0], dw[1], dw[2], dw[3]);
+    *reinterpret_cast<uint4*>(&sdep[tid * 16 + 8]) = make_uint4(dw[4], dw[5], dw[6], dw[7]);
     // A 64-entry block is four consecutive threads.
     mn = min(mn, __shfl_xor(mn, 1));
     mn = min(mn, __shfl_xor(mn, 2));
@@ -123,10 +135,9 @@ __global__ __launch_bounds__(WG) void k_nest_local(unsigned long long* tape, uin
 
     uint32_t mis = NT_NONE;
     auto pair = [&](uint32_t ki, uint32_t kj, uint32_t open_kind) {
-        const uint32_t gi = e0 + ki, gj = e0 + kj;
-        tape[gi] = ((unsigned long long)gj << 32) | spos[ki];
-        tape[gj] = ((unsigned long long)gi << 32) | spos[kj];
-        if (!nt_pair_ok(open_kind, skind[kj])) mis = min(mis, gj);
+        shi[ki] = e0 + kj;
+        shi[kj] = e0 + ki;
+        if (!nt_pair_ok(open_kind, skind[kj])) mis = min(mis, e0 + kj);
     };
     for (uint32_t b = wave; b < nb; b += NWAVE) {
         const uint32_t k = b * 64u + lane;
@@ -167,7 +178,7 @@ __global__ __launch_bounds__(WG) void k_nest_local(unsigned long long* tape, uin
                 const uint64_t ce = ballot(k2 < cnt && (int32_t)sdep[k2] < L);
                 if (ce && lane == 0) pair(ki, b2 * 64u + (uint32_t)__ffsll((long long)ce) - 1u, okind);
             } else if (lane == 0) {
-                tape[e0 + ki] = ((unsigned long long)NT_NONE << 32) | spos[ki];
+                shi[ki] = NT_NONE;
                 const uint32_t slot = atomicAdd(&nres, 1u);
                 S.res[(size_t)tile * NT + slot] = nt_res(ki, okind, L);
             }
@@ -177,6 +188,14 @@ __global__ __launch_bounds__(WG) void k_nest_local(unsigned long long* tape, uin
     for (int d = 32; d >= 1; d >>= 1) mis = min(mis, (uint32_t)__shfl_xor((int)mis, d));
     if (lane == 0 && mis != NT_NONE) atomicMin(&smis, mis);
     __syncthreads();
+    // The words that changed (paired or listed brackets); a close paired
+    // outside the tile keeps its bracket byte for k_nest_far's kind check.
+#pragma unroll
+    for (uint32_t j = 0; j < PER; ++j) {
+        const uint32_t k = tid + WG * j;
+        const uint32_t hi = shi[k];
+        if (k < cnt && hi != (uint32_t)(word[j] >> 32)) tape[e0 + k] = ((unsigned long long)hi << 32) | (uint32_t)word[j];
+    }
     if (tid == 0) {
         int32_t tm = 0x7FFF;
         for (uint32_t b = 0; b < nb; ++b) tm = min(tm, sbmin[b]);
@@ -188,20 +207,37 @@ __global__ __launch_bounds__(WG) void k_nest_local(unsigned long long* tape, uin
 }
 
 // One workgroup: depth before each tile (an exclusive scan in rounds of
-// 1024 tiles), the tile minima as min-tree leaves, and the result slots.
+// 4096 tiles, four per thread), the tile minima as min-tree leaves, and the
+// result slots.
 constexpr int NT_PREFIX_WG = 1024;
 
 __global__ __launch_bounds__(NT_PREFIX_WG) void k_nest_prefix(uint32_t n_tiles, NestScratch S) {
     __shared__ long long wsum[NT_PREFIX_WG / 64];
     __shared__ uint32_t wmis[NT_PREFIX_WG / 64];
     const uint32_t tid = threadIdx.x, lane = __lane_id(), w = tid >> 6;
+    constexpr uint32_t ROUND = 4 * NT_PREFIX_WG;   // four consecutive tiles per thread per round
     long long carry = 0;
     uint32_t mis = NT_NONE;
-    for (uint32_t t0 = 0; t0 < n_tiles; t0 += NT_PREFIX_WG) {
-        const uint32_t t = t0 + tid;
-        const long long v = t < n_tiles ? (long long)S.tsum[t] : 0;
-        if (t < n_tiles) mis = min(mis, S.tmis[t]);
-        long long inc = v;
+    for (uint32_t r0 = 0; r0 < n_tiles; r0 += ROUND) {
+        const uint32_t t = r0 + 4 * tid;
+        int32_t v[4] = {0, 0, 0, 0}, mn[4] = {0, 0, 0, 0};
+        if (t + 4 <= n_tiles) {
+            const int4 a = *reinterpret_cast<const int4*>(S.tsum + t);
+            const int4 b = *reinterpret_cast<const int4*>(S.tmin + t);
+            const uint4 c = *reinterpret_cast<const uint4*>(S.tmis + t);
+            v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+            mn[0] = b.x; mn[1] = b.y; mn[2] = b.z; mn[3] = b.w;
+            mis = min(mis, min(min(c.x, c.y), min(c.z, c.w)));
+        } else {
+            for (uint32_t q = 0; q < 4; ++q)
+                if (t + q < n_tiles) {
+                    v[q] = S.tsum[t + q];
+                    mn[q] = S.tmin[t + q];
+                    mis = min(mis, S.tmis[t + q]);
+                }
+        }
+        const long long sum = (long long)v[0] + v[1] + v[2] + v[3];
+        long long inc = sum;
 #pragma unroll
         for (int d = 1; d < 64; d <<= 1) {
             const long long u = __shfl_up(inc, d);
@@ -209,16 +245,18 @@ __global__ __launch_bounds__(NT_PREFIX_WG) void k_nest_prefix(uint32_t n_tiles, 
         }
         if (lane == 63) wsum[w] = inc;
         __syncthreads();
-        long long wb = 0, all = 0;
+        long long run = carry + inc - sum, all = 0;
         for (uint32_t k = 0; k < NT_PREFIX_WG / 64; ++k) {
-            if (k < w) wb += wsum[k];
+            if (k < w) run += wsum[k];
             all += wsum[k];
         }
         __syncthreads();
-        if (t < n_tiles) {
-            const long long b = carry + wb + inc - v;
-            S.base[t] = b;
-            S.lv[0][t] = b + S.tmin[t];
+        for (uint32_t q = 0; q < 4; ++q) {
+            if (t + q < n_tiles) {
+                S.base[t + q] = run;
+                S.lv[0][t + q] = run + mn[q];
+            }
+            run += v[q];
         }
         carry += all;
     }

@@ -14,6 +14,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <locale.h>
+#include <memory>
 #include <thread>
 #include <vector>
 
@@ -1145,8 +1146,10 @@ constexpr uint64_t kNoPartner = 0xFFFFFFFFull;
 
 uint8_t close_of(uint8_t b) { return b == '{' ? '}' : b == '[' ? ']' : 0; }
 
+// The tape is written into caller memory of at least idx.size() entries.
 struct Nest {
-    std::vector<uint64_t> tape;
+    uint64_t* tape = nullptr;
+    uint64_t n = 0;          // tape entries
     int32_t error = 0;       // falx_nest_error
     uint64_t error_pos = 0;
 };
@@ -1156,30 +1159,35 @@ struct Nest {
 // close byte it expects and appends (no partner << 32) | pos; a close pops
 // and patches both entries, or stops the tape at an empty stack or a wrong
 // kind. Then the innermost open left on the stack (json.rs:651-653).
-void nest_serial(const uint8_t* data, const std::vector<uint32_t>& idx, Nest& out) {
-    std::vector<uint64_t>& tape = out.tape;
-    tape.clear();
-    tape.reserve(idx.size());
+void nest_serial(const uint8_t* data, const uint32_t* idx, uint64_t n_idx, Nest& out) {
+    uint64_t* tape = out.tape;
+    uint64_t t = 0;
+    out.error = 0;
+    out.error_pos = 0;
     std::vector<uint64_t> stack;
-    for (uint32_t pos : idx) {
+    for (uint64_t k = 0; k < n_idx; ++k) {
+        const uint32_t pos = idx[k];
         const uint8_t b = data[pos];
         if (b == '{' || b == '[') {
-            stack.push_back(((uint64_t)tape.size() << 8) | close_of(b));
-            tape.push_back((kNoPartner << 32) | pos);
+            stack.push_back((t << 8) | close_of(b));
+            tape[t++] = (kNoPartner << 32) | pos;
         } else if (b == '}' || b == ']') {
             if (stack.empty() || (uint8_t)stack.back() != b) {
                 out.error = FALX_NEST_UNMATCHED_CLOSE;
                 out.error_pos = pos;
+                out.n = t;
                 return;
             }
             const uint64_t open = stack.back() >> 8;
             stack.pop_back();
-            tape[open] = ((uint64_t)tape.size() << 32) | (uint32_t)tape[open];
-            tape.push_back((open << 32) | pos);
+            tape[open] = (t << 32) | (uint32_t)tape[open];
+            tape[t] = (open << 32) | pos;
+            ++t;
         } else {
-            tape.push_back(pos);
+            tape[t++] = pos;
         }
     }
+    out.n = t;
     if (!stack.empty()) {
         out.error = FALX_NEST_UNCLOSED_OPEN;
         out.error_pos = (uint32_t)tape[stack.back() >> 8];
@@ -1193,12 +1201,10 @@ void nest_serial(const uint8_t* data, const std::vector<uint32_t>& idx, Nest& ou
 // close is matched in order against the opens of earlier chunks (the
 // parenthesis reduction). Any definite mismatch falls back to the serial
 // pass, as the reference does.
-void nest_par(const uint8_t* data, const std::vector<uint32_t>& idx, int threads, Nest& out) {
-    const uint64_t n = idx.size();
+void nest_par(const uint8_t* data, const uint32_t* idx, uint64_t n, int threads, Nest& out) {
     const int T = std::max(1, std::min<int>(threads, (int)(n / 64 + 1)));
-    if (T == 1) return nest_serial(data, idx, out);
-    std::vector<uint64_t>& tape = out.tape;
-    tape.assign(n, 0);
+    if (T == 1) return nest_serial(data, idx, n, out);
+    uint64_t* tape = out.tape;
     struct Part {
         bool bad = false;
         std::vector<uint64_t> opens;     // (tape index << 8) | close byte, bottom to top
@@ -1237,13 +1243,13 @@ void nest_par(const uint8_t* data, const std::vector<uint32_t>& idx, int threads
         });
     for (auto& x : th) x.join();
     for (const Part& P : parts)
-        if (P.bad) return nest_serial(data, idx, out);
+        if (P.bad) return nest_serial(data, idx, n, out);
     std::vector<uint64_t> g;
     for (const Part& P : parts) {
         for (uint64_t pend : P.pending) {
             const uint64_t ci = pend >> 32;
             const uint32_t cpos = (uint32_t)pend;
-            if (g.empty() || (uint8_t)g.back() != data[cpos]) return nest_serial(data, idx, out);
+            if (g.empty() || (uint8_t)g.back() != data[cpos]) return nest_serial(data, idx, n, out);
             const uint64_t oi = g.back() >> 8;
             g.pop_back();
             tape[oi] = (ci << 32) | (uint32_t)tape[oi];
@@ -1251,6 +1257,9 @@ void nest_par(const uint8_t* data, const std::vector<uint32_t>& idx, int threads
         }
         g.insert(g.end(), P.opens.begin(), P.opens.end());
     }
+    out.n = n;
+    out.error = 0;
+    out.error_pos = 0;
     if (!g.empty()) {
         out.error = FALX_NEST_UNCLOSED_OPEN;
         out.error_pos = (uint32_t)tape[g.back() >> 8];
@@ -1583,16 +1592,19 @@ extern "C" {
 uint64_t ora_parse_nested(const uint8_t* data, uint64_t len, int threads, uint64_t* tape, uint64_t cap,
                           int32_t* error, uint64_t* error_pos) {
     falx_spec sp = json_spec();
-    std::vector<uint32_t> idx(len + 1);
-    idx.resize(ora_index_structurals(&sp, data, len, threads, idx.data()));
+    // Uninitialised: pages are touched only as the index fills them.
+    std::unique_ptr<uint32_t[]> idx(new uint32_t[len + 1]);
+    const uint64_t n_idx = ora_index_structurals(&sp, data, len, threads, idx.get());
+    *error = 0;
+    *error_pos = 0;
+    if (n_idx > cap) return n_idx;   // the tape needs at most n_idx entries
     Nest out;
-    if (threads <= 1) nest_serial(data, idx, out);
-    else nest_par(data, idx, threads, out);
-    const uint64_t n = out.tape.size();
-    std::memcpy(tape, out.tape.data(), std::min(n, cap) * 8);
+    out.tape = tape;
+    if (threads <= 1) nest_serial(data, idx.get(), n_idx, out);
+    else nest_par(data, idx.get(), n_idx, threads, out);
     *error = out.error;
     *error_pos = out.error_pos;
-    return n;
+    return out.n;
 }
 
 }  // extern "C"

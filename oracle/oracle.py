@@ -239,7 +239,9 @@ def clean(spec: S.Spec, cell: bytes) -> bytes:
 def parse_nested(data, threads: int = 1):
     """(tape uint64 array, error, error_pos): json::parse_nested[_par]."""
     d = _buf(data)
-    tape = np.zeros(max(1, d.size), dtype=np.uint64)
+    # At most one entry per input byte; np.empty leaves untouched pages
+    # uncommitted.
+    tape = np.empty(max(1, d.size), dtype=np.uint64)
     err, pos = C.c_int32(), C.c_uint64()
     n = lib().ora_parse_nested(_ptr(d), d.size, threads, _ptr(tape), tape.size, C.byref(err), C.byref(pos))
     return tape[:n].copy(), err.value, pos.value
