@@ -229,6 +229,14 @@ hipError_t fill_u64(hipStream_t s, unsigned long long* p, uint64_t n, unsigned l
     return hipGetLastError();
 }
 
+// A device-to-host read on the context's stream (never the null stream,
+// whose implicit synchronisation would break graph captures running on
+// other streams of the process).
+hipError_t read_back(falx_ctx* c, void* h, const void* d, size_t n) {
+    hipError_t e = hipMemcpyAsync(h, d, n, hipMemcpyDeviceToHost, c->stream);
+    return e == hipSuccess ? hipStreamSynchronize(c->stream) : e;
+}
+
 falx_status begin(falx_ctx* c) {
     tl_hip_error = 0;
     (void)hipGetLastError();  // launches below report their own errors only
@@ -540,9 +548,9 @@ static falx_status reduce_stats(falx_ctx* c, const falx_spec* spec, uint32_t n_t
     if (st != FALX_OK) return st;
     StatPart host[FALX_MAX_COLUMNS];
     const char* red = static_cast<const char*>(c->d_red) + RED_OUT;
-    if (spec->n_columns) HIP_TRY(hipMemcpy(host, red, sizeof(StatPart) * spec->n_columns, hipMemcpyDeviceToHost));
+    if (spec->n_columns) HIP_TRY(read_back(c, host, red, sizeof(StatPart) * spec->n_columns));
     unsigned long long records = 0;
-    HIP_TRY(hipMemcpy(&records, red + rec_off, 8, hipMemcpyDeviceToHost));
+    HIP_TRY(read_back(c, &records, red + rec_off, 8));
     out->records = records;
     for (uint32_t k = 0; k < spec->n_columns; ++k) {
         out->cols[k].values = host[k].values;
@@ -614,7 +622,7 @@ falx_status falx_parse_field_bytes(falx_ctx* c, const falx_spec* spec, const uin
     const char* red = static_cast<const char*>(c->d_red) + RED_OUT;
     if ((st = finish(c)) != FALX_OK) return st;
     unsigned long long h[2];
-    HIP_TRY(hipMemcpy(h, red, 16, hipMemcpyDeviceToHost));
+    HIP_TRY(read_back(c, h, red, 16));
     out->fields = h[0];
     out->bytes = h[1];
     return FALX_OK;
@@ -648,7 +656,7 @@ falx_status falx_parse_ndjson_lines(falx_ctx* c, const uint8_t* d, uint64_t len,
     HIP_TRY(mark_end(c));
     if ((st = finish(c)) != FALX_OK) return st;
     uint8_t last = 0;
-    HIP_TRY(hipMemcpy(&last, d + len - 1, 1, hipMemcpyDeviceToHost));
+    HIP_TRY(read_back(c, &last, d + len - 1, 1));
     *records = c->h_result[R_N_TERMS] + (last != '\n' ? 1 : 0);  // ndjson_has_trailing_record (ndjson.rs:853-855)
     return FALX_OK;
 }
@@ -686,7 +694,7 @@ falx_status falx_parse_ndjson_id_score(falx_ctx* c, const uint8_t* d, uint64_t l
     const NdPart* red = reinterpret_cast<const NdPart*>(static_cast<const char*>(c->d_red) + RED_OUT);
     if ((st = finish(c)) != FALX_OK) return st;
     NdPart h;
-    HIP_TRY(hipMemcpy(&h, red, sizeof h, hipMemcpyDeviceToHost));
+    HIP_TRY(read_back(c, &h, red, sizeof h));
     out->records = h.records;
     out->invalid = h.invalid;
     out->id_sum = h.id_sum;
@@ -716,7 +724,7 @@ falx_status falx_parse_logfmt_pairs(falx_ctx* c, const uint8_t* d, uint64_t len,
     const char* red = static_cast<const char*>(c->d_red) + RED_OUT;
     if ((st = finish(c)) != FALX_OK) return st;
     unsigned long long h[4];
-    HIP_TRY(hipMemcpy(h, red, 32, hipMemcpyDeviceToHost));
+    HIP_TRY(read_back(c, h, red, 32));
     out->pairs = h[0];
     out->key_bytes = h[1];
     out->value_bytes = h[2];
@@ -846,8 +854,7 @@ falx_status falx_parse_nested(falx_ctx* c, const uint8_t* d, uint64_t len, uint6
     }
     if (at != UINT64_MAX) {
         unsigned long long v = 0;
-        HIP_TRY(hipMemcpyAsync(&v, tape + at, 8, hipMemcpyDeviceToHost, c->stream));
-        HIP_TRY(hipStreamSynchronize(c->stream));
+        HIP_TRY(read_back(c, &v, tape + at, 8));
         *error_pos = (uint32_t)v;
     }
     return FALX_OK;

@@ -15,6 +15,12 @@ served form of those calls needs on a GPU:
   outputs fit its HBM budget, submits them as one batch and is polled; a
   request that arrives while batches are on the device waits for the next
   free slot, so batches of different slots overlap on the GPU.
+* Requests of ``large`` bytes or more skip batching: they fill the device
+  on their own, and the plain entry points run them with the fast column
+  passes. On an idle device the loop makes the call itself; otherwise a
+  host thread with a context of its own makes those calls in arrival order
+  (ctypes releases the GIL for each), so batches of small requests keep
+  running beside them.
 * :class:`PlainServer` is the baseline: one plain entry-point call (one set
   of launches, one synchronisation) per request, in arrival order.
 
@@ -25,6 +31,9 @@ from __future__ import annotations
 
 import ctypes as C
 import math
+import queue as _queue
+import sys
+import threading
 import time
 from collections import deque
 from dataclasses import dataclass, field
@@ -155,6 +164,80 @@ def _request(corpus: Corpus, r: Req, arena: Arena, chunk: int = 0) -> gpu.Reques
     return gpu.Request(op, view, r.length, spec=spec, columns=cols, chunk=chunk)
 
 
+def plain_call(ctx: gpu.Context, arena: Arena, corpus: Corpus, r: Req) -> Dict:
+    """One request through its plain entry point (outputs in `arena`)."""
+    name, op, spec, _ = SERVED[r.kind]
+    view = _View(corpus.dev[r.kind], r.offset, r.length)
+    lib = gpu.load()
+    if op == S.OP_COLUMNS_STATS:
+        arena.reset()
+        cols = arena_columns(spec, r.length, arena)
+        st = S.CCsvStats()
+        cs = spec.to_c()
+        status = lib.falx_parse_columns_with_stats(ctx.h, C.byref(cs), view.ptr, r.length,
+                                                   C.byref(cols), C.byref(st))
+        return {"status": status, "stats": gpu._stats_dict(spec, st)}
+    if op == S.OP_LOGFMT_PAIRS:
+        st = S.CLogfmtStats()
+        status = lib.falx_parse_logfmt_pairs(ctx.h, view.ptr, r.length, C.byref(st))
+        return {"status": status, "logfmt": {"pairs": st.pairs, "key_bytes": st.key_bytes,
+                                             "value_bytes": st.value_bytes, "checksum": st.checksum}}
+    st = S.CNdjsonIdScore()
+    status = lib.falx_parse_ndjson_id_score(ctx.h, view.ptr, r.length, C.byref(st))
+    return {"status": status, "ndjson": {"records": st.records, "invalid": st.invalid, "id_sum": st.id_sum,
+                                         "score_fixed6_sum": st.score_fixed6_sum,
+                                         "score_fixed6_cells": st.score_fixed6_cells,
+                                         "score_sum": st.score_sum}}
+
+
+class _LargeLane:
+    """Large requests as plain calls on a host thread and context of their own."""
+
+    def __init__(self, corpus: Corpus, out_budget: int):
+        self.corpus = corpus
+        self.ctx = gpu.Context(0)
+        self.arena = Arena(self.ctx, out_budget)
+        self.todo: _queue.Queue = _queue.Queue()
+        self.finished: _queue.Queue = _queue.Queue()
+        self.busy = 0
+        self.thread = threading.Thread(target=self._work, daemon=True)
+        self.thread.start()
+
+    def _work(self):
+        while True:
+            item = self.todo.get()
+            if item is None:
+                return
+            r, t0 = item
+            try:
+                x = plain_call(self.ctx, self.arena, self.corpus, r)
+            except Exception as e:  # reported as the request's status
+                x = {"status": S.E_DEVICE, "error": repr(e)}
+            r.done, r.status, r.result = time.perf_counter() - t0, x["status"], x
+            self.finished.put(r)
+
+    def submit(self, r: Req, t0: float):
+        self.busy += 1
+        self.todo.put((r, t0))
+
+    def reap(self) -> int:
+        n = 0
+        while True:
+            try:
+                self.finished.get_nowait()
+            except _queue.Empty:
+                break
+            n += 1
+        self.busy -= n
+        return n
+
+    def close(self):
+        self.todo.put(None)
+        self.thread.join()
+        self.arena.buf.free()
+        self.ctx.close()
+
+
 @dataclass
 class _Slot:
     ctx: gpu.Context
@@ -168,8 +251,12 @@ class BatchServer:
     """Continuous batching over `slots` executors (see module docstring)."""
 
     def __init__(self, corpus: Corpus, slots: int = 2, hbm_budget: int = 8 * GiB, max_batch: int = 256,
-                 tuning: Optional[Dict] = None):
+                 tuning: Optional[Dict] = None, large: int = 0, large_out_budget: int = 2 * GiB,
+                 exclusive: bool = False):
         self.corpus = corpus
+        self.large = large
+        self.exclusive = exclusive
+        self.lane = _LargeLane(corpus, large_out_budget) if large else None
         self.max_batch = max_batch
         self.tuning = tuning or {}
         per = hbm_budget // slots
@@ -190,6 +277,8 @@ class BatchServer:
         self._scratch = {}
 
     def close(self):
+        if self.lane:
+            self.lane.close()
         for s in self.slots:
             s.batch.close()
             s.ctx.close()
@@ -236,8 +325,19 @@ class BatchServer:
         return True
 
     def run(self, reqs: List[Req]) -> Dict:
+        # The lane's thread waits for the GIL when its call returns: a short
+        # switch interval keeps that wait small while this loop polls.
+        old_switch = sys.getswitchinterval()
+        sys.setswitchinterval(1e-4)
+        try:
+            return self._run(reqs)
+        finally:
+            sys.setswitchinterval(old_switch)
+
+    def _run(self, reqs: List[Req]) -> Dict:
         pending = deque(sorted(reqs, key=lambda r: r.arrival))
         queue: deque = deque()
+        large_q: deque = deque()
         left = len(reqs)
         batches = 0
         max_inflight = 0
@@ -245,8 +345,25 @@ class BatchServer:
         while left:
             now = time.perf_counter() - t0
             while pending and pending[0].arrival <= now:
-                queue.append(pending.popleft())
+                r = pending.popleft()
+                (large_q if self.lane and r.length >= self.large else queue).append(r)
             progressed = False
+            if self.lane and self.lane.busy:
+                k = self.lane.reap()
+                left -= k
+                progressed = progressed or k > 0
+            if large_q:
+                if not self.lane.busy and not any(s.reqs for s in self.slots):
+                    # Idle device: the call is made here, as the plain path
+                    # would, with no hand-over to the lane's thread.
+                    r = large_q.popleft()
+                    x = plain_call(self.lane.ctx, self.lane.arena, self.corpus, r)
+                    r.done, r.status, r.result = time.perf_counter() - t0, x["status"], x
+                    left -= 1
+                    progressed = True
+                    continue
+                while large_q:
+                    self.lane.submit(large_q.popleft(), t0)
             for s in self.slots:
                 if s.reqs and s.batch.done():
                     res = s.batch.wait()
@@ -256,19 +373,25 @@ class BatchServer:
                     left -= len(s.reqs)
                     s.reqs = []
                     progressed = True
+            # With `exclusive`, small requests wait while a large one runs
+            # (as they would behind it in arrival order) instead of sharing
+            # the device with it.
+            hold = self.exclusive and self.lane is not None and self.lane.busy > 0
             for s in self.slots:
-                if not s.reqs and queue:
+                if not s.reqs and queue and not hold:
                     if self._fill(s, queue, now):
                         batches += 1
                         progressed = True
             max_inflight = max(max_inflight, sum(1 for s in self.slots if s.reqs))
             # Idle device: sleep until the next arrival. A batch in flight is
             # polled instead, so its completion is seen when it happens.
-            busy = any(s.reqs for s in self.slots)
+            busy = any(s.reqs for s in self.slots) or bool(self.lane and self.lane.busy)
             if not progressed and not queue and pending and not busy:
                 wait = pending[0].arrival - (time.perf_counter() - t0)
                 if wait > 2e-4:
                     time.sleep(wait - 1e-4)
+            elif not progressed and self.lane and self.lane.busy:
+                time.sleep(0)   # lets the lane's thread take the GIL when its call returns
         return summarize(reqs, extra={"batches": batches, "slots": len(self.slots), "max_inflight": max_inflight})
 
 
@@ -281,28 +404,7 @@ class PlainServer:
         self.arena = Arena(self.ctx, out_budget)
 
     def call(self, r: Req) -> Dict:
-        name, op, spec, _ = SERVED[r.kind]
-        view = _View(self.corpus.dev[r.kind], r.offset, r.length)
-        lib = gpu.load()
-        if op == S.OP_COLUMNS_STATS:
-            self.arena.reset()
-            cols = arena_columns(spec, r.length, self.arena)
-            st = S.CCsvStats()
-            cs = spec.to_c()
-            status = lib.falx_parse_columns_with_stats(self.ctx.h, C.byref(cs), view.ptr, r.length,
-                                                       C.byref(cols), C.byref(st))
-            return {"status": status, "stats": gpu._stats_dict(spec, st)}
-        if op == S.OP_LOGFMT_PAIRS:
-            st = S.CLogfmtStats()
-            status = lib.falx_parse_logfmt_pairs(self.ctx.h, view.ptr, r.length, C.byref(st))
-            return {"status": status, "logfmt": {"pairs": st.pairs, "key_bytes": st.key_bytes,
-                                                 "value_bytes": st.value_bytes, "checksum": st.checksum}}
-        st = S.CNdjsonIdScore()
-        status = lib.falx_parse_ndjson_id_score(self.ctx.h, view.ptr, r.length, C.byref(st))
-        return {"status": status, "ndjson": {"records": st.records, "invalid": st.invalid, "id_sum": st.id_sum,
-                                             "score_fixed6_sum": st.score_fixed6_sum,
-                                             "score_fixed6_cells": st.score_fixed6_cells,
-                                             "score_sum": st.score_sum}}
+        return plain_call(self.ctx, self.arena, self.corpus, r)
 
     def run(self, reqs: List[Req]) -> Dict:
         t0 = time.perf_counter()

@@ -1,6 +1,7 @@
 """The serving loop (falx_amd/serve.py): requests batched by continuous
-batching over two overlapping slots get exactly the results that one plain
-entry-point call per request gives, FP64 sums bit for bit."""
+batching over overlapping slots, or sent down the large-request lane (its
+thread, or the loop itself on an idle device), get exactly the results that
+one plain entry-point call per request gives, FP64 sums bit for bit."""
 import numpy as np
 import pytest
 
@@ -18,10 +19,12 @@ def _same(a, b):
 
 
 @pytest.mark.gpu
-def test_served_results_equal_plain_results(ctx):
+@pytest.mark.parametrize("slots,large", [(2, 0), (3, 1 << 20)], ids=["batched", "large_lane"])
+def test_served_results_equal_plain_results(ctx, slots, large):
     corpus = serve.Corpus(ctx, 6 << 20, seed=11)
     reqs = serve.request_stream(corpus, 80, rate=4000.0, seed=5, lo=4096, hi=6 << 20)
-    bs = serve.BatchServer(corpus, slots=2, hbm_budget=2 << 30, max_batch=12)
+    bs = serve.BatchServer(corpus, slots=slots, hbm_budget=2 << 30, max_batch=12, large=large,
+                           large_out_budget=256 << 20)
     try:
         a = serve.fresh(reqs)
         sa = bs.run(a)
