@@ -10,7 +10,7 @@ the CPU oracle's parallel baseline timed on this host. Extra lanes are
 reported under "lanes", each with the oracle's parallel CPU path on the same
 input beside it:
   geo CSV 1 GiB: lat/lon columns, fused count/sum, City + lat/lon columns,
-    structural index, pinned-H2D end-to-end;
+    structural index, pinned-H2D end-to-end, streamed tape in 64 MiB feeds;
   NDJSON 1 GiB: sum(id + nested.score), JSON nested tape;
   csv_hash 1 GiB: structural index, typed columns (i64, f64, string),
     field bytes.
@@ -211,6 +211,55 @@ def nested_lane(ctx, nd, nd_buf, args, lanes, thr) -> None:
     lanes["json_nested_tape"] = lane
 
 
+def stream_lane(ctx, data, dbuf, n_in, rows, args, lanes) -> None:
+    """Streaming ingestion (StreamParser) of the 1 GiB geo CSV in 64 MiB feeds
+    from device memory: each feed's tape of completed records. Verified by
+    the closed forms (one terminator per record, three separators each).
+    The CPU leg is the oracle's serial tape (the reference's stream is
+    serial) on a 256 MiB prefix."""
+    import numpy as np
+
+    from falx_amd import gpu
+    from falx_amd import spec as S
+
+    lib = gpu.load()
+    spec = S.GEO_LATLON
+    cs = spec.to_c()
+    feed = 64 << 20
+    h = C.c_void_p()
+    gpu._check("falx_stream_create", lib.falx_stream_create(ctx.h, C.byref(cs), 2 * feed, C.byref(h)))
+    d_seps, d_ends = ctx.alloc(4 * (feed // 4)), ctx.alloc(8 * (feed // 8))
+    tot = [0, 0]
+
+    def run():
+        gpu._check("falx_stream_reset", lib.falx_stream_reset(h))
+        ns = ne = 0
+        out = S.CStreamOut()
+        for k in range(0, n_in, feed):
+            gpu._check("falx_stream_feed", lib.falx_stream_feed(h, dbuf.ptr + k, min(feed, n_in - k), d_seps.ptr,
+                                                                d_seps.nbytes // 4, d_ends.ptr, d_ends.nbytes // 8,
+                                                                C.byref(out)))
+            ns, ne = ns + out.n_seps, ne + out.n_ends
+        gpu._check("falx_stream_finish", lib.falx_stream_finish(h, d_seps.ptr, d_seps.nbytes // 4, d_ends.ptr,
+                                                                d_ends.nbytes // 8, C.byref(out)))
+        tot[0], tot[1] = ns + out.n_seps, ne + out.n_ends + (1 if out.data_end else 0)
+    log("timing lane stream_tape_64mib_feeds")
+    lms, _ = time_steps(run, max(3, args.steps // 2), 1, ctx)
+    lib.falx_stream_destroy(h)
+    lane = {"gib_s": round(n_in / GIB / (lms / 1000.0), 2), "ms_per_step": round(lms, 3), "feed_bytes": feed,
+            "records": int(tot[1]), "verified": tot[1] == rows and tot[0] == 3 * rows}
+    if not args.no_cpu:
+        from oracle import oracle as O
+        sample = data[: 256 << 20]
+        log("timing CPU oracle lane stream_tape_64mib_feeds (serial tape, 256 MiB prefix)")
+        lane["cpu_gib_s"] = round(sample.size / GIB / cpu_time(lambda: O.parse(spec, sample, 1), 3, warmups=1), 3)
+        lane["cpu_threads"] = 1
+        lane["cpu_sample"] = "first 268435456 B, serial tape (ora_parse at 1 thread), 1 warm-up then 3 runs, median"
+    lanes["stream_tape_64mib_feeds"] = lane
+    d_seps.free()
+    d_ends.free()
+
+
 def served_stream(ctx) -> dict:
     """The mixed request stream (geo CSV, csv_hash, TSV, logfmt, NDJSON;
     sizes log-uniform 4 KiB-256 MiB; Poisson arrivals) through continuous
@@ -363,6 +412,7 @@ def main() -> int:
                     lanes[name]["cpu_gib_s"] = round(n_in / GIB / cpu_time(fn, args.cpu_runs), 3)
                     lanes[name]["cpu_threads"] = thr
             del cols3, _k3, idx_out
+            stream_lane(ctx, data, dbuf, n_in, rows, args, lanes)
 
         if "ndjson" in want:
             # NDJSON sum(id + nested.score), 1 GiB

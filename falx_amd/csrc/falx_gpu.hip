@@ -861,6 +861,142 @@ falx_status falx_parse_nested(falx_ctx* c, const uint8_t* d, uint64_t len, uint6
 }
 
 // ---------------------------------------------------------------------------
+// Streaming ingestion (StreamParser, csv_geo.rs:688-807): the unfinished
+// record of the previous feed is kept at the start of a device buffer, the
+// next chunk is appended behind it, and the tape of that buffer is built
+// with the plain pipeline. The bytes after a structural terminator start in
+// the dialect's entry state (outside quotes, no pending escape, at a line
+// start), so re-indexing the kept record from its first byte gives the
+// whole document's records exactly. Records completed by the feed are
+// returned; the bytes after the last terminator move to the other buffer.
+// ---------------------------------------------------------------------------
+struct falx_stream {
+    falx_ctx* ctx = nullptr;
+    falx_spec spec{};
+    uint8_t* buf[2] = {nullptr, nullptr};
+    uint64_t cap = 0;
+    int cur = 0;          // the buffer holding the unfinished record
+    uint64_t tail = 0;    // its length, at buf[cur][0, tail)
+    bool finished = false;
+};
+
+falx_status falx_stream_create(falx_ctx* c, const falx_spec* spec, uint64_t buf_cap, falx_stream** out) {
+    if (!c || !out) return FALX_E_INVALID;
+    *out = nullptr;
+    falx_status st = falx_spec_validate(spec);
+    if (st != FALX_OK) return st;
+    if (buf_cap == 0) return FALX_E_INVALID;
+    if (buf_cap >= (1ULL << 32)) return FALX_E_TOO_LARGE;
+    falx_stream* s = new (std::nothrow) falx_stream();
+    if (!s) return FALX_E_DEVICE;
+    s->ctx = c;
+    s->spec = *spec;
+    s->cap = buf_cap;
+    if (hipSetDevice(c->device) != hipSuccess || hipMalloc(&s->buf[0], buf_cap) != hipSuccess ||
+        hipMalloc(&s->buf[1], buf_cap) != hipSuccess) {
+        falx_stream_destroy(s);
+        return FALX_E_DEVICE;
+    }
+    *out = s;
+    return FALX_OK;
+}
+
+falx_status falx_stream_reset(falx_stream* s) {
+    if (!s) return FALX_E_INVALID;
+    s->tail = 0;
+    s->cur = 0;
+    s->finished = false;
+    return FALX_OK;
+}
+
+void falx_stream_destroy(falx_stream* s) {
+    if (!s) return;
+    (void)hipSetDevice(s->ctx->device);
+    (void)hipStreamSynchronize(s->ctx->stream);
+    for (uint8_t* b : s->buf)
+        if (b) (void)hipFree(b);
+    delete s;
+}
+
+namespace {
+
+// The tape of buf[cur][0, total) into the caller's arrays; `last` true for
+// the final call (the unterminated last record is emitted too).
+falx_status stream_emit(falx_stream* s, uint64_t total, bool last, uint32_t* d_seps, uint64_t seps_cap,
+                        uint64_t* d_ends, uint64_t ends_cap, falx_stream_out* out) {
+    falx_ctx* c = s->ctx;
+    const uint8_t* b = s->buf[s->cur];
+    uint64_t n_seps = 0, n_ends = 0;
+    falx_status st = falx_parse(c, &s->spec, b, total, d_seps, seps_cap, d_ends, ends_cap, &n_seps, &n_ends);
+    out->d_buf = b;
+    if (st == FALX_E_CAPACITY) {
+        out->n_seps = n_seps;
+        out->n_ends = n_ends;
+        return st;
+    }
+    if (st != FALX_OK) return st;
+    uint64_t keep_from = total;   // bytes from here on wait for the next feed
+    if (!last) {
+        if (n_ends == 0) {
+            keep_from = 0;
+            n_seps = 0;
+        } else {
+            unsigned long long w = 0;
+            HIP_TRY(read_back(c, &w, d_ends + n_ends - 1, 8));
+            keep_from = (w & 0xFFFFFFFFull) + 1;
+            n_seps = w >> 32;
+        }
+    }
+    out->n_seps = n_seps;
+    out->n_ends = n_ends;
+    out->data_end = keep_from;
+    const uint64_t tail = total - keep_from;
+    if (tail && keep_from) {
+        const int nxt = 1 - s->cur;
+        HIP_TRY(hipMemcpyAsync(s->buf[nxt], b + keep_from, tail, hipMemcpyDeviceToDevice, c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        s->cur = nxt;
+    }
+    s->tail = tail;
+    return FALX_OK;
+}
+
+}  // namespace
+
+falx_status falx_stream_feed(falx_stream* s, const uint8_t* d_chunk, uint64_t len, uint32_t* d_seps,
+                             uint64_t seps_cap, uint64_t* d_ends, uint64_t ends_cap, falx_stream_out* out) {
+    if (!s || !out || s->finished || (len && !d_chunk)) return FALX_E_INVALID;
+    memset(out, 0, sizeof *out);
+    falx_ctx* c = s->ctx;
+    tl_hip_error = 0;
+    const uint64_t total = s->tail + len;
+    if (total > s->cap) {
+        out->need_bytes = total;
+        return FALX_E_CAPACITY;
+    }
+    HIP_TRY(hipSetDevice(c->device));
+    if (len) HIP_TRY(hipMemcpyAsync(s->buf[s->cur] + s->tail, d_chunk, len, hipMemcpyDeviceToDevice, c->stream));
+    // A capacity failure leaves the stream as it was (the chunk is copied
+    // again by the retry).
+    return stream_emit(s, total, false, d_seps, seps_cap, d_ends, ends_cap, out);
+}
+
+falx_status falx_stream_finish(falx_stream* s, uint32_t* d_seps, uint64_t seps_cap, uint64_t* d_ends,
+                               uint64_t ends_cap, falx_stream_out* out) {
+    if (!s || !out || s->finished) return FALX_E_INVALID;
+    memset(out, 0, sizeof *out);
+    tl_hip_error = 0;
+    out->d_buf = s->buf[s->cur];
+    if (s->tail == 0) {
+        s->finished = true;
+        return FALX_OK;
+    }
+    falx_status st = stream_emit(s, s->tail, true, d_seps, seps_cap, d_ends, ends_cap, out);
+    if (st == FALX_OK) s->finished = true;
+    return st;
+}
+
+// ---------------------------------------------------------------------------
 // Index without allocation: the scratch of falx_index_structurals (tile
 // summaries, group summaries, prefixes, result slots) in one file-scope
 // device array, so the call makes no allocation at all.

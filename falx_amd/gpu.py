@@ -48,6 +48,11 @@ _SIGS = [
     ("falx_parse_ndjson_id_score", C.c_int, [_VP, _VP, _U64, C.POINTER(S.CNdjsonIdScore)]),
     ("falx_parse_logfmt_pairs", C.c_int, [_VP, _VP, _U64, C.POINTER(S.CLogfmtStats)]),
     ("falx_parse_nested", C.c_int, [_VP, _VP, _U64, _VP, _U64, _U64P, C.POINTER(C.c_int32), _U64P]),
+    ("falx_stream_create", C.c_int, [_VP, C.POINTER(S.CSpec), _U64, C.POINTER(C.c_void_p)]),
+    ("falx_stream_destroy", None, [_VP]),
+    ("falx_stream_reset", C.c_int, [_VP]),
+    ("falx_stream_feed", C.c_int, [_VP, _VP, _U64, _VP, _U64, _VP, _U64, C.POINTER(S.CStreamOut)]),
+    ("falx_stream_finish", C.c_int, [_VP, _VP, _U64, _VP, _U64, C.POINTER(S.CStreamOut)]),
     ("falx_export_arrow", C.c_int, [C.POINTER(S.CSpec), C.POINTER(S.CColumns), C.c_uint32, C.c_int32, C.c_int64,
                                     _VP, _VP]),
     ("falx_batch_create", C.c_int, [_VP, _U64, C.POINTER(C.c_void_p)]),
@@ -330,6 +335,56 @@ class Context:
             tape.free()
             return out, cnt, err, pos
         return tape, cnt, err, pos
+
+
+class Stream:
+    """A device StreamParser (falx_stream_*): feed() and finish() return the
+    records each call completes as a records.Parsed over a host copy of the
+    stream buffer (its `records()` yields them, comment records skipped)."""
+
+    def __init__(self, ctx: Context, spec: S.Spec, buf_cap: int):
+        self.ctx, self.spec = ctx, spec
+        self._cs = spec.to_c()
+        h = C.c_void_p()
+        ctx._ck("falx_stream_create", load().falx_stream_create(ctx.h, C.byref(self._cs), buf_cap, C.byref(h)))
+        self.h = h.value
+        self._seps = ctx.alloc(4 * 1024)
+        self._ends = ctx.alloc(8 * 1024)
+
+    def close(self):
+        if self.h:
+            load().falx_stream_destroy(self.h)
+            self.h = None
+
+    def _call(self, fn: str, *args):
+        from .records import Parsed
+        while True:
+            out = S.CStreamOut()
+            st = getattr(load(), fn)(self.h, *args, self._seps.ptr, self._seps.nbytes // 4, self._ends.ptr,
+                                     self._ends.nbytes // 8, C.byref(out))
+            if st == S.E_CAPACITY and not out.need_bytes:
+                if out.n_seps > self._seps.nbytes // 4:
+                    self._seps.free()
+                    self._seps = self.ctx.alloc(4 * (2 * out.n_seps + 1024))
+                if out.n_ends > self._ends.nbytes // 8:
+                    self._ends.free()
+                    self._ends = self.ctx.alloc(8 * (2 * out.n_ends + 1024))
+                continue
+            self.ctx._ck(fn, st)
+            n = out.data_end
+            data = np.empty(n, dtype=np.uint8)
+            if n:
+                _check("falx_download", load().falx_download(self.ctx.h, data.ctypes.data, out.d_buf, n))
+            seps = self._seps.download(np.uint32, out.n_seps)
+            ends = self._ends.download(np.uint64, out.n_ends)
+            return Parsed(data.tobytes(), seps, ends, self.spec)
+
+    def feed(self, chunk: "DeviceBuffer", length: Optional[int] = None):
+        n = chunk.nbytes if length is None else length
+        return self._call("falx_stream_feed", chunk.ptr, n)
+
+    def finish(self):
+        return self._call("falx_stream_finish")
 
 
 def _result_dict(op: int, spec: Optional[S.Spec], r: S.CResult) -> Dict:

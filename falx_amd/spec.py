@@ -127,6 +127,11 @@ class CResult(C.Structure):
     ]
 
 
+class CStreamOut(C.Structure):
+    _fields_ = [("d_buf", C.c_void_p), ("n_seps", C.c_uint64), ("n_ends", C.c_uint64), ("data_end", C.c_uint64),
+                ("need_bytes", C.c_uint64)]
+
+
 class CBatchTuning(C.Structure):
     _fields_ = [
         ("tile_workgroups", C.c_uint32), ("unit_workgroups", C.c_uint32),

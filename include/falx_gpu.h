@@ -300,6 +300,46 @@ falx_status falx_parse_nested(falx_ctx* ctx, const uint8_t* d_data, uint64_t len
                               uint64_t* d_tape, uint64_t tape_cap, uint64_t* n_tape,
                               int32_t* error, uint64_t* error_pos);
 
+/* ---- streaming ingestion ---------------------------------------------------
+ * `StreamParser` (csv_geo.rs:688-807 and every generated module's
+ * `stream()`): feed chunks of unbounded input, receive the records each feed
+ * completes. The unfinished last record stays in a device buffer of
+ * `buf_cap` bytes (a record plus a chunk must fit) and is indexed again with
+ * the next chunk behind it, so quotes, escapes and comments across feed
+ * boundaries parse exactly as in one document.
+ *
+ * Each call returns the tape of the records it completes in the caller's
+ * arrays, over `out->d_buf` (device; valid until the next call on the
+ * stream): ends[i] = (separators before it << 32) | terminator offset, the
+ * first record starting at offset 0 and record i + 1 after terminator i,
+ * seps the separators of those records. `data_end` is where the emitted
+ * records stop: the byte after the last terminator (feed), or the end of
+ * the buffer (finish, whose last record is unterminated when data_end is
+ * past the last terminator). Records are skipped as the tape iterators skip
+ * them (comment records). FALX_E_CAPACITY leaves the stream unchanged:
+ * `need_bytes` is the buffer size the feed needs when the kept record and
+ * the chunk do not fit (0 otherwise), else the counts are the tape sizes
+ * needed. */
+typedef struct falx_stream falx_stream;
+typedef struct falx_stream_out {
+    const uint8_t* d_buf;
+    uint64_t n_seps;
+    uint64_t n_ends;
+    uint64_t data_end;
+    uint64_t need_bytes;
+} falx_stream_out;
+
+falx_status falx_stream_create(falx_ctx* ctx, const falx_spec* spec, uint64_t buf_cap,
+                               falx_stream** out);
+void falx_stream_destroy(falx_stream* stream);
+/* Start a new stream on the same buffers (no allocation). */
+falx_status falx_stream_reset(falx_stream* stream);
+falx_status falx_stream_feed(falx_stream* stream, const uint8_t* d_chunk, uint64_t len,
+                             uint32_t* d_seps, uint64_t seps_cap, uint64_t* d_ends,
+                             uint64_t ends_cap, falx_stream_out* out);
+falx_status falx_stream_finish(falx_stream* stream, uint32_t* d_seps, uint64_t seps_cap,
+                               uint64_t* d_ends, uint64_t ends_cap, falx_stream_out* out);
+
 /* ---------------------------------------------------------------------
  * Arrow export. The reference hands its columns to Arrow by wrapping the
  * buffers, since they are Arrow's own layouts (examples/arrow_interop.rs:
