@@ -10,7 +10,8 @@ the CPU oracle's parallel baseline timed on this host. Extra lanes are
 reported under "lanes", each with the oracle's parallel CPU path on the same
 input beside it:
   geo CSV 1 GiB: lat/lon columns, fused count/sum, City + lat/lon columns,
-    structural index, pinned-H2D end-to-end, streamed tape in 64 MiB feeds;
+    structural index, pinned-H2D end-to-end, streamed tape in 64 MiB feeds,
+    BGZF inflation and BGZF end-to-end (compressed upload + inflate + columns);
   NDJSON 1 GiB: sum(id + nested.score), JSON nested tape;
   csv_hash 1 GiB: structural index, typed columns (i64, f64, string),
     field bytes.
@@ -260,6 +261,89 @@ def stream_lane(ctx, data, dbuf, n_in, rows, args, lanes) -> None:
     d_ends.free()
 
 
+def bgzf_lanes(ctx, data, truth, spec, cols, args, lanes, thr) -> None:
+    """The 1 GiB geo CSV as BGZF (bgzip framing, zlib level 6): inflation of
+    the HBM-resident compressed bytes (GB/s of output; algorithmic bytes =
+    compressed + inflated), and end to end from pinned host memory: the
+    compressed upload, inflation and the headline columns + sums. CPU legs:
+    zlib (the reference's flate2 backend family) over the blocks on `thr`
+    threads, and that plus the oracle's parse_columns_par."""
+    import zlib
+    from concurrent.futures import ThreadPoolExecutor
+
+    import numpy as np
+
+    from falx_amd import bgzf, gpu
+    from falx_amd import spec as S
+
+    n_in, rows = int(data.size), int(truth["rows"])
+    log("bgzipping the geo CSV (zlib level 6)")
+    comp = bgzf.bgzip(memoryview(data), 6, threads=thr)
+    blocks, total = bgzf.scan(comp)
+    assert total == n_in
+    c_n = len(comp)
+    d_comp = ctx.upload(np.frombuffer(comp, dtype=np.uint8))
+    d_out = ctx.alloc(n_in + 16)
+    lib = gpu.load()
+    b = np.ascontiguousarray(blocks)
+    bad = C.c_uint64()
+
+    def inflate():
+        gpu._check("falx_bgzf_inflate", lib.falx_bgzf_inflate(ctx.h, d_comp.ptr, c_n, b.ctypes.data, b.size,
+                                                               d_out.ptr, d_out.nbytes, C.byref(bad)))
+    log("timing lane bgzf_inflate")
+    lms, lkms = time_steps(inflate, max(3, args.steps // 2), 1, ctx)
+    ab = c_n + n_in
+    same = bool(np.array_equal(d_out.download(np.uint8, n_in), data))
+    lane = {"gib_s": round(n_in / GIB / (lms / 1000.0), 2), "ms_per_step": round(lms, 3), "kernel_ms": round(lkms, 3),
+            "hbm_frac": round(ab / (lkms / 1000.0) / 1e9 / HBM_PEAK_GBS, 3), "compressed_bytes": c_n,
+            "blocks": int(b.size), "verified": same}
+    # End to end: pinned compressed bytes -> HBM -> inflate -> columns + sums.
+    hp = C.c_void_p()
+    gpu._check("falx_host_alloc_pinned", lib.falx_host_alloc_pinned(ctx.h, c_n, C.byref(hp)))
+    C.memmove(hp, comp, c_n)
+    res = [None]
+
+    def e2e():
+        gpu._check("falx_upload", lib.falx_upload(ctx.h, d_comp.ptr, hp, c_n))
+        inflate()
+        res[0] = ctx.parse_columns_with_stats_into(spec, d_out, n_in, cols)
+    log("timing lane bgzf_e2e_columns_stats")
+    ems, _ = time_steps(e2e, max(3, args.steps // 2), 1, ctx)
+    e_lane = {"gib_s": round(n_in / GIB / (ems / 1000.0), 2), "ms_per_step": round(ems, 3),
+              "verified": (res[0]["records"] == rows and res[0]["cols"][0]["fixed6_sum"] == truth["lat_fixed6_sum"]
+                           and res[0]["cols"][1]["fixed6_sum"] == truth["lon_fixed6_sum"])}
+    lib.falx_host_free_pinned(ctx.h, hp)
+    if not args.no_cpu:
+        payloads = [(comp[int(x["payload"]):int(x["payload"]) + int(x["payload_len"])], int(x["out"]),
+                     int(x["isize"])) for x in blocks]
+        out = bytearray(n_in)
+        mv = memoryview(out)
+
+        def cpu_inflate():
+            def one(p):
+                mv[p[1]:p[1] + p[2]] = zlib.decompress(p[0], -15)
+            with ThreadPoolExecutor(thr) as ex:
+                list(ex.map(one, payloads, chunksize=64))
+        log("timing CPU lane bgzf_inflate (zlib)")
+        lane["cpu_gib_s"] = round(n_in / GIB / cpu_time(cpu_inflate, 3, warmups=1), 3)
+        lane["cpu_threads"] = thr
+        lane["cpu_kind"] = f"zlib {zlib.ZLIB_RUNTIME_VERSION} raw inflate per block, {thr} threads"
+        lane["cpu_equal"] = bytes(out) == data.tobytes()
+        run_cols = cpu_columns(spec, np.frombuffer(out, dtype=np.uint8), rows, 16, thr)
+
+        def cpu_e2e():
+            cpu_inflate()
+            run_cols()
+        log("timing CPU lane bgzf_e2e_columns_stats (zlib + oracle parse_columns_par)")
+        e_lane["cpu_gib_s"] = round(n_in / GIB / cpu_time(cpu_e2e, 3, warmups=1), 3)
+        e_lane["cpu_threads"] = thr
+    lanes["bgzf_inflate"] = lane
+    lanes["bgzf_e2e_columns_stats"] = e_lane
+    d_comp.free()
+    d_out.free()
+
+
 def served_stream(ctx) -> dict:
     """The mixed request stream (geo CSV, csv_hash, TSV, logfmt, NDJSON;
     sizes log-uniform 4 KiB-256 MiB; Poisson arrivals) through continuous
@@ -413,6 +497,7 @@ def main() -> int:
                     lanes[name]["cpu_threads"] = thr
             del cols3, _k3, idx_out
             stream_lane(ctx, data, dbuf, n_in, rows, args, lanes)
+            bgzf_lanes(ctx, data, truth, spec, cols, args, lanes, thr)
 
         if "ndjson" in want:
             # NDJSON sum(id + nested.score), 1 GiB

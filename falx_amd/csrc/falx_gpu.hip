@@ -3,6 +3,7 @@
 // (tile summaries and prefixes, per-tile partials, result slots).
 #include "scan.hip"
 #include "nested.h"
+#include "inflate.h"
 
 #include <algorithm>
 #include <cstdio>
@@ -994,6 +995,91 @@ falx_status falx_stream_finish(falx_stream* s, uint32_t* d_seps, uint64_t seps_c
     falx_status st = stream_emit(s, s->tail, true, d_seps, seps_cap, d_ends, ends_cap, out);
     if (st == FALX_OK) s->finished = true;
     return st;
+}
+
+// ---------------------------------------------------------------------------
+// BGZF (bgzf.rs:61-290): block boundaries on the host, inflation on the
+// device (inflate.h).
+// ---------------------------------------------------------------------------
+static_assert(sizeof(falx_bgzf_block) == sizeof(InfBlock), "block table layout");
+
+falx_status falx_bgzf_scan(const uint8_t* h, uint64_t len, falx_bgzf_block* blocks, uint64_t cap,
+                           uint64_t* n_blocks, uint64_t* total_out) {
+    if ((!h && len) || !n_blocks || !total_out) return FALX_E_INVALID;
+    *n_blocks = *total_out = 0;
+    constexpr uint64_t HEADER = 12, TRAILER = 8;
+    uint64_t pos = 0, n = 0, out = 0;
+    while (pos < len) {
+        // gzip member with FEXTRA (bgzf.rs:64-86)
+        if (pos + HEADER > len || h[pos] != 0x1f || h[pos + 1] != 0x8b || h[pos + 2] != 8 || !(h[pos + 3] & 0x04))
+            return FALX_E_INVALID;
+        const uint64_t xlen = (uint64_t)h[pos + 10] | ((uint64_t)h[pos + 11] << 8);
+        const uint64_t extra_end = pos + HEADER + xlen;
+        if (extra_end > len) return FALX_E_INVALID;
+        // the BC subfield holds BSIZE = block length - 1 (bgzf.rs:40-57)
+        uint64_t bsize = UINT64_MAX;
+        for (uint64_t i = pos + HEADER; i + 4 <= extra_end;) {
+            const uint64_t slen = (uint64_t)h[i + 2] | ((uint64_t)h[i + 3] << 8);
+            if (h[i] == 'B' && h[i + 1] == 'C' && slen == 2 && i + 6 <= extra_end) {
+                bsize = (uint64_t)h[i + 4] | ((uint64_t)h[i + 5] << 8);
+                break;
+            }
+            i += 4 + slen;
+        }
+        if (bsize == UINT64_MAX) return FALX_E_INVALID;
+        const uint64_t block_len = bsize + 1;
+        if (block_len <= HEADER + xlen + TRAILER || pos + block_len > len) return FALX_E_INVALID;
+        const uint64_t end = pos + block_len;
+        const uint32_t isize = (uint32_t)h[end - 4] | ((uint32_t)h[end - 3] << 8) | ((uint32_t)h[end - 2] << 16) |
+                               ((uint32_t)h[end - 1] << 24);
+        if (isize > 0) {   // the empty EOF block is dropped
+            if (n < cap) {
+                blocks[n].payload = extra_end;
+                blocks[n].payload_len = (uint32_t)(end - TRAILER - extra_end);
+                blocks[n].isize = isize;
+                blocks[n].out = out;
+            }
+            ++n;
+            out += isize;
+        }
+        pos = end;
+    }
+    *n_blocks = n;
+    *total_out = out;
+    return n > cap ? FALX_E_CAPACITY : FALX_OK;
+}
+
+falx_status falx_bgzf_inflate(falx_ctx* c, const uint8_t* d_comp, uint64_t comp_len, const falx_bgzf_block* blocks,
+                              uint64_t n_blocks, uint8_t* d_out, uint64_t out_cap, uint64_t* first_bad) {
+    if (!c || (n_blocks && (!blocks || !d_comp || !d_out))) return FALX_E_INVALID;
+    if (first_bad) *first_bad = UINT64_MAX;
+    if (n_blocks == 0) return FALX_OK;
+    if (n_blocks >= (1ull << 31)) return FALX_E_TOO_LARGE;
+    // Every payload inside the compressed bytes, every slot inside the output.
+    for (uint64_t k = 0; k < n_blocks; ++k) {
+        const falx_bgzf_block& b = blocks[k];
+        if (b.payload > comp_len || b.payload_len > comp_len - b.payload || b.out > out_cap ||
+            b.isize > out_cap - b.out)
+            return FALX_E_INVALID;
+    }
+    falx_status st;
+    if ((st = begin(c)) != FALX_OK) return st;
+    const size_t table = n_blocks * sizeof(falx_bgzf_block);
+    if ((st = ensure_part(c, table)) != FALX_OK) return st;
+    HIP_TRY(hipMemcpyAsync(c->d_part, blocks, table, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(fill_u64(c->stream, c->d_result + R_INF_BAD, 1, ~0ull));
+    HIP_TRY(mark_start(c));
+    hipLaunchKernelGGL(k_inflate, dim3((uint32_t)n_blocks), dim3(INF_WG), 0, c->stream, d_comp,
+                       static_cast<const InfBlock*>(c->d_part), d_out, c->d_result + R_INF_BAD);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(mark_end(c));
+    if ((st = finish(c)) != FALX_OK) return st;
+    const unsigned long long bad = c->h_result[R_INF_BAD];
+    if (bad != ~0ull) {
+        if (first_bad) *first_bad = bad;
+        return FALX_E_INVALID;   // bgzf.rs: InvalidData
+    }
+    return FALX_OK;
 }
 
 // ---------------------------------------------------------------------------

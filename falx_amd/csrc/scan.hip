@@ -93,6 +93,7 @@ enum : int {
     // nested tape: first entry below depth 0, first mismatched close, innermost
     // open never closed + 1, final depth
     R_NT_NEG = 56, R_NT_MIS, R_NT_OPEN, R_NT_DEPTH,
+    R_INF_BAD = 60,   // first malformed BGZF block
     R_SLOTS = 64
 };
 

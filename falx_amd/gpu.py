@@ -53,6 +53,8 @@ _SIGS = [
     ("falx_stream_reset", C.c_int, [_VP]),
     ("falx_stream_feed", C.c_int, [_VP, _VP, _U64, _VP, _U64, _VP, _U64, C.POINTER(S.CStreamOut)]),
     ("falx_stream_finish", C.c_int, [_VP, _VP, _U64, _VP, _U64, C.POINTER(S.CStreamOut)]),
+    ("falx_bgzf_scan", C.c_int, [_VP, _U64, _VP, _U64, _U64P, _U64P]),
+    ("falx_bgzf_inflate", C.c_int, [_VP, _VP, _U64, _VP, _U64, _VP, _U64, _U64P]),
     ("falx_export_arrow", C.c_int, [C.POINTER(S.CSpec), C.POINTER(S.CColumns), C.c_uint32, C.c_int32, C.c_int64,
                                     _VP, _VP]),
     ("falx_batch_create", C.c_int, [_VP, _U64, C.POINTER(C.c_void_p)]),

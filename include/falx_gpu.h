@@ -340,6 +340,32 @@ falx_status falx_stream_feed(falx_stream* stream, const uint8_t* d_chunk, uint64
 falx_status falx_stream_finish(falx_stream* stream, uint32_t* d_seps, uint64_t seps_cap,
                                uint64_t* d_ends, uint64_t ends_cap, falx_stream_out* out);
 
+/* ---- BGZF ------------------------------------------------------------------
+ * `bgzf::scan` / `bgzf::decompress_par` (bgzf.rs:61-290): a BGZF stream is a
+ * sequence of gzip members of at most 64 KiB output each, independently
+ * inflatable, with the payload length in the `BC` extra subfield and the
+ * inflated size (ISIZE) in the trailer. falx_bgzf_scan walks the headers of
+ * host bytes into a block table (empty EOF blocks dropped; `out` is each
+ * block's offset in the inflated stream, *total_out their sum);
+ * FALX_E_INVALID for a malformed header (the reference's InvalidData),
+ * FALX_E_CAPACITY when more than `cap` blocks (*n_blocks = the count).
+ * falx_bgzf_inflate inflates those blocks (raw DEFLATE, RFC 1951) from the
+ * compressed bytes in HBM into d_out, one wave per block; a malformed block
+ * or one whose inflated size differs from ISIZE gives FALX_E_INVALID and
+ * *first_bad its index (the smallest). The parsers then read d_out. */
+typedef struct falx_bgzf_block {
+    uint64_t payload;      /* raw-DEFLATE payload offset in the compressed bytes */
+    uint32_t payload_len;
+    uint32_t isize;        /* inflated size */
+    uint64_t out;          /* offset of its output */
+} falx_bgzf_block;
+
+falx_status falx_bgzf_scan(const uint8_t* h_comp, uint64_t len, falx_bgzf_block* blocks,
+                           uint64_t cap, uint64_t* n_blocks, uint64_t* total_out);
+falx_status falx_bgzf_inflate(falx_ctx* ctx, const uint8_t* d_comp, uint64_t comp_len,
+                              const falx_bgzf_block* blocks, uint64_t n_blocks,
+                              uint8_t* d_out, uint64_t out_cap, uint64_t* first_bad);
+
 /* ---------------------------------------------------------------------
  * Arrow export. The reference hands its columns to Arrow by wrapping the
  * buffers, since they are Arrow's own layouts (examples/arrow_interop.rs:

@@ -1266,6 +1266,153 @@ void nest_par(const uint8_t* data, const uint32_t* idx, uint64_t n, int threads,
     }
 }
 
+// ---- DEFLATE (RFC 1951) and BGZF framing (bgzf.rs:61-290) ---------------
+// The reference inflates with flate2 1.x (miniz_oxide) or libdeflater
+// 1.25.2, neither in this tree; this is the published algorithm restated:
+// block headers, stored/fixed/dynamic blocks, canonical Huffman codes read
+// most significant bit first, length/distance pairs with the RFC's tables.
+struct Inflater {
+    const uint8_t* src;
+    uint64_t n, pos = 0;
+    uint64_t bb = 0;
+    int bc = 0;
+    bool err = false;
+
+    uint32_t bits(int k) {
+        while (bc < k) {
+            if (pos >= n) { err = true; return 0; }
+            bb |= (uint64_t)src[pos++] << bc;
+            bc += 8;
+        }
+        const uint32_t v = (uint32_t)(bb & ((1ull << k) - 1));
+        bb >>= k;
+        bc -= k;
+        return v;
+    }
+};
+
+struct Canon {
+    int count[16];
+    int sym[288];
+    // Per length counts and symbols in code order; false when over-subscribed
+    // or incomplete (unless `single` and it is one code of length 1, or empty).
+    bool build(const uint8_t* len, int n, bool single) {
+        for (int l = 0; l < 16; ++l) count[l] = 0;
+        for (int s = 0; s < n; ++s) ++count[len[s]];
+        if (count[0] == n) return single;
+        int left = 1;
+        for (int l = 1; l < 16; ++l) {
+            left = 2 * left - count[l];
+            if (left < 0) return false;
+        }
+        if (left > 0 && !(single && n - count[0] == 1 && count[1] == 1)) return false;
+        int offs[16];
+        offs[1] = 0;
+        for (int l = 1; l < 15; ++l) offs[l + 1] = offs[l] + count[l];
+        for (int s = 0; s < n; ++s)
+            if (len[s]) sym[offs[len[s]]++] = s;
+        return true;
+    }
+    // Next symbol: codes are canonical, first bit sent is the top bit.
+    int decode(Inflater& in) const {
+        int code = 0, first = 0, idx = 0;
+        for (int l = 1; l < 16; ++l) {
+            code |= (int)in.bits(1);
+            if (in.err) return -1;
+            if (code - first < count[l]) return sym[idx + code - first];
+            idx += count[l];
+            first = (first + count[l]) << 1;
+            code <<= 1;
+        }
+        return -1;
+    }
+};
+
+const uint16_t kOraLenBase[29] = {3, 4, 5, 6, 7, 8, 9, 10, 11, 13, 15, 17, 19, 23, 27, 31, 35, 43, 51, 59, 67, 83, 99, 115, 131, 163, 195, 227, 258};
+const uint8_t kOraLenExtra[29] = {0, 0, 0, 0, 0, 0, 0, 0, 1, 1, 1, 1, 2, 2, 2, 2, 3, 3, 3, 3, 4, 4, 4, 4, 5, 5, 5, 5, 0};
+const uint16_t kOraDistBase[30] = {1, 2, 3, 4, 5, 7, 9, 13, 17, 25, 33, 49, 65, 97, 129, 193, 257, 385, 513, 769, 1025, 1537, 2049, 3073, 4097, 6145, 8193, 12289, 16385, 24577};
+const uint8_t kOraDistExtra[30] = {0, 0, 0, 0, 1, 1, 2, 2, 3, 3, 4, 4, 5, 5, 6, 6, 7, 7, 8, 8, 9, 9, 10, 10, 11, 11, 12, 12, 13, 13};
+const uint8_t kOraClOrder[19] = {16, 17, 18, 0, 8, 7, 9, 6, 10, 5, 11, 4, 12, 3, 13, 2, 14, 1, 15};
+
+// Raw DEFLATE into out[0, cap); the output length, or -1 when malformed or
+// longer than cap.
+int64_t inflate_raw(const uint8_t* src, uint64_t n, uint8_t* out, uint64_t cap) {
+    Inflater in{src, n};
+    uint64_t o = 0;
+    bool last = false;
+    Canon lit, dist;
+    while (!last) {
+        last = in.bits(1) != 0;
+        const uint32_t type = in.bits(2);
+        if (in.err) return -1;
+        if (type == 0) {
+            in.bb = 0;   // to the byte boundary (whole bytes are never buffered ahead)
+            in.bc = 0;
+            if (in.pos + 4 > n) return -1;
+            const uint32_t ln = src[in.pos] | (src[in.pos + 1] << 8), nl = src[in.pos + 2] | (src[in.pos + 3] << 8);
+            in.pos += 4;
+            if ((ln ^ 0xFFFFu) != nl || in.pos + ln > n || o + ln > cap) return -1;
+            std::memcpy(out + o, src + in.pos, ln);
+            in.pos += ln;
+            o += ln;
+            continue;
+        }
+        if (type == 3) return -1;
+        uint8_t lens[320];
+        if (type == 1) {
+            for (int s = 0; s < 320; ++s) lens[s] = s < 144 ? 8 : s < 256 ? 9 : s < 280 ? 7 : s < 288 ? 8 : 5;
+        } else {
+            const int nlen = (int)in.bits(5) + 257, ndist = (int)in.bits(5) + 1, ncl = (int)in.bits(4) + 4;
+            if (in.err || nlen > 286 || ndist > 30) return -1;
+            uint8_t cl[19] = {0};
+            for (int k = 0; k < ncl; ++k) cl[kOraClOrder[k]] = (uint8_t)in.bits(3);
+            Canon cc;
+            if (in.err || !cc.build(cl, 19, false)) return -1;
+            std::memset(lens, 0, sizeof lens);
+            int i = 0, prev = 0;
+            while (i < nlen + ndist) {
+                const int sym = cc.decode(in);
+                if (sym < 0) return -1;
+                int rep = 1, val = sym;
+                if (sym == 16) {
+                    if (i == 0) return -1;
+                    val = prev;
+                    rep = 3 + (int)in.bits(2);
+                } else if (sym == 17) {
+                    val = 0;
+                    rep = 3 + (int)in.bits(3);
+                } else if (sym == 18) {
+                    val = 0;
+                    rep = 11 + (int)in.bits(7);
+                }
+                if (in.err || i + rep > nlen + ndist) return -1;
+                for (int k = 0; k < rep; ++k, ++i) lens[i < nlen ? i : 288 + (i - nlen)] = (uint8_t)val;
+                prev = val;
+            }
+            if (lens[256] == 0) return -1;
+        }
+        if (!lit.build(lens, 288, true) || !dist.build(lens + 288, 32, true)) return -1;
+        while (true) {
+            const int sym = lit.decode(in);
+            if (sym < 0) return -1;
+            if (sym < 256) {
+                if (o >= cap) return -1;
+                out[o++] = (uint8_t)sym;
+                continue;
+            }
+            if (sym == 256) break;
+            if (sym > 285) return -1;
+            const uint32_t len = kOraLenBase[sym - 257] + in.bits(kOraLenExtra[sym - 257]);
+            const int ds = dist.decode(in);
+            if (ds < 0 || ds > 29) return -1;
+            const uint32_t d = kOraDistBase[ds] + in.bits(kOraDistExtra[ds]);
+            if (in.err || d > o || o + len > cap) return -1;
+            for (uint32_t k = 0; k < len; ++k, ++o) out[o] = out[o - d];
+        }
+    }
+    return (int64_t)o;
+}
+
 }  // namespace
 
 // ===========================================================================
@@ -1605,6 +1752,42 @@ uint64_t ora_parse_nested(const uint8_t* data, uint64_t len, int threads, uint64
     *error = out.error;
     *error_pos = out.error_pos;
     return out.n;
+}
+
+}  // extern "C"
+
+extern "C" {
+
+int64_t ora_inflate(const uint8_t* src, uint64_t n, uint8_t* out, uint64_t cap) {
+    return inflate_raw(src, n, out, cap);
+}
+
+// bgzf::decompress_par (bgzf.rs:209-260) over a block table from
+// falx_bgzf_scan's rules: the blocks in contiguous runs per thread, each
+// block into its slot; -1 when any block is malformed or its inflated size
+// differs from ISIZE.
+int64_t ora_bgzf_inflate(const uint8_t* comp, const falx_bgzf_block* blocks, uint64_t n_blocks, int threads,
+                         uint8_t* out) {
+    const int T = std::max(1, std::min<int>(threads, (int)std::max<uint64_t>(1, n_blocks)));
+    std::vector<int> bad(T, 0);
+    std::vector<std::thread> th;
+    const uint64_t per = (n_blocks + T - 1) / T;
+    for (int t = 0; t < T; ++t)
+        th.emplace_back([&, t] {
+            for (uint64_t k = t * per; k < std::min(n_blocks, (t + 1) * per); ++k) {
+                const falx_bgzf_block& b = blocks[k];
+                if (inflate_raw(comp + b.payload, b.payload_len, out + b.out, b.isize) != (int64_t)b.isize) {
+                    bad[t] = 1;
+                    return;
+                }
+            }
+        });
+    for (auto& x : th) x.join();
+    for (int t = 0; t < T; ++t)
+        if (bad[t]) return -1;
+    uint64_t total = 0;
+    for (uint64_t k = 0; k < n_blocks; ++k) total += blocks[k].isize;
+    return (int64_t)total;
 }
 
 }  // extern "C"

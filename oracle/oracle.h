@@ -64,6 +64,13 @@ uint64_t ora_clean(const falx_spec* spec, const uint8_t* s, uint64_t n, uint8_t*
 uint64_t ora_parse_nested(const uint8_t* data, uint64_t len, int threads, uint64_t* tape,
                           uint64_t cap, int32_t* error, uint64_t* error_pos);
 
+/* Raw DEFLATE (RFC 1951) into out[0, cap): the output length, -1 when
+ * malformed (the reference's flate2/libdeflater backends, restated). */
+int64_t ora_inflate(const uint8_t* src, uint64_t n, uint8_t* out, uint64_t cap);
+/* bgzf::decompress_par over a falx_bgzf_scan block table: the total, or -1. */
+int64_t ora_bgzf_inflate(const uint8_t* comp, const falx_bgzf_block* blocks, uint64_t n_blocks,
+                         int threads, uint8_t* out);
+
 /* Which block-classifier tier the CPU dispatches ("avx512bw" or "scalar"). */
 const char* ora_isa(void);
 

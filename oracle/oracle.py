@@ -61,6 +61,10 @@ def lib():
         L.ora_parse_nested.restype = C.c_uint64
         L.ora_parse_nested.argtypes = [u8p, C.c_uint64, C.c_int, u64p, C.c_uint64, P(C.c_int32),
                                        P(C.c_uint64)]
+        L.ora_inflate.restype = C.c_int64
+        L.ora_inflate.argtypes = [u8p, C.c_uint64, u8p, C.c_uint64]
+        L.ora_bgzf_inflate.restype = C.c_int64
+        L.ora_bgzf_inflate.argtypes = [u8p, C.c_void_p, C.c_uint64, C.c_int, u8p]
         L.ora_isa.restype = C.c_char_p
         _lib = L
     return _lib
@@ -245,3 +249,20 @@ def parse_nested(data, threads: int = 1):
     err, pos = C.c_int32(), C.c_uint64()
     n = lib().ora_parse_nested(_ptr(d), d.size, threads, _ptr(tape), tape.size, C.byref(err), C.byref(pos))
     return tape[:n].copy(), err.value, pos.value
+
+
+def inflate(payload: bytes, cap: int):
+    """Raw DEFLATE -> bytes, or None when malformed."""
+    d = _buf(payload)
+    out = np.zeros(max(1, cap), dtype=np.uint8)
+    n = lib().ora_inflate(_ptr(d), d.size, _ptr(out), cap)
+    return None if n < 0 else out[:n].tobytes()
+
+
+def bgzf_inflate(comp, blocks: np.ndarray, total: int, threads: int = 1):
+    """Inflated bytes of a block table (falx_bgzf_scan's), or None."""
+    d = _buf(comp)
+    out = np.zeros(max(1, total), dtype=np.uint8)
+    b = np.ascontiguousarray(blocks)
+    n = lib().ora_bgzf_inflate(_ptr(d), C.c_void_p(b.ctypes.data) if b.size else None, b.size, threads, _ptr(out))
+    return None if n < 0 else out[:total]
