@@ -584,6 +584,90 @@ DEV uint32_t block_fn(const BM& m, uint64_t ls) {
 }
 
 // --------------------------------------------------------------------------
+// SWAR classification of 32 bytes (a thread's run in the SWAR front)
+// --------------------------------------------------------------------------
+// Quote, candidate and newline bytes of a run, bit i for byte i.
+struct Masks32 {
+    uint32_t q, c, n;
+};
+
+// Byte transpose of four words: byte k of o_j is byte j of a_k (v_perm_b32:
+// selector bytes 0-3 pick bytes of the second operand, 4-7 of the first).
+DEV void transpose4(uint32_t a0, uint32_t a1, uint32_t a2, uint32_t a3, uint32_t& o0, uint32_t& o1, uint32_t& o2,
+                    uint32_t& o3) {
+    const uint32_t u0 = __builtin_amdgcn_perm(a1, a0, 0x05010400u);   // a0.0 a1.0 a0.1 a1.1
+    const uint32_t u1 = __builtin_amdgcn_perm(a1, a0, 0x07030602u);   // a0.2 a1.2 a0.3 a1.3
+    const uint32_t u2 = __builtin_amdgcn_perm(a3, a2, 0x05010400u);   // a2.0 a3.0 a2.1 a3.1
+    const uint32_t u3 = __builtin_amdgcn_perm(a3, a2, 0x07030602u);   // a2.2 a3.2 a2.3 a3.3
+    o0 = __builtin_amdgcn_perm(u2, u0, 0x05040100u);
+    o1 = __builtin_amdgcn_perm(u2, u0, 0x07060302u);
+    o2 = __builtin_amdgcn_perm(u3, u1, 0x05040100u);
+    o3 = __builtin_amdgcn_perm(u3, u1, 0x07060302u);
+}
+
+// Inclusive prefix XOR over the bits of a word (bit i = parity of bits 0..i).
+DEV uint32_t prefix_xor32(uint32_t x) {
+    x ^= x << 1;
+    x ^= x << 2;
+    x ^= x << 4;
+    x ^= x << 8;
+    x ^= x << 16;
+    return x;
+}
+
+// Bit 7 of each byte of y set when the byte equals c (< 0x80), from the
+// word's low seven bits of each byte (y7) and its bytes below 0x80 (lo):
+// (b ^ c) + 0x7F carries into bit 7 exactly when the low bits differ, and
+// no byte carries into the next.
+DEV uint32_t eq_ascii(uint32_t y7, uint32_t lo, uint32_t c4) { return lo & ~((y7 ^ c4) + 0x7F7F7F7Fu); }
+
+// Words x[0..7] hold the run's bytes in order (byte i = byte i & 3 of
+// x[i >> 2]); NSEP separator bytes besides '\n' (all below 0x80, as is the
+// quote byte).
+template <bool HQ, int NSEP>
+DEV Masks32 classify32(const uint32_t (&x)[8], const Params& P) {
+    // y[j] byte k = run byte 8k + j: words x0 x2 x4 x6 hold bytes 8k + 0..3,
+    // x1 x3 x5 x7 bytes 8k + 4..7.
+    uint32_t y[8];
+    transpose4(x[0], x[2], x[4], x[6], y[0], y[1], y[2], y[3]);
+    transpose4(x[1], x[3], x[5], x[7], y[4], y[5], y[6], y[7]);
+    const uint32_t q4 = HQ ? (uint32_t)P.quote * 0x01010101u : 0u;
+    const uint32_t s0 = (NSEP > 0 ? P.sepb[0] : 0u) * 0x01010101u, s1 = (NSEP > 1 ? P.sepb[1] : 0u) * 0x01010101u;
+    const uint32_t s2 = (NSEP > 2 ? P.sepb[2] : 0u) * 0x01010101u, s3 = (NSEP > 3 ? P.sepb[3] : 0u) * 0x01010101u;
+    const bool nl_cand = P.nl_cand != 0u;
+    Masks32 m{0u, 0u, 0u};
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        const uint32_t y7 = y[j] & 0x7F7F7F7Fu, lo = ~y[j] & 0x80808080u;
+        const uint32_t en = eq_ascii(y7, lo, 0x0A0A0A0Au);
+        uint32_t ec = nl_cand ? en : 0u;
+        if (NSEP > 0) ec |= eq_ascii(y7, lo, s0);
+        if (NSEP > 1) ec |= eq_ascii(y7, lo, s1);
+        if (NSEP > 2) ec |= eq_ascii(y7, lo, s2);
+        if (NSEP > 3) ec |= eq_ascii(y7, lo, s3);
+        m.n |= (en >> 7) << j;
+        m.c |= (ec >> 7) << j;
+        if (HQ) m.q |= (eq_ascii(y7, lo, q4) >> 7) << j;
+    }
+    return m;
+}
+
+// The same masks from the 256-entry candidate table, a byte at a time (more
+// than four separator bytes, or a byte of 0x80 or above).
+template <bool HQ>
+DEV Masks32 classify32_table(const uint32_t (&x)[8], const Params& P, const uint8_t* cls) {
+    Masks32 m{0u, 0u, 0u};
+#pragma unroll
+    for (int i = 0; i < 32; ++i) {
+        const uint32_t c = (x[i >> 2] >> (8 * (i & 3))) & 0xFFu;
+        m.c |= (uint32_t)cls[c] << i;
+        m.n |= (uint32_t)(c == '\n') << i;
+        if (HQ) m.q |= (uint32_t)(c == (uint32_t)P.quote) << i;
+    }
+    return m;
+}
+
+// --------------------------------------------------------------------------
 // the tile kernel
 // --------------------------------------------------------------------------
 // Exact conversion of the f64 cells the fast paths leave undecided (see
@@ -754,148 +838,10 @@ __device__ __forceinline__ void scan_tile(const Params& P, const uint32_t tile, 
     const uint32_t lds_hi = tile_start + min((uint32_t)(HALOED ? TILE + HALO : TILE), P.len - tile_start);
     uint8_t* const tb = sh.lds_bytes + PRE;
 
-    // ---- load: 16-byte coalesced reads into LDS, zero padded --------------
-    // LDS index i holds document byte tile_start - PRE + i for i in [lo_i, hi_i).
-    const uint32_t lo_i = lds_lo + PRE - tile_start, hi_i = lds_hi + PRE - tile_start;
-    for (uint32_t i = (HALOED ? 0u : (uint32_t)PRE) + tid * 16; i < (uint32_t)(HALOED ? PRE + TILE + HALO : PRE + TILE);
-         i += WG * 16) {
-        uint4 v = make_uint4(0, 0, 0, 0);
-        if (i >= lo_i && i + 16 <= hi_i) {
-            v = *reinterpret_cast<const uint4*>(P.data + (tile_start - PRE + i));
-        } else if (i >= lo_i && i < hi_i) {
-            uint8_t tmp[16];
-            for (int k = 0; k < 16; ++k) tmp[k] = (i + k < hi_i) ? P.data[tile_start - PRE + i + k] : 0;
-            memcpy(&v, tmp, 16);
-        }
-        *reinterpret_cast<uint4*>(&sh.lds_bytes[i]) = v;
-    }
-    __syncthreads();
-
-    // line-start carry entering this wave's first block
-    uint64_t ls0;
-    {
-        uint32_t wb = w * BPW * 64;
-        uint32_t v;
-        if (wb > 0) v = tb[wb - 1] == '\n';
-        else v = tile_start == 0 ? 1u : (uint32_t)(P.data[tile_start - 1] == '\n');
-        ls0 = uni(v);
-    }
-    const uint8_t qb = (uint8_t)P.quote, eb = (uint8_t)P.escape, cb = (uint8_t)P.comment;
-
-    const auto live_mask = [&](uint32_t off) -> uint64_t {
-        return off + 64 <= tile_len ? ~0ULL : (off < tile_len ? ((1ULL << (tile_len - off)) - 1) : 0ULL);
-    };
-
-    // ---- classification: ballots per block, then lane b owns block b ------
-    // Each wave classifies its BPW blocks with one byte per lane and
-    // __ballot, and writes block b's masks into lane b: the mask algebra,
-    // the counters and the compaction then run for all blocks at once, one
-    // per lane, instead of block by block on the scalar unit.
-    // The candidate bytes other than '\n' are tested by up to four compares,
-    // each its own ballot ORed as scalar masks (NSEP = their number; '\n'
-    // reuses the newline ballot), or by the 256-entry table (NSEP < 0);
-    // eight blocks' bytes are read before any is used.
-    BM mine{0ULL, 0ULL, 0ULL, 0ULL, 0ULL};
-    const bool nl_cand = P.nl_cand != 0u;
-    auto classify_all = [&](auto nsep) __attribute__((always_inline)) {
-        constexpr int NSEP = decltype(nsep)::value;
-        const uint32_t s0 = P.sepb[0], s1 = P.sepb[1], s2 = P.sepb[2], s3 = P.sepb[3];
-#pragma unroll
-        for (int b0 = 0; b0 < BPW; b0 += 8) {
-            uint32_t cc[8];
-#pragma unroll
-            for (int i = 0; i < 8; ++i) cc[i] = tb[(w * BPW + b0 + i) * 64 + ln];
-#pragma unroll
-            for (int i = 0; i < 8; ++i) {
-                const uint32_t b = (uint32_t)(b0 + i);
-                const uint32_t c = cc[i];
-                const uint64_t nl = ballot(c == '\n');
-                uint64_t cand;
-                if (NSEP < 0) {
-                    cand = ballot(sh.cls[c] != 0);
-                } else {
-                    cand = nl_cand ? nl : 0ULL;
-                    if (NSEP > 0) cand |= ballot(c == s0);
-                    if (NSEP > 1) cand |= ballot(c == s1);
-                    if (NSEP > 2) cand |= ballot(c == s2);
-                    if (NSEP > 3) cand |= ballot(c == s3);
-                }
-                mine.cand = put_lane(mine.cand, cand, b);
-                mine.nl = put_lane(mine.nl, nl, b);
-                if (HQ) mine.q = put_lane(mine.q, ballot(c == qb), b);
-                if (ESC) mine.esc = put_lane(mine.esc, ballot(c == eb), b);
-                if (CMT) mine.cm = put_lane(mine.cm, ballot(c == cb), b);
-            }
-        }
-    };
-    switch (P.n_sepb) {
-    case 0: classify_all(std::integral_constant<int, 0>{}); break;
-    case 1: classify_all(std::integral_constant<int, 1>{}); break;
-    case 2: classify_all(std::integral_constant<int, 2>{}); break;
-    case 3: classify_all(std::integral_constant<int, 3>{}); break;
-    case 4: classify_all(std::integral_constant<int, 4>{}); break;
-    default: classify_all(std::integral_constant<int, -1>{}); break;
-    }
-    const bool own = ln < (uint32_t)BPW;
-    const uint32_t off = (w * BPW + (own ? ln : 0u)) * 64;      // this lane's block in the tile
-    const uint64_t live = own ? live_mask(off) : 0ULL;
-    // Line-start carry: the previous block ends in '\n' (ls0 for the first).
-    const uint32_t prev_hi = __shfl_up((uint32_t)(mine.nl >> 32), 1);
-    const uint64_t ls = ln == 0 ? ls0 : (uint64_t)(prev_hi >> 31);
-    // Exclusive scan of the blocks' transfer functions over the lanes.
-    // Comment dialects resolve regions per entry state: each state's step of
-    // this lane's block is computed once, for its transfer function and for
-    // the passes below, which pick the step of the block's entry state.
-    constexpr bool STEPS = CMT;
-    BlockSteps<STEPS ? NS : 1> steps;
-    const auto step_of = [&](uint32_t entry) __attribute__((always_inline)) -> StepOut {
-        if constexpr (STEPS) return steps.at(entry);
-        else return block_step<HQ, ESC, CMT>(mine, entry, ls);
-    };
-    // Exclusive scan of the blocks' functions over the lanes (f_excl), and
-    // the wave's whole function (f_incl in lane 63).
-    uint32_t f_incl, f_excl;
-    if constexpr (NS == 1) {
-        f_incl = f_excl = 0u;
-    } else if constexpr (HQ && !ESC && !CMT) {
-        // Outside and inside quotes: a block flips the state when it holds
-        // an odd number of quotes, so a lane's entry function is the parity
-        // of the odd blocks below it.
-        const uint64_t odd = ballot(own && (popc(mine.q) & 1u));
-        f_excl = (mbcnt(odd) & 1u) ? 0x01u : 0x10u;
-        f_incl = (popc(odd) & 1u) ? 0x01u : 0x10u;
-    } else {
-        if constexpr (STEPS) {
-            steps.template fill<HQ, ESC, CMT>(mine, ls);
-            f_incl = own ? steps.fn() : fn_ident<NS>();
-        } else {
-            f_incl = own ? block_fn<HQ, ESC, CMT>(mine, ls) : fn_ident<NS>();
-        }
-#pragma unroll
-        for (int d = 1; d < 64; d <<= 1) {
-            const uint32_t g = __shfl_up(f_incl, d);
-            if (ln >= (uint32_t)d) f_incl = fn_then<NS>(g, f_incl);
-        }
-        f_excl = __shfl_up(f_incl, 1);
-        if (ln == 0) f_excl = fn_ident<NS>();
-    }
-
-    // ---- summary mode: every entry state's trajectory through the tile ----
-    if (MODE == M_SUMMARY) {
-        Counts* wsum = reinterpret_cast<Counts*>(sh.list);   // the list is unused here
-#pragma unroll
-        for (int s = 0; s < NS; ++s) {
-            uint64_t S = 0, T = 0, F = 0;
-            if (own) {
-                const StepOut o = step_of(fn_apply(f_excl, (uint32_t)s));
-                S = o.S & live;
-                T = o.T & live;
-                F = o.F & live;
-            }
-            const Counts c = wave_counts(S, T, F, tile_start + off);
-            if (ln == 0) wsum[w * NS + s] = c;
-        }
-        if (ln == 63) sh.wfn[w] = f_incl;
+    // Summary mode's end: the tile summary from the waves' counters
+    // (wsum[w * NS + s]: wave w entered in state s) and functions (sh.wfn).
+    Counts* const wsum = reinterpret_cast<Counts*>(sh.list);   // the list is unused in summary mode
+    auto publish_summary = [&]() __attribute__((always_inline)) {
         __syncthreads();
         uint32_t* out = P.tsum + (size_t)tile * P.ts_stride;
         if (tid < (uint32_t)NS) {
@@ -915,49 +861,301 @@ __device__ __forceinline__ void scan_tile(const Params& P, const uint32_t tile, 
             }
             out[0] = f;
         }
-        return;
-    }
-
-    // ---- entry states: tile prefix from the scan, then through the waves --
-    if (tid == 0) {
-        const uint32_t* tp = P.tpre + (size_t)tile * 8;
-        const uint32_t* ts = P.tsum + (size_t)tile * P.ts_stride;
-        const uint32_t s0 = tp[0];
-        uint32_t s = s0;
-        for (int k = 0; k < NWAVE; ++k) {
-            sh.wentry[k] = s;
-            s = fn_apply(ts[1 + k], s);
+    };
+    // Emit modes: each wave's entry state from the tile prefix and the wave
+    // functions of the summary, and the tile's exclusive and inclusive
+    // counters (the prefix, then the tile's own counters for its entry state).
+    auto load_entry_states = [&]() __attribute__((always_inline)) {
+        if (tid == 0) {
+            const uint32_t* tp = P.tpre + (size_t)tile * 8;
+            const uint32_t* ts = P.tsum + (size_t)tile * P.ts_stride;
+            const uint32_t s0 = tp[0];
+            uint32_t s = s0;
+            for (int k = 0; k < NWAVE; ++k) {
+                sh.wentry[k] = s;
+                s = fn_apply(ts[1 + k], s);
+            }
+            sh.pre = c7_load(tp + 1);
+            sh.inc = compose(sh.pre, c7_load(ts + TS_COUNTS + C7 * s0));
         }
-        // The tile's inclusive counters: the prefix, then the tile's own
-        // counters from its summary for its entry state.
-        sh.pre = c7_load(tp + 1);
-        sh.inc = compose(sh.pre, c7_load(ts + TS_COUNTS + C7 * s0));
-    }
-    __syncthreads();
-
-    // ---- pass 2: masks, compaction, counters (one block per lane) ---------
-    {
-        const uint32_t entry = fn_apply(f_excl, uni(sh.wentry[w]));
-        uint64_t S = 0, T = 0, F = 0;
-        if (own) {
-            const StepOut o = step_of(entry);
-            S = o.S & live;
-            T = o.T & live;
-            F = o.F & live;
-        }
-        const uint32_t n = popc(S);
-        const uint32_t incl = wave_incl_sum(n);
-        if (ln == 63) sh.wcount[w] = incl;
         __syncthreads();
-        // One list for the tile, in byte order: this lane's structurals go
-        // after the earlier waves' and the earlier lanes' entries.
-        uint32_t wb = 0;
-        for (uint32_t k = 0; k < w; ++k) wb += sh.wcount[k];
-        uint16_t* lst = sh.list + uni(wb) + (incl - n);
-        for (uint64_t rem = S; rem; rem &= rem - 1) {
-            const uint32_t bit = (uint32_t)__ffsll((long long)rem) - 1u;
-            *lst++ = (uint16_t)((off + bit) | (uint32_t)(((T >> bit) & 1ULL) << 14) |
-                                (uint32_t)(((F >> bit) & 1ULL) << 15));
+    };
+
+    // Dialects with neither escape nor comment take the SWAR front below; the
+    // others classify by ballots with one lane per 64-byte block.
+    constexpr bool SWAR = !ESC && !CMT;
+    if constexpr (SWAR) {
+        // ---- SWAR front: thread t holds tile bytes [32t, 32t + 32) ---------
+        // The bytes come from HBM into registers (and into the LDS copy the
+        // cell readers use) and are classified four to a word. A byte
+        // transpose puts bytes 8k + j of the thread's run into byte k of
+        // word j, so a candidate's per-byte equality flags (bit 7 of each
+        // byte) fold into one 32-bit mask in byte order with a shift-or per
+        // word. Each wave covers the same 2 KiB as in the ballot form, so
+        // the two forms' tile summaries are interchangeable.
+        const uint32_t t0 = tid * 32u;
+        uint32_t x[8];
+        if (t0 + 32u <= tile_len) {
+            const uint4 a = *reinterpret_cast<const uint4*>(P.data + tile_start + t0);
+            const uint4 b = *reinterpret_cast<const uint4*>(P.data + tile_start + t0 + 16u);
+            x[0] = a.x; x[1] = a.y; x[2] = a.z; x[3] = a.w;
+            x[4] = b.x; x[5] = b.y; x[6] = b.z; x[7] = b.w;
+        } else {
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                uint32_t v = 0;
+                for (uint32_t i = 0; i < 4; ++i) {
+                    const uint32_t at = t0 + 4u * (uint32_t)k + i;
+                    if (at < tile_len) v |= (uint32_t)P.data[tile_start + at] << (8 * i);
+                }
+                x[k] = v;
+            }
+        }
+        reinterpret_cast<uint4*>(tb + t0)[0] = make_uint4(x[0], x[1], x[2], x[3]);
+        reinterpret_cast<uint4*>(tb + t0)[1] = make_uint4(x[4], x[5], x[6], x[7]);
+        if (HALOED && tid < (uint32_t)((PRE + HALO) / 16)) {
+            // PRE bytes before the tile and HALO bytes after it, zero outside
+            // the document. LDS index i holds document byte tile_start - PRE + i.
+            const uint32_t lo_i = lds_lo + PRE - tile_start, hi_i = lds_hi + PRE - tile_start;
+            const uint32_t i = tid < (uint32_t)(PRE / 16) ? tid * 16u : (uint32_t)(PRE + TILE) + (tid - PRE / 16) * 16u;
+            uint4 v = make_uint4(0, 0, 0, 0);
+            if (i >= lo_i && i + 16 <= hi_i) {
+                v = *reinterpret_cast<const uint4*>(P.data + (tile_start - PRE + i));
+            } else if (i >= lo_i && i < hi_i) {
+                uint8_t tmp[16];
+                for (int k = 0; k < 16; ++k) tmp[k] = (i + k < hi_i) ? P.data[tile_start - PRE + i + k] : 0;
+                memcpy(&v, tmp, 16);
+            }
+            *reinterpret_cast<uint4*>(&sh.lds_bytes[i]) = v;
+        }
+        const uint32_t live32 = t0 + 32u <= tile_len ? ~0u : (t0 < tile_len ? (1u << (tile_len - t0)) - 1u : 0u);
+        Masks32 m{0u, 0u, 0u};
+        // Candidate and quote bytes below 0x80 with at most four separator
+        // bytes take the word compares; other dialects the byte table.
+        const uint32_t qa = HQ ? (uint32_t)P.quote : 0u;
+        const bool ascii = qa < 0x80u && (P.n_sepb < 1 || P.sepb[0] < 0x80u) && (P.n_sepb < 2 || P.sepb[1] < 0x80u) &&
+                           (P.n_sepb < 3 || P.sepb[2] < 0x80u) && (P.n_sepb < 4 || P.sepb[3] < 0x80u);
+        switch (ascii ? P.n_sepb : 5u) {
+        case 0: m = classify32<HQ, 0>(x, P); break;
+        case 1: m = classify32<HQ, 1>(x, P); break;
+        case 2: m = classify32<HQ, 2>(x, P); break;
+        case 3: m = classify32<HQ, 3>(x, P); break;
+        case 4: m = classify32<HQ, 4>(x, P); break;
+        default: m = classify32_table<HQ>(x, P, sh.cls); break;
+        }
+        m.q &= live32;
+        m.c &= live32;
+        m.n &= live32;
+        // Quote parity: inclusive prefix from the thread's first byte, the
+        // quotes of the wave's earlier threads (mbcnt over the odd ones) and
+        // the wave's function (swap when its quote count is odd).
+        const uint32_t pref = HQ ? prefix_xor32(m.q) : 0u;
+        const uint64_t odd = ballot(HQ && (__popc(m.q) & 1));
+        const uint32_t ex = mbcnt(odd) & 1u;
+        const uint32_t wave_fn = NS == 2 ? ((popc(odd) & 1u) ? 0x01u : 0x10u) : 0u;
+        __syncthreads();   // the LDS copy is complete
+        if (MODE == M_SUMMARY) {
+#pragma unroll
+            for (int s = 0; s < NS; ++s) {
+                const uint32_t inside = pref ^ (((uint32_t)s ^ ex) ? ~0u : 0u);
+                const uint32_t S = HQ ? (m.c & ~inside) : m.c, T = S & m.n;
+                const Counts c = wave_counts((uint64_t)S, (uint64_t)T, 0ULL, tile_start + t0);
+                if (ln == 0) wsum[w * NS + s] = c;
+            }
+            if (ln == 0) sh.wfn[w] = wave_fn;
+            publish_summary();
+            return;
+        }
+        load_entry_states();
+        {
+            const uint32_t entry = HQ ? (uni(sh.wentry[w]) ^ ex) : 0u;
+            const uint32_t inside = pref ^ (entry ? ~0u : 0u);
+            const uint32_t S = HQ ? (m.c & ~inside) : m.c, T = S & m.n;
+            const uint32_t n = (uint32_t)__popc(S);
+            const uint32_t incl = wave_incl_sum(n);
+            if (ln == 63) sh.wcount[w] = incl;
+            __syncthreads();
+            uint32_t wb = 0;
+            for (uint32_t k = 0; k < w; ++k) wb += sh.wcount[k];
+            uint16_t* lst = sh.list + uni(wb) + (incl - n);
+            for (uint32_t rem = S; rem; rem &= rem - 1u) {
+                const uint32_t bit = (uint32_t)__ffs((int)rem) - 1u;
+                *lst++ = (uint16_t)((t0 + bit) | (((T >> bit) & 1u) << 14));
+            }
+        }
+    } else {
+        // ---- load: 16-byte coalesced reads into LDS, zero padded --------------
+        // LDS index i holds document byte tile_start - PRE + i for i in [lo_i, hi_i).
+        const uint32_t lo_i = lds_lo + PRE - tile_start, hi_i = lds_hi + PRE - tile_start;
+        for (uint32_t i = (HALOED ? 0u : (uint32_t)PRE) + tid * 16; i < (uint32_t)(HALOED ? PRE + TILE + HALO : PRE + TILE);
+             i += WG * 16) {
+            uint4 v = make_uint4(0, 0, 0, 0);
+            if (i >= lo_i && i + 16 <= hi_i) {
+                v = *reinterpret_cast<const uint4*>(P.data + (tile_start - PRE + i));
+            } else if (i >= lo_i && i < hi_i) {
+                uint8_t tmp[16];
+                for (int k = 0; k < 16; ++k) tmp[k] = (i + k < hi_i) ? P.data[tile_start - PRE + i + k] : 0;
+                memcpy(&v, tmp, 16);
+            }
+            *reinterpret_cast<uint4*>(&sh.lds_bytes[i]) = v;
+        }
+        __syncthreads();
+
+        // line-start carry entering this wave's first block
+        uint64_t ls0;
+        {
+            uint32_t wb = w * BPW * 64;
+            uint32_t v;
+            if (wb > 0) v = tb[wb - 1] == '\n';
+            else v = tile_start == 0 ? 1u : (uint32_t)(P.data[tile_start - 1] == '\n');
+            ls0 = uni(v);
+        }
+        const uint8_t qb = (uint8_t)P.quote, eb = (uint8_t)P.escape, cb = (uint8_t)P.comment;
+
+        const auto live_mask = [&](uint32_t off) -> uint64_t {
+            return off + 64 <= tile_len ? ~0ULL : (off < tile_len ? ((1ULL << (tile_len - off)) - 1) : 0ULL);
+        };
+
+        // ---- classification: ballots per block, then lane b owns block b ------
+        // Each wave classifies its BPW blocks with one byte per lane and
+        // __ballot, and writes block b's masks into lane b: the mask algebra,
+        // the counters and the compaction then run for all blocks at once, one
+        // per lane, instead of block by block on the scalar unit.
+        // The candidate bytes other than '\n' are tested by up to four compares,
+        // each its own ballot ORed as scalar masks (NSEP = their number; '\n'
+        // reuses the newline ballot), or by the 256-entry table (NSEP < 0);
+        // eight blocks' bytes are read before any is used.
+        BM mine{0ULL, 0ULL, 0ULL, 0ULL, 0ULL};
+        const bool nl_cand = P.nl_cand != 0u;
+        auto classify_all = [&](auto nsep) __attribute__((always_inline)) {
+            constexpr int NSEP = decltype(nsep)::value;
+            const uint32_t s0 = P.sepb[0], s1 = P.sepb[1], s2 = P.sepb[2], s3 = P.sepb[3];
+    #pragma unroll
+            for (int b0 = 0; b0 < BPW; b0 += 8) {
+                uint32_t cc[8];
+    #pragma unroll
+                for (int i = 0; i < 8; ++i) cc[i] = tb[(w * BPW + b0 + i) * 64 + ln];
+    #pragma unroll
+                for (int i = 0; i < 8; ++i) {
+                    const uint32_t b = (uint32_t)(b0 + i);
+                    const uint32_t c = cc[i];
+                    const uint64_t nl = ballot(c == '\n');
+                    uint64_t cand;
+                    if (NSEP < 0) {
+                        cand = ballot(sh.cls[c] != 0);
+                    } else {
+                        cand = nl_cand ? nl : 0ULL;
+                        if (NSEP > 0) cand |= ballot(c == s0);
+                        if (NSEP > 1) cand |= ballot(c == s1);
+                        if (NSEP > 2) cand |= ballot(c == s2);
+                        if (NSEP > 3) cand |= ballot(c == s3);
+                    }
+                    mine.cand = put_lane(mine.cand, cand, b);
+                    mine.nl = put_lane(mine.nl, nl, b);
+                    if (HQ) mine.q = put_lane(mine.q, ballot(c == qb), b);
+                    if (ESC) mine.esc = put_lane(mine.esc, ballot(c == eb), b);
+                    if (CMT) mine.cm = put_lane(mine.cm, ballot(c == cb), b);
+                }
+            }
+        };
+        switch (P.n_sepb) {
+        case 0: classify_all(std::integral_constant<int, 0>{}); break;
+        case 1: classify_all(std::integral_constant<int, 1>{}); break;
+        case 2: classify_all(std::integral_constant<int, 2>{}); break;
+        case 3: classify_all(std::integral_constant<int, 3>{}); break;
+        case 4: classify_all(std::integral_constant<int, 4>{}); break;
+        default: classify_all(std::integral_constant<int, -1>{}); break;
+        }
+        const bool own = ln < (uint32_t)BPW;
+        const uint32_t off = (w * BPW + (own ? ln : 0u)) * 64;      // this lane's block in the tile
+        const uint64_t live = own ? live_mask(off) : 0ULL;
+        // Line-start carry: the previous block ends in '\n' (ls0 for the first).
+        const uint32_t prev_hi = __shfl_up((uint32_t)(mine.nl >> 32), 1);
+        const uint64_t ls = ln == 0 ? ls0 : (uint64_t)(prev_hi >> 31);
+        // Exclusive scan of the blocks' transfer functions over the lanes.
+        // Comment dialects resolve regions per entry state: each state's step of
+        // this lane's block is computed once, for its transfer function and for
+        // the passes below, which pick the step of the block's entry state.
+        constexpr bool STEPS = CMT;
+        BlockSteps<STEPS ? NS : 1> steps;
+        const auto step_of = [&](uint32_t entry) __attribute__((always_inline)) -> StepOut {
+            if constexpr (STEPS) return steps.at(entry);
+            else return block_step<HQ, ESC, CMT>(mine, entry, ls);
+        };
+        // Exclusive scan of the blocks' functions over the lanes (f_excl), and
+        // the wave's whole function (f_incl in lane 63).
+        uint32_t f_incl, f_excl;
+        if constexpr (NS == 1) {
+            f_incl = f_excl = 0u;
+        } else if constexpr (HQ && !ESC && !CMT) {
+            // Outside and inside quotes: a block flips the state when it holds
+            // an odd number of quotes, so a lane's entry function is the parity
+            // of the odd blocks below it.
+            const uint64_t odd = ballot(own && (popc(mine.q) & 1u));
+            f_excl = (mbcnt(odd) & 1u) ? 0x01u : 0x10u;
+            f_incl = (popc(odd) & 1u) ? 0x01u : 0x10u;
+        } else {
+            if constexpr (STEPS) {
+                steps.template fill<HQ, ESC, CMT>(mine, ls);
+                f_incl = own ? steps.fn() : fn_ident<NS>();
+            } else {
+                f_incl = own ? block_fn<HQ, ESC, CMT>(mine, ls) : fn_ident<NS>();
+            }
+    #pragma unroll
+            for (int d = 1; d < 64; d <<= 1) {
+                const uint32_t g = __shfl_up(f_incl, d);
+                if (ln >= (uint32_t)d) f_incl = fn_then<NS>(g, f_incl);
+            }
+            f_excl = __shfl_up(f_incl, 1);
+            if (ln == 0) f_excl = fn_ident<NS>();
+        }
+
+        // ---- summary mode: every entry state's trajectory through the tile ----
+        if (MODE == M_SUMMARY) {
+    #pragma unroll
+            for (int s = 0; s < NS; ++s) {
+                uint64_t S = 0, T = 0, F = 0;
+                if (own) {
+                    const StepOut o = step_of(fn_apply(f_excl, (uint32_t)s));
+                    S = o.S & live;
+                    T = o.T & live;
+                    F = o.F & live;
+                }
+                const Counts c = wave_counts(S, T, F, tile_start + off);
+                if (ln == 0) wsum[w * NS + s] = c;
+            }
+            if (ln == 63) sh.wfn[w] = f_incl;
+            publish_summary();
+            return;
+        }
+
+        // ---- entry states: tile prefix from the scan, then through the waves --
+        load_entry_states();
+
+        // ---- pass 2: masks, compaction, counters (one block per lane) ---------
+        {
+            const uint32_t entry = fn_apply(f_excl, uni(sh.wentry[w]));
+            uint64_t S = 0, T = 0, F = 0;
+            if (own) {
+                const StepOut o = step_of(entry);
+                S = o.S & live;
+                T = o.T & live;
+                F = o.F & live;
+            }
+            const uint32_t n = popc(S);
+            const uint32_t incl = wave_incl_sum(n);
+            if (ln == 63) sh.wcount[w] = incl;
+            __syncthreads();
+            // One list for the tile, in byte order: this lane's structurals go
+            // after the earlier waves' and the earlier lanes' entries.
+            uint32_t wb = 0;
+            for (uint32_t k = 0; k < w; ++k) wb += sh.wcount[k];
+            uint16_t* lst = sh.list + uni(wb) + (incl - n);
+            for (uint64_t rem = S; rem; rem &= rem - 1) {
+                const uint32_t bit = (uint32_t)__ffsll((long long)rem) - 1u;
+                *lst++ = (uint16_t)((off + bit) | (uint32_t)(((T >> bit) & 1ULL) << 14) |
+                                    (uint32_t)(((F >> bit) & 1ULL) << 15));
+            }
         }
     }
     __syncthreads();
@@ -1121,7 +1319,7 @@ __device__ __forceinline__ void scan_tile(const Params& P, const uint32_t tile, 
 
     const Counts inc = sh.inc;
     const bool trailer = last_tile && inc.last_term < P.len;
-    const bool trailer_comment = trailer && CMT && P.data[inc.last_term] == cb;
+    const bool trailer_comment = trailer && CMT && P.data[inc.last_term] == (uint8_t)P.comment;
     // Virtual terminator word (local position, term bit, comment flag).
     const uint32_t trailer_word =
         trailer ? ((P.len - tile_start) | 0x4000u | (trailer_comment ? 0x8000u : 0u)) : 0u;
