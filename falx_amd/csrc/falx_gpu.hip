@@ -135,6 +135,23 @@ hipError_t launch_fast(falx_ctx* c, Params& P, int kind) {
     return e;
 }
 
+// Dialects with neither escape nor comment (kinds 0 and 2) have numeric fast
+// forms, for passes whose columns [first, first + n) are all i64/f64.
+bool numeric_fast(const falx_spec* spec, int kind, uint32_t first, uint32_t n) {
+    if (kind != 0 && kind != 2) return false;
+    for (uint32_t k = first; k < first + n && k < spec->n_columns; ++k) {
+        const uint32_t t = spec->columns[k].type;
+        if (t != FALX_COL_I64 && t != FALX_COL_F64) return false;
+    }
+    return true;
+}
+
+// The statistics pass over columns [P.col_group, + STAT_GROUP).
+hipError_t launch_stats_group(falx_ctx* c, const falx_spec* spec, Params& P, int kind) {
+    if (numeric_fast(spec, kind, P.col_group, STAT_GROUP)) return launch_fast<M_NUM_STATS_FAST, M_STATS>(c, P, kind);
+    return launch_fast<M_STATS_FAST, M_STATS>(c, P, kind);
+}
+
 // The tile-prefix pipeline: tile summaries (every entry state), then the
 // three-level scan to each tile's entry state and exclusive counters, and
 // the grand totals in the result slots. Emit passes read P.tpre.
@@ -498,16 +515,22 @@ static falx_status columns_impl(falx_ctx* c, const falx_spec* spec, const uint8_
         if ((st = ensure_part(c, part_bytes)) != FALX_OK) return st;
         P.partials = c->d_part;
     }
+    const int kind = dialect_kind(spec);
     HIP_TRY(mark_start(c));
-    HIP_TRY(run_prefix(c, P, dialect_kind(spec)));
-    if (stats) HIP_TRY((launch_fast<M_COLUMNS_FAST_STATS, M_COLUMNS>(c, P, dialect_kind(spec))));
-    else HIP_TRY((launch_fast<M_COLUMNS_FAST, M_COLUMNS>(c, P, dialect_kind(spec))));
+    HIP_TRY(run_prefix(c, P, kind));
+    if (numeric_fast(spec, kind, 0, spec->n_columns)) {
+        if (stats) HIP_TRY((launch_fast<M_NUM_FAST_STATS, M_COLUMNS>(c, P, kind)));
+        else HIP_TRY((launch_fast<M_NUM_FAST, M_COLUMNS>(c, P, kind)));
+    } else {
+        if (stats) HIP_TRY((launch_fast<M_COLUMNS_FAST_STATS, M_COLUMNS>(c, P, kind)));
+        else HIP_TRY((launch_fast<M_COLUMNS_FAST, M_COLUMNS>(c, P, kind)));
+    }
     // Statistics of columns past the first group come from extra passes.
     if (stats) {
         Params Q = P;
         for (uint32_t g = COL_STAT_GROUP; g < spec->n_columns; g += STAT_GROUP) {
             Q.col_group = g;
-            HIP_TRY((launch_fast<M_STATS_FAST, M_STATS>(c, Q, dialect_kind(spec))));
+            HIP_TRY(launch_stats_group(c, spec, Q, kind));
         }
     }
     // String columns: arena bases from the per-tile byte counts, then the
@@ -592,11 +615,12 @@ falx_status falx_parse_csv_stats(falx_ctx* c, const falx_spec* spec, const uint8
     P.partials = c->d_part;
     P.n_cols = spec->n_columns;
     uint32_t groups = spec->n_columns ? (spec->n_columns + STAT_GROUP - 1) / STAT_GROUP : 1;
+    const int kind = dialect_kind(spec);
     HIP_TRY(mark_start(c));
-    HIP_TRY(run_prefix(c, P, dialect_kind(spec)));
+    HIP_TRY(run_prefix(c, P, kind));
     for (uint32_t g = 0; g < groups; ++g) {
         P.col_group = g * STAT_GROUP;
-        HIP_TRY((launch_fast<M_STATS_FAST, M_STATS>(c, P, dialect_kind(spec))));
+        HIP_TRY(launch_stats_group(c, spec, P, kind));
     }
     HIP_TRY(mark_end(c));
     return reduce_stats(c, spec, P.n_tiles, out);

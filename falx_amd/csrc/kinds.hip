@@ -14,6 +14,8 @@ namespace {
 constexpr bool K_HQ = FX_KIND >= 2;
 constexpr bool K_ESC = FX_KIND == 3 || FX_KIND == 5;
 constexpr bool K_CMT = FX_KIND == 1 || FX_KIND == 4 || FX_KIND == 5;
+// Kinds with neither escape nor comment: the SWAR front and its numeric fast forms.
+constexpr bool K_SWAR = !K_ESC && !K_CMT;
 
 template <int MODE>
 hipError_t plain(dim3 g, hipStream_t s, const Params* P) {
@@ -48,6 +50,15 @@ hipError_t launch_kind<FX_KIND>(int form, int mode, dim3 g, hipStream_t s, const
         case M_COLUMNS_FAST: return plain<M_COLUMNS_FAST>(g, s, P);
         case M_COLUMNS_FAST_STATS: return plain<M_COLUMNS_FAST_STATS>(g, s, P);
         case M_STATS_FAST: return plain<M_STATS_FAST>(g, s, P);
+        case M_NUM_FAST:
+            if constexpr (K_SWAR) return plain<M_NUM_FAST>(g, s, P);
+            break;
+        case M_NUM_FAST_STATS:
+            if constexpr (K_SWAR) return plain<M_NUM_FAST_STATS>(g, s, P);
+            break;
+        case M_NUM_STATS_FAST:
+            if constexpr (K_SWAR) return plain<M_NUM_STATS_FAST>(g, s, P);
+            break;
         case M_STRINGS: return plain<M_STRINGS>(g, s, P);
         case M_FIELDBYTES: return plain<M_FIELDBYTES>(g, s, P);
         case M_LOGFMT:

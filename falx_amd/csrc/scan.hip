@@ -51,6 +51,8 @@ constexpr int TILE = NWAVE * BPW * 64;    // 8192 bytes per workgroup
 constexpr int PRE = 64;
 constexpr int HALO = 64;
 constexpr int TILE_LDS = PRE + TILE + HALO + 16;
+// Terminators the numeric fast forms list per tile (more: the full pass).
+constexpr int TL_CAP = 1024;
 constexpr uint64_t EVEN_BITS = 0x5555555555555555ULL;
 constexpr int STAT_GROUP = 2;             // stat columns accumulated per STATS pass
 constexpr int COL_STAT_GROUP = 2;         // stat columns fused into a COLUMNS pass
@@ -61,7 +63,11 @@ enum Mode : int {
     // a cell they leave undecided is flagged in Params::fixup and redone by
     // the full pass, whose launch skips the unflagged tiles.
     M_COLUMNS_FAST, M_STATS_FAST,
-    M_COLUMNS_FAST_STATS   // M_COLUMNS_FAST with the fused statistics (the plain fast form has none)
+    M_COLUMNS_FAST_STATS,  // M_COLUMNS_FAST with the fused statistics (the plain fast form has none)
+    // The fast forms for dialects without escape or comment when every
+    // column of the pass is numeric (i64/f64): the terminators are listed
+    // while the tile is compacted, and records go straight to threads.
+    M_NUM_FAST, M_NUM_FAST_STATS, M_NUM_STATS_FAST
 };
 
 // Record/field counters of a byte range; `compose(a, b)` = a then b.
@@ -812,10 +818,14 @@ __device__ __forceinline__ void scan_tile(const Params& P, const uint32_t tile, 
     constexpr int NS = Dial<HQ, ESC, CMT>::NS;
     // A *_FAST pass is its pass without the stream parsers (and their
     // register and code footprint): cells they would take flag the tile.
-    constexpr bool FAST = KMODE == M_COLUMNS_FAST || KMODE == M_COLUMNS_FAST_STATS || KMODE == M_STATS_FAST;
-    constexpr int MODE = (KMODE == M_COLUMNS_FAST || KMODE == M_COLUMNS_FAST_STATS)
+    constexpr bool NUMFAST = KMODE == M_NUM_FAST || KMODE == M_NUM_FAST_STATS || KMODE == M_NUM_STATS_FAST;
+    static_assert(!NUMFAST || (!ESC && !CMT), "numeric fast forms: dialects without escape or comment");
+    constexpr bool FAST =
+        NUMFAST || KMODE == M_COLUMNS_FAST || KMODE == M_COLUMNS_FAST_STATS || KMODE == M_STATS_FAST;
+    constexpr int MODE = (KMODE == M_COLUMNS_FAST || KMODE == M_COLUMNS_FAST_STATS || KMODE == M_NUM_FAST ||
+                          KMODE == M_NUM_FAST_STATS)
                              ? M_COLUMNS
-                             : (KMODE == M_STATS_FAST ? M_STATS : KMODE);
+                             : ((KMODE == M_STATS_FAST || KMODE == M_NUM_STATS_FAST) ? M_STATS : KMODE);
     const uint32_t tid = threadIdx.x, ln = tid & 63u, w = uni(tid >> 6);  // w is wave-uniform
 
     if (tid == 0) {
@@ -971,17 +981,45 @@ __device__ __forceinline__ void scan_tile(const Params& P, const uint32_t tile, 
             const uint32_t entry = HQ ? (uni(sh.wentry[w]) ^ ex) : 0u;
             const uint32_t inside = pref ^ (entry ? ~0u : 0u);
             const uint32_t S = HQ ? (m.c & ~inside) : m.c, T = S & m.n;
-            const uint32_t n = (uint32_t)__popc(S);
+            // Structurals and terminators counted in the two halves of one
+            // word (a tile holds at most 8192 of each).
+            const uint32_t n = (uint32_t)__popc(S) | ((uint32_t)__popc(T) << 16);
             const uint32_t incl = wave_incl_sum(n);
             if (ln == 63) sh.wcount[w] = incl;
             __syncthreads();
-            uint32_t wb = 0;
-            for (uint32_t k = 0; k < w; ++k) wb += sh.wcount[k];
-            uint16_t* lst = sh.list + uni(wb) + (incl - n);
-            for (uint32_t rem = S; rem; rem &= rem - 1u) {
-                const uint32_t bit = (uint32_t)__ffs((int)rem) - 1u;
-                *lst++ = (uint16_t)((t0 + bit) | (((T >> bit) & 1u) << 14));
+            uint32_t wb = 0, tot = 0;
+#pragma unroll
+            for (uint32_t k = 0; k < (uint32_t)NWAVE; ++k) {
+                const uint32_t c = sh.wcount[k];
+                if (k < w) wb += c;
+                tot += c;
             }
+            const uint32_t at = uni(wb) + (incl - n);
+            uint16_t* lst = sh.list + (at & 0xFFFFu);
+            if constexpr (NUMFAST) {
+                // The numeric fast forms also list the terminators (list
+                // index by rank) at the list's far end; a tile whose lists
+                // would meet is left to the full pass.
+                const bool fits = (tot & 0xFFFFu) <= (uint32_t)(TILE - TL_CAP) && (tot >> 16) <= (uint32_t)TL_CAP;
+                uint16_t* tl = sh.list + (TILE - TL_CAP);
+                uint32_t rank = at >> 16, idx = at & 0xFFFFu;
+                for (uint32_t rem = S; rem; rem &= rem - 1u) {
+                    const uint32_t bit = (uint32_t)__ffs((int)rem) - 1u;
+                    const uint32_t t = (T >> bit) & 1u;
+                    *lst++ = (uint16_t)((t0 + bit) | (t << 14));
+                    if (t && fits) tl[rank] = (uint16_t)idx;
+                    rank += t;
+                    idx += 1u;
+                }
+                if (tid == 0 && !fits) sh.fix_bad = 1u;
+            } else {
+                for (uint32_t rem = S; rem; rem &= rem - 1u) {
+                    const uint32_t bit = (uint32_t)__ffs((int)rem) - 1u;
+                    *lst++ = (uint16_t)((t0 + bit) | (((T >> bit) & 1u) << 14));
+                }
+            }
+            __syncthreads();   // every thread has read sh.wcount
+            if (ln == 63) sh.wcount[w] = incl & 0xFFFFu;
         }
     } else {
         // ---- load: 16-byte coalesced reads into LDS, zero padded --------------
@@ -1551,6 +1589,108 @@ __device__ __forceinline__ void scan_tile(const Params& P, const uint32_t tile, 
         }
         return v;
     };
+    // ---- numeric fast forms: records straight to threads -------------------
+    // Record r of the tile ends at its r-th terminator, list entry tl[r] (the
+    // SWAR front listed them); the record after the last terminator ends at
+    // the virtual terminator (last tile) or is still open, and then only its
+    // fields that end here are written (the next tile writes the rest).
+    // Thread r takes record r: its fields that end in this tile are entries
+    // e0 .. e0 + n - 1, so column c's cell is entry e0 + (c.index - ord0).
+    // Rows are consecutive across a wave's lanes, so a column's validity bits
+    // are one ballot, ORed into at most two words.
+    if constexpr (NUMFAST) {
+        constexpr bool IS_COLS = MODE == M_COLUMNS;
+        constexpr uint32_t NACC = 2;
+        static_assert(NACC == STAT_GROUP && NACC == COL_STAT_GROUP, "accumulators per pass");
+        constexpr bool with_stats = KMODE != M_NUM_FAST;
+        const uint16_t* const tl = sh.list + (TILE - TL_CAP);
+        const uint32_t c0 = IS_COLS ? 0u : P.col_group;
+        const uint32_t nc = IS_COLS ? P.n_cols : min((uint32_t)STAT_GROUP, P.n_cols - c0);
+        const uint64_t rows_cap = P.rows_cap;
+        const auto word_at = [&](uint32_t k) -> uint32_t { return k < n_list ? (uint32_t)sh.list[k] : trailer_word; };
+        const auto pos_of = [&](uint32_t wd) -> uint32_t { return tile_start + (wd & 0x3FFFu); };
+        Acc acc[NACC];
+#pragma unroll
+        for (uint32_t j = 0; j < NACC; ++j) acc[j] = Acc{0, 0, 0, 0, 0, 0.0};
+        unsigned long long records = 0;
+        if (!uni(sh.fix_bad)) {
+            const uint32_t nt = inc.n_terms - pre.n_terms;   // the tile's real terminators
+            const uint32_t e_after = nt ? (uint32_t)tl[nt - 1] + 1u : 0u;
+            const uint32_t n_rec = nt + ((trailer || e_after < n_list) ? 1u : 0u);
+            for (uint32_t r0 = 0; r0 < n_rec; r0 += WG) {
+                const uint32_t r = r0 + tid;
+                const bool has = r < n_rec;
+                uint32_t e0 = 0, n = 0, fs0 = 0, rs = 0, ord0 = 0;
+                bool term = false;
+                if (has) {
+                    e0 = r == 0 ? 0u : (uint32_t)tl[r - 1] + 1u;
+                    if (r < nt) {
+                        term = true;
+                        n = (uint32_t)tl[r] - e0 + 1u;
+                    } else {
+                        term = trailer;
+                        n = n_list - e0 + (trailer ? 1u : 0u);
+                    }
+                    if (e0 == 0) {
+                        fs0 = pre.last_struct;
+                        rs = pre.last_term;
+                        ord0 = pre.ord;
+                    } else {
+                        fs0 = pos_of(sh.list[e0 - 1]) + 1u;
+                        rs = fs0;
+                    }
+                }
+                if (has && term) records += 1;
+                const uint32_t row = pre.n_rows + r;
+                const bool live = row < rows_cap;
+                // The wave's first row (lanes with a record are the lowest).
+                const uint32_t rfirst = pre.n_rows + r0 + 64u * w;
+                for (uint32_t jj = 0; jj < nc; ++jj) {
+                    const ColDesc& c = P.cols[c0 + jj];
+                    const int32_t i = (int32_t)c.index - (int32_t)ord0;
+                    const bool here = has && i >= 0 && (uint32_t)i < n;
+                    const bool missing = has && term && i >= (int32_t)n;
+                    uint32_t fs = 0, fe = 0;
+                    if (here) {
+                        const uint32_t k = e0 + (uint32_t)i;
+                        fe = pos_of(word_at(k));
+                        fs = i == 0 ? fs0 : pos_of(word_at(k - 1)) + 1u;
+                        if (term && (uint32_t)i + 1u == n && fe > rs && src.at(fe - 1) == '\r') fe -= 1;
+                    }
+                    const bool as_f64 = c.type == FALX_COL_F64;
+                    NumVal nv{};
+                    if (here) nv = num_value(fs, fe, as_f64);
+                    const bool vb = here && (as_f64 ? nv.f.ok : nv.iok);
+                    if (IS_COLS && (here || missing) && live) {
+                        if (as_f64) reinterpret_cast<double*>(c.values)[row] = vb ? nv.f.value : 0.0;
+                        else reinterpret_cast<long long*>(c.values)[row] = vb ? nv.iv : 0;
+                    }
+                    if (with_stats && jj < NACC && (IS_COLS ? live : true)) {
+                        CellStat o{false, false, 0, 0, 0, 0.0};
+                        if (vb) {
+                            o = as_f64 ? CellStat{true, nv.f.fixed6, f64_to_bits(nv.f.value), 0, nv.f.mantissa, nv.f.value}
+                                       : CellStat{true, false, (unsigned long long)nv.iv, 0, 0, (double)nv.iv};
+                        }
+                        acc_add(acc, jj, o);
+                    }
+                    if (IS_COLS) {
+                        const uint64_t W = ballot(vb && live);
+                        if (W && ln == 0) {
+                            const uint32_t wi = rfirst >> 6, sb = rfirst & 63u;
+                            atomicOr(&c.valid[wi], (unsigned long long)(W << sb));
+                            if (sb && (W >> (64u - sb))) atomicOr(&c.valid[wi + 1], (unsigned long long)(W >> (64u - sb)));
+                        }
+                    }
+                }
+            }
+        }
+        if (IS_COLS && tid == 0 && last_tile) P.result[R_N_ROWS] = inc.n_rows + (trailer ? 1 : 0);
+        __syncthreads();
+        if (tid == 0) P.fixup[tile] = sh.fix_bad;
+        if (with_stats) acc_publish(acc, c0, min(NACC, nc), records);
+        return;
+    }
+
     // ---- COLUMNS and STATS: one record per thread --------------------------
     // The tile's records end at its terminator entries (and at the virtual
     // terminator of an unterminated last record); a record still open after
@@ -1927,7 +2067,11 @@ __device__ __forceinline__ void scan_tile(const Params& P, const uint32_t tile, 
 // over the register budget of 4 waves/SIMD otherwise); 1 = no constraint.
 template <int MODE>
 constexpr int plain_waves() {
-    return MODE == M_STATS ? 4 : ((MODE == M_STATS_FAST || MODE == M_COLUMNS_FAST_STATS) ? 5 : 1);
+    return MODE == M_STATS ? 4
+                           : ((MODE == M_STATS_FAST || MODE == M_COLUMNS_FAST_STATS || MODE == M_NUM_FAST_STATS ||
+                               MODE == M_NUM_STATS_FAST)
+                                  ? 5
+                                  : 1);
 }
 
 // A document's Params read through the constant address space: nothing

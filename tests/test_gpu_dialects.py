@@ -28,6 +28,11 @@ HASH4 = S.CSV_HASH_DIALECT.with_columns([(0, S.I64), (1, S.F64), (2, S.STR), (3,
 BS_CMT_TYPED = S.Spec("backslash_comment_typed", b",\n", ord('"'), ord("\\"), ord("#"),
                       (S.Column(0, S.I64), S.Column(1, S.F64), S.Column(2, S.STR), S.Column(3, S.STR)))  # kind 5
 KIND_SPECS = [TSV_TYPED, CMT_TYPED, CSV_TYPED4, BS_TYPED, HASH4, BS_CMT_TYPED]
+# Numeric-only projections of the kinds with neither escape nor comment (the
+# numeric fast forms), with a third column so statistics take a second pass.
+TSV_NUM = S.TSV.with_columns([(0, S.I64), (1, S.F64), (4, S.F64)], "tsv_num")                          # kind 0
+CSV_NUM = S.CSV.with_columns([(4, S.I64), (1, S.F64), (0, S.I64)], "csv_num")                          # kind 2
+NUM_SPECS = [TSV_NUM, CSV_NUM]
 LOGFMT_ALPHA = b'ab= =  ""\\\\\r\n\n'
 
 
@@ -118,7 +123,7 @@ def _check_all(ctx, ora, spec, doc, label):
     assert np.array_equal(ctx.index_structurals(spec, buf, n), ora.index_structurals(spec, doc)), label + " index"
 
 
-@pytest.mark.parametrize("spec", KIND_SPECS, ids=lambda s: s.name)
+@pytest.mark.parametrize("spec", KIND_SPECS + NUM_SPECS, ids=lambda s: s.name)
 def test_dialect_kinds_columns_stats_field_bytes(ctx, ora, spec):
     """Columns, statistics (separate and fused), field bytes and the index of
     every dialect kind against the oracle: small documents and ones of ~40
@@ -299,3 +304,26 @@ def test_error_statuses(ctx, ora):
         want = ora.ndjson_id_score(doc)
         assert status == (S.E_SCHEMA if want["invalid"] else S.OK), doc
         assert (r.records, r.invalid, r.id_sum) == (want["records"], want["invalid"], want["id_sum"]), doc
+
+
+@pytest.mark.parametrize("spec", NUM_SPECS, ids=lambda s: s.name)
+def test_numeric_fast_forms_crowded_tiles(ctx, ora, spec):
+    """Tiles with more records (over 1024) or structurals (over 7168) than
+    the numeric fast forms list go to the full pass; runs of them between
+    ordinary tiles, records crossing the seams, and no final newline."""
+    sep = bytes([spec.structural[0]])
+    rng = Rng(0xC20D_7ED + len(spec.name))
+    for i in range(6):
+        parts = []
+        for _ in range(3 + rng.below(4)):
+            k = rng.below(3)
+            if k == 0:
+                parts.append(b"".join(b"%d\n" % rng.below(10) for _ in range(2000 + rng.below(6000))))
+            elif k == 1:
+                parts.append((sep * 4 + b"\n") * (1500 + rng.below(3000)))
+            else:
+                parts.append(dialect_doc(rng, spec, 200 + rng.below(400)))
+        doc = b"".join(parts)
+        if i % 2:
+            doc = doc.rstrip(b"\n")
+        _check_all(ctx, ora, spec, doc, f"{spec.name} crowded {i}")
