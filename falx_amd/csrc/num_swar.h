@@ -60,6 +60,50 @@ FX_HD uint32_t digits4(uint32_t d) {
     return (t & 0xFFu) * 100u + (t >> 16);
 }
 
+// m / 10^6 correctly rounded, for m < 10^9, without a divide: the product
+// q = m * RN(10^-6), then one Markstein correction q + RN(m - 10^6 q) * RN(10^-6)
+// with the remainder exact by FMA. Equal to the IEEE quotient for every
+// m < 10^9 (tests/native/fixed6_div_check.cpp runs them all).
+FX_HD double div_1e6(uint32_t m) {
+    const double a = (double)m;
+    const double q = a * 1e-6;
+    const double r = __builtin_fma(-q, 1e6, a);
+    return __builtin_fma(r, 1e-6, q);
+}
+
+// Bit 7 of a byte of w set when the byte is not an ASCII digit.
+FX_HD uint32_t nondigit_bits(uint32_t w) {
+    const uint32_t x = w ^ 0x30303030u;
+    return (((x & 0x7F7F7F7Fu) + 0x76767676u) | x) & 0x80808080u;
+}
+
+// The fixed `[-]d{1,3}.d{6}` shape (csv_geo.rs:1414-1453) of an n-byte cell
+// from the 12 bytes that end at its last byte (w0 w1 w2, little-endian):
+// w1 = [d '.' f1 f2], w2 = [f3 f4 f5 f6], and the n - 8 bytes before them
+// are the top of w0. Decides exactly the cells fixed6_tail decides, with
+// the same value, flag and mantissa; false for any other shape.
+FX_HD bool fixed6_win(uint32_t w0, uint32_t w1, uint32_t w2, uint32_t n, F64Result& r) {
+    if (n - 8u > 3u || (w1 & 0xFF00u) != 0x2E00u) return false;
+    if (nondigit_bits((w1 & 0xFFFF00FFu) | 0x3000u) | nondigit_bits(w2)) return false;
+    const uint32_t npre = n - 8u;
+    const uint32_t pre = npre ? w0 >> (32u - 8u * npre) : 0u;   // the cell's first bytes, first one lowest
+    const bool neg = npre != 0u && (pre & 0xFFu) == '-';
+    const uint32_t dg = neg ? pre >> 8 : pre;
+    const uint32_t nd = npre - (neg ? 1u : 0u);
+    const uint32_t p0 = (dg & 0xFFu) - '0', p1 = ((dg >> 8) & 0xFFu) - '0';
+    if (nd > 2u || (nd >= 1u && p0 > 9u) || (nd == 2u && p1 > 9u)) return false;
+    const uint32_t d = (w1 & 0xFFu) - '0';
+    const uint32_t iv = nd == 2u ? p0 * 100u + p1 * 10u + d : (nd == 1u ? p0 * 10u + d : d);
+    const uint32_t f12 = (((w1 >> 16) & 0xFFu) - '0') * 10u + ((w1 >> 24) - '0');
+    const uint32_t m = iv * 1000000u + f12 * 10000u + digits4(w2 ^ 0x30303030u);
+    r.ok = true;
+    r.fixed6 = true;
+    r.deferred = false;
+    r.mantissa = neg ? -(int64_t)m : (int64_t)m;
+    r.value = bits_to_f64(f64_to_bits(div_1e6(m)) | ((uint64_t)neg << 63));
+    return true;
+}
+
 // 0x0F in each byte of a word that is at or above `below` bytes into it.
 FX_HD uint32_t digit_mask(int32_t below) {
     return below <= 0 ? 0x0F0F0F0Fu : below >= 4 ? 0u : 0x0F0F0F0Fu << (8 * below);

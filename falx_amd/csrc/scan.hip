@@ -1651,16 +1651,29 @@ __device__ __forceinline__ void scan_tile(const Params& P, const uint32_t tile, 
                     const bool here = has && i >= 0 && (uint32_t)i < n;
                     const bool missing = has && term && i >= (int32_t)n;
                     uint32_t fs = 0, fe = 0;
+                    bool odd = false;   // a '\r' check outside the LDS copy, or a trim past the field start
                     if (here) {
                         const uint32_t k = e0 + (uint32_t)i;
                         fe = pos_of(word_at(k));
                         fs = i == 0 ? fs0 : pos_of(word_at(k - 1)) + 1u;
-                        if (term && (uint32_t)i + 1u == n && fe > rs && src.at(fe - 1) == '\r') fe -= 1;
+                        if (term && (uint32_t)i + 1u == n && fe > rs) {
+                            if (fe - 1u < lds_lo) odd = true;
+                            else if (sh.lds_bytes[fe - 1u - tile_start + PRE] == '\r') fe -= 1;
+                        }
+                        odd = odd || fe < fs;
                     }
+                    if (odd) sh.fix_bad = 1u;
                     const bool as_f64 = c.type == FALX_COL_F64;
                     NumVal nv{};
-                    if (here) nv = num_value(fs, fe, as_f64);
-                    const bool vb = here && (as_f64 ? nv.f.ok : nv.iok);
+                    // The fixed-6 shape from the 12 bytes ending at the cell,
+                    // then the general register parsers.
+                    bool done = false;
+                    if (here && !odd && as_f64 && fs >= lds_lo && fe <= lds_hi) {
+                        const Win<3> W = win_lds<3>(sh.lds_bytes, fe - tile_start + PRE);
+                        done = fixed6_win(W.w[0], W.w[1], W.w[2], fe - fs, nv.f);
+                    }
+                    if (here && !odd && !done) nv = num_value(fs, fe, as_f64);
+                    const bool vb = here && !odd && (as_f64 ? nv.f.ok : nv.iok);
                     if (IS_COLS && (here || missing) && live) {
                         if (as_f64) reinterpret_cast<double*>(c.values)[row] = vb ? nv.f.value : 0.0;
                         else reinterpret_cast<long long*>(c.values)[row] = vb ? nv.iv : 0;

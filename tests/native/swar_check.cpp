@@ -46,7 +46,7 @@ std::string cell(Rng& r) {
     if (k < 30) {  // d.ddd shapes, fixed-6 and near misses
         digits(r, s, (int)r.below(6));
         s.push_back('.');
-        digits(r, s, (int)r.below(12));
+        digits(r, s, r.below(2) ? 6 : (int)r.below(12));
     } else if (k < 45) {  // integers up to and past 19 digits, leading zeros
         if (r.below(4) == 0) s += std::string(1 + r.below(4), '0');
         digits(r, s, 1 + (int)r.below(21));
@@ -97,7 +97,8 @@ int fail(const char* what, const std::string& c) {
 }  // namespace
 
 template <int NW>
-int sweep(uint64_t seed, uint64_t count, uint64_t& f_dec, uint64_t& f_ok, uint64_t& i_dec, uint64_t& i_ok) {
+int sweep(uint64_t seed, uint64_t count, uint64_t& f_dec, uint64_t& f_ok, uint64_t& i_dec, uint64_t& i_ok,
+          uint64_t& f6_dec) {
     constexpr uint32_t B = 4 * NW;
     Rng r{seed | 1};
     static fx::BigDec big;
@@ -140,6 +141,23 @@ int sweep(uint64_t seed, uint64_t count, uint64_t& f_dec, uint64_t& f_ok, uint64
                 std::count(c.begin(), c.end(), '.') <= 1 && c.find_first_of("+-", 1) == std::string::npos)
                 return fail("f64 undecided", c);
         }
+        // The fixed-6 shape from the last 12 bytes (fixed6_win): what it
+        // decides agrees with the oracle, and it decides every fixed-6 cell.
+        if (NW == 3) {
+            fx::F64Result f6{};
+            double ov;
+            int ofx;
+            int64_t om;
+            const int ook = ora_parse_f64(p, n, &ov, &ofx, &om);
+            if (fx::fixed6_win(W.w[0], W.w[1], W.w[2], n, f6)) {
+                ++f6_dec;
+                if (!ook || !ofx) return fail("fixed6_win decided a cell of another shape", c);
+                if (fx::f64_to_bits(f6.value) != fx::f64_to_bits(ov) || !f6.fixed6 || f6.mantissa != om)
+                    return fail("fixed6_win value", c);
+            } else if (ook && ofx && n <= 11) {
+                return fail("fixed6_win missed a fixed-6 cell", c);
+            }
+        }
         // i64
         if (fx::num_swar(W, n, false, quote, at, fr, iok, iv)) {
             ++i_dec;
@@ -158,11 +176,11 @@ int sweep(uint64_t seed, uint64_t count, uint64_t& f_dec, uint64_t& f_ok, uint64
 int main(int argc, char** argv) {
     const uint64_t seed = argc > 1 ? std::strtoull(argv[1], nullptr, 0) : 1;
     const uint64_t count = argc > 2 ? std::strtoull(argv[2], nullptr, 0) : 200000;
-    uint64_t f_dec = 0, i_dec = 0, f_ok = 0, i_ok = 0;
-    if (sweep<6>(seed, count, f_dec, f_ok, i_dec, i_ok)) return 1;
-    if (sweep<3>(seed + 1, count, f_dec, f_ok, i_dec, i_ok)) return 1;
+    uint64_t f_dec = 0, i_dec = 0, f_ok = 0, i_ok = 0, f6_dec = 0;
+    if (sweep<6>(seed, count, f_dec, f_ok, i_dec, i_ok, f6_dec)) return 1;
+    if (sweep<3>(seed + 1, count, f_dec, f_ok, i_dec, i_ok, f6_dec)) return 1;
     std::printf("swar_check: %" PRIu64 " cells agree (f64 decided %" PRIu64 ", %" PRIu64
-                " valid; i64 decided %" PRIu64 ", %" PRIu64 " valid)\n",
-                2 * count, f_dec, f_ok, i_dec, i_ok);
+                " valid; i64 decided %" PRIu64 ", %" PRIu64 " valid; fixed-6 windows decided %" PRIu64 ")\n",
+                2 * count, f_dec, f_ok, i_dec, i_ok, f6_dec);
     return 0;
 }

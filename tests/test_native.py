@@ -44,6 +44,15 @@ def test_register_number_parser_matches_oracle(ora, seed):
     assert "agree" in r.stdout
 
 
+def test_fixed6_divide_free_quotient_exhaustive(ora):
+    """div_1e6, the fixed-6 cells' m / 10^6 without a divide, equals the IEEE
+    quotient for every mantissa below 10^9."""
+    exe = _build("fixed6_div_check", ora)
+    r = subprocess.run([exe], capture_output=True, text=True)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "agree" in r.stdout
+
+
 @pytest.mark.parametrize("seed", [5, 0xFEED])
 def test_block_regions_match_byte_automaton(ora, seed):
     exe = _build("regions_check", ora)
