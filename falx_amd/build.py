@@ -19,7 +19,7 @@ HOST_TU = "csrc/falx_gpu.hip"
 KIND_TU = "csrc/kinds.hip"
 N_KINDS = 6
 GPU_DEPS = ["csrc/falx_gpu.hip", "csrc/kinds.hip", "csrc/scan.hip", "csrc/batch.hip", "csrc/cells.h", "csrc/ndjson_walk.h",
-            "csrc/num_swar.h", "csrc/regions.h", "csrc/pow5_table.h"]
+            "csrc/num_swar.h", "csrc/regions.h", "csrc/pow5_table.h", "csrc/swar_tile.h"]
 # Packed-FP32 instructions stay off in every device compile.
 HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

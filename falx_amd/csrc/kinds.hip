@@ -23,6 +23,14 @@ hipError_t plain(dim3 g, hipStream_t s, const Params* P) {
     return hipGetLastError();
 }
 
+// The SWAR kinds' dedicated summary and numeric fast passes (swar_tile.h).
+template <int MODE>
+hipError_t swar(dim3 g, hipStream_t s, const Params* P) {
+    if constexpr (MODE == M_SUMMARY) hipLaunchKernelGGL((k_swar_summary<K_HQ>), g, dim3(WG), 0, s, *P);
+    else hipLaunchKernelGGL((k_numfast<K_HQ, MODE>), g, dim3(WG), 0, s, *P);
+    return hipGetLastError();
+}
+
 template <int MODE>
 hipError_t redo(dim3 g, hipStream_t s, const Params* P) {
     hipLaunchKernelGGL((k_redo<K_HQ, K_ESC, K_CMT, MODE>), g, dim3(WG), 0, s, *P);
@@ -44,20 +52,22 @@ hipError_t launch_kind<FX_KIND>(int form, int mode, dim3 g, hipStream_t s, const
     constexpr bool LOGFMT = FX_KIND == 3;
     if (form == F_PLAIN) {
         switch (mode) {
-        case M_SUMMARY: return plain<M_SUMMARY>(g, s, P);
+        case M_SUMMARY:
+            if constexpr (K_SWAR) return swar<M_SUMMARY>(g, s, P);
+            else return plain<M_SUMMARY>(g, s, P);
         case M_INDEX: return plain<M_INDEX>(g, s, P);
         case M_TAPE: return plain<M_TAPE>(g, s, P);
         case M_COLUMNS_FAST: return plain<M_COLUMNS_FAST>(g, s, P);
         case M_COLUMNS_FAST_STATS: return plain<M_COLUMNS_FAST_STATS>(g, s, P);
         case M_STATS_FAST: return plain<M_STATS_FAST>(g, s, P);
         case M_NUM_FAST:
-            if constexpr (K_SWAR) return plain<M_NUM_FAST>(g, s, P);
+            if constexpr (K_SWAR) return swar<M_NUM_FAST>(g, s, P);
             break;
         case M_NUM_FAST_STATS:
-            if constexpr (K_SWAR) return plain<M_NUM_FAST_STATS>(g, s, P);
+            if constexpr (K_SWAR) return swar<M_NUM_FAST_STATS>(g, s, P);
             break;
         case M_NUM_STATS_FAST:
-            if constexpr (K_SWAR) return plain<M_NUM_STATS_FAST>(g, s, P);
+            if constexpr (K_SWAR) return swar<M_NUM_STATS_FAST>(g, s, P);
             break;
         case M_STRINGS: return plain<M_STRINGS>(g, s, P);
         case M_FIELDBYTES: return plain<M_FIELDBYTES>(g, s, P);

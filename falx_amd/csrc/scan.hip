@@ -748,6 +748,122 @@ struct Shared {
 
 DEV uint32_t list_at(const Shared& sh, uint32_t k) { return sh.list[k]; }
 
+// Per-cell outcome folded into the column statistics.
+struct CellStat {
+    bool ok, fx;
+    unsigned long long csum, bytes;
+    long long fxm;
+    double d;
+};
+
+// One thread's share of one tile: counts and byte totals fit 32 bits.
+struct Acc {
+    uint32_t values, bytes, fx_cells;
+    unsigned long long csum;
+    long long fx_sum;
+    double dsum;
+    DEV void add(const CellStat& o) {
+        if (!o.ok) return;
+        values += 1; csum += o.csum; bytes += (uint32_t)o.bytes; dsum += o.d;
+        if (o.fx) { fx_sum += o.fxm; fx_cells += 1; }
+    }
+};
+
+// Lane value v summed over the wave's four rows of 16 lanes (DPP within a
+// row, then the rows in order); the result is wave-uniform. All lanes must
+// be active.
+DEV uint32_t rows_sum_u32(uint32_t v) {
+    v = row_sum16(v);
+    return ((lane_value(v, 0) + lane_value(v, 16)) + lane_value(v, 32)) + lane_value(v, 48);
+}
+DEV unsigned long long lane_u64(unsigned long long v, uint32_t l) {
+    return ((unsigned long long)lane_value((uint32_t)(v >> 32), l) << 32) | lane_value((uint32_t)v, l);
+}
+DEV unsigned long long rows_sum_u64(unsigned long long v) {
+    v = row_sum16(v);
+    return ((lane_u64(v, 0) + lane_u64(v, 16)) + lane_u64(v, 32)) + lane_u64(v, 48);
+}
+DEV double rows_sum_f64(double v) {
+    v = row_sum16(v);
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    const auto at = [&](uint32_t l) { return __longlong_as_double((long long)lane_u64(b, l)); };
+    return ((at(0) + at(16)) + at(32)) + at(48);
+}
+
+// Per-tile statistics of up to N columns [first, first + nc) plus the record
+// count, reduced in a fixed order (the same in every pass, so a tile's
+// partials are bit-identical whichever pass wrote them): waves 1-3 leave
+// their lanes' accumulators in the LDS exchange area xs, wave 0 adds them to
+// its own lane by lane (waves in order), sums each row of 16 lanes by DPP and
+// then the four row sums in order, and writes the StatParts. Only the fields
+// a column's type can make non-zero are reduced; counts of values and of
+// fixed-6 cells travel as the 16-bit halves of one word (a tile holds at most
+// 8193 records). Called by every thread; xs must not be in use.
+template <uint32_t N>
+DEV void publish_stats(const Params& P, uint32_t tile, const Acc (&acc)[N], uint32_t first, uint32_t nc,
+                       unsigned long long records, unsigned long long* xs) {
+    constexpr uint32_t NSLOT = 4 * N + 1;   // per column: counts | bytes, checksum, fixed-6 sum, FP64 sum; records
+    const uint32_t ln = __lane_id(), w = uni(threadIdx.x >> 6);
+    __syncthreads();   // every thread is done with what xs held
+    if (w > 0) {
+        unsigned long long* o = xs + (w - 1) * NSLOT * 64 + ln;
+#pragma unroll
+        for (uint32_t j = 0; j < N; ++j) {
+            if (j >= nc) break;
+            o[(4 * j + 0) * 64] =
+                (unsigned long long)(acc[j].values | (acc[j].fx_cells << 16)) | ((unsigned long long)acc[j].bytes << 32);
+            o[(4 * j + 1) * 64] = acc[j].csum;
+            o[(4 * j + 2) * 64] = (unsigned long long)acc[j].fx_sum;
+            o[(4 * j + 3) * 64] = (unsigned long long)__double_as_longlong(acc[j].dsum);
+        }
+        o[4 * N * 64] = records;
+    }
+    __syncthreads();
+    if (w != 0) return;
+    const unsigned long long* in = xs + ln;
+    StatPart* parts = reinterpret_cast<StatPart*>(P.partials);
+#pragma unroll
+    for (uint32_t j = 0; j < N; ++j) {
+        if (j >= nc) break;
+        const uint32_t type = P.cols[first + j].type;
+        const bool is_f64 = type == FALX_COL_F64, is_num = is_f64 || type == FALX_COL_I64;
+        uint32_t vc = acc[j].values | (acc[j].fx_cells << 16), by = acc[j].bytes;
+        unsigned long long cs = acc[j].csum, fx = (unsigned long long)acc[j].fx_sum;
+        double ds = acc[j].dsum;
+#pragma unroll
+        for (uint32_t k = 0; k < (uint32_t)NWAVE - 1; ++k) {
+            const unsigned long long* q = in + k * NSLOT * 64;
+            const unsigned long long c0 = q[(4 * j + 0) * 64];
+            vc += (uint32_t)c0;
+            by += (uint32_t)(c0 >> 32);
+            cs += q[(4 * j + 1) * 64];
+            if (is_f64) fx += q[(4 * j + 2) * 64];
+            if (is_num) ds += __longlong_as_double((long long)q[(4 * j + 3) * 64]);
+        }
+        const uint32_t tvc = rows_sum_u32(vc);
+        const unsigned long long tcs = rows_sum_u64(cs);
+        const uint32_t tby = is_num ? 0u : rows_sum_u32(by);
+        const unsigned long long tfx = is_f64 ? rows_sum_u64(fx) : 0ULL;
+        const double tds = is_num ? rows_sum_f64(ds) : 0.0;
+        if (ln == 0) {
+            StatPart& o = parts[(size_t)tile * P.n_cols + first + j];
+            o.values = tvc & 0xFFFFu;
+            o.checksum = tcs;
+            o.bytes = tby;
+            o.fixed6_sum = (long long)tfx;
+            o.fixed6_cells = tvc >> 16;
+            o.sum = tds;
+        }
+    }
+    if (first == 0) {
+        unsigned long long rec = records;
+#pragma unroll
+        for (uint32_t k = 0; k < (uint32_t)NWAVE - 1; ++k) rec += in[k * NSLOT * 64 + 4 * N * 64];
+        const unsigned long long t = rows_sum_u64(rec);
+        if (ln == 0) reinterpret_cast<unsigned long long*>(parts + (size_t)P.n_tiles * P.n_cols)[tile] = t;
+    }
+}
+
 // Fixed-6 cells read straight from the tile's LDS copy: the last 8 bytes
 // come from three aligned words (fixed6_tail in cells.h decides). [lf, lt)
 // are offsets into the copy.
@@ -1454,25 +1570,6 @@ __device__ __forceinline__ void scan_tile(const Params& P, const uint32_t tile, 
         return;
     }
 
-    // Per-cell outcome folded into the column statistics.
-    struct CellStat {
-        bool ok, fx;
-        unsigned long long csum, bytes;
-        long long fxm;
-        double d;
-    };
-    // One thread's share of one tile: counts and byte totals fit 32 bits.
-    struct Acc {
-        uint32_t values, bytes, fx_cells;
-        unsigned long long csum;
-        long long fx_sum;
-        double dsum;
-        DEV void add(const CellStat& o) {
-            if (!o.ok) return;
-            values += 1; csum += o.csum; bytes += (uint32_t)o.bytes; dsum += o.d;
-            if (o.fx) { fx_sum += o.fxm; fx_cells += 1; }
-        }
-    };
     // acc[j] += o for a uniform j < N, without indexing registers.
     auto acc_add = [](auto& acc, uint32_t j, const CellStat& o) __attribute__((always_inline)) {
         constexpr uint32_t N = sizeof(acc) / sizeof(acc[0]);
@@ -1480,61 +1577,9 @@ __device__ __forceinline__ void scan_tile(const Params& P, const uint32_t tile, 
         for (uint32_t jj = 0; jj < N; ++jj)
             if (jj == j) acc[jj].add(o);
     };
-    // Per-tile statistics of up to STAT_GROUP columns plus the record count,
-    // reduced in a fixed order: sums over each row of 16 lanes by DPP, then
-    // the 16 row sums of the workgroup (waves in order, rows in order) from
-    // one exchange through LDS. StatPart field f of column j is slot 6j + f.
-    constexpr uint32_t NROW = NWAVE * 4;
-    __shared__ unsigned long long stat_red[NROW][STAT_GROUP * 6 + 1];
     auto acc_publish = [&](auto& acc, uint32_t first, uint32_t nc, unsigned long long records) {
-        constexpr uint32_t N = sizeof(acc) / sizeof(acc[0]);
-        const bool row_head = (ln & 15u) == 0u;
-        unsigned long long* red = &stat_red[w * 4 + (ln >> 4)][0];
-#pragma unroll
-        for (uint32_t j = 0; j < N; ++j) {
-            if (j >= nc) break;
-            // Only the fields the column's type can make non-zero are
-            // reduced (the others are zero in every lane): counts of values
-            // and of fixed-6 cells travel as the 16-bit halves of one word
-            // (a thread holds at most a few dozen cells of a tile).
-            const uint32_t type = P.cols[first + j].type;
-            const bool is_f64 = type == FALX_COL_F64, is_num = is_f64 || type == FALX_COL_I64;
-            const uint32_t vc = row_sum16(acc[j].values | (acc[j].fx_cells << 16));
-            const unsigned long long v1 = row_sum16(acc[j].csum);
-            uint32_t v2 = 0;
-            unsigned long long v3 = 0;
-            double v5 = 0.0;
-            if (!is_num) v2 = row_sum16(acc[j].bytes);
-            if (is_f64) v3 = row_sum16((unsigned long long)acc[j].fx_sum);
-            if (is_num) v5 = row_sum16(acc[j].dsum);
-            if (row_head) {
-                unsigned long long* r = red + 6 * j;
-                r[0] = vc & 0xFFFFu; r[1] = v1; r[2] = v2; r[3] = v3; r[4] = vc >> 16;
-                r[5] = (unsigned long long)__double_as_longlong(v5);
-            }
-        }
-        const unsigned long long rec = row_sum16(records);
-        if (row_head) red[STAT_GROUP * 6] = rec;
-        __syncthreads();
-        StatPart* parts = reinterpret_cast<StatPart*>(P.partials);
-        if (tid < nc * 6) {
-            const uint32_t j = tid / 6, f = tid % 6;
-            unsigned long long* dst = reinterpret_cast<unsigned long long*>(&parts[(size_t)tile * P.n_cols + first + j]);
-            if (f == 5) {
-                double t = __longlong_as_double((long long)stat_red[0][tid]);
-                for (uint32_t k = 1; k < NROW; ++k) t += __longlong_as_double((long long)stat_red[k][tid]);
-                dst[5] = (unsigned long long)__double_as_longlong(t);
-            } else {
-                unsigned long long t = stat_red[0][tid];
-                for (uint32_t k = 1; k < NROW; ++k) t += stat_red[k][tid];
-                dst[f] = t;
-            }
-        }
-        if (tid == 0 && first == 0) {
-            unsigned long long t = 0;
-            for (uint32_t k = 0; k < NROW; ++k) t += stat_red[k][STAT_GROUP * 6];
-            reinterpret_cast<unsigned long long*>(parts + (size_t)P.n_tiles * P.n_cols)[tile] = t;
-        }
+        static_assert((NWAVE - 1) * (4 * STAT_GROUP + 1) * 64 * 8 <= sizeof(sh.list), "statistics exchange fits the list");
+        publish_stats(P, tile, acc, first, nc, records, reinterpret_cast<unsigned long long*>(sh.list));
     };
     // Typed value of a present cell as f64 (as_f64) or i64: for cells inside
     // the tile the fixed-6 test, then the register parser; the stream parser
@@ -3008,3 +3053,5 @@ inline hipError_t launch_tile_kernel(int kind, int form, int mode, dim3 g, hipSt
 }
 
 }  // namespace fx
+
+#include "swar_tile.h"
