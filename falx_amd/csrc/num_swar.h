@@ -81,27 +81,30 @@ FX_HD uint32_t nondigit_bits(uint32_t w) {
 // from the 12 bytes that end at its last byte (w0 w1 w2, little-endian):
 // w1 = [d '.' f1 f2], w2 = [f3 f4 f5 f6], and the n - 8 bytes before them
 // are the top of w0. Decides exactly the cells fixed6_tail decides, with
-// the same value, flag and mantissa; false for any other shape.
+// the same value, flag and mantissa; false for any other shape. Straight-
+// line code: every condition is evaluated and the value computed whatever
+// the shape, so a wavefront's lanes never branch apart here.
 FX_HD bool fixed6_win(uint32_t w0, uint32_t w1, uint32_t w2, uint32_t n, F64Result& r) {
-    if (n - 8u > 3u || (w1 & 0xFF00u) != 0x2E00u) return false;
-    if (nondigit_bits((w1 & 0xFFFF00FFu) | 0x3000u) | nondigit_bits(w2)) return false;
-    const uint32_t npre = n - 8u;
-    const uint32_t pre = npre ? w0 >> (32u - 8u * npre) : 0u;   // the cell's first bytes, first one lowest
-    const bool neg = npre != 0u && (pre & 0xFFu) == '-';
+    const uint32_t npre = n - 8u;   // 0..3 for a fixed-6 cell
+    // the cell's first npre bytes, first one lowest (npre <= 3 here, so the shift is 8..32)
+    const uint32_t pre = npre - 1u < 3u ? w0 >> (32u - 8u * npre) : 0u;
+    const bool neg = npre - 1u < 3u && (pre & 0xFFu) == '-';
     const uint32_t dg = neg ? pre >> 8 : pre;
     const uint32_t nd = npre - (neg ? 1u : 0u);
     const uint32_t p0 = (dg & 0xFFu) - '0', p1 = ((dg >> 8) & 0xFFu) - '0';
-    if (nd > 2u || (nd >= 1u && p0 > 9u) || (nd == 2u && p1 > 9u)) return false;
+    const bool ok = npre <= 3u && (w1 & 0xFF00u) == 0x2E00u &&
+                    (nondigit_bits((w1 & 0xFFFF00FFu) | 0x3000u) | nondigit_bits(w2)) == 0u && nd <= 2u &&
+                    (nd < 1u || p0 <= 9u) && (nd < 2u || p1 <= 9u);
     const uint32_t d = (w1 & 0xFFu) - '0';
-    const uint32_t iv = nd == 2u ? p0 * 100u + p1 * 10u + d : (nd == 1u ? p0 * 10u + d : d);
+    const uint32_t iv = (nd == 2u ? p0 * 100u + p1 * 10u : (nd == 1u ? p0 * 10u : 0u)) + d;
     const uint32_t f12 = (((w1 >> 16) & 0xFFu) - '0') * 10u + ((w1 >> 24) - '0');
-    const uint32_t m = iv * 1000000u + f12 * 10000u + digits4(w2 ^ 0x30303030u);
-    r.ok = true;
-    r.fixed6 = true;
+    const uint32_t m = (iv & 0x3FFu) * 1000000u + f12 * 10000u + digits4(w2 ^ 0x30303030u);
+    r.ok = ok;
+    r.fixed6 = ok;
     r.deferred = false;
     r.mantissa = neg ? -(int64_t)m : (int64_t)m;
     r.value = bits_to_f64(f64_to_bits(div_1e6(m)) | ((uint64_t)neg << 63));
-    return true;
+    return ok;
 }
 
 // 0x0F in each byte of a word that is at or above `below` bytes into it.

@@ -282,30 +282,21 @@ __global__ __launch_bounds__(WG) void k_numfast(Params P) {
         for (uint32_t r0 = 0; r0 < n_rec; r0 += WG) {
             // Record r ends at terminator r, or is the trailer's or the open
             // one after the tile's last terminator; its fields that end here
-            // are entries e0 .. e0 + n - 1, the first of ordinal ord0.
+            // are entries e0 .. e0 + n - 1, the first of ordinal ord0. The
+            // steps below are straight-line (selects, no branches) except
+            // for rare cells, so the lanes of a wave stay together.
             const uint32_t r = r0 + tid;
             const bool has = r < n_rec;
-            uint32_t e0 = 0, n = 0, fs0 = 0, rs = 0, ord0 = 0;
-            bool term = false;
-            if (has) {
-                e0 = r == 0 ? 0u : (uint32_t)sh.t.tl[r - 1] + 1u;
-                if (r < nt) {
-                    term = true;
-                    n = (uint32_t)sh.t.tl[r] - e0 + 1u;
-                } else {
-                    term = trailer;
-                    n = n_list - e0 + (trailer ? 1u : 0u);
-                }
-                if (e0 == 0) {
-                    fs0 = pre.last_struct;
-                    rs = pre.last_term;
-                    ord0 = pre.ord;
-                } else {
-                    fs0 = pos_of(sh.t.list[e0 - 1]) + 1u;
-                    rs = fs0;
-                }
-            }
-            if (has && term) records += 1;
+            const uint32_t rr = has ? r : 0u;
+            const uint32_t e0 = rr == 0 ? 0u : (uint32_t)sh.t.tl[rr - 1] + 1u;
+            const bool real = rr < nt;
+            const uint32_t e1 = real ? (uint32_t)sh.t.tl[rr] : n_list;   // the terminator entry (n_list: the trailer's)
+            const bool term = has && (real || trailer);
+            const uint32_t n = has ? e1 - e0 + ((real || trailer) ? 1u : 0u) : 0u;
+            const uint32_t fs0 = e0 == 0 ? pre.last_struct : pos_of(sh.t.list[e0 - 1]) + 1u;
+            const uint32_t rs = e0 == 0 ? pre.last_term : fs0;
+            const uint32_t ord0 = e0 == 0 ? pre.ord : 0u;
+            records += (has && term) ? 1u : 0u;
             const uint32_t row = pre.n_rows + r;
             const bool live = row < rows_cap;
             const uint32_t rfirst = pre.n_rows + r0 + 64u * w;   // the wave's first row
@@ -317,32 +308,28 @@ __global__ __launch_bounds__(WG) void k_numfast(Params P) {
                 const bool as_f64 = c.type == FALX_COL_F64;
                 // The cell [fs, fe): the record's last field without its
                 // trailing '\r' (csv_geo.rs:1056-1060).
-                uint32_t fs = 0, fe = 0;
-                bool flag = false;
-                if (here) {
-                    const uint32_t k = e0 + (uint32_t)i;
-                    fe = pos_of(word_at(k));
-                    fs = i == 0 ? fs0 : pos_of(word_at(k - 1)) + 1u;
-                    if (term && (uint32_t)i + 1u == n && fe > rs) {
-                        if (fe - 1u < lds_lo) flag = true;
-                        else if (lds[fe - 1u - tile_start + PRE] == '\r') fe -= 1;
-                    }
-                    flag = flag || fe < fs || fs < lds_lo || fe > lds_hi;
+                const uint32_t k = here ? e0 + (uint32_t)i : 0u;
+                uint32_t fe = pos_of(word_at(k));
+                const uint32_t fs = i == 0 ? fs0 : pos_of(word_at(k - (k ? 1u : 0u))) + 1u;
+                const bool trim = here && term && (uint32_t)i + 1u == n && fe > rs;
+                const bool cr_out = trim && fe - 1u < lds_lo;   // the byte before lies outside the LDS copy
+                const uint32_t crb = lds[(trim && !cr_out) ? fe - 1u - tile_start + PRE : 0u];
+                fe -= (trim && !cr_out && crb == '\r') ? 1u : 0u;
+                bool flag = here && (cr_out || fe < fs || fs < lds_lo || fe > lds_hi);
+                const bool go = here && !flag;
+                uint32_t lf = go ? fs - tile_start + PRE : PRE, lt = go ? fe - tile_start + PRE : PRE;
+                Win<3> W = win_lds<3>(lds, lt);
+                // A cell quoted at both ends is read inside the quotes.
+                if (HQ && go && lt - lf >= 2u && (W.w[2] >> 24) == (uint32_t)quote && lds[lf] == (uint8_t)quote) {
+                    lf += 1;
+                    lt -= 1;
+                    W = win_lds<3>(lds, lt);
                 }
+                const uint32_t cn = lt - lf;
                 NumVal nv{};
-                if (here && !flag) {
-                    uint32_t lf = fs - tile_start + PRE, lt = fe - tile_start + PRE;
-                    Win<3> W = win_lds<3>(lds, lt);
-                    // A cell quoted at both ends is read inside the quotes.
-                    if (HQ && lt - lf >= 2u && (W.w[2] >> 24) == (uint32_t)quote && lds[lf] == (uint8_t)quote) {
-                        lf += 1;
-                        lt -= 1;
-                        W = win_lds<3>(lds, lt);
-                    }
-                    const uint32_t cn = lt - lf;
-                    bool done = false;
-                    if (as_f64) done = fixed6_win(W.w[0], W.w[1], W.w[2], cn, nv.f);
-                    if (!done && swar_ok && cn <= Win12::BYTES) {
+                bool done = as_f64 && fixed6_win(W.w[0], W.w[1], W.w[2], cn, nv.f);
+                if (go && !done) {
+                    if (swar_ok && cn <= Win12::BYTES) {
                         auto at = [&](uint32_t p) { return (uint32_t)lds[lt - Win12::BYTES + p]; };
                         done = num_swar(W, cn, as_f64, quote, at, nv.f, nv.iok, nv.iv) &&
                                !(as_f64 && nv.f.ok && nv.f.deferred);
@@ -350,20 +337,29 @@ __global__ __launch_bounds__(WG) void k_numfast(Params P) {
                     flag = !done;
                 }
                 if (flag) sh.fix_bad = 1u;
-                const bool vb = here && !flag && (as_f64 ? nv.f.ok : nv.iok);
+                const bool vb = go && done && (as_f64 ? nv.f.ok : nv.iok);
+                const double fval = vb ? nv.f.value : 0.0;
+                const long long ival = vb ? nv.iv : 0;
                 if (IS_COLS && (here || missing) && live) {
-                    if (as_f64) reinterpret_cast<double*>(c.values)[row] = vb ? nv.f.value : 0.0;
-                    else reinterpret_cast<long long*>(c.values)[row] = vb ? nv.iv : 0;
+                    if (as_f64) reinterpret_cast<double*>(c.values)[row] = fval;
+                    else reinterpret_cast<long long*>(c.values)[row] = ival;
                 }
                 if (STATS && jj < NACC && (IS_COLS ? live : true)) {
-                    CellStat o{false, false, 0, 0, 0, 0.0};
-                    if (vb) {
-                        o = as_f64 ? CellStat{true, nv.f.fixed6, f64_to_bits(nv.f.value), 0, nv.f.mantissa, nv.f.value}
-                                   : CellStat{true, false, (unsigned long long)nv.iv, 0, 0, (double)nv.iv};
-                    }
+                    // The cell's share of the column statistics (adding +0.0
+                    // for an invalid cell leaves the FP64 sum's bits as they
+                    // are: it starts at +0.0 and cannot become -0.0).
+                    const bool fx = vb && as_f64 && nv.f.fixed6;
+                    const unsigned long long bits = as_f64 ? f64_to_bits(fval) : (unsigned long long)ival;
+                    const double dv = as_f64 ? fval : (double)ival;
 #pragma unroll
-                    for (uint32_t j = 0; j < NACC; ++j)
-                        if (j == jj) acc[j].add(o);
+                    for (uint32_t j = 0; j < NACC; ++j) {
+                        if (j != jj) continue;
+                        acc[j].values += vb ? 1u : 0u;
+                        acc[j].csum += vb ? bits : 0ULL;
+                        acc[j].dsum += vb ? dv : 0.0;
+                        acc[j].fx_sum += fx ? nv.f.mantissa : 0LL;
+                        acc[j].fx_cells += fx ? 1u : 0u;
+                    }
                 }
                 if (IS_COLS) {
                     // Rows are consecutive across the lanes holding a record
