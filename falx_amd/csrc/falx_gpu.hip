@@ -152,6 +152,15 @@ hipError_t launch_stats_group(falx_ctx* c, const falx_spec* spec, Params& P, int
     return launch_fast<M_STATS_FAST, M_STATS>(c, P, kind);
 }
 
+// Levels 2 and 3 of the tile-prefix scan for NS entry states.
+template <int NS>
+void launch_scan_par(falx_ctx* c, const uint32_t* tsum, const uint32_t* gsum, uint32_t stride, uint32_t n_tiles,
+                     uint32_t groups, uint32_t* gpre, uint32_t* tpre, unsigned long long* result) {
+    hipLaunchKernelGGL(k_scan_groups_par<NS>, dim3(1), dim3(GB_THREADS), 0, c->stream, gsum, stride, groups, gpre,
+                       result);
+    hipLaunchKernelGGL(k_scan_tiles_par<NS>, dim3(groups), dim3(SG), 0, c->stream, tsum, stride, n_tiles, gpre, tpre);
+}
+
 // The tile-prefix pipeline: tile summaries (every entry state), then the
 // three-level scan to each tile's entry state and exclusive counters, and
 // the grand totals in the result slots. Emit passes read P.tpre.
@@ -172,9 +181,13 @@ hipError_t run_prefix(falx_ctx* c, Params& P, int kind, char* base = nullptr) {
     hipError_t e = launch_scan<M_SUMMARY>(c, P, kind);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_scan_reduce, dim3(groups), dim3(SG), 0, c->stream, tsum, stride, P.n_tiles, ns, gsum);
-    hipLaunchKernelGGL(k_scan_groups, dim3(1), dim3(64), 0, c->stream, gsum, stride, groups, ns, gpre, P.result);
-    hipLaunchKernelGGL(k_scan_tiles, dim3(groups), dim3(SG), 0, c->stream, tsum, stride, P.n_tiles, ns, gpre,
-                       tpre);
+    switch (ns) {
+    case 1: launch_scan_par<1>(c, tsum, gsum, stride, P.n_tiles, groups, gpre, tpre, P.result); break;
+    case 2: launch_scan_par<2>(c, tsum, gsum, stride, P.n_tiles, groups, gpre, tpre, P.result); break;
+    case 3: launch_scan_par<3>(c, tsum, gsum, stride, P.n_tiles, groups, gpre, tpre, P.result); break;
+    case 4: launch_scan_par<4>(c, tsum, gsum, stride, P.n_tiles, groups, gpre, tpre, P.result); break;
+    default: launch_scan_par<6>(c, tsum, gsum, stride, P.n_tiles, groups, gpre, tpre, P.result); break;
+    }
     P.tpre = tpre;
     return hipGetLastError();
 }

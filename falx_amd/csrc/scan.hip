@@ -2314,6 +2314,161 @@ __global__ __launch_bounds__(64) void k_scan_groups(const uint32_t* gs, uint32_t
 }
 #endif
 
+// Levels 2 and 3 as parallel scans, for a known number of entry states NS:
+// a summary (transfer function + NS counter sets) is held in registers and
+// the workgroup scans them by Kogge-Stone steps through LDS. Level 2 takes
+// up to GB_PER groups per thread in one workgroup; level 3 one workgroup per
+// group of SG tiles. The results equal those of the serial walks above.
+template <int NS>
+struct Summ {
+    uint32_t fn;
+    uint32_t c[NS * C7];
+};
+
+template <int NS>
+DEV Summ<NS> summ_ident() {
+    Summ<NS> r;
+    r.fn = fn_ident<NS>();
+#pragma unroll
+    for (int k = 0; k < NS * C7; ++k) r.c[k] = 0u;
+    return r;
+}
+
+template <int NS>
+DEV Summ<NS> summ_load(const uint32_t* p) {
+    Summ<NS> r;
+    r.fn = p[0];
+#pragma unroll
+    for (int k = 0; k < NS * C7; ++k) r.c[k] = p[TS_COUNTS + k];
+    return r;
+}
+
+// The counters of entry state s (run time) by selects, not register indexing.
+template <int NS>
+DEV Counts summ_counts(const Summ<NS>& a, uint32_t s) {
+    uint32_t w[C7];
+#pragma unroll
+    for (int k = 0; k < C7; ++k) w[k] = a.c[k];
+#pragma unroll
+    for (int t = 1; t < NS; ++t)
+#pragma unroll
+        for (int k = 0; k < C7; ++k) w[k] = s == (uint32_t)t ? a.c[C7 * t + k] : w[k];
+    return c7_load(w);
+}
+
+// a then b.
+template <int NS>
+DEV Summ<NS> summ_then(const Summ<NS>& a, const Summ<NS>& b) {
+    Summ<NS> r;
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+        uint32_t w[C7];
+#pragma unroll
+        for (int k = 0; k < C7; ++k) w[k] = a.c[C7 * s + k];
+        const Counts x = compose(c7_load(w), summ_counts(b, fn_apply(a.fn, (uint32_t)s)));
+        c7_store(w, x);
+#pragma unroll
+        for (int k = 0; k < C7; ++k) r.c[C7 * s + k] = w[k];
+    }
+    r.fn = fn_then<NS>(a.fn, b.fn);
+    return r;
+}
+
+// Inclusive scan of the workgroup's summaries (thread t holds element t);
+// `buf` holds blockDim.x summaries.
+template <int NS>
+DEV Summ<NS> summ_scan(Summ<NS> e, uint32_t (*buf)[1 + NS * C7]) {
+    const uint32_t t = threadIdx.x, n = blockDim.x;
+    for (uint32_t d = 1; d < n; d <<= 1) {
+        buf[t][0] = e.fn;
+#pragma unroll
+        for (int k = 0; k < NS * C7; ++k) buf[t][1 + k] = e.c[k];
+        __syncthreads();
+        if (t >= d) {
+            Summ<NS> p;
+            p.fn = buf[t - d][0];
+#pragma unroll
+            for (int k = 0; k < NS * C7; ++k) p.c[k] = buf[t - d][1 + k];
+            e = summ_then<NS>(p, e);
+        }
+        __syncthreads();
+    }
+    return e;
+}
+
+// The summary before thread t's (the identity for thread 0), given the
+// inclusive scan.
+template <int NS>
+DEV Summ<NS> summ_exclusive(const Summ<NS>& incl, uint32_t (*buf)[1 + NS * C7]) {
+    const uint32_t t = threadIdx.x;
+    buf[t][0] = incl.fn;
+#pragma unroll
+    for (int k = 0; k < NS * C7; ++k) buf[t][1 + k] = incl.c[k];
+    __syncthreads();
+    Summ<NS> x = summ_ident<NS>();
+    if (t > 0) {
+        x.fn = buf[t - 1][0];
+#pragma unroll
+        for (int k = 0; k < NS * C7; ++k) x.c[k] = buf[t - 1][1 + k];
+    }
+    return x;
+}
+
+constexpr uint32_t GB_THREADS = 256;   // level 2 workgroup
+
+#ifndef FX_KIND   // defined once, in the host translation unit
+template <int NS>
+__global__ __launch_bounds__(GB_THREADS) void k_scan_groups_par(const uint32_t* gs, uint32_t stride, uint32_t n_groups,
+                                                                uint32_t* gpre, unsigned long long* result) {
+    __shared__ uint32_t buf[GB_THREADS][1 + NS * C7];
+    const uint32_t t = threadIdx.x;
+    const uint32_t per = (n_groups + GB_THREADS - 1) / GB_THREADS;
+    const uint32_t g0 = min(n_groups, t * per), g1 = min(n_groups, g0 + per);
+    Summ<NS> e = summ_ident<NS>();
+    for (uint32_t g = g0; g < g1; ++g) e = summ_then<NS>(e, summ_load<NS>(gs + (size_t)g * stride));
+    const Summ<NS> incl = summ_scan<NS>(e, buf);
+    const Summ<NS> ex = summ_exclusive<NS>(incl, buf);
+    // Walk this thread's groups from the state entering them (state 0 at
+    // the document's start).
+    uint32_t s = fn_apply(ex.fn, 0u);
+    Counts C = summ_counts<NS>(ex, 0u);
+    for (uint32_t g = g0; g < g1; ++g) {
+        const Summ<NS> q = summ_load<NS>(gs + (size_t)g * stride);
+        uint32_t* o = gpre + (size_t)g * 8;
+        o[0] = s;
+        c7_store(o + 1, C);
+        C = compose(C, summ_counts<NS>(q, s));
+        s = fn_apply(q.fn, s);
+    }
+    if (t == GB_THREADS - 1) {
+        const Counts T = summ_counts<NS>(incl, 0u);
+        result[R_N_STRUCT] = T.n_struct;
+        result[R_N_SEPS] = T.n_seps;
+        result[R_N_TERMS] = T.n_terms;
+        result[R_N_ROWS] = T.n_rows;
+    }
+}
+
+template <int NS>
+__global__ __launch_bounds__(SG) void k_scan_tiles_par(const uint32_t* ts, uint32_t stride, uint32_t n_tiles,
+                                                        const uint32_t* gpre, uint32_t* tpre) {
+    __shared__ uint32_t buf[SG][1 + NS * C7];
+    const uint32_t t = threadIdx.x, tile = blockIdx.x * SG + t;
+    const bool have = tile < n_tiles;
+    const Summ<NS> e = have ? summ_load<NS>(ts + (size_t)tile * stride) : summ_ident<NS>();
+    const Summ<NS> incl = summ_scan<NS>(e, buf);
+    const Summ<NS> ex = summ_exclusive<NS>(incl, buf);
+    const uint32_t* g = gpre + (size_t)blockIdx.x * 8;
+    const uint32_t s0 = g[0];
+    const Counts C = compose(c7_load(g + 1), summ_counts<NS>(ex, s0));
+    if (have) {
+        uint32_t* o = tpre + (size_t)tile * 8;
+        o[0] = fn_apply(ex.fn, s0);
+        c7_store(o + 1, C);
+    }
+}
+#endif
+
 // String arena bases: exclusive prefix over tiles of one string slot's
 // cleaned byte counts (blockIdx.x = slot). Each thread sums a run of tiles,
 // the block scans the runs, and each thread writes its tiles' offsets; the
