@@ -381,6 +381,57 @@ DEV Counts wave_counts(uint64_t S, uint64_t T, uint64_t F, uint32_t base) {
     return c;
 }
 
+// The counters of a wave entered in state 0 and in state 1 of a quote-only
+// dialect (the two entry states are complementary: what is inside quotes
+// from one is outside from the other). Lane l holds a run of up to 32 bytes
+// at `base`: its candidates C, newline candidates N (within C) and `inside`,
+// the bytes inside quotes when the wave is entered in state 0. The same
+// results as wave_counts on each state's masks, with the wave sums of both
+// states shared (three scans instead of four) and 32-bit masks.
+DEV void wave_counts_pair(uint32_t C, uint32_t N, uint32_t inside, uint32_t base, Counts& c0, Counts& c1) {
+    const uint32_t ln = __lane_id();
+    const uint32_t S0 = C & ~inside, S1 = C & inside, T0 = S0 & N, T1 = S1 & N;
+    // Separators after the wave's last terminator, per state.
+    auto ord_of = [&](uint32_t S, uint32_t T, uint64_t has_t) -> uint32_t {
+        const uint32_t seps = S & ~T;
+        if (!has_t) return (uint32_t)__popc(seps);
+        const uint32_t L = hibit(has_t);
+        if (ln < L) return 0u;
+        if (ln > L) return (uint32_t)__popc(seps);
+        const uint32_t hb = 31u - (uint32_t)__clz(T);
+        return hb == 31u ? 0u : (uint32_t)__popc(seps >> (hb + 1u));
+    };
+    const uint64_t ht0 = ballot(T0 != 0u), ht1 = ballot(T1 != 0u);
+    const uint32_t a0 = lane_value(wave_incl_sum((uint32_t)__popc(S0) | ((uint32_t)__popc(T0) << 16)), 63);
+    const uint32_t a1 = lane_value(wave_incl_sum((uint32_t)__popc(S1) | ((uint32_t)__popc(T1) << 16)), 63);
+    const uint32_t od = lane_value(wave_incl_sum(ord_of(S0, T0, ht0) | (ord_of(S1, T1, ht1) << 16)), 63);
+    auto finish = [&](uint32_t S, uint32_t T, uint64_t ht, uint32_t a, uint32_t ord, Counts& c) {
+        c = counts_zero();
+        const uint64_t hs = ballot(S != 0u);
+        if (!hs) return;
+        c.n_struct = a & 0xFFFFu;
+        c.n_terms = a >> 16;
+        c.n_seps = c.n_struct - c.n_terms;
+        c.n_rows = c.n_terms;
+        c.ord = ord;
+        if (ht) {
+            const uint32_t L = hibit(ht);
+            c.has_term = 1;
+            c.last_term = lane_value(base + (T ? 31u - (uint32_t)__clz(T) : 0u) + 1u, L);
+        }
+        // Last and second-to-last structural of the wave.
+        const uint32_t H = hibit(hs);
+        const uint32_t h = S ? 31u - (uint32_t)__clz(S) : 0u;
+        const uint32_t S2 = S & ~(1u << h);
+        c.last_struct = lane_value(base + h + 1u, H);
+        const uint32_t p2 = lane_value(S2 ? base + (31u - (uint32_t)__clz(S2)) + 1u : 0u, H);
+        const uint64_t below = hs & ~(1ULL << H);
+        c.prev_struct = p2 ? p2 : (below ? lane_value(base + h + 1u, hibit(below)) : 0u);
+    };
+    finish(S0, T0, ht0, a0, od & 0xFFFFu, c0);
+    finish(S1, T1, ht1, a1, od >> 16, c1);
+}
+
 // --------------------------------------------------------------------------
 // block-wide scans over 256 threads
 // --------------------------------------------------------------------------
@@ -640,13 +691,13 @@ DEV Masks32 classify32(const uint32_t (&x)[8], const Params& P) {
     const uint32_t q4 = HQ ? (uint32_t)P.quote * 0x01010101u : 0u;
     const uint32_t s0 = (NSEP > 0 ? P.sepb[0] : 0u) * 0x01010101u, s1 = (NSEP > 1 ? P.sepb[1] : 0u) * 0x01010101u;
     const uint32_t s2 = (NSEP > 2 ? P.sepb[2] : 0u) * 0x01010101u, s3 = (NSEP > 3 ? P.sepb[3] : 0u) * 0x01010101u;
-    const bool nl_cand = P.nl_cand != 0u;
+    const uint32_t nl_keep = P.nl_cand != 0u ? ~0u : 0u;   // '\n' flags kept among the candidates
     Masks32 m{0u, 0u, 0u};
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
         const uint32_t y7 = y[j] & 0x7F7F7F7Fu, lo = ~y[j] & 0x80808080u;
         const uint32_t en = eq_ascii(y7, lo, 0x0A0A0A0Au);
-        uint32_t ec = nl_cand ? en : 0u;
+        uint32_t ec = en & nl_keep;
         if (NSEP > 0) ec |= eq_ascii(y7, lo, s0);
         if (NSEP > 1) ec |= eq_ascii(y7, lo, s1);
         if (NSEP > 2) ec |= eq_ascii(y7, lo, s2);

@@ -120,12 +120,16 @@ __global__ __launch_bounds__(WG) void k_swar_summary(Params P) {
     const uint32_t pref = HQ ? prefix_xor32(m.q) : 0u;
     const uint64_t odd = ballot(HQ && (__popc(m.q) & 1));
     const uint32_t ex = mbcnt(odd) & 1u;
-#pragma unroll
-    for (int s = 0; s < NS; ++s) {
-        const uint32_t inside = pref ^ (((uint32_t)s ^ ex) ? ~0u : 0u);
-        const uint32_t S = HQ ? (m.c & ~inside) : m.c, T = S & m.n;
-        const Counts c = wave_counts((uint64_t)S, (uint64_t)T, 0ULL, tile_start + t0);
-        if (ln == 0) wsum[w * NS + s] = c;
+    if constexpr (HQ) {
+        Counts c0, c1;
+        wave_counts_pair(m.c, m.c & m.n, pref ^ (ex ? ~0u : 0u), tile_start + t0, c0, c1);
+        if (ln == 0) {
+            wsum[w * NS + 0] = c0;
+            wsum[w * NS + 1] = c1;
+        }
+    } else {
+        const Counts c = wave_counts((uint64_t)m.c, (uint64_t)(m.c & m.n), 0ULL, tile_start + t0);
+        if (ln == 0) wsum[w * NS] = c;
     }
     if (ln == 0) wfn[w] = NS == 2 ? ((popc(odd) & 1u) ? 0x01u : 0x10u) : 0u;
     __syncthreads();
@@ -288,6 +292,10 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(8, 8))) void
             // are entries e0 .. e0 + n - 1, the first of ordinal ord0. The
             // steps below are straight-line (selects, no branches) except
             // for rare cells, so the lanes of a wave stay together.
+            // A wave whose lanes all lie past the tile's records skips the
+            // round (the steps below are straight-line and would otherwise
+            // be issued for no lane).
+            if (r0 + 64u * w >= n_rec) continue;
             const uint32_t r = r0 + tid;
             const bool has = r < n_rec;
             const uint32_t rr = has ? r : 0u;
