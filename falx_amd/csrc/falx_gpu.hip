@@ -25,6 +25,7 @@ struct falx_ctx {
     void* d_part = nullptr;
     size_t part_cap = 0;
     void* d_red = nullptr;       // reduction outputs (StatPart[32] + extras)
+    unsigned char* h_red = nullptr;   // pinned home of the reduction outputs a call reads
     bool timed = false;
 };
 
@@ -32,6 +33,8 @@ namespace {
 
 constexpr size_t RED_BYTES = 4 << 20;   // reduction scratch: level-1 partials, then results
 constexpr size_t RED_OUT = 2 << 20;     // offset of the level-2 results inside it
+constexpr size_t REC_OFF = 64 * 1024;   // offset of the record count behind the StatParts (from RED_OUT)
+constexpr size_t H_RED_BYTES = 4096;    // the StatParts and the record count, copied home together
 
 inline falx_status hip_status(hipError_t e) { return e == hipSuccess ? FALX_OK : FALX_E_DEVICE; }
 
@@ -343,6 +346,7 @@ falx_status falx_ctx_create(int device, falx_ctx** out) {
         hipMalloc(&c->d_result, R_SLOTS * 8) != hipSuccess ||
         hipMalloc(&c->d_red, RED_BYTES) != hipSuccess ||
         hipHostMalloc(&c->h_result, R_SLOTS * 8, hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc(reinterpret_cast<void**>(&c->h_red), H_RED_BYTES, hipHostMallocDefault) != hipSuccess ||
         hipDeviceSynchronize() != hipSuccess) {
         falx_ctx_destroy(c);
         return FALX_E_DEVICE;
@@ -361,6 +365,7 @@ void falx_ctx_destroy(falx_ctx* c) {
     if (c->d_red) (void)hipFree(c->d_red);
     if (c->d_result) (void)hipFree(c->d_result);
     if (c->h_result) (void)hipHostFree(c->h_result);
+    if (c->h_red) (void)hipHostFree(c->h_red);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->ev_count) (void)hipEventDestroy(c->ev_count);
@@ -578,16 +583,22 @@ static falx_status reduce_stats(falx_ctx* c, const falx_spec* spec, uint32_t n_t
     const StatPart* parts = static_cast<const StatPart*>(c->d_part);
     if (spec->n_columns) HIP_TRY(reduce_stats_2level(c, parts, n_tiles, spec->n_columns));
     // record counts: a u64 per tile after the StatParts, summed the same way
-    const size_t rec_off = 64 * 1024;
     HIP_TRY(reduce_u64_2level(c, reinterpret_cast<const unsigned long long*>(parts + (size_t)n_tiles * spec->n_columns),
-                              n_tiles, 1u, rec_off));
+                              n_tiles, 1u, REC_OFF));
+    // The reduced StatParts and record count come home with the result
+    // slots, under the one synchronisation of finish().
+    const char* red = static_cast<const char*>(c->d_red) + RED_OUT;
+    const size_t sp_bytes = sizeof(StatPart) * spec->n_columns;
+    static_assert(sizeof(StatPart) * FALX_MAX_COLUMNS + 8 <= H_RED_BYTES, "reduction outputs fit their pinned home");
+    if (sp_bytes) HIP_TRY(hipMemcpyAsync(c->h_red, red, sp_bytes, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipMemcpyAsync(c->h_red + sizeof(StatPart) * FALX_MAX_COLUMNS, red + REC_OFF, 8, hipMemcpyDeviceToHost,
+                           c->stream));
     falx_status st = finish(c);
     if (st != FALX_OK) return st;
     StatPart host[FALX_MAX_COLUMNS];
-    const char* red = static_cast<const char*>(c->d_red) + RED_OUT;
-    if (spec->n_columns) HIP_TRY(read_back(c, host, red, sizeof(StatPart) * spec->n_columns));
+    if (sp_bytes) memcpy(host, c->h_red, sp_bytes);
     unsigned long long records = 0;
-    HIP_TRY(read_back(c, &records, red + rec_off, 8));
+    memcpy(&records, c->h_red + sizeof(StatPart) * FALX_MAX_COLUMNS, 8);
     out->records = records;
     for (uint32_t k = 0; k < spec->n_columns; ++k) {
         out->cols[k].values = host[k].values;
