@@ -377,6 +377,17 @@ class BatchServer:
             # (as they would behind it in arrival order) instead of sharing
             # the device with it.
             hold = self.exclusive and self.lane is not None and self.lane.busy > 0
+            # One queued request on an idle device: a batch of one would only
+            # add the batch set-up to its latency, so it is made as a plain
+            # call here (on the first slot's context and output arena).
+            if (len(queue) == 1 and not hold and not any(s.reqs for s in self.slots)
+                    and not (self.lane and self.lane.busy)):
+                r = queue.popleft()
+                s0 = self.slots[0]
+                x = plain_call(s0.ctx, s0.arena, self.corpus, r)
+                r.done, r.status, r.result = time.perf_counter() - t0, x["status"], x
+                left -= 1
+                continue
             for s in self.slots:
                 if not s.reqs and queue and not hold:
                     if self._fill(s, queue, now):
