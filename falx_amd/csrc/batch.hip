@@ -313,7 +313,9 @@ falx_status falx_batch_submit(falx_batch* b, const falx_request* reqs, uint32_t 
             uint2* far = reinterpret_cast<uint2*>(base + far_off);
             NdPart* tmp = reinterpret_cast<NdPart*>(base + far_off + align_up((size_t)n_tiles * sizeof(uint2), 256));
             B.nd.push_back(BNd{r.d_data, (uint32_t)r.len, n_tiles, parts, far});
-            B.tab[T_NDJ].add((uint32_t)B.nd.size() - 1, (n_tiles + chunk - 1) / chunk, chunk);
+            // Units of whole blocks of the NDJSON kernel (ND_WAVES tiles).
+            const uint32_t nd_chunk = (chunk + ND_WAVES - 1) / ND_WAVES * ND_WAVES;
+            B.tab[T_NDJ].add((uint32_t)B.nd.size() - 1, (n_tiles + nd_chunk - 1) / nd_chunk, nd_chunk);
             add_red(B, RJ_ND, parts, 2 * n_tiles, 1, tmp, reinterpret_cast<void*>(res_rel + RES_ND_OFF));
             continue;
         }

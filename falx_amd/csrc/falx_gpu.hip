@@ -725,7 +725,8 @@ falx_status falx_parse_ndjson_id_score(falx_ctx* c, const uint8_t* d, uint64_t l
     NdPart* parts = static_cast<NdPart*>(c->d_part);
     uint2* far = reinterpret_cast<uint2*>(parts + 2 * (size_t)n_tiles);
     HIP_TRY(mark_start(c));
-    hipLaunchKernelGGL(k_ndjson_id_score, dim3(n_tiles), dim3(ND_WG), 0, c->stream, d, (uint32_t)len, n_tiles,
+    hipLaunchKernelGGL(k_ndjson_id_score, dim3((n_tiles + ND_WAVES - 1) / ND_WAVES), dim3(ND_WG), 0, c->stream, d,
+                       (uint32_t)len, n_tiles,
                        parts, far);
     hipLaunchKernelGGL(k_ndjson_far, dim3((n_tiles + 63) / 64), dim3(64), 0, c->stream, d, n_tiles,
                        static_cast<const uint2*>(far), parts + n_tiles);

@@ -387,6 +387,257 @@ FX_HD void ndjson_resolve(const Src& s, NdRecord& r, BigDec& b) {
     r.score_deferred = false;
 }
 
+// ---------------------------------------------------------------------------
+// The common record layout, read in straight-line steps.
+//
+// The documents of this sink repeat one key order (the layout ndjson.rs:696-711
+// reads by position):
+//   {"id":INT,"name":STRING,"nested":{"score":NUMBER,"tags":VALUE}}
+// with spaces around the punctuation. ndjson_common reads a record of that
+// layout in a fixed sequence of steps, the same for every record, so the lanes
+// of a wave take them together instead of each running the key loop of
+// ndjson_record on its own. Each step is one of ndjson_record's steps on that
+// layout; the first byte a step does not expect clears `ok` and freezes the
+// cursor, and the record then goes to ndjson_record. Whitespace other than
+// ' ', exponents, numbers of more than 15 digits and any other key order are
+// left to ndjson_record. The source reads aligned 8-byte words (word8) up to
+// 24 bytes past `to`, and the byte at `to` is '\n' or 0, which no step takes,
+// so the cursor never passes `to`.
+// ---------------------------------------------------------------------------
+
+FX_HD uint32_t ctz64(uint64_t x) { return (uint32_t)__builtin_ctzll(x); }
+
+// Bytes sh/8 .. sh/8 + 7 of the 16 bytes hi:lo (sh a multiple of 8, 0..64).
+FX_HD uint64_t nd_funnel(uint64_t lo, uint64_t hi, uint32_t sh) {
+    return sh == 0 ? lo : (sh >= 64 ? hi : (lo >> sh) | (hi << (64u - sh)));
+}
+
+template <class Src>
+FX_HD uint64_t nd_bytes8(const Src& s, uint32_t i) {
+    const uint32_t a = i & ~7u;
+    return nd_funnel(s.word8(a), s.word8(a + 8), 8u * (i - a));
+}
+
+// A little-endian word of up to 8 literal bytes.
+constexpr uint64_t nd_lit8(const char* t, uint32_t n) {
+    return n == 0 ? 0ULL : (uint64_t)(uint8_t)t[0] | (nd_lit8(t + 1, n - 1) << 8);
+}
+
+constexpr uint64_t ND_HI = 0x8080808080808080ULL;
+// The keys of the common layout with their quotes.
+constexpr uint64_t ND_K_ID = nd_lit8("\"id\"", 4), ND_K_NAME = nd_lit8("\"name\"", 6),
+                   ND_K_NESTED = nd_lit8("\"nested\"", 8), ND_K_SCORE = nd_lit8("\"score\"", 7),
+                   ND_K_TAGS = nd_lit8("\"tags\"", 6);
+
+// Bit 7 of every byte of x that is not ' '.
+FX_HD uint64_t nd_nonspace(uint64_t x) { return ~zero_bytes64(x ^ 0x2020202020202020ULL) & ND_HI; }
+
+// ' '* P ' '* at i: the position of the byte after the spaces that follow P.
+// Both runs of spaces and P lie in 8 bytes.
+template <class Src>
+FX_HD uint32_t nd_punct(const Src& s, uint32_t i, uint32_t P, bool& ok) {
+    const uint64_t x = nd_bytes8(s, i);
+    const uint64_t ns = nd_nonspace(x);
+    const uint32_t k = ns ? ctz64(ns) >> 3 : 0u;
+    const uint64_t after = (ns >> (8u * k)) & ~0xFFULL;
+    const bool go = ok && ns != 0 && ((uint32_t)(x >> (8u * k)) & 0xFFu) == P && after != 0;
+    ok = go;
+    return go ? i + k + (ctz64(after | (1ULL << 63)) >> 3) : i;
+}
+
+// ' '* '}' at i: the position of the '}'.
+template <class Src>
+FX_HD uint32_t nd_close(const Src& s, uint32_t i, bool& ok) {
+    const uint64_t x = nd_bytes8(s, i);
+    const uint64_t ns = nd_nonspace(x);
+    const uint32_t k = ns ? ctz64(ns) >> 3 : 0u;
+    const bool go = ok && ns != 0 && ((uint32_t)(x >> (8u * k)) & 0xFFu) == '}';
+    ok = go;
+    return go ? i + k : i;
+}
+
+// The n (1..8) bytes of `lit` at i (a key with its quotes).
+template <class Src>
+FX_HD uint32_t nd_key(const Src& s, uint32_t i, uint64_t lit, uint32_t n, bool& ok) {
+    const uint64_t mask = n >= 8 ? ~0ULL : (1ULL << (8u * n)) - 1u;
+    const bool go = ok && ((nd_bytes8(s, i) ^ lit) & mask) == 0;
+    ok = go;
+    return go ? i + n : i;
+}
+
+// The first byte at or after byte `from` of a 16-byte window whose flags
+// (bit 7 of each byte) are fx (bytes 0-7) and fy (bytes 8-15); 16 when none.
+FX_HD uint32_t nd_first(uint64_t fx, uint64_t fy, uint32_t from) {
+    if (from >= 16) return 16;
+    const uint64_t mx = from >= 8 ? 0ULL : fx & (~0ULL << (8u * from));
+    const uint64_t my = from <= 8 ? fy : fy & (~0ULL << (8u * (from - 8u)));
+    return mx ? ctz64(mx) >> 3 : (my ? 8u + (ctz64(my) >> 3) : 16u);
+}
+
+// 10^k for k < 16, exact.
+FX_HD uint64_t nd_pow10(uint32_t k) {
+    uint64_t p = (k & 1u) ? 10ULL : 1ULL;
+    if (k & 2u) p *= 100ULL;
+    if (k & 4u) p *= 10000ULL;
+    if (k & 8u) p *= 100000000ULL;
+    return p;
+}
+
+// The value of the first n (1..16) bytes of the window x:y, all digits, the
+// first the most significant: the digits are moved to the top of the 16
+// bytes (the zero bytes below them read as leading zeros) and both halves
+// folded (fold_digits8).
+FX_HD uint64_t nd_value16(uint64_t x, uint64_t y, uint32_t n) {
+    const uint32_t s = 8u * (16u - n);
+    uint64_t lo, hi;
+    if (s >= 64) {
+        lo = 0;
+        hi = x << (s - 64u);
+    } else if (s == 0) {
+        lo = x;
+        hi = y;
+    } else {
+        lo = x << s;
+        hi = (y << s) | (x >> (64u - s));
+    }
+    return fold_digits8(lo, 8) * 100000000ULL + fold_digits8(hi, 8);
+}
+
+// A number at i: its sign and the 16 bytes after the sign.
+template <class Src>
+FX_HD bool nd_signed16(const Src& s, uint32_t i, uint64_t& x, uint64_t& y) {
+    const uint32_t a = i & ~7u, sh = 8u * (i - a);
+    const uint64_t w0 = s.word8(a), w1 = s.word8(a + 8), w2 = s.word8(a + 16);
+    const bool neg = (nd_funnel(w0, w1, sh) & 0xFFu) == '-';
+    const uint32_t sh2 = sh + (neg ? 8u : 0u);
+    x = nd_funnel(w0, w1, sh2);
+    y = nd_funnel(w1, w2, sh2);
+    return neg;
+}
+
+// j_int on -?[0-9]{1,15}: the position after the digits (the byte there is
+// checked by the next step, which takes only ' ' or ',').
+template <class Src>
+FX_HD uint32_t nd_int(const Src& s, uint32_t i, int64_t& v, bool& ok) {
+    uint64_t x, y;
+    const bool neg = nd_signed16(s, i, x, y);
+    const uint32_t n = nd_first(nondigit_bytes64(x), nondigit_bytes64(y), 0);
+    const bool go = ok && n >= 1 && n <= 15;
+    const uint64_t val = nd_value16(x, y, go ? n : 1u);
+    v = neg ? (int64_t)(0 - val) : (int64_t)val;
+    ok = go;
+    return go ? i + (neg ? 1u : 0u) + n : i;
+}
+
+// j_number on -?(0|[1-9][0-9]*)(.[0-9]+)? with at most 15 digits in all
+// (its fast branch: m / 10^frac_digits, one correctly rounded division, and
+// the 1e-6 mantissa for <= 6 decimals and <= 12 integer digits).
+template <class Src>
+FX_HD uint32_t nd_number(const Src& s, uint32_t i, NdRecord& r, bool& ok) {
+    uint64_t x, y;
+    const bool neg = nd_signed16(s, i, x, y);
+    const uint64_t fx = nondigit_bytes64(x), fy = nondigit_bytes64(y);
+    const uint32_t p1 = nd_first(fx, fy, 0);   // integer digits
+    const uint32_t b1 = p1 < 8 ? (uint32_t)(x >> (8u * p1)) & 0xFFu : (p1 < 16 ? (uint32_t)(y >> (8u * (p1 - 8u))) & 0xFFu : 0u);
+    const bool dot = b1 == '.';
+    const uint32_t p2 = dot ? nd_first(fx, fy, p1 + 1u) : p1;   // end of the number
+    const uint32_t fd = dot ? p2 - p1 - 1u : 0u;                 // fraction digits
+    const uint32_t nd = p1 + fd;
+    const bool go = ok && p1 >= 1 && !((x & 0xFFu) == '0' && p1 > 1) && (!dot || fd >= 1) && p2 < 16 && nd <= 15;
+    // The digits without the '.': bytes [0, p1), then those from p1 + 1 on.
+    const uint64_t xs = (x >> 8) | (y << 56), ys = y >> 8;
+    const uint64_t kx = dot ? (p1 >= 8 ? ~0ULL : (1ULL << (8u * p1)) - 1u) : ~0ULL;
+    const uint64_t ky = dot ? (p1 <= 8 ? 0ULL : (1ULL << (8u * (p1 - 8u))) - 1u) : ~0ULL;
+    const uint64_t dx = (x & kx) | (xs & ~kx), dy = (y & ky) | (ys & ~ky);
+    const uint64_t m = nd_value16(dx, dy, go ? nd : 1u);
+    const double mag = (double)m / (double)nd_pow10(go ? fd : 0u);
+    r.score = neg ? -mag : mag;
+    r.score_deferred = false;
+    r.sfrom = r.sto = 0;
+    r.fx6 = fd <= 6 && p1 <= 12;
+    const int64_t mant = (int64_t)(m * nd_pow10(r.fx6 ? 6u - fd : 0u));
+    r.mant = r.fx6 ? (neg ? -mant : mant) : 0;
+    ok = go;
+    return go ? i + (neg ? 1u : 0u) + p2 : i;
+}
+
+// j_skip_value on a string, array or object at i: strings by their quotes
+// and backslashes, brackets by depth, 8 bytes per step between the bytes
+// that matter. The position after the value.
+template <class Src>
+FX_HD uint32_t nd_skip(const Src& s, uint32_t i, uint32_t to, bool& ok) {
+    const uint32_t b0 = (uint32_t)(nd_bytes8(s, i) & 0xFFu);
+    bool bad = !ok || !(b0 == '"' || b0 == '[' || b0 == '{');
+    bool instr = false, done = false;
+    int depth = 0;
+    uint32_t p = i;
+    while (!done && !bad) {
+        const uint32_t a = p & ~7u;
+        const uint64_t x = s.word8(a);
+        const uint64_t zq = zero_bytes64(x ^ 0x2222222222222222ULL);
+        // In a string: '\\'. Outside: '[', ']', '{', '}' (and 'Y', '_',
+        // 'y', 0x7F, which share their bits under 0xD9 and pass as plain).
+        const uint64_t zo = instr ? zero_bytes64(x ^ 0x5C5C5C5C5C5C5C5CULL)
+                                  : zero_bytes64((x & 0xD9D9D9D9D9D9D9D9ULL) ^ 0x5959595959595959ULL);
+        const uint64_t hit = (zq | zo) & (~0ULL << (8u * (p - a)));
+        const uint32_t sb = hit ? ctz64(hit) : 71u;
+        const uint32_t pos = a + (sb >> 3);   // a + 8 when the word has none
+        if (pos >= to) {
+            bad = true;
+        } else if (hit) {
+            const uint32_t ch = (uint32_t)(x >> (sb & ~7u)) & 0xFFu;
+            if (instr) {
+                const bool q = ch == '"';
+                instr = !q;
+                done = q && depth == 0;
+                p = pos + (q ? 1u : 2u);
+            } else {
+                const bool open = ch == '[' || ch == '{', close = ch == ']' || ch == '}';
+                instr = ch == '"';
+                depth += open ? 1 : (close ? -1 : 0);
+                done = close && depth == 0;
+                p = pos + 1u;
+            }
+        } else {
+            p = pos;
+        }
+    }
+    ok = !bad;
+    return bad ? i : p;
+}
+
+// The record data[from, to) in the common layout: true with `out` filled as
+// ndjson_record fills it (which then returns 1), false when the record is of
+// another form.
+template <class Src>
+FX_HD bool ndjson_common(const Src& s, uint32_t from, uint32_t to, NdRecord& out) {
+    bool ok = true;
+    uint32_t i = from;
+    out.id = 0;
+    i = nd_punct(s, i, '{', ok);
+    i = nd_key(s, i, ND_K_ID, 4, ok);
+    i = nd_punct(s, i, ':', ok);
+    i = nd_int(s, i, out.id, ok);
+    i = nd_punct(s, i, ',', ok);
+    i = nd_key(s, i, ND_K_NAME, 6, ok);
+    i = nd_punct(s, i, ':', ok);
+    i = nd_skip(s, i, to, ok);
+    i = nd_punct(s, i, ',', ok);
+    i = nd_key(s, i, ND_K_NESTED, 8, ok);
+    i = nd_punct(s, i, ':', ok);
+    i = nd_punct(s, i, '{', ok);
+    i = nd_key(s, i, ND_K_SCORE, 7, ok);
+    i = nd_punct(s, i, ':', ok);
+    i = nd_number(s, i, out, ok);
+    i = nd_punct(s, i, ',', ok);
+    i = nd_key(s, i, ND_K_TAGS, 6, ok);
+    i = nd_punct(s, i, ':', ok);
+    i = nd_skip(s, i, to, ok);
+    i = nd_punct(s, i, '}', ok);
+    i = nd_close(s, i, ok);
+    return ok && i < to;
+}
+
 template <class Src>
 FX_HD bool blank_span(const Src& s, uint32_t from, uint32_t to) {
     for (uint32_t p = from; p < to; ++p)

@@ -2661,24 +2661,31 @@ struct NdPart {
 };
 
 // NDJSON id+score (ndjson.rs:679-723): records are the bytes between raw
-// '\n's. A tile owns the records that start in it and holds in LDS its 8 KiB
-// plus ND_HALO bytes after them, so a record is read from LDS alone unless it
-// runs past the halo (then one lane walks it with HBM reads). One wave per
-// tile: the newline positions are listed in order 1 KiB per step (16 bytes
-// per lane, SWAR byte tests, ranks by a wave scan), and lanes take records
-// from a per-wave queue as they finish their previous one, so no lane waits
-// for a long neighbour while records remain.
-constexpr int ND_WG = 64;
+// '\n's. A block of ND_WAVES waves owns ND_WAVES 8 KiB tiles and holds in LDS
+// their bytes plus ND_HALO bytes after them, so a record is read from LDS
+// alone unless it runs past the halo (then one lane walks it with HBM reads).
+// Each wave lists the newlines that end the records starting in its tile, in
+// order, 1 KiB per step (16 bytes per lane, SWAR byte tests, ranks by a wave
+// scan); then the block's 256 lanes read the records of all its tiles, lane t
+// records t, t + 256, ... of the block's list: first in the straight-line
+// steps of the common layout (ndjson_common), which every lane takes together,
+// and with the record walker (ndjson_record) when a record has another form.
+// Pooling the tiles keeps lanes busy: a tile holds about 1.2 records per lane
+// of one wave, so a wave per tile would run a second round mostly idle. The
+// fixed assignment also fixes the order of the f64 score sum.
+constexpr uint32_t ND_WAVES = 4;
+constexpr int ND_WG = 64 * ND_WAVES;
+constexpr uint32_t ND_SPAN = ND_WAVES * TILE;     // bytes of the tiles of a block
 constexpr uint32_t ND_HALO = 512;
-constexpr uint32_t ND_WIN = TILE + ND_HALO;
-constexpr uint32_t ND_MAXNL = 512;      // newlines listed per round
+constexpr uint32_t ND_WIN = ND_SPAN + ND_HALO;
+constexpr uint32_t ND_MAXNL = 512;      // newlines listed per wave and round
 // Zero bytes after the window: the walker's word reads (three aligned words
 // from a byte of the record) end at most 24 bytes past its last byte.
 constexpr uint32_t ND_PAD = 32;
 constexpr uint32_t ND_NONE = 0xFFFFFFFFu;
 
-// The tile's LDS copy as a walker source (ndjson_walk.h): bytes, and
-// aligned 8-byte words for skipping strings (tile starts are multiples of 8).
+// The block's LDS copy as a walker source (ndjson_walk.h): bytes, and
+// aligned 8-byte words for skipping strings (block starts are multiples of 8).
 struct NdLds {
     static constexpr bool kWord8 = true;
     const uint8_t* b;   // document bytes [lo, lo + ND_WIN), then ND_PAD zero bytes
@@ -2752,52 +2759,57 @@ DEV uint32_t nl_mask16(uint4 v) {
     return m;
 }
 
-// far[tile] receives the one record of the tile that runs past its LDS
-// window, (start, end), or (ND_NONE, 0); k_ndjson_far walks those records.
-DEV void ndjson_tile(const uint8_t* data, uint32_t len, uint32_t n_tiles, NdPart* parts, uint2* far, uint32_t tile) {
+// far[tile] receives the one record of the tile that runs past the block's
+// LDS window, (start, end), or (ND_NONE, 0); k_ndjson_far walks those
+// records. The block's sums go to parts[first tile of the block], zeros to
+// the parts of its other tiles.
+DEV void ndjson_block(const uint8_t* data, uint32_t len, uint32_t n_tiles, NdPart* parts, uint2* far, uint32_t blk) {
     __shared__ __align__(16) uint8_t bytes[ND_WIN + ND_PAD];
-    __shared__ uint16_t nls[ND_MAXNL];   // newline positions - tile_start, in order
-    __shared__ uint32_t qnext;
+    __shared__ uint16_t nls[ND_WAVES][ND_MAXNL];   // per wave: newline positions - block start, in order
+    __shared__ uint32_t w_n[ND_WAVES], w_items[ND_WAVES], w_cur[ND_WAVES];
     __shared__ SlowF64 slow;
-    __shared__ unsigned long long ush[1];
-    __shared__ long long lsh[1];
-    __shared__ double dsh[1];
-    (void)n_tiles;
-    const uint32_t ln = threadIdx.x;
-    const uint32_t tile_start = tile * (uint32_t)TILE;
-    const uint32_t tile_end = min(tile_start + (uint32_t)TILE, len);
-    const uint32_t win_hi = min(tile_start + ND_WIN, len);
-    const uint32_t win_len = win_hi - tile_start;
-    for (uint32_t i = ln * 16; i < ND_WIN + ND_PAD; i += ND_WG * 16) {
+    __shared__ unsigned long long ush[ND_WAVES];
+    __shared__ long long lsh[ND_WAVES];
+    __shared__ double dsh[ND_WAVES];
+    const uint32_t tid = threadIdx.x, ln = tid & 63u, w = uni(tid >> 6);
+    const uint32_t blk_start = blk * ND_SPAN;
+    const uint32_t win_hi = (uint32_t)min((uint64_t)blk_start + ND_WIN, (uint64_t)len);
+    const uint32_t win_len = win_hi - blk_start;
+    for (uint32_t i = tid * 16; i < ND_WIN + ND_PAD; i += ND_WG * 16) {
         uint4 v = make_uint4(0, 0, 0, 0);
         if (i + 16 <= win_len) {
-            v = *reinterpret_cast<const uint4*>(data + tile_start + i);
+            v = *reinterpret_cast<const uint4*>(data + blk_start + i);
         } else if (i < win_len) {
             uint8_t tmp[16];
-            for (int k = 0; k < 16; ++k) tmp[k] = (i + k < win_len) ? data[tile_start + i + k] : 0;
+            for (int k = 0; k < 16; ++k) tmp[k] = (i + k < win_len) ? data[blk_start + i + k] : 0;
             memcpy(&v, tmp, 16);
         }
         *reinterpret_cast<uint4*>(&bytes[i]) = v;
     }
-    if (ln == 0) slow.lock = 0;
+    if (tid == 0) slow.lock = 0;
     __syncthreads();
-    const NdLds src{bytes, tile_start};
+    const NdLds src{bytes, blk_start};
     NdAcc acc{0, 0, 0, 0, 0, 0.0};
     uint2 far_rec = make_uint2(ND_NONE, 0u);
+    // This wave's tile.
+    const uint32_t tile = blk * ND_WAVES + w;
+    const bool live = tile < n_tiles;
+    const uint32_t tile_start = blk_start + w * (uint32_t)TILE;
+    const uint32_t tile_end = live ? (uint32_t)min((uint64_t)tile_start + TILE, (uint64_t)len) : tile_start;
     // Start of the record being listed: the tile's first byte when a record
     // starts there, else it is the byte after the tile's first newline.
-    uint32_t cur = (tile_start == 0 || data[tile_start - 1] == '\n') ? tile_start : ND_NONE;
+    uint32_t cur = (live && (tile_start == 0 || data[tile_start - 1] == '\n')) ? tile_start : ND_NONE;
     uint32_t scan = tile_start;     // first byte not yet searched for newlines
-    bool done = tile_start >= len;
-    while (!done) {
+    bool done = !live;
+    while (true) {
         // List newlines from `scan` on, stopping after the first one at or
         // past tile_end - 1 (the record after it is the next tile's), or
         // when the list is full.
         uint32_t n = 0;
         bool stop = false, full = false;
-        while (!stop && !full && scan < win_hi) {
+        while (!done && !stop && !full && scan < win_hi) {
             const uint32_t base = scan & ~15u;
-            const uint32_t off = base - tile_start + 16 * ln;   // LDS offset of this lane's 16 bytes
+            const uint32_t off = base - blk_start + 16 * ln;   // LDS offset of this lane's 16 bytes
             const uint32_t p0 = base + 16 * ln;
             uint32_t m = 0;
             if (off + 16 <= ND_WIN + 16) m = nl_mask16(*reinterpret_cast<const uint4*>(&bytes[off]));
@@ -2823,7 +2835,7 @@ DEV void ndjson_tile(const uint8_t* data, uint32_t len, uint32_t n_tiles, NdPart
             const uint32_t total = lane_value(incl, 63);
             uint32_t rank = n + incl - cnt;
             for (uint32_t mm = m; mm; mm &= mm - 1) {
-                if (rank < ND_MAXNL) nls[rank] = (uint16_t)(p0 + (uint32_t)__ffs((int)mm) - 1u - tile_start);
+                if (rank < ND_MAXNL) nls[w][rank] = (uint16_t)(p0 + (uint32_t)__ffs((int)mm) - 1u - blk_start);
                 ++rank;
             }
             if (n + total >= ND_MAXNL && (n + total > ND_MAXNL || !stop)) {
@@ -2833,34 +2845,54 @@ DEV void ndjson_tile(const uint8_t* data, uint32_t len, uint32_t n_tiles, NdPart
                 stop = false;
             } else {
                 n += total;
-                scan = base + 16 * ND_WG;
+                scan = base + 16 * 64;
             }
         }
         __syncthreads();
-        if (full) scan = tile_start + nls[ND_MAXNL - 1] + 1u;
-        const uint32_t next_cur = n ? tile_start + nls[n - 1] + 1u : cur;
-        const bool open_end = !stop && !full && scan >= win_hi;   // no newline ends the record at next_cur
-        // The data's last record, without a final newline, is one more item.
-        const bool last_rec = open_end && win_hi == len && next_cur != ND_NONE && next_cur < tile_end &&
-                              next_cur < len;
-        const uint32_t items = n + (last_rec ? 1u : 0u);
-        // Item k ends at newline k (at len for the last record); its start is
-        // `cur` for k = 0, else the byte after newline k - 1. Lanes take
-        // items from the queue.
-        if (ln == 0) qnext = ND_WG;
-        __syncthreads();
-        for (uint32_t k = ln; k < items;) {
-            const uint32_t from = k == 0 ? cur : tile_start + nls[k - 1] + 1u;
-            const uint32_t to = k < n ? tile_start + nls[k] : len;
-            if (from != ND_NONE) nd_add(acc, nd_record(src, from, to, slow));
-            k = atomicAdd(&qnext, 1u);
+        // Item k of a wave ends at its newline k (at len for the data's last
+        // record, which has none); its start is the wave's `cur` for k = 0,
+        // else the byte after newline k - 1.
+        uint32_t items = 0, next_cur = cur;
+        bool open_end = false;
+        if (!done) {
+            if (full) scan = blk_start + nls[w][ND_MAXNL - 1] + 1u;
+            next_cur = n ? blk_start + nls[w][n - 1] + 1u : cur;
+            open_end = !stop && !full && scan >= win_hi;   // no newline ends the record at next_cur
+            const bool last_rec = open_end && win_hi == len && next_cur != ND_NONE && next_cur < tile_end &&
+                                  next_cur < len;
+            items = n + (last_rec ? 1u : 0u);
+        }
+        if (ln == 0) {
+            w_n[w] = n;
+            w_items[w] = items;
+            w_cur[w] = cur;
         }
         __syncthreads();
-        cur = next_cur;
-        if (stop) break;
-        if (open_end) {
-            if (cur != ND_NONE && cur < tile_end) {
-                if (win_hi < len) {
+        // The items of all waves, in wave order, ND_WG lanes at a time.
+        uint32_t pre[ND_WAVES + 1];
+        pre[0] = 0;
+#pragma unroll
+        for (uint32_t v = 0; v < ND_WAVES; ++v) pre[v + 1] = pre[v] + w_items[v];
+        for (uint32_t g = tid; g < pre[ND_WAVES]; g += ND_WG) {
+            uint32_t v = 0;
+#pragma unroll
+            for (uint32_t q = 1; q < ND_WAVES; ++q) v += g >= pre[q] ? 1u : 0u;
+            const uint32_t k = g - pre[v];
+            const uint32_t from = k == 0 ? w_cur[v] : blk_start + nls[v][k - 1] + 1u;
+            const uint32_t to = k < w_n[v] ? blk_start + nls[v][k] : len;
+            const bool have = from != ND_NONE;
+            NdRecord r;
+            const bool common = ndjson_common(src, have ? from : to, to, r);
+            if (common) nd_add(acc, NdOne{1u, r.fx6, r.id, r.mant, r.score});
+            else if (have) nd_add(acc, nd_record(src, from, to, slow));
+        }
+        __syncthreads();   // the lists are read before the next round writes them
+        if (!done) {
+            cur = next_cur;
+            if (stop) {
+                done = true;
+            } else if (open_end) {
+                if (cur != ND_NONE && cur < tile_end && win_hi < len) {
                     // A record running past the halo: its end is searched in
                     // HBM 512 bytes per step, and one lane walks it there.
                     uint32_t end = len;
@@ -2884,20 +2916,23 @@ DEV void ndjson_tile(const uint8_t* data, uint32_t len, uint32_t n_tiles, NdPart
                     }
                     far_rec = make_uint2(cur, end);
                 }
+                done = true;
             }
-            break;
         }
+        if (!__syncthreads_or(done ? 0 : 1)) break;
     }
-    acc.recs = block_reduce_sum<unsigned long long, 1>(acc.recs, ush);
-    acc.inval = block_reduce_sum<unsigned long long, 1>(acc.inval, ush);
-    acc.fxc = block_reduce_sum<unsigned long long, 1>(acc.fxc, ush);
-    acc.ids = block_reduce_sum<unsigned long long, 1>(acc.ids, ush);
-    acc.fxs = block_reduce_sum<long long, 1>(acc.fxs, lsh);
-    acc.ss = block_reduce_sum<double, 1>(acc.ss, dsh);
-    if (ln == 0) {
+    acc.recs = block_reduce_sum<unsigned long long, ND_WAVES>(acc.recs, ush);
+    acc.inval = block_reduce_sum<unsigned long long, ND_WAVES>(acc.inval, ush);
+    acc.fxc = block_reduce_sum<unsigned long long, ND_WAVES>(acc.fxc, ush);
+    acc.ids = block_reduce_sum<unsigned long long, ND_WAVES>(acc.ids, ush);
+    acc.fxs = block_reduce_sum<long long, ND_WAVES>(acc.fxs, lsh);
+    acc.ss = block_reduce_sum<double, ND_WAVES>(acc.ss, dsh);
+    if (ln == 0 && live) {
         NdPart& p = parts[tile];
-        p.records = acc.recs; p.invalid = acc.inval; p.id_sum = (long long)acc.ids;
-        p.fx_sum = acc.fxs; p.fx_cells = acc.fxc; p.score_sum = acc.ss;
+        const bool first = w == 0;
+        p.records = first ? acc.recs : 0; p.invalid = first ? acc.inval : 0;
+        p.id_sum = first ? (long long)acc.ids : 0;
+        p.fx_sum = first ? acc.fxs : 0; p.fx_cells = first ? acc.fxc : 0; p.score_sum = first ? acc.ss : 0.0;
         far[tile] = far_rec;
     }
 }
@@ -2919,7 +2954,7 @@ DEV void ndjson_far_tile(const uint8_t* data, const uint2* far, NdPart* fparts, 
 #ifndef FX_KIND   // defined once, in the host translation unit
 __global__ __launch_bounds__(ND_WG) void k_ndjson_id_score(const uint8_t* data, uint32_t len, uint32_t n_tiles,
                                                         NdPart* parts, uint2* far) {
-    ndjson_tile(data, len, n_tiles, parts, far, blockIdx.x);
+    ndjson_block(data, len, n_tiles, parts, far, blockIdx.x);
 }
 
 __global__ __launch_bounds__(64) void k_ndjson_far(const uint8_t* data, uint32_t n_tiles, const uint2* far,
@@ -3186,8 +3221,8 @@ __global__ __launch_bounds__(ND_WG) void k_ndjson_batch(const BatchHdr* H) {
         const uint32_t pi = uni(s_p), t0 = uni(s_t0), t1 = uni(s_t1);
         __syncthreads();
         const BNd& A = H->nd[pi];
-        for (uint32_t t = t0; t < t1; ++t) {
-            ndjson_tile(A.data, A.len, A.n_tiles, A.parts, A.far, t);
+        for (uint32_t t = t0; t < t1; t += ND_WAVES) {   // t0 and the chunk are multiples of ND_WAVES
+            ndjson_block(A.data, A.len, A.n_tiles, A.parts, A.far, t / ND_WAVES);
             __syncthreads();
         }
     }

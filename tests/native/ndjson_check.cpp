@@ -121,6 +121,102 @@ std::string jvalue(Rng& r, int depth) {
     return jobject(r, depth + 1, false);
 }
 
+// Records in the common layout (as the data generator writes them). With
+// probability 1/2 one part is changed to a form the straight-line reader
+// must refuse or still read right; `plain` says no part was changed.
+std::string canon(Rng& r, bool& plain) {
+    auto sp = [&]() {
+        const uint64_t k = r.below(10);
+        return k == 0 ? std::string(" ") : (k == 1 ? std::string("  ") : std::string());
+    };
+    const int mut = r.below(2) ? -1 : (int)r.below(30);
+    plain = mut < 0;
+    auto ws_at = [&](int slot) {
+        if (mut == 0 && slot == 3) return std::string("\t");
+        if (mut == 1 && slot == 7) return std::string(" \r");
+        if (mut == 2 && slot == 5) return std::string(8, ' ');
+        if (mut == 3 && slot == 9) return std::string(5, ' ');
+        return sp();
+    };
+    std::string id = std::to_string(r.below(1ULL << 40));
+    if (mut == 4) id = "-" + id;
+    if (mut == 5) id = "000" + id;
+    if (mut == 6) id = std::string(16, '7');
+    if (mut == 7) id = "123456789012345";
+    if (mut == 8) id = "-";
+    if (mut == 9) id = "12x";
+    std::string name = "\"";
+    for (int i = 0, n = 4 + (int)r.below(29); i < n; ++i) {
+        const uint64_t k = r.below(40);
+        if (k == 0) name += "\\\"";
+        else if (k == 1) name += "\\\\";
+        else if (k == 2) name += "[{";
+        else name.push_back((char)('a' + r.below(26)));
+    }
+    name += "\"";
+    if (mut == 10) name = "12";
+    if (mut == 11) name = "{\"a\":[1,{\"b\":\"}\"}],\"c\":\"]\"}";
+    if (mut == 12) name = "\"abc\\\"";
+    const int dec = 1 + (int)r.below(6);
+    std::string score = std::to_string(r.below(1000)) + ".";
+    for (int i = 0; i < dec; ++i) score.push_back((char)('0' + r.below(10)));
+    if (mut == 13) score += "e" + std::to_string(r.below(20));
+    if (mut == 14) score = "012.5";
+    if (mut == 15) score = "0.25";
+    if (mut == 16) score = ".5";
+    if (mut == 17) score = "5.";
+    if (mut == 18) score = "123456789.1234567";
+    if (mut == 19) score = "-" + score;
+    if (mut == 20) score = std::to_string(r.below(100000));
+    if (mut == 21) score = "1234567890123.5";
+    std::string tags = "[";
+    for (int k = 0, n = (int)r.below(6); k < n; ++k) {
+        if (k) tags += "," + sp();
+        tags += "\"";
+        for (int i = 0, tl = 1 + (int)r.below(8); i < tl; ++i) tags.push_back((char)('a' + r.below(26)));
+        tags += "\"";
+    }
+    tags += "]";
+    if (mut == 22) tags = "[[1, 2], {\"x\": [\"]\"]}, \"y\\\\\"]";
+    if (mut == 23) tags = "\"t\"";
+    std::string k_id = "\"id\"", k_name = "\"name\"", k_nested = "\"nested\"", k_score = "\"score\"";
+    if (mut == 24) k_id = "\"Id\"";
+    if (mut == 25) k_score = "\"scores\"";
+    if (mut == 26) k_name = "\"id\"";
+    std::string s = ws_at(0) + "{" + ws_at(1) + k_id + ws_at(2) + ":" + ws_at(3) + id + ws_at(4) + "," + ws_at(5) +
+                    k_name + ws_at(6) + ":" + ws_at(7) + name + ws_at(8) + "," + ws_at(9) + k_nested + sp() + ":" +
+                    sp() + "{" + sp() + k_score + sp() + ":" + sp() + score + sp() + "," + sp() + "\"tags\"" +
+                    sp() + ":" + sp() + tags + sp() + "}" + sp() + "}";
+    if (mut == 27) s += " trailing";
+    if (mut == 28) s = s.substr(0, r.below(s.size() + 1));
+    if (mut == 29) s.erase(s.size() - 1);
+    return s;
+}
+
+// The straight-line reader on `line` at a random alignment of an 8-aligned
+// buffer, with '\n' after the record (as the kernel sees it) and quotes,
+// backslashes, digits and spaces after that. When it accepts, the walker must
+// return 1 with the same fields. Returns -1 on a mismatch, else whether it
+// accepted.
+int common_agrees(Rng& r, const std::string& line, int ok, const fx::NdRecord& rec) {
+    alignas(8) static uint8_t buf[1 << 16];
+    const uint32_t off = (uint32_t)r.below(8);
+    if (off + line.size() + 40 > sizeof buf) return 0;
+    std::memcpy(buf + off, line.data(), line.size());
+    buf[off + line.size()] = '\n';
+    for (int k = 1; k < 40; ++k) buf[off + line.size() + k] = " \"\\7,}:"[r.below(7)];
+    fx::NdRecord c{};
+    if (!fx::ndjson_common(WordSrc{buf}, off, off + (uint32_t)line.size(), c)) return 0;
+    if (ok != 1 || c.id != rec.id || c.fx6 != rec.fx6 || c.mant != rec.mant || c.score_deferred ||
+        fx::f64_to_bits(c.score) != fx::f64_to_bits(rec.score)) {
+        std::fprintf(stderr, "COMMON LAYOUT MISMATCH on record [%s]: walker %d id %" PRId64 "/%" PRId64
+                             " mant %" PRId64 "/%" PRId64 "\n",
+                     line.c_str(), ok, c.id, rec.id, c.mant, rec.mant);
+        return -1;
+    }
+    return 1;
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
@@ -168,7 +264,27 @@ int main(int argc, char** argv) {
             return 1;
         }
         valid += g_rec;
+        if (common_agrees(r, line, ok, rec) < 0) return 1;
     }
-    std::printf("ndjson_check: %" PRIu64 " records agree (%" PRIu64 " valid)\n", count, valid);
+    // The common layout: every plain record is taken by the straight-line
+    // reader, and whatever it takes agrees with the walker.
+    uint64_t plain_n = 0, taken = 0;
+    for (uint64_t i = 0; i < count; ++i) {
+        bool plain = false;
+        const std::string line = canon(r, plain);
+        fx::NdRecord rec;
+        const int ok = walk(BufSrc{(const uint8_t*)line.data()}, 0, (uint32_t)line.size(), rec);
+        const int c = common_agrees(r, line, ok, rec);
+        if (c < 0) return 1;
+        if (plain && c != 1) {
+            std::fprintf(stderr, "COMMON LAYOUT REFUSED a plain record [%s]\n", line.c_str());
+            return 1;
+        }
+        plain_n += plain;
+        taken += (uint64_t)c;
+    }
+    std::printf("ndjson_check: %" PRIu64 " records agree (%" PRIu64 " valid); common layout: %" PRIu64
+                " records, %" PRIu64 " plain, %" PRIu64 " taken by the straight-line reader\n",
+                count, valid, count, plain_n, taken);
     return 0;
 }

@@ -274,6 +274,67 @@ def test_ndjson_records_longer_than_a_tile(ctx, ora):
         assert ctx.ndjson_lines(ctx.upload(doc), len(doc)) == ora.ndjson_lines(doc)
 
 
+def _nd_common(rng, mut):
+    """A record in the common layout {"id","name","nested":{"score","tags"}}
+    with spaces, or (mut >= 0) one part changed to a form the kernel's
+    straight-line reader must hand to the walker or still read right."""
+    def sp():
+        k = rng.below(10)
+        return b" " if k == 0 else (b"  " if k == 1 else b"")
+    ws = [sp() for _ in range(10)]
+    if mut == 0:
+        ws[3] = b"\t"
+    elif mut == 1:
+        ws[7] = b" \r"
+    elif mut == 2:
+        ws[5] = b" " * 8
+    ident = b"%d" % rng.below(1 << 40)
+    ident = {3: b"-" + ident, 4: b"000" + ident, 5: b"7" * 16, 6: b"123456789012345", 7: b"-", 8: b"12x"}.get(mut, ident)
+    name = bytearray(b'"')
+    for _ in range(4 + rng.below(29)):
+        k = rng.below(40)
+        name += b'\\"' if k == 0 else (b"\\\\" if k == 1 else (b"[{" if k == 2 else bytes([97 + rng.below(26)])))
+    name = {9: b"12", 10: b'{"a":[1,{"b":"}"}],"c":"]"}', 11: b'"abc\\"'}.get(mut, bytes(name + b'"'))
+    score = b"%d." % rng.below(1000) + _digits(rng, 1 + rng.below(6))
+    score = {12: score + b"e%d" % rng.below(20), 13: b"012.5", 14: b"0.25", 15: b".5", 16: b"5.",
+             17: b"123456789.1234567", 18: b"-" + score, 19: b"%d" % rng.below(100000),
+             20: b"1234567890123.5"}.get(mut, score)
+    tags = b"[" + b",".join(b'"' + bytes(97 + rng.below(26) for _ in range(1 + rng.below(8))) + b'"'
+                            for _ in range(rng.below(6))) + b"]"
+    tags = {21: b'[[1, 2], {"x": ["]"]}, "y\\\\"]', 22: b'"t"'}.get(mut, tags)
+    k_id = b'"Id"' if mut == 23 else b'"id"'
+    k_score = b'"scores"' if mut == 24 else b'"score"'
+    k_name = b'"id"' if mut == 25 else b'"name"'
+    s = (ws[0] + b"{" + ws[1] + k_id + ws[2] + b":" + ws[3] + ident + ws[4] + b"," + ws[5] + k_name + ws[6] + b":" +
+         ws[7] + name + ws[8] + b"," + ws[9] + b'"nested"' + sp() + b":" + sp() + b"{" + sp() + k_score + sp() +
+         b":" + sp() + score + sp() + b"," + sp() + b'"tags"' + sp() + b":" + sp() + tags + sp() + b"}" + sp() + b"}")
+    if mut == 26:
+        s += b" trailing"
+    elif mut == 27:
+        s = s[:rng.below(len(s) + 1)]
+    elif mut == 28:
+        s = s[:-1]
+    return s
+
+
+def test_ndjson_common_layout_and_near_misses(ctx, ora):
+    """Documents of records in the common layout, a quarter of them changed
+    in one part (whitespace other than spaces, signs, leading zeros, 15 and 16
+    digit numbers, exponents, nested tags, other keys, truncation): the
+    kernel reads the plain ones in straight-line steps and hands the others
+    to its record walker, and every field agrees with the oracle. A mutation
+    class per document, so a wrong reading of one class shows in its sums."""
+    rng = Rng(0xC0_4417_0A7E)
+    for mut in range(-1, 29):
+        recs = [_nd_common(rng, mut if rng.below(4) == 0 else -1) for _ in range(700)]
+        doc = b"\n".join(recs) + (b"\n" if mut % 2 else b"")
+        got = ctx.ndjson_id_score(ctx.upload(doc), len(doc))
+        want = ora.ndjson_id_score(doc)
+        for k in ("records", "invalid", "id_sum", "score_fixed6_sum", "score_fixed6_cells"):
+            assert got[k] == want[k], f"mutation {mut}: {k} {got[k]} != {want[k]}"
+        assert got["score_sum"] == pytest.approx(want["score_sum"], rel=1e-12, abs=1e-9), f"mutation {mut}"
+
+
 # ---- error statuses through the C ABI ----------------------------------------
 def test_error_statuses(ctx, ora):
     """FALX_E_TOO_LARGE for inputs of 2^32 bytes or more (checked before any
