@@ -2746,16 +2746,15 @@ struct GlobalSrc {
     DEV uint8_t at(uint32_t p) const { return g[p]; }
 };
 
-// Newline bytes of four little-endian words as a 16-bit mask.
-DEV uint32_t nl_mask16(uint4 v) {
+// Newline bytes of 32 bytes (x[0..7], little-endian words) as a 32-bit mask,
+// bit i for byte i (byte transpose, then one byte test per word).
+DEV uint32_t nl_mask32(const uint32_t (&x)[8]) {
+    uint32_t y[8];   // y[j] byte k = byte 8k + j
+    transpose4(x[0], x[2], x[4], x[6], y[0], y[1], y[2], y[3]);
+    transpose4(x[1], x[3], x[5], x[7], y[4], y[5], y[6], y[7]);
     uint32_t m = 0;
-    const uint32_t w[4] = {v.x, v.y, v.z, v.w};
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        const uint32_t x = w[k] ^ 0x0A0A0A0Au;
-        const uint32_t z = ~(((x & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | x | 0x7F7F7F7Fu);
-        m |= (((z >> 7) & 1u) | ((z >> 14) & 2u) | ((z >> 21) & 4u) | ((z >> 28) & 8u)) << (4 * k);
-    }
+    for (int j = 0; j < 8; ++j) m |= (eq_ascii(y[j] & 0x7F7F7F7Fu, ~y[j] & 0x80808080u, 0x0A0A0A0Au) >> 7) << j;
     return m;
 }
 
@@ -2804,30 +2803,35 @@ DEV void ndjson_block(const uint8_t* data, uint32_t len, uint32_t n_tiles, NdPar
     while (true) {
         // List newlines from `scan` on, stopping after the first one at or
         // past tile_end - 1 (the record after it is the next tile's), or
-        // when the list is full.
+        // when the list is full. 32 bytes per lane and step.
         uint32_t n = 0;
         bool stop = false, full = false;
         while (!done && !stop && !full && scan < win_hi) {
-            const uint32_t base = scan & ~15u;
-            const uint32_t off = base - blk_start + 16 * ln;   // LDS offset of this lane's 16 bytes
-            const uint32_t p0 = base + 16 * ln;
+            const uint32_t base = scan & ~31u;
+            const uint32_t off = base - blk_start + 32 * ln;   // LDS offset of this lane's 32 bytes
+            const uint32_t p0 = base + 32 * ln;
             uint32_t m = 0;
-            if (off + 16 <= ND_WIN + 16) m = nl_mask16(*reinterpret_cast<const uint4*>(&bytes[off]));
+            if (off + 32 <= ND_WIN + ND_PAD) {
+                const uint4 v0 = *reinterpret_cast<const uint4*>(&bytes[off]);
+                const uint4 v1 = *reinterpret_cast<const uint4*>(&bytes[off + 16]);
+                const uint32_t x[8] = {v0.x, v0.y, v0.z, v0.w, v1.x, v1.y, v1.z, v1.w};
+                m = nl_mask32(x);
+            }
             // Positions >= scan and < win_hi only.
-            if (p0 < scan) m &= scan - p0 >= 16 ? 0u : (0xFFFFu << (scan - p0)) & 0xFFFFu;
-            if (p0 + 16 > win_hi) m &= win_hi <= p0 ? 0u : (1u << (win_hi - p0)) - 1u;
+            if (p0 < scan) m &= scan - p0 >= 32 ? 0u : ~0u << (scan - p0);
+            if (p0 + 32 > win_hi) m &= win_hi <= p0 ? 0u : (1u << (win_hi - p0)) - 1u;
             // The stopping newline: the first one at or past tile_end - 1.
             const uint32_t lim = tile_end - 1;
             uint32_t cand = m;
-            if (p0 + 16 <= lim) cand = 0;
-            else if (p0 < lim) cand &= (0xFFFFu << (lim - p0)) & 0xFFFFu;
+            if (p0 + 32 <= lim) cand = 0;
+            else if (p0 < lim) cand &= ~0u << (lim - p0);
             const uint64_t cl = ballot(cand != 0);
             uint32_t stop_pos = ND_NONE;
             if (cl) {
                 const uint32_t L = (uint32_t)__ffsll((long long)cl) - 1u;
                 stop_pos = lane_value(p0 + (uint32_t)__ffs((int)cand) - 1u, L);
                 if (p0 > stop_pos) m = 0;
-                else if (p0 + 16 > stop_pos) m &= (2u << (stop_pos - p0)) - 1u;
+                else if (p0 + 32 > stop_pos) m &= (2u << (stop_pos - p0)) - 1u;   // 2u << 31 wraps to 0: all bits
                 stop = true;
             }
             const uint32_t cnt = popc((uint64_t)m);
@@ -2845,7 +2849,7 @@ DEV void ndjson_block(const uint8_t* data, uint32_t len, uint32_t n_tiles, NdPar
                 stop = false;
             } else {
                 n += total;
-                scan = base + 16 * 64;
+                scan = base + 32 * 64;
             }
         }
         __syncthreads();
