@@ -554,7 +554,8 @@ static falx_status columns_impl(falx_ctx* c, const falx_spec* spec, const uint8_
     // String columns: arena bases from the per-tile byte counts, then the
     // placement pass.
     if (P.n_str) {
-        hipLaunchKernelGGL(k_slot_scan, dim3(P.n_str), dim3(SLOT_WG), 0, c->stream, P.str_len, P.n_tiles, P.n_str,
+        hipLaunchKernelGGL(k_slot_scan_wide, dim3((P.n_tiles + SLOT_CHUNK - 1) / SLOT_CHUNK, P.n_str), dim3(SLOT_WG), 0,
+                           c->stream, P.str_len, P.n_tiles, P.n_str,
                            const_cast<uint32_t*>(P.str_base), P.result);
         HIP_TRY(hipGetLastError());
         HIP_TRY(launch_scan<M_STRINGS>(c, P, dialect_kind(spec)));

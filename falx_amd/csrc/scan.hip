@@ -1022,6 +1022,16 @@ __device__ __forceinline__ void scan_tile(const Params& P, const uint32_t tile, 
     const uint32_t lds_lo = HALOED ? tile_start - min((uint32_t)PRE, tile_start) : tile_start;
     const uint32_t lds_hi = tile_start + min((uint32_t)(HALOED ? TILE + HALO : TILE), P.len - tile_start);
     uint8_t* const tb = sh.lds_bytes + PRE;
+    // The tile's quote bytes as a bitmap (bit i of word k: tile byte 32k + i)
+    // for the cell passes of doubled-quote dialects: a cell quoted at both
+    // ends with no other quote byte cleans to its inner bytes, which are then
+    // copied like an unquoted cell's instead of streamed byte by byte.
+    constexpr bool QBITS = HQ && !ESC && HALOED;
+    uint32_t* qbits = nullptr;
+    if constexpr (QBITS) {
+        __shared__ uint32_t qbits_arr[TILE / 32];
+        qbits = qbits_arr;
+    }
 
     // Summary mode's end: the tile summary from the waves' counters
     // (wsum[w * NS + s]: wave w entered in state s) and functions (sh.wfn).
@@ -1131,6 +1141,7 @@ __device__ __forceinline__ void scan_tile(const Params& P, const uint32_t tile, 
         m.q &= live32;
         m.c &= live32;
         m.n &= live32;
+        if constexpr (QBITS) qbits[tid] = m.q;
         // Quote parity: inclusive prefix from the thread's first byte, the
         // quotes of the wave's earlier threads (mbcnt over the odd ones) and
         // the wave's function (swap when its quote count is odd).
@@ -1282,6 +1293,13 @@ __device__ __forceinline__ void scan_tile(const Params& P, const uint32_t tile, 
         const bool own = ln < (uint32_t)BPW;
         const uint32_t off = (w * BPW + (own ? ln : 0u)) * 64;      // this lane's block in the tile
         const uint64_t live = own ? live_mask(off) : 0ULL;
+        if constexpr (QBITS) {
+            if (own) {
+                const uint64_t qm = mine.q & live;
+                qbits[off >> 5] = (uint32_t)qm;
+                qbits[(off >> 5) + 1] = (uint32_t)(qm >> 32);
+            }
+        }
         // Line-start carry: the previous block ends in '\n' (ls0 for the first).
         const uint32_t prev_hi = __shfl_up((uint32_t)(mine.nl >> 32), 1);
         const uint64_t ls = ln == 0 ? ls0 : (uint64_t)(prev_hi >> 31);
@@ -1523,8 +1541,45 @@ __device__ __forceinline__ void scan_tile(const Params& P, const uint32_t tile, 
         if (cd.escape >= 0) return false;
         return !quoted_both(src, from, to, (uint8_t)cd.quote);
     };
+    // A cell of the tile quoted at both ends with no other quote byte (read
+    // from the tile's quote bitmap, a word per 32 bytes): it cleans to
+    // [from + 1, to - 1).
+    auto simple_quoted = [&](uint32_t from, uint32_t to) -> bool {
+        if constexpr (!QBITS) {
+            return false;
+        } else {
+            if (from < tile_start || to > tile_start + tile_len || to - from < 2u) return false;
+            const uint32_t a = from - tile_start, z = to - 1u - tile_start;   // first and last byte
+            if (((qbits[a >> 5] >> (a & 31u)) & (qbits[z >> 5] >> (z & 31u)) & 1u) == 0u) return false;
+            uint32_t n = 0;
+            for (uint32_t k = a >> 5; k <= (z >> 5) && n <= 2u; ++k) {
+                uint32_t wv = qbits[k];
+                if (k == (a >> 5)) wv &= ~0u << (a & 31u);
+                if (k == (z >> 5)) wv &= (z & 31u) == 31u ? ~0u : (2u << (z & 31u)) - 1u;
+                n += (uint32_t)__popc(wv);
+            }
+            return n == 2u;
+        }
+    };
+    // The bytes a cell cleans to when they are a run of the document: the
+    // cell itself (raw), or its inside (simple_quoted). False: it needs the
+    // cleaning stream.
+    auto run_cell = [&](uint32_t from, uint32_t to, uint32_t& rf, uint32_t& rt) -> bool {
+        if (raw_cell(from, to)) {
+            rf = from;
+            rt = to;
+            return true;
+        }
+        if (simple_quoted(from, to)) {
+            rf = from + 1u;
+            rt = to - 1u;
+            return true;
+        }
+        return false;
+    };
     auto clean_len = [&](uint32_t from, uint32_t to) -> uint32_t {
-        return raw_cell(from, to) ? to - from : clean_cell(src, from, to, cd, [](uint8_t) {});
+        uint32_t rf, rt;
+        return run_cell(from, to, rf, rt) ? rt - rf : clean_cell(src, from, to, cd, [](uint8_t) {});
     };
     auto field_end = [&](const Entry& e) -> uint32_t {
         return (e.is_term && e.pos > e.rs && src.at(e.pos - 1) == '\r') ? e.pos - 1 : e.pos;
@@ -1998,17 +2053,20 @@ __device__ __forceinline__ void scan_tile(const Params& P, const uint32_t tile, 
                     bool here, missing;
                     uint32_t fs, fe;
                     cell_of(R, c.index, here, missing, fs, fe);
-                    const bool raw = here && raw_cell(fs, fe);
-                    const uint32_t L = here ? (raw ? fe - fs : clean_len(fs, fe)) : 0u;
+                    // A cell that cleans to a run of the document [rf, rt)
+                    // is copied as it stands.
+                    uint32_t rf = 0, rt = 0;
+                    const bool raw = here && run_cell(fs, fe, rf, rt);
+                    const uint32_t L = here ? (raw ? rt - rf : clean_len(fs, fe)) : 0u;
                     uint32_t rtot;
                     const uint32_t at = run + block_excl_sum(L, sh.sum_sh, rtot);
                     const bool live = R.row < rows_cap;
                     // A live raw cell inside the tile's LDS copy joins the
                     // shared copy below: work per byte, not per longest cell.
-                    const bool coop = coop_tile && raw && live && fs >= lds_lo && fe <= lds_hi;
+                    const bool coop = coop_tile && raw && live && rf >= lds_lo && rt <= lds_hi;
                     if (coop_tile) {
                         rt_end[tid] = (uint16_t)(at + L);
-                        rt_delta[tid] = coop ? (int16_t)((int32_t)(fs - tile_start + PRE) - (int32_t)at) : (int16_t)RAW_SKIP;
+                        rt_delta[tid] = coop ? (int16_t)((int32_t)(rf - tile_start + PRE) - (int32_t)at) : (int16_t)RAW_SKIP;
                     }
                     if (here && !coop) {
                         // Bytes of rows past rows_cap are not placed: the call
@@ -2020,7 +2078,7 @@ __device__ __forceinline__ void scan_tile(const Params& P, const uint32_t tile, 
                         };
                         if (raw) {
                             for (uint32_t k = 0; k < L; ++k) {
-                                const uint8_t b = src.at(fs + k);
+                                const uint8_t b = src.at(rf + k);
                                 put(at + k, b);
                                 s_sum += b;
                             }
@@ -2555,6 +2613,54 @@ DEV void slot_scan_body(const uint32_t* len, uint32_t n_tiles, uint32_t ns, uint
 __global__ __launch_bounds__(SLOT_WG) void k_slot_scan(const uint32_t* len, uint32_t n_tiles, uint32_t ns,
                                                        uint32_t* base, unsigned long long* result) {
     slot_scan_body(len, n_tiles, ns, base, result, blockIdx.x);
+}
+
+// The same scan for a whole document in many workgroups: workgroup
+// (chunk, slot) takes SLOT_CHUNK tiles, SLOT_PER per thread. Its base is the
+// sum of the lengths of every tile before the chunk, which it reads itself
+// (coalesced, at most n_tiles words), so no workgroup waits for another.
+constexpr uint32_t SLOT_PER = 4;
+constexpr uint32_t SLOT_CHUNK = SLOT_WG * SLOT_PER;
+
+__global__ __launch_bounds__(SLOT_WG) void k_slot_scan_wide(const uint32_t* len, uint32_t n_tiles, uint32_t ns,
+                                                            uint32_t* base, unsigned long long* result) {
+    __shared__ unsigned long long wsh[SLOT_WG / 64];
+    __shared__ uint32_t wsh32[SLOT_WG / 64];
+    const uint32_t t = threadIdx.x, ln = __lane_id(), w = t >> 6, slot = blockIdx.y;
+    const uint32_t c0 = blockIdx.x * SLOT_CHUNK;
+    // Lengths of the tiles before the chunk.
+    unsigned long long pre = 0;
+    for (uint32_t i = t; i < c0; i += SLOT_WG) pre += len[(size_t)i * ns + slot];
+    for (int d = 32; d > 0; d >>= 1) pre += __shfl_xor(pre, d);
+    if (ln == 0) wsh[w] = pre;
+    // The chunk: SLOT_PER consecutive tiles per thread, scanned in the block.
+    const uint32_t i0 = c0 + t * SLOT_PER;
+    uint32_t v[SLOT_PER], sum = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < SLOT_PER; ++k) {
+        v[k] = i0 + k < n_tiles ? len[(size_t)(i0 + k) * ns + slot] : 0u;
+        sum += v[k];
+    }
+    const uint32_t incl = wave_incl_sum(sum);
+    if (ln == 63) wsh32[w] = incl;
+    __syncthreads();
+    unsigned long long run = 0;
+    uint32_t wb = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < SLOT_WG / 64; ++k) {
+        run += wsh[k];
+        if (k < w) wb += wsh32[k];
+    }
+    run += wb + incl - sum;
+#pragma unroll
+    for (uint32_t k = 0; k < SLOT_PER; ++k) {
+        const uint32_t i = i0 + k;
+        if (i < n_tiles) {
+            base[(size_t)i * ns + slot] = (uint32_t)run;
+            run += v[k];
+            if (i + 1 == n_tiles) result[R_STR_LEN + slot] = run;
+        }
+    }
 }
 #endif
 
