@@ -223,6 +223,27 @@ def test_columns_region_hazard_docs(ctx, ora):
         assert fb == ora.field_bytes(S.CSV_HASH, doc), f"field bytes case {i}"
 
 
+@pytest.mark.parametrize("spec", [S.CSV_HASH, S.GEO_CITY_LATLON], ids=lambda s: s.name)
+def test_quoted_string_cells_across_tile_seam(ctx, ora, spec):
+    """String cells quoted at both ends, with and without doubled quotes, at
+    every offset from 80 bytes before the 8 KiB tile seam to 8 after it. A
+    cell that lies in one tile and holds no quote but its outer two is
+    cleaned as a run of the document; one crossing the seam, starting
+    before its record's tile or holding a doubled quote takes the cleaning
+    stream. Both kinds of tile front run: csv_hash (comment dialect, one
+    byte per lane and ballots) and the geo CSV (32 bytes per thread)."""
+    cells = [b'"ab,cd"', b'"a""b"', b'""', b'"x"', b'"' + b"q" * 70 + b'"', b'"' + b"r" * 40 + b'""s"']
+    for cell in cells:
+        for off in range(TILE - 80, TILE + 8):
+            if spec is S.CSV_HASH:
+                doc = b"k" * (off - 1) + b"\n" + cell + b",7\n" + cell + b",8\n"
+            else:
+                doc = b"9" * (off - 3) + b"\n1," + cell + b",1.500000,-2.250000\n2," + cell + b",3.000000,4.000000\n"
+            want = ora.parse_columns(spec, doc)
+            got = ctx.parse_columns(spec, ctx.upload(doc), len(doc))
+            assert_columns_equal(got, want, spec, f"cell {cell[:12]!r} at {off}")
+
+
 @pytest.mark.parametrize("spec", [S.GEO_LATLON, S.GEO_CITY_LATLON, S.GEO_ALL], ids=lambda s: s.name)
 def test_geo_columns_and_stats(ctx, ora, spec):
     data, truth = datagen.geo_csv(3 * 1024 * 1024 + 777)
