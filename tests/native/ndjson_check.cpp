@@ -129,7 +129,7 @@ std::string canon(Rng& r, bool& plain) {
         const uint64_t k = r.below(10);
         return k == 0 ? std::string(" ") : (k == 1 ? std::string("  ") : std::string());
     };
-    const int mut = r.below(2) ? -1 : (int)r.below(30);
+    const int mut = r.below(2) ? -1 : (int)r.below(32);
     plain = mut < 0;
     auto ws_at = [&](int slot) {
         if (mut == 0 && slot == 3) return std::string("\t");
@@ -173,12 +173,24 @@ std::string canon(Rng& r, bool& plain) {
     for (int k = 0, n = (int)r.below(6); k < n; ++k) {
         if (k) tags += "," + sp();
         tags += "\"";
-        for (int i = 0, tl = 1 + (int)r.below(8); i < tl; ++i) tags.push_back((char)('a' + r.below(26)));
+        for (int i = 0, tl = 1 + (int)r.below(8); i < tl; ++i) {
+            // Now and then a byte that matters to a skip: brackets and braces
+            // inside the string, an escaped quote or backslash.
+            const uint64_t t = r.below(60);
+            if (t == 0) tags += "]";
+            else if (t == 1) tags += "}";
+            else if (t == 2) tags += "[";
+            else if (t == 3) tags += "\\\"";
+            else if (t == 4) tags += "\\\\";
+            else tags.push_back((char)('a' + r.below(26)));
+        }
         tags += "\"";
     }
     tags += "]";
     if (mut == 22) tags = "[[1, 2], {\"x\": [\"]\"]}, \"y\\\\\"]";
     if (mut == 23) tags = "\"t\"";
+    if (mut == 30) tags = "[1, \"a]\", null,true ]";
+    if (mut == 31) tags = "[\"" + std::string(r.below(40), 'x') + "\\\"\"]";
     std::string k_id = "\"id\"", k_name = "\"name\"", k_nested = "\"nested\"", k_score = "\"score\"";
     if (mut == 24) k_id = "\"Id\"";
     if (mut == 25) k_score = "\"scores\"";
