@@ -643,9 +643,10 @@ DEV uint32_t block_fn(const BM& m, uint64_t ls) {
 // --------------------------------------------------------------------------
 // SWAR classification of 32 bytes (a thread's run in the SWAR front)
 // --------------------------------------------------------------------------
-// Quote, candidate and newline bytes of a run, bit i for byte i.
+// Quote, candidate, newline and comment bytes of a run, bit i for byte i
+// (comment bytes only for comment dialects).
 struct Masks32 {
-    uint32_t q, c, n;
+    uint32_t q, c, n, h;
 };
 
 // Byte transpose of four words: byte k of o_j is byte j of a_k (v_perm_b32:
@@ -680,8 +681,8 @@ DEV uint32_t eq_ascii(uint32_t y7, uint32_t lo, uint32_t c4) { return lo & ~((y7
 
 // Words x[0..7] hold the run's bytes in order (byte i = byte i & 3 of
 // x[i >> 2]); NSEP separator bytes besides '\n' (all below 0x80, as is the
-// quote byte).
-template <bool HQ, int NSEP>
+// quote byte and the comment byte).
+template <bool HQ, int NSEP, bool CMT = false>
 DEV Masks32 classify32(const uint32_t (&x)[8], const Params& P) {
     // y[j] byte k = run byte 8k + j: words x0 x2 x4 x6 hold bytes 8k + 0..3,
     // x1 x3 x5 x7 bytes 8k + 4..7.
@@ -692,7 +693,8 @@ DEV Masks32 classify32(const uint32_t (&x)[8], const Params& P) {
     const uint32_t s0 = (NSEP > 0 ? P.sepb[0] : 0u) * 0x01010101u, s1 = (NSEP > 1 ? P.sepb[1] : 0u) * 0x01010101u;
     const uint32_t s2 = (NSEP > 2 ? P.sepb[2] : 0u) * 0x01010101u, s3 = (NSEP > 3 ? P.sepb[3] : 0u) * 0x01010101u;
     const uint32_t nl_keep = P.nl_cand != 0u ? ~0u : 0u;   // '\n' flags kept among the candidates
-    Masks32 m{0u, 0u, 0u};
+    const uint32_t h4 = CMT ? (uint32_t)P.comment * 0x01010101u : 0u;
+    Masks32 m{0u, 0u, 0u, 0u};
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
         const uint32_t y7 = y[j] & 0x7F7F7F7Fu, lo = ~y[j] & 0x80808080u;
@@ -705,15 +707,16 @@ DEV Masks32 classify32(const uint32_t (&x)[8], const Params& P) {
         m.n |= (en >> 7) << j;
         m.c |= (ec >> 7) << j;
         if (HQ) m.q |= (eq_ascii(y7, lo, q4) >> 7) << j;
+        if (CMT) m.h |= (eq_ascii(y7, lo, h4) >> 7) << j;
     }
     return m;
 }
 
 // The same masks from the 256-entry candidate table, a byte at a time (more
 // than four separator bytes, or a byte of 0x80 or above).
-template <bool HQ>
+template <bool HQ, bool CMT = false>
 DEV Masks32 classify32_table(const uint32_t (&x)[8], const Params& P, const uint8_t* cls) {
-    Masks32 m{0u, 0u, 0u};
+    Masks32 m{0u, 0u, 0u, 0u};
     // A word at a time (not unrolled: four table reads in flight, not 32).
 #pragma unroll 1
     for (int k = 0; k < 8; ++k) {
@@ -727,6 +730,7 @@ DEV Masks32 classify32_table(const uint32_t (&x)[8], const Params& P, const uint
             m.c |= (uint32_t)cls[c] << i;
             m.n |= (uint32_t)(c == '\n') << i;
             if (HQ) m.q |= (uint32_t)(c == (uint32_t)P.quote) << i;
+            if (CMT) m.h |= (uint32_t)(c == (uint32_t)P.comment) << i;
         }
     }
     return m;
@@ -1078,7 +1082,7 @@ __device__ __forceinline__ void scan_tile(const Params& P, const uint32_t tile, 
 
     // Dialects with neither escape nor comment take the SWAR front below; the
     // others classify by ballots with one lane per 64-byte block.
-    constexpr bool SWAR = !ESC && !CMT;
+    constexpr bool SWAR = !ESC;
     if constexpr (SWAR) {
         // ---- SWAR front: thread t holds tile bytes [32t, 32t + 32) ---------
         // The bytes come from HBM into registers (and into the LDS copy the
@@ -1127,21 +1131,79 @@ __device__ __forceinline__ void scan_tile(const Params& P, const uint32_t tile, 
         Masks32 m{0u, 0u, 0u};
         // Candidate and quote bytes below 0x80 with at most four separator
         // bytes take the word compares; other dialects the byte table.
-        const uint32_t qa = HQ ? (uint32_t)P.quote : 0u;
-        const bool ascii = qa < 0x80u && (P.n_sepb < 1 || P.sepb[0] < 0x80u) && (P.n_sepb < 2 || P.sepb[1] < 0x80u) &&
-                           (P.n_sepb < 3 || P.sepb[2] < 0x80u) && (P.n_sepb < 4 || P.sepb[3] < 0x80u);
+        const uint32_t qa = HQ ? (uint32_t)P.quote : 0u, ha = CMT ? (uint32_t)P.comment : 0u;
+        const bool ascii = qa < 0x80u && ha < 0x80u && (P.n_sepb < 1 || P.sepb[0] < 0x80u) &&
+                           (P.n_sepb < 2 || P.sepb[1] < 0x80u) && (P.n_sepb < 3 || P.sepb[2] < 0x80u) &&
+                           (P.n_sepb < 4 || P.sepb[3] < 0x80u);
         switch (ascii ? P.n_sepb : 5u) {
-        case 0: m = classify32<HQ, 0>(x, P); break;
-        case 1: m = classify32<HQ, 1>(x, P); break;
-        case 2: m = classify32<HQ, 2>(x, P); break;
-        case 3: m = classify32<HQ, 3>(x, P); break;
-        case 4: m = classify32<HQ, 4>(x, P); break;
-        default: m = classify32_table<HQ>(x, P, sh.cls); break;
+        case 0: m = classify32<HQ, 0, CMT>(x, P); break;
+        case 1: m = classify32<HQ, 1, CMT>(x, P); break;
+        case 2: m = classify32<HQ, 2, CMT>(x, P); break;
+        case 3: m = classify32<HQ, 3, CMT>(x, P); break;
+        case 4: m = classify32<HQ, 4, CMT>(x, P); break;
+        default: m = classify32_table<HQ, CMT>(x, P, sh.cls); break;
         }
         m.q &= live32;
         m.c &= live32;
         m.n &= live32;
+        m.h &= live32;
         if constexpr (QBITS) qbits[tid] = m.q;
+        if constexpr (CMT) {
+            // Comment dialects: the NORMAL/QUOTE/COMMENT machine runs on the
+            // thread's run as the ballot form runs it on a 64-byte block (the
+            // run's masks are the low halves of the block masks; the high half
+            // holds no byte and leaves the state as it is), then the runs'
+            // transfer functions are scanned over the wave's lanes.
+            const BM mine{(uint64_t)m.c, (uint64_t)m.n, (uint64_t)m.q, 0ULL, (uint64_t)m.h};
+            // Line start at the run's first byte: the previous run ends in
+            // '\n' (the wave's first run reads the byte before it).
+            const uint32_t prev_n = __shfl_up(m.n, 1);
+            const uint32_t at0 = tile_start + t0;
+            // (A run past the tile's end holds no byte: its line start is moot.)
+            const bool peek = ln == 0 && at0 > 0u && t0 <= tile_len;
+            const uint32_t ls_first = at0 == 0u ? 1u : (peek ? (uint32_t)(P.data[at0 - 1u] == '\n') : 0u);
+            const uint64_t ls = ln == 0 ? (uint64_t)ls_first : (uint64_t)(prev_n >> 31);
+            BlockSteps<NS> steps;
+            steps.template fill<HQ, false, true>(mine, ls);
+            uint32_t f_incl = steps.fn();
+#pragma unroll
+            for (int d = 1; d < 64; d <<= 1) {
+                const uint32_t g = __shfl_up(f_incl, d);
+                if (ln >= (uint32_t)d) f_incl = fn_then<NS>(g, f_incl);
+            }
+            uint32_t f_excl = __shfl_up(f_incl, 1);
+            if (ln == 0) f_excl = fn_ident<NS>();
+            __syncthreads();   // the LDS copy is complete
+            if (MODE == M_SUMMARY) {
+#pragma unroll
+                for (int s = 0; s < NS; ++s) {
+                    const StepOut o = steps.at(fn_apply(f_excl, (uint32_t)s));
+                    const Counts c = wave_counts(o.S, o.T, o.F, tile_start + t0);
+                    if (ln == 0) wsum[w * NS + s] = c;
+                }
+                if (ln == 63) sh.wfn[w] = f_incl;
+                publish_summary();
+                return;
+            }
+            load_entry_states();
+            {
+                const StepOut o = steps.at(fn_apply(f_excl, uni(sh.wentry[w])));
+                const uint32_t S = (uint32_t)o.S, T = (uint32_t)o.T, F = (uint32_t)o.F;
+                const uint32_t n = (uint32_t)__popc(S);
+                const uint32_t incl = wave_incl_sum(n);
+                if (ln == 63) sh.wcount[w] = incl;
+                __syncthreads();
+                uint32_t wb = 0;
+#pragma unroll
+                for (uint32_t k = 0; k < (uint32_t)NWAVE; ++k)
+                    if (k < w) wb += sh.wcount[k];
+                uint16_t* lst = sh.list + uni(wb) + (incl - n);
+                for (uint32_t rem = S; rem; rem &= rem - 1u) {
+                    const uint32_t bit = (uint32_t)__ffs((int)rem) - 1u;
+                    *lst++ = (uint16_t)((t0 + bit) | (((T >> bit) & 1u) << 14) | (((F >> bit) & 1u) << 15));
+                }
+            }
+        } else {
         // Quote parity: inclusive prefix from the thread's first byte, the
         // quotes of the wave's earlier threads (mbcnt over the odd ones) and
         // the wave's function (swap when its quote count is odd).
@@ -1206,6 +1268,7 @@ __device__ __forceinline__ void scan_tile(const Params& P, const uint32_t tile, 
             }
             __syncthreads();   // every thread has read sh.wcount
             if (ln == 63) sh.wcount[w] = incl & 0xFFFFu;
+        }
         }
     } else {
         // ---- load: 16-byte coalesced reads into LDS, zero padded --------------
