@@ -149,6 +149,14 @@ bool numeric_fast(const falx_spec* spec, int kind, uint32_t first, uint32_t n) {
     return true;
 }
 
+// Whether the statistics of a columns call need the pass over columns
+// [g, g + STAT_GROUP): the STRINGS pass fuses those of string columns.
+bool stats_group_needed(const falx_spec* spec, uint32_t g) {
+    for (uint32_t k = g; k < g + STAT_GROUP && k < spec->n_columns; ++k)
+        if (spec->columns[k].type != FALX_COL_STR) return true;
+    return false;
+}
+
 // The statistics pass over columns [P.col_group, + STAT_GROUP).
 hipError_t launch_stats_group(falx_ctx* c, const falx_spec* spec, Params& P, int kind) {
     if (numeric_fast(spec, kind, P.col_group, STAT_GROUP)) return launch_fast<M_NUM_STATS_FAST, M_STATS>(c, P, kind);
@@ -547,6 +555,7 @@ static falx_status columns_impl(falx_ctx* c, const falx_spec* spec, const uint8_
     if (stats) {
         Params Q = P;
         for (uint32_t g = COL_STAT_GROUP; g < spec->n_columns; g += STAT_GROUP) {
+            if (!stats_group_needed(spec, g)) continue;
             Q.col_group = g;
             HIP_TRY(launch_stats_group(c, spec, Q, kind));
         }

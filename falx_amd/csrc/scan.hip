@@ -2107,9 +2107,10 @@ __device__ __forceinline__ void scan_tile(const Params& P, const uint32_t tile, 
                 // tile's cells are long: its mean cleaned length decides.
                 const bool coop_tile = gather && total > 13u * (inc.n_rows - pre.n_rows + 1u);
                 if (tile == 0 && tid == 0) offs[0] = 0;
-                // Fused statistics of a string column among the first
-                // COL_STAT_GROUP: present cells, cleaned bytes, their byte sum.
-                const bool stat_col = P.partials != nullptr && j < (uint32_t)COL_STAT_GROUP;
+                // Fused statistics of every string column (present cells,
+                // cleaned bytes, their byte sum): no statistics pass runs for
+                // a group of string columns.
+                const bool stat_col = P.partials != nullptr;
                 unsigned long long s_cells = 0, s_bytes = 0, s_sum = 0;
                 uint32_t run = 0;   // bytes of this tile's range placed by earlier rounds
                 for_records([&](const Rec& R) {
