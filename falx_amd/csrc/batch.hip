@@ -387,7 +387,7 @@ falx_status falx_batch_submit(falx_batch* b, const falx_request* reqs, uint32_t 
         case FALX_OP_COLUMNS_STATS:
             emit(E_COLUMNS, D.pidx);
             if (r.op == FALX_OP_COLUMNS_STATS)
-                for (uint32_t g = COL_STAT_GROUP; g < D.spec.n_columns; g += STAT_GROUP)
+                for (uint32_t g = 0; g < D.spec.n_columns; g += STAT_GROUP)
                     if (stats_group_needed(&D.spec, g)) stats_pass(g);
             if (P.n_str) {
                 B.tab[T_SLOT].add(D.pidx, P.n_str, 1);

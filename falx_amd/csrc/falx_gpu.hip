@@ -150,10 +150,16 @@ bool numeric_fast(const falx_spec* spec, int kind, uint32_t first, uint32_t n) {
 }
 
 // Whether the statistics of a columns call need the pass over columns
-// [g, g + STAT_GROUP): the STRINGS pass fuses those of string columns.
+// [g, g + STAT_GROUP): the columns pass fuses those of the first
+// COL_STAT_GROUP columns that are not strings, the STRINGS pass those of
+// every string column; a group holding any other column needs its pass.
 bool stats_group_needed(const falx_spec* spec, uint32_t g) {
-    for (uint32_t k = g; k < g + STAT_GROUP && k < spec->n_columns; ++k)
-        if (spec->columns[k].type != FALX_COL_STR) return true;
+    uint32_t rank = 0;   // columns before k that are not strings
+    for (uint32_t k = 0; k < g + STAT_GROUP && k < spec->n_columns; ++k) {
+        if (spec->columns[k].type == FALX_COL_STR) continue;
+        if (k >= g && rank >= (uint32_t)COL_STAT_GROUP) return true;
+        ++rank;
+    }
     return false;
 }
 
@@ -551,10 +557,11 @@ static falx_status columns_impl(falx_ctx* c, const falx_spec* spec, const uint8_
         if (stats) HIP_TRY((launch_fast<M_COLUMNS_FAST_STATS, M_COLUMNS>(c, P, kind)));
         else HIP_TRY((launch_fast<M_COLUMNS_FAST, M_COLUMNS>(c, P, kind)));
     }
-    // Statistics of columns past the first group come from extra passes.
+    // Statistics of the columns neither fused pass covers come from extra
+    // passes over their groups.
     if (stats) {
         Params Q = P;
-        for (uint32_t g = COL_STAT_GROUP; g < spec->n_columns; g += STAT_GROUP) {
+        for (uint32_t g = 0; g < spec->n_columns; g += STAT_GROUP) {
             if (!stats_group_needed(spec, g)) continue;
             Q.col_group = g;
             HIP_TRY(launch_stats_group(c, spec, Q, kind));

@@ -853,8 +853,8 @@ DEV double rows_sum_f64(double v) {
     return ((at(0) + at(16)) + at(32)) + at(48);
 }
 
-// Per-tile statistics of up to N columns [first, first + nc) plus the record
-// count, reduced in a fixed order (the same in every pass, so a tile's
+// Per-tile statistics of up to N columns (accumulator j holds column
+// cols[j], j < nc) plus the record count (with_records), reduced in a fixed order (the same in every pass, so a tile's
 // partials are bit-identical whichever pass wrote them): waves 1-3 leave
 // their lanes' accumulators in the LDS exchange area xs, wave 0 adds them to
 // its own lane by lane (waves in order), sums each row of 16 lanes by DPP and
@@ -863,8 +863,8 @@ DEV double rows_sum_f64(double v) {
 // fixed-6 cells travel as the 16-bit halves of one word (a tile holds at most
 // 8193 records). Called by every thread; xs must not be in use.
 template <uint32_t N>
-DEV void publish_stats(const Params& P, uint32_t tile, const Acc (&acc)[N], uint32_t first, uint32_t nc,
-                       unsigned long long records, unsigned long long* xs) {
+DEV void publish_stats(const Params& P, uint32_t tile, const Acc (&acc)[N], const uint32_t (&cols)[N], uint32_t nc,
+                       unsigned long long records, bool with_records, unsigned long long* xs) {
     constexpr uint32_t NSLOT = 4 * N + 1;   // per column: counts | bytes, checksum, fixed-6 sum, FP64 sum; records
     const uint32_t ln = __lane_id(), w = uni(threadIdx.x >> 6);
     __syncthreads();   // every thread is done with what xs held
@@ -888,7 +888,7 @@ DEV void publish_stats(const Params& P, uint32_t tile, const Acc (&acc)[N], uint
 #pragma unroll
     for (uint32_t j = 0; j < N; ++j) {
         if (j >= nc) break;
-        const uint32_t type = P.cols[first + j].type;
+        const uint32_t type = P.cols[cols[j]].type;
         const bool is_f64 = type == FALX_COL_F64, is_num = is_f64 || type == FALX_COL_I64;
         uint32_t vc = acc[j].values | (acc[j].fx_cells << 16), by = acc[j].bytes;
         unsigned long long cs = acc[j].csum, fx = (unsigned long long)acc[j].fx_sum;
@@ -909,7 +909,7 @@ DEV void publish_stats(const Params& P, uint32_t tile, const Acc (&acc)[N], uint
         const unsigned long long tfx = is_f64 ? rows_sum_u64(fx) : 0ULL;
         const double tds = is_num ? rows_sum_f64(ds) : 0.0;
         if (ln == 0) {
-            StatPart& o = parts[(size_t)tile * P.n_cols + first + j];
+            StatPart& o = parts[(size_t)tile * P.n_cols + cols[j]];
             o.values = tvc & 0xFFFFu;
             o.checksum = tcs;
             o.bytes = tby;
@@ -918,7 +918,7 @@ DEV void publish_stats(const Params& P, uint32_t tile, const Acc (&acc)[N], uint
             o.sum = tds;
         }
     }
-    if (first == 0) {
+    if (with_records) {
         unsigned long long rec = records;
 #pragma unroll
         for (uint32_t k = 0; k < (uint32_t)NWAVE - 1; ++k) rec += in[k * NSLOT * 64 + 4 * N * 64];
@@ -1754,9 +1754,9 @@ __device__ __forceinline__ void scan_tile(const Params& P, const uint32_t tile, 
         for (uint32_t jj = 0; jj < N; ++jj)
             if (jj == j) acc[jj].add(o);
     };
-    auto acc_publish = [&](auto& acc, uint32_t first, uint32_t nc, unsigned long long records) {
+    auto acc_publish = [&](auto& acc, const auto& cols, uint32_t nc, unsigned long long records, bool with_records) {
         static_assert((NWAVE - 1) * (4 * STAT_GROUP + 1) * 64 * 8 <= sizeof(sh.list), "statistics exchange fits the list");
-        publish_stats(P, tile, acc, first, nc, records, reinterpret_cast<unsigned long long*>(sh.list));
+        publish_stats(P, tile, acc, cols, nc, records, with_records, reinterpret_cast<unsigned long long*>(sh.list));
     };
     // Typed value of a present cell as f64 (as_f64) or i64: for cells inside
     // the tile the fixed-6 test, then the register parser; the stream parser
@@ -1922,7 +1922,10 @@ __device__ __forceinline__ void scan_tile(const Params& P, const uint32_t tile, 
         if (IS_COLS && tid == 0 && last_tile) P.result[R_N_ROWS] = inc.n_rows + (trailer ? 1 : 0);
         __syncthreads();
         if (tid == 0) P.fixup[tile] = sh.fix_bad;
-        if (with_stats) acc_publish(acc, c0, min(NACC, nc), records);
+        if (with_stats) {
+            const uint32_t cols[NACC] = {c0, c0 + 1u};
+            acc_publish(acc, cols, min(NACC, nc), records, c0 == 0u);
+        }
         return;
     }
 
@@ -2233,6 +2236,26 @@ __device__ __forceinline__ void scan_tile(const Params& P, const uint32_t tile, 
         Acc acc[NACC];
 #pragma unroll
         for (uint32_t j = 0; j < NACC; ++j) acc[j] = Acc{0, 0, 0, 0, 0, 0.0};
+        // Accumulator j of a COLUMNS pass holds the j-th column that is not a
+        // string (string columns' statistics come from the STRINGS pass); a
+        // STATS pass holds its group's columns in order.
+        uint32_t acc_col[NACC];
+        uint32_t n_acc = 0;
+        if (IS_COLS) {
+#pragma unroll
+            for (uint32_t j = 0; j < NACC; ++j) acc_col[j] = 0u;
+            for (uint32_t k = 0; k < P.n_cols && n_acc < NACC; ++k)
+                if (P.cols[k].type != FALX_COL_STR) {
+#pragma unroll
+                    for (uint32_t j = 0; j < NACC; ++j)
+                        if (j == n_acc) acc_col[j] = k;
+                    ++n_acc;
+                }
+        } else {
+#pragma unroll
+            for (uint32_t j = 0; j < NACC; ++j) acc_col[j] = c0 + j;
+            n_acc = min(NACC, nc);
+        }
         unsigned long long records = 0;
         if (IS_COLS && tid < FALX_MAX_COLUMNS) sh.str_len[tid] = 0;   // read after the segment scans' barriers
         for_records([&](const Rec& R) {
@@ -2241,14 +2264,17 @@ __device__ __forceinline__ void scan_tile(const Params& P, const uint32_t tile, 
             const uint32_t row = R.row;
             const bool live = row < rows_cap;
             const uint64_t RL = IS_COLS ? ballot(live_rec) : 0ULL;
+            uint32_t slot = 0;   // accumulator of the next column that is not a string
             for (uint32_t jj = 0; jj < nc; ++jj) {
                 const uint32_t j = c0 + jj;
                 const ColDesc& c = P.cols[j];
                 bool here, missing;
                 uint32_t fs, fe;
                 cell_of(R, c.index, here, missing, fs, fe);
+                const uint32_t acc_j = IS_COLS ? (c.type == FALX_COL_STR ? NACC : slot) : jj;
+                if (IS_COLS && c.type != FALX_COL_STR) ++slot;
                 // Statistics of COLUMNS cover the rows that fit rows_cap.
-                const bool stat_ok = with_stats && jj < NACC && (IS_COLS ? live : true);
+                const bool stat_ok = with_stats && acc_j < NACC && (IS_COLS ? live : true);
                 bool vb = false;
                 if (c.type == FALX_COL_F64 || c.type == FALX_COL_I64) {
                     const bool as_f64 = c.type == FALX_COL_F64;
@@ -2266,7 +2292,7 @@ __device__ __forceinline__ void scan_tile(const Params& P, const uint32_t tile, 
                             reinterpret_cast<long long*>(c.values)[row] = vb ? nv.iv : 0;
                         if (vb) o = CellStat{true, false, (unsigned long long)nv.iv, 0, 0, (double)nv.iv};
                     }
-                    if (stat_ok) acc_add(acc, jj, o);
+                    if (stat_ok) acc_add(acc, acc_j, o);
                 } else if (c.type == FALX_COL_STR) {
                     if (IS_COLS) {
                         // Cleaned byte counts only; the STRINGS pass places
@@ -2279,7 +2305,7 @@ __device__ __forceinline__ void scan_tile(const Params& P, const uint32_t tile, 
                     } else if (here) {
                         unsigned long long cs = 0;
                         const uint32_t nb = clean_cell(src, fs, fe, cd, [&](uint8_t b) { cs += b; });
-                        if (stat_ok) acc_add(acc, jj, CellStat{true, false, cs, nb, 0, 0.0});
+                        if (stat_ok) acc_add(acc, acc_j, CellStat{true, false, cs, nb, 0, 0.0});
                     }
                 } else {
                     vb = here && fe != fs;
@@ -2288,7 +2314,7 @@ __device__ __forceinline__ void scan_tile(const Params& P, const uint32_t tile, 
                     if (vb && stat_ok) {
                         unsigned long long cs = 0;
                         for (uint32_t q = fs; q < fe; ++q) cs += src.at(q);
-                        acc_add(acc, jj, CellStat{true, false, cs, fe - fs, 0, 0.0});
+                        acc_add(acc, acc_j, CellStat{true, false, cs, fe - fs, 0, 0.0});
                     }
                 }
                 if (IS_COLS && c.type != FALX_COL_STR) wave_valid(c, vb && live, row, RL);
@@ -2297,7 +2323,7 @@ __device__ __forceinline__ void scan_tile(const Params& P, const uint32_t tile, 
         if (IS_COLS && tid == 0 && last_tile) P.result[R_N_ROWS] = inc.n_rows + (trailer_row_live ? 1 : 0);
         __syncthreads();
         if (FAST && tid == 0) P.fixup[tile] = sh.fix_bad;
-        if (with_stats) acc_publish(acc, c0, min(NACC, nc), records);
+        if (with_stats) acc_publish(acc, acc_col, n_acc, records, IS_COLS || c0 == 0);
         if (IS_COLS && P.n_str > 0 && tid < P.n_str) P.str_len[(size_t)tile * P.n_str + tid] = sh.str_len[tid];
     }
 }

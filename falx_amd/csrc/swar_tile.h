@@ -386,7 +386,10 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(8, 8))) void
         }
     }
     if (IS_COLS && tid == 0 && last_tile) P.result[R_N_ROWS] = inc.n_rows + (trailer ? 1 : 0);
-    if (STATS) publish_stats(P, tile, acc, c0, min(NACC, nc), records, sh.xs);
+    if (STATS) {
+        const uint32_t cols[NACC] = {c0, c0 + 1u};
+        publish_stats(P, tile, acc, cols, min(NACC, nc), records, c0 == 0u, sh.xs);
+    }
     else __syncthreads();   // every thread's flag is in
     if (tid == 0) P.fixup[tile] = sh.fix_bad;
 }
