@@ -643,10 +643,10 @@ DEV uint32_t block_fn(const BM& m, uint64_t ls) {
 // --------------------------------------------------------------------------
 // SWAR classification of 32 bytes (a thread's run in the SWAR front)
 // --------------------------------------------------------------------------
-// Quote, candidate, newline and comment bytes of a run, bit i for byte i
-// (comment bytes only for comment dialects).
+// Quote, candidate, newline, comment and escape bytes of a run, bit i for
+// byte i (comment and escape bytes only for the dialects that have them).
 struct Masks32 {
-    uint32_t q, c, n, h;
+    uint32_t q, c, n, h, e;
 };
 
 // Byte transpose of four words: byte k of o_j is byte j of a_k (v_perm_b32:
@@ -681,8 +681,8 @@ DEV uint32_t eq_ascii(uint32_t y7, uint32_t lo, uint32_t c4) { return lo & ~((y7
 
 // Words x[0..7] hold the run's bytes in order (byte i = byte i & 3 of
 // x[i >> 2]); NSEP separator bytes besides '\n' (all below 0x80, as is the
-// quote byte and the comment byte).
-template <bool HQ, int NSEP, bool CMT = false>
+// quote, comment and escape bytes).
+template <bool HQ, int NSEP, bool CMT = false, bool ESC = false>
 DEV Masks32 classify32(const uint32_t (&x)[8], const Params& P) {
     // y[j] byte k = run byte 8k + j: words x0 x2 x4 x6 hold bytes 8k + 0..3,
     // x1 x3 x5 x7 bytes 8k + 4..7.
@@ -694,7 +694,8 @@ DEV Masks32 classify32(const uint32_t (&x)[8], const Params& P) {
     const uint32_t s2 = (NSEP > 2 ? P.sepb[2] : 0u) * 0x01010101u, s3 = (NSEP > 3 ? P.sepb[3] : 0u) * 0x01010101u;
     const uint32_t nl_keep = P.nl_cand != 0u ? ~0u : 0u;   // '\n' flags kept among the candidates
     const uint32_t h4 = CMT ? (uint32_t)P.comment * 0x01010101u : 0u;
-    Masks32 m{0u, 0u, 0u, 0u};
+    const uint32_t e4 = ESC ? (uint32_t)P.escape * 0x01010101u : 0u;
+    Masks32 m{0u, 0u, 0u, 0u, 0u};
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
         const uint32_t y7 = y[j] & 0x7F7F7F7Fu, lo = ~y[j] & 0x80808080u;
@@ -708,15 +709,16 @@ DEV Masks32 classify32(const uint32_t (&x)[8], const Params& P) {
         m.c |= (ec >> 7) << j;
         if (HQ) m.q |= (eq_ascii(y7, lo, q4) >> 7) << j;
         if (CMT) m.h |= (eq_ascii(y7, lo, h4) >> 7) << j;
+        if (ESC) m.e |= (eq_ascii(y7, lo, e4) >> 7) << j;
     }
     return m;
 }
 
 // The same masks from the 256-entry candidate table, a byte at a time (more
 // than four separator bytes, or a byte of 0x80 or above).
-template <bool HQ, bool CMT = false>
+template <bool HQ, bool CMT = false, bool ESC = false>
 DEV Masks32 classify32_table(const uint32_t (&x)[8], const Params& P, const uint8_t* cls) {
-    Masks32 m{0u, 0u, 0u, 0u};
+    Masks32 m{0u, 0u, 0u, 0u, 0u};
     // A word at a time (not unrolled: four table reads in flight, not 32).
 #pragma unroll 1
     for (int k = 0; k < 8; ++k) {
@@ -731,10 +733,43 @@ DEV Masks32 classify32_table(const uint32_t (&x)[8], const Params& P, const uint
             m.n |= (uint32_t)(c == '\n') << i;
             if (HQ) m.q |= (uint32_t)(c == (uint32_t)P.quote) << i;
             if (CMT) m.h |= (uint32_t)(c == (uint32_t)P.comment) << i;
+            if (ESC) m.e |= (uint32_t)(c == (uint32_t)P.escape) << i;
         }
     }
     return m;
 }
+
+// The escape-dialect step (quote parity with escapes, no comment) of a
+// thread's 32-byte run from each of its four entry states s = c + 2r (c:
+// the previous byte escapes this run's first, r: inside quotes). The escape
+// add-carry of escaped_positions (formats.rs:334-343) and the quote parity
+// prefix are block_step's on 64-byte blocks cut at bit 32: both are chains
+// of carries and parities that run upward only, so two runs stepped in turn
+// give the block's masks, carry and state.
+struct EscRun {
+    uint32_t px0, px1, co0, co1;   // quote parity prefix and carry out, for c = 0 and c = 1
+    DEV void fill(const Masks32& m) {
+        constexpr uint32_t EVEN32 = 0x55555555u;
+        const uint32_t partial = EVEN32 + (EVEN32 ^ m.e);
+        const uint32_t c1 = partial < EVEN32 ? 1u : 0u;
+        const uint32_t sum1 = partial + 1u;
+        px0 = prefix_xor32(m.q & (EVEN32 ^ partial));
+        px1 = prefix_xor32(m.q & (EVEN32 ^ sum1));
+        co0 = c1;
+        co1 = c1 | (sum1 < partial ? 1u : 0u);
+    }
+    DEV uint32_t out(uint32_t s) const {
+        const uint32_t c = s & 1u, r = s >> 1;
+        return (c ? co1 : co0) + 2u * (((c ? px1 : px0) >> 31) ^ r);
+    }
+    DEV uint32_t fn() const { return out(0u) | (out(1u) << 4) | (out(2u) << 8) | (out(3u) << 12); }
+    // Structurals and terminators of the run entered in state s.
+    DEV void masks(const Masks32& m, uint32_t s, uint32_t& S, uint32_t& T) const {
+        const uint32_t inside = ((s & 1u) ? px1 : px0) ^ ((s >> 1) ? ~0u : 0u);
+        S = m.c & ~inside;
+        T = S & m.n;
+    }
+};
 
 // --------------------------------------------------------------------------
 // the tile kernel
@@ -1082,7 +1117,7 @@ __device__ __forceinline__ void scan_tile(const Params& P, const uint32_t tile, 
 
     // Dialects with neither escape nor comment take the SWAR front below; the
     // others classify by ballots with one lane per 64-byte block.
-    constexpr bool SWAR = !ESC;
+    constexpr bool SWAR = !(ESC && CMT);
     if constexpr (SWAR) {
         // ---- SWAR front: thread t holds tile bytes [32t, 32t + 32) ---------
         // The bytes come from HBM into registers (and into the LDS copy the
@@ -1132,21 +1167,23 @@ __device__ __forceinline__ void scan_tile(const Params& P, const uint32_t tile, 
         // Candidate and quote bytes below 0x80 with at most four separator
         // bytes take the word compares; other dialects the byte table.
         const uint32_t qa = HQ ? (uint32_t)P.quote : 0u, ha = CMT ? (uint32_t)P.comment : 0u;
-        const bool ascii = qa < 0x80u && ha < 0x80u && (P.n_sepb < 1 || P.sepb[0] < 0x80u) &&
+        const uint32_t ea = ESC ? (uint32_t)P.escape : 0u;
+        const bool ascii = qa < 0x80u && ha < 0x80u && ea < 0x80u && (P.n_sepb < 1 || P.sepb[0] < 0x80u) &&
                            (P.n_sepb < 2 || P.sepb[1] < 0x80u) && (P.n_sepb < 3 || P.sepb[2] < 0x80u) &&
                            (P.n_sepb < 4 || P.sepb[3] < 0x80u);
         switch (ascii ? P.n_sepb : 5u) {
-        case 0: m = classify32<HQ, 0, CMT>(x, P); break;
-        case 1: m = classify32<HQ, 1, CMT>(x, P); break;
-        case 2: m = classify32<HQ, 2, CMT>(x, P); break;
-        case 3: m = classify32<HQ, 3, CMT>(x, P); break;
-        case 4: m = classify32<HQ, 4, CMT>(x, P); break;
-        default: m = classify32_table<HQ, CMT>(x, P, sh.cls); break;
+        case 0: m = classify32<HQ, 0, CMT, ESC>(x, P); break;
+        case 1: m = classify32<HQ, 1, CMT, ESC>(x, P); break;
+        case 2: m = classify32<HQ, 2, CMT, ESC>(x, P); break;
+        case 3: m = classify32<HQ, 3, CMT, ESC>(x, P); break;
+        case 4: m = classify32<HQ, 4, CMT, ESC>(x, P); break;
+        default: m = classify32_table<HQ, CMT, ESC>(x, P, sh.cls); break;
         }
         m.q &= live32;
         m.c &= live32;
         m.n &= live32;
         m.h &= live32;
+        m.e &= live32;
         if constexpr (QBITS) qbits[tid] = m.q;
         if constexpr (CMT) {
             // Comment dialects: the NORMAL/QUOTE/COMMENT machine runs on the
@@ -1201,6 +1238,51 @@ __device__ __forceinline__ void scan_tile(const Params& P, const uint32_t tile, 
                 for (uint32_t rem = S; rem; rem &= rem - 1u) {
                     const uint32_t bit = (uint32_t)__ffs((int)rem) - 1u;
                     *lst++ = (uint16_t)((t0 + bit) | (((T >> bit) & 1u) << 14) | (((F >> bit) & 1u) << 15));
+                }
+            }
+        } else if constexpr (ESC) {
+            // Escape dialects (no comment): each run's four entry states
+            // (EscRun), the runs' functions scanned over the wave's lanes.
+            static_assert(NS == 4, "escape dialects without comment: four entry states");
+            EscRun er;
+            er.fill(m);
+            uint32_t f_incl = er.fn();
+#pragma unroll
+            for (int d = 1; d < 64; d <<= 1) {
+                const uint32_t g = __shfl_up(f_incl, d);
+                if (ln >= (uint32_t)d) f_incl = fn_then<NS>(g, f_incl);
+            }
+            uint32_t f_excl = __shfl_up(f_incl, 1);
+            if (ln == 0) f_excl = fn_ident<NS>();
+            __syncthreads();   // the LDS copy is complete
+            if (MODE == M_SUMMARY) {
+#pragma unroll
+                for (int s = 0; s < NS; ++s) {
+                    uint32_t S, T;
+                    er.masks(m, fn_apply(f_excl, (uint32_t)s), S, T);
+                    const Counts c = wave_counts((uint64_t)S, (uint64_t)T, 0ULL, tile_start + t0);
+                    if (ln == 0) wsum[w * NS + s] = c;
+                }
+                if (ln == 63) sh.wfn[w] = f_incl;
+                publish_summary();
+                return;
+            }
+            load_entry_states();
+            {
+                uint32_t S, T;
+                er.masks(m, fn_apply(f_excl, uni(sh.wentry[w])), S, T);
+                const uint32_t n = (uint32_t)__popc(S);
+                const uint32_t incl = wave_incl_sum(n);
+                if (ln == 63) sh.wcount[w] = incl;
+                __syncthreads();
+                uint32_t wb = 0;
+#pragma unroll
+                for (uint32_t k = 0; k < (uint32_t)NWAVE; ++k)
+                    if (k < w) wb += sh.wcount[k];
+                uint16_t* lst = sh.list + uni(wb) + (incl - n);
+                for (uint32_t rem = S; rem; rem &= rem - 1u) {
+                    const uint32_t bit = (uint32_t)__ffs((int)rem) - 1u;
+                    *lst++ = (uint16_t)((t0 + bit) | (((T >> bit) & 1u) << 14));
                 }
             }
         } else {
