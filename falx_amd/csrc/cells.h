@@ -88,6 +88,28 @@ struct DoubledStream {
     }
 };
 
+// The second quotes of the doubled pairs among the quote bytes Q of a
+// 32-byte word (bit i: byte i): the odd offsets within each run of quotes,
+// counted from the run's first byte, which is what a left-to-right
+// doubled-quote stream drops (DoubledStream in cells.h pairs a quote with the
+// next one, then goes on after the pair). A run's start parity is found by
+// adding its first bit to Q (the carry clears the run), so runs starting at
+// even bytes drop their odd bytes and the others their even bytes.
+// `in_run` and `odd_next` carry a run from the previous word: whether that
+// word ended in a quote, and whether the offset of this word's first byte in
+// that run is odd.
+FX_HD uint32_t dq_dropped(uint32_t Q, uint32_t& in_run, uint32_t& odd_next) {
+    constexpr uint32_t EVEN32 = 0x55555555u, ODD32 = 0xAAAAAAAAu;
+    const uint32_t cont = in_run & Q & 1u;                        // a run goes on into byte 0
+    const uint32_t starts = Q & ~((Q << 1) | cont);
+    const uint32_t s_even = (starts & EVEN32) | (cont & (odd_next ^ 1u));
+    const uint32_t r_even = Q & ~(Q + s_even);                    // runs of even start parity
+    const uint32_t r_odd = Q & ~r_even;
+    in_run = Q >> 31;
+    odd_next = (r_odd >> 31) & 1u;
+    return (r_even & ODD32) | (r_odd & EVEN32);
+}
+
 // Backslash dialect: outer quotes stripped when present at both ends, then
 // `E x` -> `x` (a trailing lone escape byte is kept).
 template <class Src>

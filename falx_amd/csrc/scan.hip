@@ -1689,33 +1689,45 @@ __device__ __forceinline__ void scan_tile(const Params& P, const uint32_t tile, 
     // A cell of the tile quoted at both ends with no other quote byte (read
     // from the tile's quote bitmap, a word per 32 bytes): it cleans to
     // [from + 1, to - 1).
-    auto simple_quoted = [&](uint32_t from, uint32_t to) -> bool {
+    // The inner quote bytes of the tile bytes [a, z] (tile-relative, a <= z)
+    // in word k, from the bitmap.
+    auto qword = [&](uint32_t k, uint32_t a, uint32_t z) -> uint32_t {
+        uint32_t wv = qbits[k];
+        if (k == (a >> 5)) wv &= ~0u << (a & 31u);
+        if (k == (z >> 5)) wv &= (z & 31u) == 31u ? ~0u : (2u << (z & 31u)) - 1u;
+        return wv;
+    };
+    // A cell of the tile quoted at both ends: `drops` is the number of quote
+    // bytes its doubled pairs lose (dq_dropped over its inside), so it cleans
+    // to its inside less those; with none it cleans to [from + 1, to - 1).
+    auto quoted_in_tile = [&](uint32_t from, uint32_t to, uint32_t& drops) -> bool {
         if constexpr (!QBITS) {
             return false;
         } else {
             if (from < tile_start || to > tile_start + tile_len || to - from < 2u) return false;
-            const uint32_t a = from - tile_start, z = to - 1u - tile_start;   // first and last byte
+            const uint32_t a = from - tile_start, z = to - 1u - tile_start;   // the outer quotes
             if (((qbits[a >> 5] >> (a & 31u)) & (qbits[z >> 5] >> (z & 31u)) & 1u) == 0u) return false;
-            uint32_t n = 0;
-            for (uint32_t k = a >> 5; k <= (z >> 5) && n <= 2u; ++k) {
-                uint32_t wv = qbits[k];
-                if (k == (a >> 5)) wv &= ~0u << (a & 31u);
-                if (k == (z >> 5)) wv &= (z & 31u) == 31u ? ~0u : (2u << (z & 31u)) - 1u;
-                n += (uint32_t)__popc(wv);
+            drops = 0;
+            if (z > a + 1u) {
+                uint32_t in_run = 0, odd_next = 0;
+                for (uint32_t k = (a + 1u) >> 5; k <= ((z - 1u) >> 5); ++k)
+                    drops += (uint32_t)__popc(dq_dropped(qword(k, a + 1u, z - 1u), in_run, odd_next));
             }
-            return n == 2u;
+            return true;
         }
     };
     // The bytes a cell cleans to when they are a run of the document: the
-    // cell itself (raw), or its inside (simple_quoted). False: it needs the
-    // cleaning stream.
-    auto run_cell = [&](uint32_t from, uint32_t to, uint32_t& rf, uint32_t& rt) -> bool {
+    // cell itself (raw), or its inside (quoted with no doubled quote). False:
+    // `drops` tells whether the doubled-quote copy (quoted in the tile) or
+    // the cleaning stream (drops = ~0) makes it.
+    auto run_cell = [&](uint32_t from, uint32_t to, uint32_t& rf, uint32_t& rt, uint32_t& drops) -> bool {
+        drops = ~0u;
         if (raw_cell(from, to)) {
             rf = from;
             rt = to;
             return true;
         }
-        if (simple_quoted(from, to)) {
+        if (quoted_in_tile(from, to, drops) && drops == 0u) {
             rf = from + 1u;
             rt = to - 1u;
             return true;
@@ -1723,8 +1735,10 @@ __device__ __forceinline__ void scan_tile(const Params& P, const uint32_t tile, 
         return false;
     };
     auto clean_len = [&](uint32_t from, uint32_t to) -> uint32_t {
-        uint32_t rf, rt;
-        return run_cell(from, to, rf, rt) ? rt - rf : clean_cell(src, from, to, cd, [](uint8_t) {});
+        uint32_t rf, rt, drops;
+        if (run_cell(from, to, rf, rt, drops)) return rt - rf;
+        if (drops != ~0u) return to - from - 2u - drops;
+        return clean_cell(src, from, to, cd, [](uint8_t) {});
     };
     auto field_end = [&](const Entry& e) -> uint32_t {
         return (e.is_term && e.pos > e.rs && src.at(e.pos - 1) == '\r') ? e.pos - 1 : e.pos;
@@ -2204,9 +2218,9 @@ __device__ __forceinline__ void scan_tile(const Params& P, const uint32_t tile, 
                     cell_of(R, c.index, here, missing, fs, fe);
                     // A cell that cleans to a run of the document [rf, rt)
                     // is copied as it stands.
-                    uint32_t rf = 0, rt = 0;
-                    const bool raw = here && run_cell(fs, fe, rf, rt);
-                    const uint32_t L = here ? (raw ? rt - rf : clean_len(fs, fe)) : 0u;
+                    uint32_t rf = 0, rt = 0, drops = ~0u;
+                    const bool raw = here && run_cell(fs, fe, rf, rt, drops);
+                    const uint32_t L = here ? (raw ? rt - rf : (drops != ~0u ? fe - fs - 2u - drops : clean_len(fs, fe))) : 0u;
                     uint32_t rtot;
                     const uint32_t at = run + block_excl_sum(L, sh.sum_sh, rtot);
                     const bool live = R.row < rows_cap;
@@ -2230,6 +2244,21 @@ __device__ __forceinline__ void scan_tile(const Params& P, const uint32_t tile, 
                                 const uint8_t b = src.at(rf + k);
                                 put(at + k, b);
                                 s_sum += b;
+                            }
+                        } else if (drops != ~0u) {
+                            // Doubled quotes, the cell in the tile: its inside
+                            // less the dropped quotes, a bitmap word at a time.
+                            const uint32_t a = fs + 1u - tile_start, z = fe - 2u - tile_start;
+                            uint32_t q = at, in_run = 0, odd_next = 0;
+                            for (uint32_t k = a >> 5; k <= (z >> 5); ++k) {
+                                const uint32_t d = dq_dropped(qword(k, a, z), in_run, odd_next);
+                                const uint32_t p0 = max(a, k << 5), p1 = min(z, (k << 5) + 31u);
+                                for (uint32_t p = p0; p <= p1; ++p) {
+                                    if ((d >> (p & 31u)) & 1u) continue;
+                                    const uint8_t b = tb[p];
+                                    put(q++, b);
+                                    s_sum += b;
+                                }
                             }
                         } else {
                             uint32_t q = at;

@@ -59,3 +59,13 @@ def test_block_regions_match_byte_automaton(ora, seed):
     r = subprocess.run([exe, str(seed), "400000"], capture_output=True, text=True)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "agree" in r.stdout
+
+
+@pytest.mark.parametrize("seed", [5, 0xD0B1E])
+def test_doubled_quote_drops_match_stream(ora, seed):
+    """dq_dropped (cells.h), the doubled-quote drops the string passes read
+    from a tile's quote bitmap, against DoubledStream on random insides."""
+    exe = _build("dq_check", ora)
+    r = subprocess.run([exe, str(seed), "200000"], capture_output=True, text=True)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "agree" in r.stdout
