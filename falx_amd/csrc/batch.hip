@@ -532,7 +532,11 @@ falx_status falx_batch_submit(falx_batch* b, const falx_request* reqs, uint32_t 
     };
     static_assert(N_TAB <= 64, "signature bits");
     if (sig) {
-        if (b->tune.use_graph) {
+        // Batches of at least use_graph documents run their launch sequence
+        // as a graph, captured the first time the sequence is needed; smaller
+        // ones (light load makes many, of varied stage sets) launch it as it
+        // is, which costs less than instantiating a graph on their latency.
+        if (b->tune.use_graph && n >= b->tune.use_graph) {
             hipGraphExec_t exec = nullptr;
             for (auto& g : b->graphs)
                 if (g.first == sig) exec = g.second;

@@ -55,6 +55,7 @@ SERVED = [
     ("ndjson", S.OP_NDJSON_ID_SCORE, None, datagen.ndjson),
 ]
 MIN_ROW = 8   # no record of the served generators is shorter: rows <= len / 8 + 2
+GRAPH_MIN = 8  # batches of at least this many requests run as captured graphs
 
 
 @dataclass
@@ -252,10 +253,11 @@ class BatchServer:
 
     def __init__(self, corpus: Corpus, slots: int = 2, hbm_budget: int = 8 * GiB, max_batch: int = 256,
                  tuning: Optional[Dict] = None, large: int = 0, large_out_budget: int = 2 * GiB,
-                 exclusive: bool = False):
+                 exclusive: bool = False, hold_below: int = 0):
         self.corpus = corpus
         self.large = large
         self.exclusive = exclusive
+        self.hold_below = hold_below
         self.lane = _LargeLane(corpus, large_out_budget) if large else None
         self.max_batch = max_batch
         self.tuning = tuning or {}
@@ -266,13 +268,16 @@ class BatchServer:
             scratch = per // 4
             b = gpu.Batch(ctx, scratch)
             t = self.tuning.get("executor")
+            bt = b.tuning()
             if t:
-                bt = b.tuning()
                 bt.tile_workgroups = t["tile_workgroups"]
                 bt.unit_workgroups = t["unit_workgroups"]
                 for k, v in enumerate(t["chunk"]):
                     bt.chunk[k] = v
-                b.set_tuning(bt)
+            # Batches of GRAPH_MIN requests or more replay captured graphs;
+            # the small batches of light load launch their stages directly.
+            bt.use_graph = GRAPH_MIN
+            b.set_tuning(bt)
             self.slots.append(_Slot(ctx, b, Arena(ctx, per - scratch), per))
         self._scratch = {}
 
@@ -375,8 +380,11 @@ class BatchServer:
                     progressed = True
             # With `exclusive`, small requests wait while a large one runs
             # (as they would behind it in arrival order) instead of sharing
-            # the device with it.
-            hold = self.exclusive and self.lane is not None and self.lane.busy > 0
+            # the device with it; with `hold_below`, they wait only while
+            # fewer than that many are queued (light load keeps the large
+            # request's latency, a backlog shares the device).
+            lane_busy = self.lane is not None and self.lane.busy > 0
+            hold = lane_busy and (self.exclusive or len(queue) < self.hold_below)
             # One queued request on an idle device: a batch of one would only
             # add the batch set-up to its latency, so it is made as a plain
             # call here (on the first slot's context and output arena).

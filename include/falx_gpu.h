@@ -490,7 +490,9 @@ typedef struct falx_batch_tuning {
     uint32_t tile_workgroups;
     uint32_t unit_workgroups;
     uint32_t chunk[8];
-    uint32_t use_graph;           /* capture and replay the launch sequence */
+    uint32_t use_graph;           /* 0: launch the stages directly; k >= 1: batches of at least k
+                                     documents capture their launch sequence in a hipGraph (once
+                                     per set of stages) and replay it */
 } falx_batch_tuning;
 
 typedef struct falx_batch falx_batch;

@@ -122,8 +122,9 @@ def _check(ctx, docs, results, label):
 def test_mixed_batch_matches_plain_entry_points(ctx):
     rng = Rng(0xBA7C4)
     b = gpu.Batch(ctx, SCRATCH)
-    # Replays of cached launch sequences with fresh documents each time.
-    for rep in range(3):
+    # Replays of cached launch sequences with fresh documents each time
+    # (use_graph = 1: every batch runs as a graph, captured the first time).
+    for rep in range(4):
         docs = [Doc(ctx, op, spec, data) for op, spec, data in _docs(rng, 6 << 20)]
         b.submit([d.request() for d in docs])
         results = b.wait()
