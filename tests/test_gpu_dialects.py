@@ -245,6 +245,45 @@ def test_columns_every_seam_offset(ctx, ora, spec):
             assert_columns_equal(ctx.parse_columns(spec, buf, len(doc)), want, spec, f"pad {pad} {rec!r}")
 
 
+def _open_record_doc(rng: Rng, lo: int, hi: int) -> bytes:
+    """Geo-schema rows whose City is lo..hi bytes, quoted and holding commas,
+    newlines and doubled quotes, or plain: tiles open inside such records."""
+    out = bytearray(b"id,City,Latitude,Longitude\n")
+    k = 0
+    while len(out) < 3 * TILE + 777:
+        n = lo + rng.below(hi - lo + 1)
+        if rng.below(2):
+            body = bytearray()
+            while len(body) < n:
+                t = rng.below(12)
+                body += b"," if t == 0 else (b"\n" if t == 1 else (b'""' if t == 2 else bytes([97 + rng.below(26)])))
+            city = b'"' + bytes(body) + b'"'
+        else:
+            city = bytes(97 + rng.below(26) for _ in range(n))
+        out += b"%d,%s,%d.%06d,-%d.%06d\n" % (k, city, rng.below(90), rng.below(10 ** 6), rng.below(180),
+                                                rng.below(10 ** 6))
+        k += 1
+    return bytes(out)
+
+
+@pytest.mark.parametrize("lo,hi", [(20, 200), (300, 2500)], ids=["within_reach", "beyond_reach"])
+def test_numeric_columns_open_records(ctx, ora, lo, hi):
+    """Numeric columns of records open across tile starts: quoted fields with
+    commas, newlines and doubled quotes just before the seam, records of up
+    to 256 bytes and records reaching several KiB back. Columns and fused
+    statistics against the oracle."""
+    rng = Rng(0x0BE2 + lo)
+    for spec in (S.GEO_LATLON,):
+        for i in range(6):
+            doc = _open_record_doc(rng, lo, hi)
+            buf = ctx.upload(doc)
+            want = ora.parse_columns(spec, doc)
+            assert_columns_equal(ctx.parse_columns(spec, buf, len(doc)), want, spec, f"doc {i}")
+            cols, keep = ctx.alloc_columns(spec, want["rows"], len(doc))
+            st = ctx.parse_columns_with_stats_into(spec, buf, len(doc), cols)
+            assert_stats_equal(st, ora.csv_stats(spec, doc), spec, f"doc {i}")
+
+
 # ---- NDJSON records longer than a tile ---------------------------------------
 def _nd_record(rng: Rng, name_len: int) -> bytes:
     name = bytearray()
