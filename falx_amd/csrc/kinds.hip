@@ -26,8 +26,7 @@ hipError_t plain(dim3 g, hipStream_t s, const Params* P) {
 // The SWAR kinds' dedicated summary and numeric fast passes (swar_tile.h).
 template <int MODE>
 hipError_t swar(dim3 g, hipStream_t s, const Params* P) {
-    if constexpr (MODE == M_SUMMARY)
-        hipLaunchKernelGGL((k_swar_summary<K_HQ>), dim3((g.x + SUM_TILES - 1) / SUM_TILES), dim3(WG), 0, s, *P);
+    if constexpr (MODE == M_SUMMARY) hipLaunchKernelGGL((k_swar_summary<K_HQ>), g, dim3(WG), 0, s, *P);
     else hipLaunchKernelGGL((k_numfast<K_HQ, MODE>), g, dim3(WG), 0, s, *P);
     return hipGetLastError();
 }

@@ -98,79 +98,56 @@ DEV uint32_t live_run(uint32_t t0, uint32_t tile_len) {
 // ---------------------------------------------------------------------------
 // Tile summaries
 // ---------------------------------------------------------------------------
-// A workgroup summarises SUM_TILES consecutive tiles: every tile's bytes are
-// requested before the first is classified, so the loads of the later ones
-// are in flight while the earlier ones are counted.
-constexpr uint32_t SUM_TILES = 2;
-static_assert(SUM_TILES <= (uint32_t)NWAVE, "a wave finishes each tile");
-
 template <bool HQ>
 __global__ __launch_bounds__(WG) void k_swar_summary(Params P) {
     constexpr int NS = HQ ? 2 : 1;
-    __shared__ Counts wsum[SUM_TILES][NWAVE * NS];
-    __shared__ uint32_t wfn[SUM_TILES][NWAVE];
+    __shared__ Counts wsum[NWAVE * NS];
+    __shared__ uint32_t wfn[NWAVE];
     __shared__ uint8_t cls[256];
-    const uint32_t tid = threadIdx.x, ln = tid & 63u, w = uni(tid >> 6);
-    const uint32_t tile0 = blockIdx.x * SUM_TILES;
+    const uint32_t tile = blockIdx.x, tid = threadIdx.x, ln = tid & 63u, w = uni(tid >> 6);
+    const uint32_t tile_start = tile * (uint32_t)TILE, tile_len = min((uint32_t)TILE, P.len - tile_start);
     const bool ascii = dialect_ascii<HQ>(P);
     if (!ascii) {
         cls[tid] = (uint8_t)((P.cand[tid >> 5] >> (tid & 31)) & 1u);
         __syncthreads();
     }
     const uint32_t t0 = tid * 32u;
-    uint32_t x[SUM_TILES][8];
-#pragma unroll
-    for (uint32_t j = 0; j < SUM_TILES; ++j) {
-        const uint32_t tile = tile0 + j;
-        if (tile < P.n_tiles) {
-            const uint32_t tile_start = tile * (uint32_t)TILE;
-            load_run(P, tile_start, min((uint32_t)TILE, P.len - tile_start), t0, x[j]);
+    uint32_t x[8];
+    load_run(P, tile_start, tile_len, t0, x);
+    const Masks32 m = classify_run<HQ>(x, P, cls, ascii, live_run(t0, tile_len));
+    // Quote parity from the thread's first byte; the quotes of the wave's
+    // earlier threads by mbcnt over the odd ones.
+    const uint32_t pref = HQ ? prefix_xor32(m.q) : 0u;
+    const uint64_t odd = ballot(HQ && (__popc(m.q) & 1));
+    const uint32_t ex = mbcnt(odd) & 1u;
+    if constexpr (HQ) {
+        Counts c0, c1;
+        wave_counts_pair(m.c, m.c & m.n, pref ^ (ex ? ~0u : 0u), tile_start + t0, c0, c1);
+        if (ln == 0) {
+            wsum[w * NS + 0] = c0;
+            wsum[w * NS + 1] = c1;
         }
+    } else {
+        const Counts c = wave_counts((uint64_t)m.c, (uint64_t)(m.c & m.n), 0ULL, tile_start + t0);
+        if (ln == 0) wsum[w * NS] = c;
     }
-#pragma unroll
-    for (uint32_t j = 0; j < SUM_TILES; ++j) {
-        const uint32_t tile = tile0 + j;
-        if (tile >= P.n_tiles) break;
-        const uint32_t tile_start = tile * (uint32_t)TILE, tile_len = min((uint32_t)TILE, P.len - tile_start);
-        const Masks32 m = classify_run<HQ>(x[j], P, cls, ascii, live_run(t0, tile_len));
-        // Quote parity from the thread's first byte; the quotes of the wave's
-        // earlier threads by mbcnt over the odd ones.
-        const uint32_t pref = HQ ? prefix_xor32(m.q) : 0u;
-        const uint64_t odd = ballot(HQ && (__popc(m.q) & 1));
-        const uint32_t ex = mbcnt(odd) & 1u;
-        if constexpr (HQ) {
-            Counts c0, c1;
-            wave_counts_pair(m.c, m.c & m.n, pref ^ (ex ? ~0u : 0u), tile_start + t0, c0, c1);
-            if (ln == 0) {
-                wsum[j][w * NS + 0] = c0;
-                wsum[j][w * NS + 1] = c1;
-            }
-        } else {
-            const Counts c = wave_counts((uint64_t)m.c, (uint64_t)(m.c & m.n), 0ULL, tile_start + t0);
-            if (ln == 0) wsum[j][w * NS] = c;
-        }
-        if (ln == 0) wfn[j][w] = NS == 2 ? ((popc(odd) & 1u) ? 0x01u : 0x10u) : 0u;
-    }
+    if (ln == 0) wfn[w] = NS == 2 ? ((popc(odd) & 1u) ? 0x01u : 0x10u) : 0u;
     __syncthreads();
-    // Tile j's counters per entry state from wave j's first lanes, its
-    // functions from wave j's lane 32.
-    const uint32_t j = w, tile = tile0 + j;
-    if (j >= SUM_TILES || tile >= P.n_tiles) return;
     uint32_t* out = P.tsum + (size_t)tile * P.ts_stride;
-    if (ln < (uint32_t)NS) {
-        uint32_t s = ln;
+    if (tid < (uint32_t)NS) {
+        uint32_t s = tid;
         Counts c = counts_zero();
         for (int k = 0; k < NWAVE; ++k) {
-            c = compose(c, wsum[j][k * NS + s]);
-            s = fn_apply(wfn[j][k], s);
+            c = compose(c, wsum[k * NS + s]);
+            s = fn_apply(wfn[k], s);
         }
-        c7_store(out + TS_COUNTS + C7 * ln, c);
+        c7_store(out + TS_COUNTS + C7 * tid, c);
     }
-    if (ln == 32) {
+    if (tid == 0) {
         uint32_t f = fn_ident<NS>();
         for (int k = 0; k < NWAVE; ++k) {
-            f = fn_then<NS>(f, wfn[j][k]);
-            out[1 + k] = wfn[j][k];
+            f = fn_then<NS>(f, wfn[k]);
+            out[1 + k] = wfn[k];
         }
         out[0] = f;
     }
