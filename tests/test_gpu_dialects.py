@@ -245,23 +245,29 @@ def test_columns_every_seam_offset(ctx, ora, spec):
             assert_columns_equal(ctx.parse_columns(spec, buf, len(doc)), want, spec, f"pad {pad} {rec!r}")
 
 
-def _open_record_doc(rng: Rng, lo: int, hi: int) -> bytes:
-    """Geo-schema rows whose City is lo..hi bytes, quoted and holding commas,
-    newlines and doubled quotes, or plain: tiles open inside such records."""
-    out = bytearray(b"id,City,Latitude,Longitude\n")
+TSV_OPEN = S.TSV.with_columns([(0, S.I64), (2, S.F64), (3, S.F64)], "tsv_open")                      # kind 0
+
+
+def _open_record_doc(rng: Rng, lo: int, hi: int, sep: bytes = b",") -> bytes:
+    """Geo-schema rows whose second field is lo..hi bytes: with sep ',' half
+    of them quoted and holding commas, newlines and doubled quotes; with a
+    tab, plain text. Tiles open inside such records."""
+    quoted = sep == b","
+    out = bytearray(sep.join([b"id", b"City", b"Latitude", b"Longitude"]) + b"\n")
     k = 0
     while len(out) < 3 * TILE + 777:
         n = lo + rng.below(hi - lo + 1)
-        if rng.below(2):
+        if quoted and rng.below(2):
             body = bytearray()
             while len(body) < n:
                 t = rng.below(12)
                 body += b"," if t == 0 else (b"\n" if t == 1 else (b'""' if t == 2 else bytes([97 + rng.below(26)])))
             city = b'"' + bytes(body) + b'"'
         else:
-            city = bytes(97 + rng.below(26) for _ in range(n))
-        out += b"%d,%s,%d.%06d,-%d.%06d\n" % (k, city, rng.below(90), rng.below(10 ** 6), rng.below(180),
-                                                rng.below(10 ** 6))
+            city = bytes(97 + rng.below(26) if rng.below(8) else 32 for _ in range(n))
+        lat = b"%d.%06d" % (rng.below(90), rng.below(10 ** 6))
+        lon = b"-%d.%06d" % (rng.below(180), rng.below(10 ** 6))
+        out += sep.join([b"%d" % k, city, lat, lon]) + b"\n"
         k += 1
     return bytes(out)
 
@@ -269,13 +275,14 @@ def _open_record_doc(rng: Rng, lo: int, hi: int) -> bytes:
 @pytest.mark.parametrize("lo,hi", [(20, 200), (300, 2500)], ids=["within_reach", "beyond_reach"])
 def test_numeric_columns_open_records(ctx, ora, lo, hi):
     """Numeric columns of records open across tile starts: quoted fields with
-    commas, newlines and doubled quotes just before the seam, records of up
-    to 256 bytes and records reaching several KiB back. Columns and fused
-    statistics against the oracle."""
+    commas, newlines and doubled quotes just before the seam (CSV, kind 2),
+    plain long fields (TSV, kind 0), records of up to 256 bytes and records
+    reaching several KiB back. Columns and fused statistics against the
+    oracle."""
     rng = Rng(0x0BE2 + lo)
-    for spec in (S.GEO_LATLON,):
+    for spec, sep in ((S.GEO_LATLON, b","), (TSV_OPEN, b"\t")):
         for i in range(6):
-            doc = _open_record_doc(rng, lo, hi)
+            doc = _open_record_doc(rng, lo, hi, sep)
             buf = ctx.upload(doc)
             want = ora.parse_columns(spec, doc)
             assert_columns_equal(ctx.parse_columns(spec, buf, len(doc)), want, spec, f"doc {i}")
